@@ -1,0 +1,129 @@
+/* isg.h -- C ABI of the MI355X MLFMA / BiCGSTAB / DBIM solver (libisg.so).
+ *
+ * Drop-in boundary for the hot path of merthidayetoglu/inverse-solver-GPU.
+ * The reference exposes a C++ call surface over mutable globals (vars.h:14-26,
+ * configuration in mom.cpp:4-69 and setup_mlfma.cpp:3-4).  This header keeps
+ * the same operations and argument meaning with an explicit context, plain
+ * pointers and sizes:
+ *
+ *   reference (vars.h)                          this ABI
+ *   setup_mlfma() + setup_gpu() + setup_bicgs() isg_create()
+ *     (setup_mlfma.cpp:50, gpu.cu:69, bicgs_gpu.cu:89)
+ *   mlfma(x, r)       matvec_gpu.cu:65-79       isg_mlfma()
+ *   mlfmah(x, r)      matvec_gpu.cu:80-101      isg_mlfmah()
+ *   aggregate(x, r)   matvec_gpu.cu:266-368     isg_aggregate()
+ *   aggregateh(x, r)  matvec_gpu.cu:369-470     isg_aggregateh()   (x is the output)
+ *   bicgs(x, o, b, h) bicgs_gpu.cu:116-224      isg_bicgs()
+ *   DBIM loop         mom.cpp:222-400           isg_dbim_*()
+ *   incident fields / measurements  setup_born.cpp:62-65,139-168  isg_dbim_setup()
+ *
+ * Complex arrays are interleaved (re, im) doubles -- the memory layout of
+ * std::complex<double> -- over unknowns in the reference's tree order (Morton
+ * leaf boxes of 8x8 pixels, setup_mlfma.cpp:124-153); isg_geometry() returns
+ * the raster-to-tree map of setup_mlfma.cpp:177-185.  Host-pointer entry points
+ * copy in and out; *_dev entry points take device pointers and run on the
+ * context's stream (isg_stream()).
+ *
+ * Every entry point returns 0 on success, a negative ISG_E* code on error
+ * (isg_last_error() gives the message), and bicgs returns a positive status
+ * when a system did not converge (the reference only prints, bicgs_gpu.cu:216-219).
+ * One context is used by one host thread at a time.
+ */
+#ifndef ISG_H
+#define ISG_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ISG_OK 0
+#define ISG_E_ARG (-1)
+#define ISG_E_DEVICE (-2)
+#define ISG_E_STATE (-3)
+
+#define ISG_SOLVE_CONVERGED 0
+#define ISG_SOLVE_NOT_CONVERGED 1
+#define ISG_SOLVE_RHO_BREAKDOWN 2
+#define ISG_SOLVE_OMEGA_BREAKDOWN 3
+
+typedef struct isg_context isg_context;
+
+/* Configuration.  Defaults (isg_config_default) equal the reference's
+ * compile-time constants: res 0.1 (mom.cpp:4), box 8 (:5), level 7 (:6),
+ * toli 1e-4 (:21), iti 300 (:22), regular 0 (:32), beta 5 and ninter 30
+ * (setup_mlfma.cpp:3-4); numrx/numtx/scale come from the environment in the
+ * reference (mom.cpp:84-97). */
+typedef struct isg_config {
+  int level;       /* quadtree levels counting the root; side = 8 * 2^(level-1) pixels */
+  int box;         /* pixels per leaf-box side; must be 8 */
+  double res;      /* pixel size in wavelengths */
+  double scale;    /* frequency scale: k0 = 2 pi scale */
+  int beta;        /* excess-bandwidth parameter of the sample-count rule */
+  int ninter;      /* Lagrange interpolation taps */
+  int numrx;       /* far-field receivers, equispaced on 360 degrees */
+  int numtx;       /* plane-wave transmitters, equispaced on 360 degrees */
+  double toli;     /* BiCGSTAB relative residual */
+  int iti;         /* BiCGSTAB iteration limit */
+  double regular;  /* Tikhonov weight: the gradient adds regular^2 * o per transmitter */
+  int max_rhs;     /* right-hand sides solved together (1 = the reference's plain loop) */
+  int precision;   /* 0: FP64 operator (plain).  Reserved for the mixed path. */
+} isg_config;
+
+/* Per-solve result. */
+typedef struct isg_solve_info {
+  int status;      /* ISG_SOLVE_* */
+  int iters;       /* BiCGSTAB iterations (bicgs_gpu.cu:220) */
+  double relres;   /* final recursive relative residual */
+} isg_solve_info;
+
+void isg_config_default(isg_config* cfg);
+int isg_create(const isg_config* cfg, isg_context** out);
+void isg_destroy(isg_context* ctx);
+const char* isg_last_error(const isg_context* ctx);
+
+/* sizes: numunk (setup: mom.cpp:137-138), numsamp per level (setup_mlfma.cpp:80-85) */
+int isg_numunk(const isg_context* ctx);
+int isg_level(const isg_context* ctx);
+int isg_numsamp(const isg_context* ctx, int level);
+double isg_k0(const isg_context* ctx);
+/* pixel centres in tree order (2*numunk doubles) and raster->tree map (numunk ints); either may be NULL */
+int isg_geometry(const isg_context* ctx, double* pos, int* raster2tree);
+/* the HIP stream (hipStream_t) all device work of this context runs on */
+void* isg_stream(const isg_context* ctx);
+
+/* ---- reference operator surface (host arrays) ---- */
+/* r = G x  (solve.cpp:25-248 / matvec_gpu.cu:65-79) */
+int isg_mlfma(isg_context* ctx, const double* x, double* r);
+/* r = conj(G conj x); x is not modified (the reference conjugates it in place and restores it) */
+int isg_mlfmah(isg_context* ctx, const double* x, double* r);
+/* r[numrx] = far-field pattern of x at the receiver angles */
+int isg_aggregate(isg_context* ctx, const double* x, double* r);
+/* x[numunk] = receiver back-projection of r[numrx] (output first, as the reference) */
+int isg_aggregateh(isg_context* ctx, double* x, const double* r);
+/* BiCGSTAB on (I - G diag(o)) x = b, or the adjoint system when herm != 0
+ * (bicgs.cpp:62-87); x is the initial guess on entry and the solution on exit.
+ * o is the weighted contrast o*k0^2 as passed by the reference's callers. */
+int isg_bicgs(isg_context* ctx, double* x, const double* o, const double* b, int herm, isg_solve_info* info);
+
+/* ---- device-pointer, batched surface (vectors [nrhs][numunk], receivers [nrhs][numrx]) ---- */
+int isg_mlfma_dev(isg_context* ctx, const void* x, void* r, int nrhs);
+int isg_aggregate_dev(isg_context* ctx, const void* x, void* r, int nrhs);
+int isg_aggregateh_dev(isg_context* ctx, void* x, const void* r, int nrhs);
+/* o_stride 0: one contrast shared by every right-hand side; numunk: one per right-hand side.
+ * x_zero != 0 promises x == 0 on entry (skips the initial matvec). info has nrhs entries. */
+int isg_bicgs_dev(isg_context* ctx, void* x, const void* o, long o_stride, const void* b, int herm, int nrhs,
+                  int x_zero, isg_solve_info* info);
+
+/* multipoles (which 0) or locals (which 1) of one level after the last operator call,
+ * 4^level * numsamp[level] complex values (stage checkpoints of solve.cpp:218-245) */
+int isg_stage(isg_context* ctx, int which, int level, int rhs, double* out);
+
+/* operator applications and BiCGSTAB counters since creation:
+ * out[0] single-RHS matvecs (mom.cpp:61), out[1] solves, out[2] solver iterations */
+int isg_counters(const isg_context* ctx, long long* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ISG_H */

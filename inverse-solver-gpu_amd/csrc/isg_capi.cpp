@@ -1,0 +1,264 @@
+// isg_capi.cpp -- extern "C" entry points of libisg.so (declared in include/isg.h).
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/isg.h"
+#include "isg_engine.hpp"
+
+struct isg_context {
+  isg_config cfg;
+  std::unique_ptr<isg::Engine> eng;
+  std::string err;
+  // host-pointer staging buffers (device), single right-hand side
+  isg::zd* dx = nullptr;
+  isg::zd* dr = nullptr;
+  isg::zd* dg = nullptr;
+  isg::zd* drx = nullptr;
+};
+
+namespace {
+
+int fail(isg_context* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+bool check(isg_context* c) {
+  if (!c || !c->eng) return false;
+  if (!c->eng->ok()) {
+    c->err = c->eng->error();
+    return false;
+  }
+  return true;
+}
+
+bool hip_ok(isg_context* c, hipError_t e, const char* what) {
+  if (e == hipSuccess) return true;
+  c->err = std::string(what) + ": " + hipGetErrorString(e);
+  return false;
+}
+
+int sync(isg_context* c) {
+  if (!hip_ok(c, hipStreamSynchronize(c->eng->stream()), "stream sync")) return ISG_E_DEVICE;
+  return check(c) ? ISG_OK : ISG_E_DEVICE;
+}
+
+size_t vbytes(isg_context* c) { return sizeof(isg::zd) * (size_t)c->eng->plan().N; }
+
+}  // namespace
+
+extern "C" {
+
+void isg_config_default(isg_config* c) {
+  if (!c) return;
+  c->level = 7;
+  c->box = 8;
+  c->res = 0.1;
+  c->scale = 1.0;
+  c->beta = 5;
+  c->ninter = 30;
+  c->numrx = 16;
+  c->numtx = 16;
+  c->toli = 1e-4;
+  c->iti = 300;
+  c->regular = 0.0;
+  c->max_rhs = 1;
+  c->precision = 0;
+}
+
+int isg_create(const isg_config* cfg, isg_context** out) {
+  if (!cfg || !out) return ISG_E_ARG;
+  *out = nullptr;
+  if (cfg->max_rhs < 1) return ISG_E_ARG;
+  if (cfg->precision != 0) return ISG_E_ARG;
+  isg_context* c = new isg_context();
+  c->cfg = *cfg;
+  isg::EngineConfig ec;
+  ec.plan.level = cfg->level;
+  ec.plan.box = cfg->box;
+  ec.plan.res = cfg->res;
+  ec.plan.scale = cfg->scale;
+  ec.plan.beta = cfg->beta;
+  ec.plan.ninter = cfg->ninter;
+  ec.plan.numrx = cfg->numrx;
+  ec.max_rhs = cfg->max_rhs;
+  ec.precision = cfg->precision;
+  c->eng.reset(new isg::Engine(ec));
+  if (!check(c)) {
+    *out = c;  // keep the context so the caller can read the error
+    return ISG_E_DEVICE;
+  }
+  size_t vb = vbytes(c);
+  if (!hip_ok(c, hipMalloc((void**)&c->dx, vb), "hipMalloc") || !hip_ok(c, hipMalloc((void**)&c->dr, vb), "hipMalloc") ||
+      !hip_ok(c, hipMalloc((void**)&c->dg, vb), "hipMalloc") ||
+      !hip_ok(c, hipMalloc((void**)&c->drx, sizeof(isg::zd) * (size_t)cfg->numrx), "hipMalloc")) {
+    *out = c;
+    return ISG_E_DEVICE;
+  }
+  *out = c;
+  return ISG_OK;
+}
+
+void isg_destroy(isg_context* c) {
+  if (!c) return;
+  c->eng.reset();
+  if (c->dx) (void)hipFree(c->dx);
+  if (c->dr) (void)hipFree(c->dr);
+  if (c->dg) (void)hipFree(c->dg);
+  if (c->drx) (void)hipFree(c->drx);
+  delete c;
+}
+
+const char* isg_last_error(const isg_context* c) { return c ? c->err.c_str() : "null context"; }
+
+int isg_numunk(const isg_context* c) { return (c && c->eng) ? c->eng->plan().N : ISG_E_ARG; }
+int isg_level(const isg_context* c) { return (c && c->eng) ? c->eng->plan().L : ISG_E_ARG; }
+int isg_numsamp(const isg_context* c, int level) {
+  if (!c || !c->eng || level < 0 || level >= c->eng->plan().L) return ISG_E_ARG;
+  return c->eng->plan().nsamp[level];
+}
+double isg_k0(const isg_context* c) { return (c && c->eng) ? c->eng->plan().k0 : 0.0; }
+
+int isg_geometry(const isg_context* c, double* pos, int* r2t) {
+  if (!c || !c->eng) return ISG_E_ARG;
+  const isg::Plan& P = c->eng->plan();
+  if (pos) std::memcpy(pos, P.pos.data(), sizeof(double) * 2 * P.N);
+  if (r2t) std::memcpy(r2t, P.raster2tree.data(), sizeof(int) * P.N);
+  return ISG_OK;
+}
+
+void* isg_stream(const isg_context* c) { return (c && c->eng) ? (void*)c->eng->stream() : nullptr; }
+
+int isg_mlfma(isg_context* c, const double* x, double* r) {
+  if (!check(c)) return ISG_E_STATE;
+  if (!x || !r) return fail(c, ISG_E_ARG, "null pointer");
+  hipStream_t s = c->eng->stream();
+  size_t vb = vbytes(c);
+  if (!hip_ok(c, hipMemcpyAsync(c->dx, x, vb, hipMemcpyHostToDevice, s), "H2D")) return ISG_E_DEVICE;
+  c->eng->mlfma(c->dx, c->dr, 1);
+  if (!hip_ok(c, hipMemcpyAsync(r, c->dr, vb, hipMemcpyDeviceToHost, s), "D2H")) return ISG_E_DEVICE;
+  return sync(c);
+}
+
+int isg_mlfmah(isg_context* c, const double* x, double* r) {
+  if (!check(c)) return ISG_E_STATE;
+  if (!x || !r) return fail(c, ISG_E_ARG, "null pointer");
+  const int N = c->eng->plan().N;
+  std::vector<double> xc(x, x + 2 * (size_t)N);
+  for (int n = 0; n < N; n++) xc[2 * n + 1] = -xc[2 * n + 1];
+  int rc = isg_mlfma(c, xc.data(), r);
+  if (rc != ISG_OK) return rc;
+  for (int n = 0; n < N; n++) r[2 * n + 1] = -r[2 * n + 1];
+  return ISG_OK;
+}
+
+int isg_aggregate(isg_context* c, const double* x, double* r) {
+  if (!check(c)) return ISG_E_STATE;
+  if (!x || !r) return fail(c, ISG_E_ARG, "null pointer");
+  hipStream_t s = c->eng->stream();
+  if (!hip_ok(c, hipMemcpyAsync(c->dx, x, vbytes(c), hipMemcpyHostToDevice, s), "H2D")) return ISG_E_DEVICE;
+  c->eng->aggregate(c->dx, c->drx, 1);
+  if (!hip_ok(c, hipMemcpyAsync(r, c->drx, sizeof(isg::zd) * c->cfg.numrx, hipMemcpyDeviceToHost, s), "D2H"))
+    return ISG_E_DEVICE;
+  return sync(c);
+}
+
+int isg_aggregateh(isg_context* c, double* x, const double* r) {
+  if (!check(c)) return ISG_E_STATE;
+  if (!x || !r) return fail(c, ISG_E_ARG, "null pointer");
+  hipStream_t s = c->eng->stream();
+  if (!hip_ok(c, hipMemcpyAsync(c->drx, r, sizeof(isg::zd) * c->cfg.numrx, hipMemcpyHostToDevice, s), "H2D"))
+    return ISG_E_DEVICE;
+  c->eng->aggregateh(c->drx, c->dx, 1);
+  if (!hip_ok(c, hipMemcpyAsync(x, c->dx, vbytes(c), hipMemcpyDeviceToHost, s), "D2H")) return ISG_E_DEVICE;
+  return sync(c);
+}
+
+int isg_bicgs(isg_context* c, double* x, const double* o, const double* b, int herm, isg_solve_info* info) {
+  if (!check(c)) return ISG_E_STATE;
+  if (!x || !o || !b) return fail(c, ISG_E_ARG, "null pointer");
+  hipStream_t s = c->eng->stream();
+  size_t vb = vbytes(c);
+  if (!hip_ok(c, hipMemcpyAsync(c->dx, x, vb, hipMemcpyHostToDevice, s), "H2D") ||
+      !hip_ok(c, hipMemcpyAsync(c->dg, o, vb, hipMemcpyHostToDevice, s), "H2D") ||
+      !hip_ok(c, hipMemcpyAsync(c->dr, b, vb, hipMemcpyHostToDevice, s), "H2D"))
+    return ISG_E_DEVICE;
+  std::vector<isg::SolveStat> st;
+  c->eng->bicgs(c->dx, c->dg, 0, c->dr, herm != 0, 1, false, c->cfg.toli, c->cfg.iti, st);
+  if (!check(c)) return ISG_E_DEVICE;
+  if (!hip_ok(c, hipMemcpyAsync(x, c->dx, vb, hipMemcpyDeviceToHost, s), "D2H")) return ISG_E_DEVICE;
+  int rc = sync(c);
+  if (rc != ISG_OK) return rc;
+  if (info) {
+    info->status = st[0].status;
+    info->iters = st[0].iters;
+    info->relres = st[0].relres;
+  }
+  return st[0].status;
+}
+
+int isg_mlfma_dev(isg_context* c, const void* x, void* r, int nrhs) {
+  if (!check(c)) return ISG_E_STATE;
+  if (!x || !r || nrhs < 1 || nrhs > c->cfg.max_rhs) return fail(c, ISG_E_ARG, "bad argument");
+  c->eng->mlfma((const isg::zd*)x, (isg::zd*)r, nrhs);
+  return check(c) ? ISG_OK : ISG_E_DEVICE;
+}
+
+int isg_aggregate_dev(isg_context* c, const void* x, void* r, int nrhs) {
+  if (!check(c)) return ISG_E_STATE;
+  if (!x || !r || nrhs < 1 || nrhs > c->cfg.max_rhs) return fail(c, ISG_E_ARG, "bad argument");
+  c->eng->aggregate((const isg::zd*)x, (isg::zd*)r, nrhs);
+  return check(c) ? ISG_OK : ISG_E_DEVICE;
+}
+
+int isg_aggregateh_dev(isg_context* c, void* x, const void* r, int nrhs) {
+  if (!check(c)) return ISG_E_STATE;
+  if (!x || !r || nrhs < 1 || nrhs > c->cfg.max_rhs) return fail(c, ISG_E_ARG, "bad argument");
+  c->eng->aggregateh((const isg::zd*)r, (isg::zd*)x, nrhs);
+  return check(c) ? ISG_OK : ISG_E_DEVICE;
+}
+
+int isg_bicgs_dev(isg_context* c, void* x, const void* o, long o_stride, const void* b, int herm, int nrhs,
+                  int x_zero, isg_solve_info* info) {
+  if (!check(c)) return ISG_E_STATE;
+  if (!x || !o || !b || nrhs < 1 || nrhs > c->cfg.max_rhs) return fail(c, ISG_E_ARG, "bad argument");
+  if (o_stride != 0 && o_stride != c->eng->plan().N) return fail(c, ISG_E_ARG, "o_stride must be 0 or numunk");
+  std::vector<isg::SolveStat> st;
+  c->eng->bicgs((isg::zd*)x, (const isg::zd*)o, o_stride, (const isg::zd*)b, herm != 0, nrhs, x_zero != 0,
+                c->cfg.toli, c->cfg.iti, st);
+  if (!check(c)) return ISG_E_DEVICE;
+  int worst = 0;
+  for (int i = 0; i < nrhs; i++) {
+    if (info) {
+      info[i].status = st[i].status;
+      info[i].iters = st[i].iters;
+      info[i].relres = st[i].relres;
+    }
+    if (st[i].status > worst) worst = st[i].status;
+  }
+  return worst;
+}
+
+int isg_stage(isg_context* c, int which, int level, int rhs, double* out) {
+  if (!check(c)) return ISG_E_STATE;
+  if (!out || level < 0 || level >= c->eng->plan().L) return fail(c, ISG_E_ARG, "bad argument");
+  std::vector<isg::zd> v;
+  c->eng->stage(which, level, rhs, v);
+  if (!check(c)) return ISG_E_DEVICE;
+  std::memcpy(out, v.data(), v.size() * sizeof(isg::zd));
+  return ISG_OK;
+}
+
+int isg_counters(const isg_context* c, long long* out) {
+  if (!c || !c->eng || !out) return ISG_E_ARG;
+  out[0] = c->eng->matvecs;
+  out[1] = c->eng->solves;
+  out[2] = c->eng->solve_iters;
+  return ISG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,96 @@
+// isg_device.hpp -- complex arithmetic and reduction helpers for gfx950 kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace isg {
+
+// Interleaved complex value; alignment lets a wavefront move one complex per
+// lane with a single 8- or 16-byte load.
+template <typename T>
+struct alignas(2 * sizeof(T)) cplx {
+  T x, y;
+};
+
+template <typename T>
+__host__ __device__ __forceinline__ cplx<T> mkc(T a, T b) {
+  cplx<T> r;
+  r.x = a;
+  r.y = b;
+  return r;
+}
+template <typename T>
+__device__ __forceinline__ cplx<T> cadd(cplx<T> a, cplx<T> b) {
+  return mkc<T>(a.x + b.x, a.y + b.y);
+}
+template <typename T>
+__device__ __forceinline__ cplx<T> csub(cplx<T> a, cplx<T> b) {
+  return mkc<T>(a.x - b.x, a.y - b.y);
+}
+template <typename T>
+__device__ __forceinline__ cplx<T> cmul(cplx<T> a, cplx<T> b) {
+  return mkc<T>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// acc + a*b
+template <typename T>
+__device__ __forceinline__ cplx<T> cfma(cplx<T> a, cplx<T> b, cplx<T> acc) {
+  return mkc<T>(acc.x + (a.x * b.x - a.y * b.y), acc.y + (a.x * b.y + a.y * b.x));
+}
+template <typename T>
+__device__ __forceinline__ cplx<T> cconj(cplx<T> a) {
+  return mkc<T>(a.x, -a.y);
+}
+// conj(a) * b
+template <typename T>
+__device__ __forceinline__ cplx<T> cconjmul(cplx<T> a, cplx<T> b) {
+  return mkc<T>(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
+}
+template <typename T>
+__device__ __forceinline__ cplx<T> cscale(T s, cplx<T> a) {
+  return mkc<T>(s * a.x, s * a.y);
+}
+template <typename To, typename Ti>
+__device__ __forceinline__ cplx<To> ccast(cplx<Ti> a) {
+  return mkc<To>((To)a.x, (To)a.y);
+}
+
+typedef cplx<double> zd;
+typedef cplx<float> zf;
+
+__host__ __device__ __forceinline__ zd zdiv(zd a, zd b) {
+  double den = b.x * b.x + b.y * b.y;
+  return mkc<double>((a.x * b.x + a.y * b.y) / den, (a.y * b.x - a.x * b.y) / den);
+}
+__host__ __device__ __forceinline__ double zabs(zd a) {
+  return sqrt(a.x * a.x + a.y * a.y);
+}
+
+// Morton code of a (column X, row Y) pair, row bit more significant.
+__host__ __device__ __forceinline__ unsigned spread2(unsigned v) {
+  v &= 0x0000ffffu;
+  v = (v | (v << 8)) & 0x00ff00ffu;
+  v = (v | (v << 4)) & 0x0f0f0f0fu;
+  v = (v | (v << 2)) & 0x33333333u;
+  v = (v | (v << 1)) & 0x55555555u;
+  return v;
+}
+__host__ __device__ __forceinline__ unsigned compact2(unsigned v) {
+  v &= 0x55555555u;
+  v = (v | (v >> 1)) & 0x33333333u;
+  v = (v | (v >> 2)) & 0x0f0f0f0fu;
+  v = (v | (v >> 4)) & 0x00ff00ffu;
+  v = (v | (v >> 8)) & 0x0000ffffu;
+  return v;
+}
+__host__ __device__ __forceinline__ unsigned morton2(unsigned X, unsigned Y) {
+  return (spread2(Y) << 1) | spread2(X);
+}
+
+// Sum of a double over the 64 lanes of a wavefront; lane 0 holds the result.
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
+}  // namespace isg

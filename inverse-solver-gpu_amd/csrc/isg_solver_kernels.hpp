@@ -1,0 +1,275 @@
+// isg_solver_kernels.hpp -- BiCGSTAB vector algebra for gfx950.
+//
+// The reference runs every BiCGSTAB step as a separate saxpy / prep / post
+// kernel and finishes each dot product on the host after a blocking copy
+// (bicgs_gpu.cu:16-58, 116-224; kernels.cu:334-413).  Here each step is one
+// fused pass that also produces the dot products the next scalar needs:
+//   k_bi_init   r = b - A x, r~ = r, partial <b,b>, <r,r>
+//   k_bi_pupd   p = r + beta (p - omega v), y = prep(p)
+//   k_bi_post1  v = post(p, G y), partial <r~,v>
+//   k_bi_supd   s = r - alpha v, y = prep(s)
+//   k_bi_post2  t = post(s, G y), partial <t,s>, <t,t>
+//   k_bi_xrupd  x += alpha p; x += omega s; r = s - omega t; partial <r,r>, <r~,r>
+// Partial sums are FP64, one per block, reduced in a fixed order by
+// k_bi_fin, which also advances the per-right-hand-side scalars on the
+// device (no host round trip).  prep/post are the contrast products of
+// kernels.cu:394-409: forward y = g x, v = x - Gy; adjoint y = conj x,
+// v = x - conj(g Gy).
+#pragma once
+
+#include "isg_device.hpp"
+#include "isg_engine.hpp"
+
+namespace isg {
+
+enum { PH_INIT = 0, PH_ALPHA = 1, PH_OMEGA = 2, PH_RESID = 3 };
+constexpr int kSlots = 3;
+
+__device__ __forceinline__ zd prep_val(zd x, zd g, bool herm) { return herm ? cconj(x) : cmul(x, g); }
+__device__ __forceinline__ zd post_val(zd x, zd gy, zd g, bool herm) {
+  return herm ? csub(x, cconj(cmul(g, gy))) : csub(x, gy);
+}
+
+// Block-level sum of up to 3 complex partials; thread 0 writes part[slot].
+template <int NS>
+__device__ __forceinline__ void block_store(zd (&v)[NS], double2* part, int nblk, int blk) {
+  __shared__ double sh[NS * 2][kBlock / 64];
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int s = 0; s < NS; s++) {
+    double a = wave_sum(v[s].x), b = wave_sum(v[s].y);
+    if (lane == 0) {
+      sh[2 * s][w] = a;
+      sh[2 * s + 1][w] = b;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+      double a = 0, b = 0;
+      for (int i = 0; i < kBlock / 64; i++) {
+        a += sh[2 * s][i];
+        b += sh[2 * s + 1][i];
+      }
+      part[s * nblk + blk] = make_double2(a, b);
+    }
+  }
+}
+
+#define ISG_VEC_LOOP(N) for (long i = (long)blockIdx.x * kBlock + threadIdx.x; i < (N); i += (long)gridDim.x * kBlock)
+
+__global__ void __launch_bounds__(kBlock) k_bi_init(const zd* __restrict__ x, const zd* __restrict__ gy,
+                                                    const zd* __restrict__ g, long gs, const zd* __restrict__ b,
+                                                    zd* __restrict__ r, zd* __restrict__ rt, long N, int herm,
+                                                    int xzero, double2* __restrict__ part) {
+  int rhs = blockIdx.y;
+  long o = rhs * N;
+  zd acc[2] = {mkc<double>(0, 0), mkc<double>(0, 0)};
+  ISG_VEC_LOOP(N) {
+    zd bv = b[o + i];
+    zd ax = xzero ? mkc<double>(0, 0) : post_val(x[o + i], gy[o + i], g[rhs * gs + i], herm);
+    zd rv = csub(bv, ax);
+    r[o + i] = rv;
+    rt[o + i] = rv;
+    acc[0].x += bv.x * bv.x + bv.y * bv.y;
+    acc[1].x += rv.x * rv.x + rv.y * rv.y;
+  }
+  block_store<2>(acc, part + (long)rhs * kSlots * gridDim.x, gridDim.x, blockIdx.x);
+}
+
+__global__ void __launch_bounds__(kBlock) k_bi_pupd(const zd* __restrict__ r, zd* __restrict__ p,
+                                                    const zd* __restrict__ v, const zd* __restrict__ g, long gs,
+                                                    zd* __restrict__ y, long N, int herm,
+                                                    const BiState* __restrict__ st) {
+  int rhs = blockIdx.y;
+  const BiState& S = st[rhs];
+  if (!S.active) return;
+  long o = rhs * N;
+  zd beta = S.beta, mom = mkc<double>(-S.omega.x, -S.omega.y);
+  int first = S.first;
+  ISG_VEC_LOOP(N) {
+    zd pv;
+    if (first) {
+      pv = r[o + i];
+    } else {
+      zd tmp = cfma(mom, v[o + i], p[o + i]);
+      pv = cfma(beta, tmp, r[o + i]);
+    }
+    p[o + i] = pv;
+    y[o + i] = prep_val(pv, g[rhs * gs + i], herm);
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_bi_post1(const zd* __restrict__ p, const zd* __restrict__ gy,
+                                                     const zd* __restrict__ g, long gs, zd* __restrict__ v,
+                                                     const zd* __restrict__ rt, long N, int herm,
+                                                     const BiState* __restrict__ st, double2* __restrict__ part) {
+  int rhs = blockIdx.y;
+  if (!st[rhs].active) return;
+  long o = rhs * N;
+  zd acc[1] = {mkc<double>(0, 0)};
+  ISG_VEC_LOOP(N) {
+    zd vv = post_val(p[o + i], gy[o + i], g[rhs * gs + i], herm);
+    v[o + i] = vv;
+    zd a = rt[o + i];
+    acc[0].x += a.x * vv.x + a.y * vv.y;
+    acc[0].y += a.x * vv.y - a.y * vv.x;
+  }
+  block_store<1>(acc, part + (long)rhs * kSlots * gridDim.x, gridDim.x, blockIdx.x);
+}
+
+__global__ void __launch_bounds__(kBlock) k_bi_supd(const zd* __restrict__ r, const zd* __restrict__ v,
+                                                    const zd* __restrict__ g, long gs, zd* __restrict__ s,
+                                                    zd* __restrict__ y, long N, int herm,
+                                                    const BiState* __restrict__ st) {
+  int rhs = blockIdx.y;
+  const BiState& S = st[rhs];
+  if (!S.active) return;
+  long o = rhs * N;
+  zd mal = mkc<double>(-S.alpha.x, -S.alpha.y);
+  ISG_VEC_LOOP(N) {
+    zd sv = cfma(mal, v[o + i], r[o + i]);
+    s[o + i] = sv;
+    y[o + i] = prep_val(sv, g[rhs * gs + i], herm);
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_bi_post2(const zd* __restrict__ s, const zd* __restrict__ gy,
+                                                     const zd* __restrict__ g, long gs, zd* __restrict__ t, long N,
+                                                     int herm, const BiState* __restrict__ st,
+                                                     double2* __restrict__ part) {
+  int rhs = blockIdx.y;
+  if (!st[rhs].active) return;
+  long o = rhs * N;
+  zd acc[2] = {mkc<double>(0, 0), mkc<double>(0, 0)};
+  ISG_VEC_LOOP(N) {
+    zd sv = s[o + i];
+    zd tv = post_val(sv, gy[o + i], g[rhs * gs + i], herm);
+    t[o + i] = tv;
+    acc[0].x += tv.x * sv.x + tv.y * sv.y;
+    acc[0].y += tv.x * sv.y - tv.y * sv.x;
+    acc[1].x += tv.x * tv.x + tv.y * tv.y;
+  }
+  block_store<2>(acc, part + (long)rhs * kSlots * gridDim.x, gridDim.x, blockIdx.x);
+}
+
+__global__ void __launch_bounds__(kBlock) k_bi_xrupd(zd* __restrict__ x, const zd* __restrict__ p,
+                                                     const zd* __restrict__ s, const zd* __restrict__ t,
+                                                     zd* __restrict__ r, const zd* __restrict__ rt, long N,
+                                                     const BiState* __restrict__ st, double2* __restrict__ part) {
+  int rhs = blockIdx.y;
+  const BiState& S = st[rhs];
+  if (!S.active) return;
+  long o = rhs * N;
+  zd al = S.alpha, om = S.omega, mom = mkc<double>(-S.omega.x, -S.omega.y);
+  zd acc[2] = {mkc<double>(0, 0), mkc<double>(0, 0)};
+  ISG_VEC_LOOP(N) {
+    zd sv = s[o + i];
+    zd xv = cfma(al, p[o + i], x[o + i]);
+    xv = cfma(om, sv, xv);
+    x[o + i] = xv;
+    zd rv = cfma(mom, t[o + i], sv);
+    r[o + i] = rv;
+    zd a = rt[o + i];
+    acc[0].x += rv.x * rv.x + rv.y * rv.y;
+    acc[1].x += a.x * rv.x + a.y * rv.y;
+    acc[1].y += a.x * rv.y - a.y * rv.x;
+  }
+  block_store<2>(acc, part + (long)rhs * kSlots * gridDim.x, gridDim.x, blockIdx.x);
+}
+
+// One block per right-hand side: fixed-order reduction of the partials, then
+// the scalar step of bicgs_gpu.cu:148-213 for that right-hand side.
+__global__ void __launch_bounds__(kBlock) k_bi_fin(int phase, BiState* __restrict__ st, int* __restrict__ active,
+                                                   const double2* __restrict__ part, int nblk, double tol, int maxit) {
+  int rhs = blockIdx.x;
+  BiState& S = st[rhs];
+  if (phase != PH_INIT && !S.active) return;
+  const double2* pr = part + (long)rhs * kSlots * nblk;
+  int ns = (phase == PH_ALPHA) ? 1 : 2;
+  __shared__ double sh[4][kBlock / 64];
+  double v[4] = {0, 0, 0, 0};
+  for (int s = 0; s < ns; s++)
+    for (int j = threadIdx.x; j < nblk; j += kBlock) {
+      double2 q = pr[s * nblk + j];
+      v[2 * s] += q.x;
+      v[2 * s + 1] += q.y;
+    }
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int s = 0; s < 4; s++) {
+    double a = wave_sum(v[s]);
+    if (lane == 0) sh[s][w] = a;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  double tot[4] = {0, 0, 0, 0};
+  for (int s = 0; s < 4; s++)
+    for (int i = 0; i < kBlock / 64; i++) tot[s] += sh[s][i];
+  zd s0 = mkc<double>(tot[0], tot[1]), s1 = mkc<double>(tot[2], tot[3]);
+  if (phase == PH_INIT) {
+    double bn = sqrt(s0.x);
+    if (bn < 1e-20) bn = 1;
+    S.bnrm = bn;
+    S.err = sqrt(s1.x) / bn;
+    S.iter = 0;
+    S.first = 1;
+    S.beta = mkc<double>(0, 0);
+    S.alpha = mkc<double>(0, 0);
+    S.omega = mkc<double>(0, 0);
+    if (S.err > tol) {
+      S.rho = mkc<double>(s1.x, 0.0);
+      S.iter = 1;
+      if (zabs(S.rho) < 1e-200) {
+        S.status = 2;
+        S.active = 0;
+      } else {
+        S.status = 1;
+        S.active = 1;
+      }
+    } else {
+      S.status = (S.err < tol) ? 0 : 1;
+      S.active = 0;
+    }
+  } else if (phase == PH_ALPHA) {
+    S.alpha = zdiv(S.rho, s0);
+  } else if (phase == PH_OMEGA) {
+    double tt = s1.x;
+    S.omega = mkc<double>(s0.x / tt, s0.y / tt);
+    if (zabs(s0) < 1e-200 && tt < 1e-200) S.omega = mkc<double>(1.0, 0.0);
+  } else {
+    S.err = sqrt(s0.x) / S.bnrm;
+    if (S.err < tol) {
+      S.status = 0;
+      S.active = 0;
+    } else if (zabs(S.omega) < 1e-200) {
+      S.status = 3;
+      S.active = 0;
+    } else if (S.iter >= maxit) {
+      S.status = 1;
+      S.active = 0;
+    } else {
+      S.rho1 = S.rho;
+      S.rho = s1;
+      S.iter++;
+      S.first = 0;
+      if (zabs(S.rho) < 1e-200) {
+        S.status = 2;
+        S.active = 0;
+      } else {
+        S.beta = cmul(zdiv(S.rho, S.rho1), zdiv(S.alpha, S.omega));
+      }
+    }
+  }
+  active[rhs] = S.active;
+}
+
+// y = prep(x)
+__global__ void __launch_bounds__(kBlock) k_prep(const zd* __restrict__ x, const zd* __restrict__ g, long gs,
+                                                 zd* __restrict__ y, long N, int herm) {
+  int rhs = blockIdx.y;
+  long o = rhs * N;
+  ISG_VEC_LOOP(N) y[o + i] = prep_val(x[o + i], g[rhs * gs + i], herm);
+}
+
+}  // namespace isg

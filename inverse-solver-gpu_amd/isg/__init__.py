@@ -1,0 +1,246 @@
+"""Python front end of libisg.so, the MI355X MLFMA / BiCGSTAB / DBIM solver.
+
+Mirrors the reference's operator surface (vars.h:14-26): ``mlfma``,
+``mlfmah``, ``aggregate``, ``aggregateh`` and ``bicgs`` on complex128 vectors
+in tree order, plus batched device-pointer variants and the DBIM driver.
+Everything runs in the HIP library; this module only marshals arguments.
+Import succeeds without a GPU, but creating a :class:`Solver` raises if the
+library is missing or no HIP device is present -- there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.normpath(os.path.join(_HERE, "..", "libisg.so"))
+HEADER_PATH = os.path.normpath(os.path.join(_HERE, "..", "..", "include", "isg.h"))
+
+ISG_OK = 0
+SOLVE_STATUS = {0: "converged", 1: "not converged", 2: "rho breakdown", 3: "omega breakdown"}
+
+
+class IsgError(RuntimeError):
+    pass
+
+
+class Config(ctypes.Structure):
+    _fields_ = [
+        ("level", ctypes.c_int),
+        ("box", ctypes.c_int),
+        ("res", ctypes.c_double),
+        ("scale", ctypes.c_double),
+        ("beta", ctypes.c_int),
+        ("ninter", ctypes.c_int),
+        ("numrx", ctypes.c_int),
+        ("numtx", ctypes.c_int),
+        ("toli", ctypes.c_double),
+        ("iti", ctypes.c_int),
+        ("regular", ctypes.c_double),
+        ("max_rhs", ctypes.c_int),
+        ("precision", ctypes.c_int),
+    ]
+
+
+class SolveInfo(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int), ("iters", ctypes.c_int), ("relres", ctypes.c_double)]
+
+
+class DbimStats(ctypes.Structure):
+    _fields_ = [
+        ("alpha", ctypes.c_double),
+        ("misfit", ctypes.c_double),
+        ("object_error", ctypes.c_double),
+        ("object_cost", ctypes.c_double),
+        ("gradient_norm2", ctypes.c_double),
+        ("object_norm2", ctypes.c_double),
+        ("beta_re", ctypes.c_double),
+        ("beta_im", ctypes.c_double),
+        ("solves", ctypes.c_longlong),
+        ("solve_iters", ctypes.c_longlong),
+        ("matvecs", ctypes.c_longlong),
+        ("not_converged", ctypes.c_longlong),
+        ("seconds", ctypes.c_double),
+    ]
+
+
+_V = ctypes.c_void_p
+_P = ctypes.POINTER
+_D = _P(ctypes.c_double)
+_I = _P(ctypes.c_int)
+
+_SIGS = {
+    "isg_config_default": (None, [_P(Config)]),
+    "isg_create": (ctypes.c_int, [_P(Config), _P(_V)]),
+    "isg_destroy": (None, [_V]),
+    "isg_last_error": (ctypes.c_char_p, [_V]),
+    "isg_numunk": (ctypes.c_int, [_V]),
+    "isg_level": (ctypes.c_int, [_V]),
+    "isg_numsamp": (ctypes.c_int, [_V, ctypes.c_int]),
+    "isg_k0": (ctypes.c_double, [_V]),
+    "isg_geometry": (ctypes.c_int, [_V, _D, _I]),
+    "isg_stream": (_V, [_V]),
+    "isg_mlfma": (ctypes.c_int, [_V, _D, _D]),
+    "isg_mlfmah": (ctypes.c_int, [_V, _D, _D]),
+    "isg_aggregate": (ctypes.c_int, [_V, _D, _D]),
+    "isg_aggregateh": (ctypes.c_int, [_V, _D, _D]),
+    "isg_bicgs": (ctypes.c_int, [_V, _D, _D, _D, ctypes.c_int, _P(SolveInfo)]),
+    "isg_mlfma_dev": (ctypes.c_int, [_V, _V, _V, ctypes.c_int]),
+    "isg_aggregate_dev": (ctypes.c_int, [_V, _V, _V, ctypes.c_int]),
+    "isg_aggregateh_dev": (ctypes.c_int, [_V, _V, _V, ctypes.c_int]),
+    "isg_bicgs_dev": (ctypes.c_int, [_V, _V, _V, ctypes.c_long, _V, ctypes.c_int, ctypes.c_int,
+                                     ctypes.c_int, _P(SolveInfo)]),
+    "isg_stage": (ctypes.c_int, [_V, ctypes.c_int, ctypes.c_int, ctypes.c_int, _D]),
+    "isg_counters": (ctypes.c_int, [_V, _P(ctypes.c_longlong)]),
+}
+
+_lib = None
+
+
+def declared_symbols():
+    """Names of every function declared in include/isg.h."""
+    import re
+    txt = open(HEADER_PATH).read()
+    return sorted(set(re.findall(r"\b(isg_[a-z0-9_]+)\s*\(", txt)))
+
+
+def load(path=None):
+    """Load libisg.so (build it first with inverse-solver-gpu_amd/build.py)."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise IsgError("libisg.so not built at %s; run inverse-solver-gpu_amd/build.py" % p)
+    L = ctypes.CDLL(p)
+    for name, (res, args) in _SIGS.items():
+        if hasattr(L, name):
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+    _lib = L
+    return L
+
+
+def _dp(a):
+    return a.ctypes.data_as(_D)
+
+
+def _c128(a, n=None):
+    a = np.ascontiguousarray(a, dtype=np.complex128)
+    if n is not None and a.size != n:
+        raise ValueError("expected %d complex values, got %d" % (n, a.size))
+    return a
+
+
+def default_config(**kw):
+    c = Config()
+    load().isg_config_default(ctypes.byref(c))
+    for k, v in kw.items():
+        if not hasattr(c, k):
+            raise KeyError(k)
+        setattr(c, k, v)
+    return c
+
+
+class Solver:
+    """One operator + solver context on the GPU (setup_mlfma + setup_gpu +
+    setup_bicgs of the reference, mom.cpp:184-190)."""
+
+    def __init__(self, **kw):
+        self.lib = load()
+        self.cfg = default_config(**kw)
+        h = _V()
+        rc = self.lib.isg_create(ctypes.byref(self.cfg), ctypes.byref(h))
+        self.h = h
+        if rc != ISG_OK:
+            msg = self.lib.isg_last_error(h).decode() if h else "invalid configuration"
+            self.close()
+            raise IsgError("isg_create failed (%d): %s" % (rc, msg))
+        self.N = self.lib.isg_numunk(h)
+        self.level = self.lib.isg_level(h)
+        self.numsamp = [self.lib.isg_numsamp(h, i) for i in range(self.level)]
+        self.k0 = self.lib.isg_k0(h)
+        self.numrx = self.cfg.numrx
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.isg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _err(self, rc, what):
+        if rc < 0:
+            raise IsgError("%s failed (%d): %s" % (what, rc, self.lib.isg_last_error(self.h).decode()))
+        return rc
+
+    def geometry(self):
+        pos = np.empty(self.N, np.complex128)
+        r2t = np.empty(self.N, np.int32)
+        self._err(self.lib.isg_geometry(self.h, _dp(pos), r2t.ctypes.data_as(_I)), "isg_geometry")
+        return pos, r2t
+
+    def mlfma(self, x):
+        x = _c128(x, self.N)
+        r = np.empty(self.N, np.complex128)
+        self._err(self.lib.isg_mlfma(self.h, _dp(x), _dp(r)), "isg_mlfma")
+        return r
+
+    def mlfmah(self, x):
+        x = _c128(x, self.N)
+        r = np.empty(self.N, np.complex128)
+        self._err(self.lib.isg_mlfmah(self.h, _dp(x), _dp(r)), "isg_mlfmah")
+        return r
+
+    def aggregate(self, x):
+        x = _c128(x, self.N)
+        r = np.empty(self.numrx, np.complex128)
+        self._err(self.lib.isg_aggregate(self.h, _dp(x), _dp(r)), "isg_aggregate")
+        return r
+
+    def aggregateh(self, r):
+        r = _c128(r, self.numrx)
+        x = np.empty(self.N, np.complex128)
+        self._err(self.lib.isg_aggregateh(self.h, _dp(x), _dp(r)), "isg_aggregateh")
+        return x
+
+    def bicgs(self, o, b, x0=None, herm=False):
+        """Solve (I - G diag(o)) x = b (or its adjoint); o is contrast * k0^2."""
+        o = _c128(o, self.N)
+        b = _c128(b, self.N)
+        x = np.zeros(self.N, np.complex128) if x0 is None else _c128(x0, self.N).copy()
+        info = SolveInfo()
+        rc = self._err(self.lib.isg_bicgs(self.h, _dp(x), _dp(o), _dp(b), int(herm), ctypes.byref(info)),
+                       "isg_bicgs")
+        return x, {"status": info.status, "iters": info.iters, "relres": info.relres, "rc": rc}
+
+    def stage(self, which, level, rhs=0):
+        out = np.empty(4 ** level * self.numsamp[level], np.complex128)
+        self._err(self.lib.isg_stage(self.h, which, level, rhs, _dp(out)), "isg_stage")
+        return out.reshape(4 ** level, self.numsamp[level])
+
+    def counters(self):
+        out = (ctypes.c_longlong * 3)()
+        self._err(self.lib.isg_counters(self.h, out), "isg_counters")
+        return {"matvecs": out[0], "solves": out[1], "solve_iters": out[2]}
+
+    # ---- device-pointer surface (pointers are integers, e.g. torch data_ptr())
+    def mlfma_dev(self, x_ptr, r_ptr, nrhs=1):
+        return self._err(self.lib.isg_mlfma_dev(self.h, _V(x_ptr), _V(r_ptr), nrhs), "isg_mlfma_dev")
+
+    def aggregate_dev(self, x_ptr, r_ptr, nrhs=1):
+        return self._err(self.lib.isg_aggregate_dev(self.h, _V(x_ptr), _V(r_ptr), nrhs), "isg_aggregate_dev")
+
+    def aggregateh_dev(self, x_ptr, r_ptr, nrhs=1):
+        return self._err(self.lib.isg_aggregateh_dev(self.h, _V(x_ptr), _V(r_ptr), nrhs), "isg_aggregateh_dev")
+
+    def bicgs_dev(self, x_ptr, o_ptr, b_ptr, nrhs=1, herm=False, o_stride=0, x_zero=False):
+        infos = (SolveInfo * nrhs)()
+        rc = self._err(self.lib.isg_bicgs_dev(self.h, _V(x_ptr), _V(o_ptr), o_stride, _V(b_ptr), int(herm),
+                                              nrhs, int(x_zero), infos), "isg_bicgs_dev")
+        return rc, [{"status": i.status, "iters": i.iters, "relres": i.relres} for i in infos]
