@@ -1,0 +1,221 @@
+#!/usr/bin/env python3
+"""Benchmark: seconds per DBIM iteration on one MI355X.
+
+Workload (the headline configuration): 1024x1024 pixels of lambda/10
+(N = 1,048,576 unknowns, quadtree level 8, 8x8-pixel leaf boxes), 64
+plane-wave transmitters, 256 far-field receivers, contrast = 20 seeded random
+ellipses (seed 3, eps_r - 1 in U(0.05, 0.8) + 0.02i), BiCGSTAB relative
+residual 1e-6 (iteration limit 300), Tikhonov regular = sqrt(1e-3).
+
+One step = one DBIM iteration (mom.cpp:277-400): per transmitter a Frechet
+derivative solve, a total-field solve and an adjoint solve, plus the line
+search, gradient and Polak-Ribiere update.  Inputs are resident in HBM before
+the timed region; the timed region brackets K iterations with device
+synchronisation on both sides (each iteration ends with a stream sync).
+
+Two paths run from the same synthetic measurements:
+  plain   FP64, one right-hand side at a time, every solve from zero -- the
+          reference's loop on this library's kernels;
+  fast    every transmitter of a phase in one multi-RHS BiCGSTAB with
+          deflation, total-field solves warm-started from the previous
+          DBIM iterate.
+`value` is the fast path's s/iteration; the plain figure is reported beside it.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "inverse-solver-gpu_amd"))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+METRIC = "s per DBIM iteration, 1024² px ×64 Tx, rel-res 1e-6: plain vs batched+learned"
+PEAK_FP64_TFLOPS = 78.6
+PEAK_FP32_TFLOPS = 157.3
+PEAK_HBM_GBS = 8000.0
+
+WORKLOADS = {
+    # name: level, transmitters, receivers, ellipses, seed
+    "dbim-1024": dict(level=8, numtx=64, numrx=256, nell=20, seed=3),
+    "dbim-256": dict(level=6, numtx=16, numrx=64, nell=8, seed=3),
+}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def near_flops_per_rhs(level, S_leaf):
+    """Algorithmic flops of one near-field + L2P application (one RHS):
+    8*64*64 per in-domain 3x3 box pair (P2P) + 8*64*S per leaf box (L2P)."""
+    nb = 1 << (level - 1)
+    pairs = (3 * nb - 2) ** 2
+    return 8 * 64 * 64 * pairs + 8 * 64 * S_leaf * nb * nb, pairs
+
+
+def m2l_bytes_per_rhs(level, numsamp, cbytes=16):
+    """Read every multipole once and write every local once, levels 2..leaf."""
+    return sum(2 * cbytes * 4 ** i * numsamp[i] for i in range(2, level))
+
+
+def run_path(isg, solver, dbim, warmup, steps, label):
+    for w in range(warmup):
+        st = dbim.iterate()
+        log("[%s] warmup iteration %d: %.3fs, %d solves, %d BiCGSTAB iterations, misfit %.4e"
+            % (label, w, st["seconds"], st["solves"], st["solve_iters"], st["misfit"]))
+    solver.profile(True)
+    t0 = time.perf_counter()
+    stats = []
+    for k in range(steps):
+        st = dbim.iterate()
+        stats.append(st)
+        log("[%s] timed iteration %d: %.3fs, %d solves, %d BiCGSTAB iterations, %d not converged, misfit %.4e"
+            % (label, k, st["seconds"], st["solves"], st["solve_iters"], st["not_converged"], st["misfit"]))
+    t1 = time.perf_counter()
+    prof = {k: solver.profile_read(k) for k in ("near", "m2l", "bicgs_update")}
+    solver.profile(False)
+    return (t1 - t0) / steps, stats, prof
+
+
+def cpu_baseline(level, numrx, matvecs_per_iter, nsample):
+    """Time the CPU oracle (FP64 C++/OpenMP restatement of the reference CPU
+    path) on `nsample` matvecs of the same size and scale by the matvec count
+    of one plain DBIM iteration measured on the GPU."""
+    from oracle.oracle import Oracle
+    o = Oracle(level, numrx=numrx)
+    rng = np.random.default_rng(0)
+    x = rng.standard_normal(o.N) + 1j * rng.standard_normal(o.N)
+    o.mlfma(x)  # first touch
+    t0 = time.perf_counter()
+    for _ in range(nsample):
+        o.mlfma(x)
+    t_mv = (time.perf_counter() - t0) / nsample
+    threads = o.L.orc_threads()
+    return {
+        "value": t_mv * matvecs_per_iter,
+        "unit": "s",
+        "cores": threads,
+        "kind": "port",
+        "sample": ("%d FP64 MLFMA matvecs at %dx%d on the CPU oracle (%.3f s each, %d OpenMP threads), "
+                   "scaled by the %d matvecs of one plain DBIM iteration counted on the GPU run; "
+                   "BiCGSTAB vector algebra and receiver operators not included (extrapolated)"
+                   % (nsample, 8 << (level - 1), 8 << (level - 1), t_mv, threads, matvecs_per_iter)),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="dbim-1024", choices=sorted(WORKLOADS))
+    ap.add_argument("--paths", default="fast,plain")
+    ap.add_argument("--cpu-sample", type=int, default=2, help="oracle matvecs timed for cpu_baseline (0: skip)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (transmitter sharding "
+              "over RCCL is described in DESIGN.md but not built); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import isg
+    wl = WORKLOADS[args.workload]
+    level, ntx, nrx = wl["level"], wl["numtx"], wl["numrx"]
+    side = 8 << (level - 1)
+    common = dict(level=level, numrx=nrx, numtx=ntx, toli=1e-6, iti=300, regular=float(np.sqrt(1e-3)))
+    paths = [p for p in args.paths.split(",") if p]
+
+    # synthetic measurements of the true contrast, made once by the batched path
+    t0 = time.perf_counter()
+    fast = isg.Solver(max_rhs=ntx, **common)
+    pos, r2t = fast.geometry()
+    otrue = isg.raster_to_tree(isg.phantom_ellipses(side, wl["nell"], wl["seed"]), r2t)
+    d_fast = isg.Dbim(fast, otrue, field_start=1)
+    mes = d_fast.get("mes")
+    log("setup (measurements + first gradient): %.2fs, %d solves, %d BiCGSTAB iterations, %d not converged"
+        % (time.perf_counter() - t0, d_fast.setup_stats["solves"], d_fast.setup_stats["solve_iters"],
+           d_fast.setup_stats["not_converged"]))
+    S_leaf = fast.numsamp[-1]
+    numsamp = list(fast.numsamp)
+    flops_mv, pairs = near_flops_per_rhs(level, S_leaf)
+
+    res = {}
+    if "fast" in paths:
+        res["fast"] = run_path(isg, fast, d_fast, args.warmup, args.steps, "fast")
+    fast.close()
+    del d_fast
+    if "plain" in paths:
+        plain = isg.Solver(max_rhs=1, **common)
+        d_plain = isg.Dbim(plain, otrue, field_start=0, measurements=mes)
+        res["plain"] = run_path(isg, plain, d_plain, args.warmup, args.steps, "plain")
+        plain.close()
+
+    main_path = "fast" if "fast" in res else "plain"
+    s_iter, stats, prof = res[main_path]
+    matvecs = sum(s["matvecs"] for s in stats)
+    near_ms, near_n = prof["near"]
+    m2l_ms, m2l_n = prof["m2l"]
+    vec_ms, vec_n = prof["bicgs_update"]
+    achieved = matvecs * flops_mv / (near_ms * 1e-3) / 1e12 if near_ms > 0 else 0.0
+    m2l_gbs = matvecs * m2l_bytes_per_rhs(level, numsamp) / (m2l_ms * 1e-3) / 1e9 if m2l_ms > 0 else 0.0
+
+    out = {
+        "metric": METRIC,
+        "value": round(s_iter, 4),
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * s_iter, 2),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded ellipse phantom, measurements synthesized by the forward solver)",
+        "config": {
+            "workload": "DBIM %dx%d px lambda/10, level %d, %d Tx, %d Rx, %d ellipses seed %d, rel-res 1e-6"
+                        % (side, side, level, ntx, nrx, wl["nell"], wl["seed"]),
+            "path": {"fast": "batched %d RHS + deflation, FP64, previous-iterate total-field start" % ntx,
+                     "plain": "1 RHS, FP64, zero start"}[main_path],
+        },
+        "roofline": {
+            "kernel": "k_near_mfma (P2P + L2P, FP64 v_mfma_f64_16x16x4)",
+            "bound": "mfma",
+            "achieved": round(achieved, 2),
+            "peak": PEAK_FP64_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": round(achieved / PEAK_FP64_TFLOPS, 4),
+            "traffic": None,
+            "flops_per_rhs_launch": flops_mv,
+            "avg_launch_ms": round(near_ms / max(near_n, 1), 4),
+            "launches": near_n,
+        },
+        "m2l": {"achieved_GBs": round(m2l_gbs, 1), "peak_GBs": PEAK_HBM_GBS,
+                "frac": round(m2l_gbs / PEAK_HBM_GBS, 4), "ms_per_matvec_launchset": round(m2l_ms / max(m2l_n, 1), 4)},
+        # x/r update reads x, p, s, t, r~ and writes x, r: 7 complex FP64 vectors per RHS iteration
+        "bicgs_update": {"GBs": round(7 * 16 * side * side * sum(s["solve_iters"] for s in stats)
+                                      / max(vec_ms * 1e-3, 1e-12) / 1e9, 1),
+                         "avg_launch_ms": round(vec_ms / max(vec_n, 1), 4)},
+        "solves": sum(s["solves"] for s in stats),
+        "bicgstab_iterations": sum(s["solve_iters"] for s in stats),
+        "matvecs": matvecs,
+    }
+    if "plain" in res and "fast" in res:
+        out["plain_s_per_iter"] = round(res["plain"][0], 4)
+        out["speedup_vs_plain"] = round(res["plain"][0] / s_iter, 3)
+    if args.cpu_sample > 0 and "plain" in res:
+        plain_mv = int(round(sum(s["matvecs"] for s in res["plain"][1]) / len(res["plain"][1])))
+        try:
+            out["cpu_baseline"] = cpu_baseline(level, nrx, plain_mv, args.cpu_sample)
+        except Exception as e:  # the CPU oracle is optional on a box without a compiler
+            out["cpu_baseline"] = {"error": str(e)}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -122,6 +122,54 @@ int isg_stage(isg_context* ctx, int which, int level, int rhs, double* out);
  * out[0] single-RHS matvecs (mom.cpp:61), out[1] solves, out[2] solver iterations */
 int isg_counters(const isg_context* ctx, long long* out);
 
+/* ---- DBIM / nonlinear-CG inversion (mom.cpp:222-400, setup_born.cpp) ---- */
+typedef struct isg_dbim_stats {
+  double alpha;          /* step length (mom.cpp:315) */
+  double misfit;         /* sum|mesd|^2 / sum|mes|^2            (hist.txt col 1, setup_born.cpp:239) */
+  double object_error;   /* sum|o - o_true|^2 / sum|o_true|^2   (col 2) */
+  double object_cost;    /* sum|regular o|^2 / sum|mes|^2       (col 3) */
+  double gradient_norm2; /* sum|od|^2                           (col 4) */
+  double object_norm2;   /* sum|o|^2                            (col 5) */
+  double beta_re, beta_im; /* Polak-Ribiere coefficient (mom.cpp:394) */
+  long long solves, solve_iters, matvecs, not_converged;
+  double seconds;        /* wall time of the call, device work included */
+} isg_dbim_stats;
+
+/* Incident plane waves for numtx transmitters (setup_born.cpp:62-65), synthetic
+ * measurements of the true contrast o_true (tree order, numunk complex; the
+ * reference's phantom file), zero initial object and the first gradient
+ * (mom.cpp:222-276).  field_start: 0 = every total-field solve starts from zero
+ * as in the reference (mom.cpp:324), 1 = from the previous total field. */
+int isg_dbim_setup(isg_context* ctx, const double* o_true, int field_start, isg_dbim_stats* st);
+/* Same start from given measurements mes (numtx x numrx complex, the reference's
+ * mes array after setup_born.cpp:148) instead of synthesizing them; o_true may be
+ * NULL (object_error then reports sum|o|^2). */
+int isg_dbim_setup_measured(isg_context* ctx, const double* mes, const double* o_true, int field_start,
+                            isg_dbim_stats* st);
+/* One DBIM iteration (mom.cpp:277-400). */
+int isg_dbim_iterate(isg_context* ctx, isg_dbim_stats* st);
+/* which: 0 o, 1 od (gradient), 2 ot (direction), 3 mes, 4 mesd, 5 inc, 6 tot.
+ * Vectors are numunk (x numtx for inc/tot) complex; mes/mesd numtx x numrx.
+ * isg_dbim_set accepts 0, 1, 2, 4 and 6 (checkpoint / resume). */
+int isg_dbim_get(isg_context* ctx, int which, double* out);
+int isg_dbim_set(isg_context* ctx, int which, const double* in);
+
+/* ---- live kernel timing (HIP events on the context's stream; the reference
+ * times whole matvecs with cudaEvents, matvec_gpu.cu:117-122) ----
+ * kernel 0: near field + L2P on the matrix cores, one launch per matvec
+ * kernel 1: M2L over all levels of one matvec
+ * kernel 2: BiCGSTAB x/r update with its two dot products
+ * isg_profile(ctx, 1) clears the totals and starts recording. */
+int isg_profile(isg_context* ctx, int enable);
+int isg_profile_read(isg_context* ctx, int kernel, double* total_ms, long long* launches);
+
+/* ---- seeded phantoms (raster order, 2*side*side doubles) ---- */
+/* n Gaussian blobs, centres in the inner 80%, sigma U(0.3,1), amplitude U(0.1,0.5)+iU(0.01,0.05) */
+int isg_phantom_blobs(int side, double res, int n, unsigned long long seed, double* out);
+/* n ellipses, semi-axes U(0.02,0.08)*side*res, value U(vmin,vmax) + i*imag, later ones on top */
+int isg_phantom_ellipses(int side, double res, int n, unsigned long long seed, double vmin, double vmax,
+                         double imag, double* out);
+
 #ifdef __cplusplus
 }
 #endif

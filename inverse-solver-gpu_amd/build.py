@@ -27,6 +27,7 @@ HIPFLAGS = ["-x", "hip", "--offload-arch=" + ARCH, "-Xclang", "-target-feature",
 # (source, is_hip)
 SOURCES = [
     ("isg_plan.cpp", False),
+    ("isg_phantom.cpp", False),
     ("isg_engine.hip", True),
     ("isg_dbim.hip", True),
     ("isg_capi.cpp", True),

@@ -7,11 +7,13 @@
 #include <vector>
 
 #include "../../include/isg.h"
+#include "isg_dbim.hpp"
 #include "isg_engine.hpp"
 
 struct isg_context {
   isg_config cfg;
   std::unique_ptr<isg::Engine> eng;
+  std::unique_ptr<isg::Dbim> dbim;
   std::string err;
   // host-pointer staging buffers (device), single right-hand side
   isg::zd* dx = nullptr;
@@ -105,6 +107,7 @@ int isg_create(const isg_config* cfg, isg_context** out) {
 
 void isg_destroy(isg_context* c) {
   if (!c) return;
+  c->dbim.reset();
   c->eng.reset();
   if (c->dx) (void)hipFree(c->dx);
   if (c->dr) (void)hipFree(c->dr);
@@ -258,6 +261,107 @@ int isg_counters(const isg_context* c, long long* out) {
   out[0] = c->eng->matvecs;
   out[1] = c->eng->solves;
   out[2] = c->eng->solve_iters;
+  return ISG_OK;
+}
+
+static void fill_stats(const isg::DbimStats& s, isg_dbim_stats* o) {
+  if (!o) return;
+  o->alpha = s.alpha;
+  o->misfit = s.hist[0];
+  o->object_error = s.hist[1];
+  o->object_cost = s.hist[2];
+  o->gradient_norm2 = s.hist[3];
+  o->object_norm2 = s.hist[4];
+  o->beta_re = s.beta_re;
+  o->beta_im = s.beta_im;
+  o->solves = s.solves;
+  o->solve_iters = s.solve_iters;
+  o->matvecs = s.matvecs;
+  o->not_converged = s.not_converged;
+  o->seconds = s.seconds;
+}
+
+int isg_dbim_setup(isg_context* c, const double* o_true, int field_start, isg_dbim_stats* st) {
+  if (!check(c)) return ISG_E_STATE;
+  if (!o_true || field_start < 0 || field_start > 1) return fail(c, ISG_E_ARG, "bad argument");
+  isg::DbimOptions opt;
+  opt.numtx = c->cfg.numtx;
+  opt.regular = c->cfg.regular;
+  opt.tol = c->cfg.toli;
+  opt.maxit = c->cfg.iti;
+  opt.field_start = field_start;
+  c->dbim.reset();
+  c->dbim.reset(new isg::Dbim(*c->eng, opt));
+  if (!c->dbim->ok()) return fail(c, ISG_E_DEVICE, c->dbim->error());
+  isg::DbimStats s;
+  c->dbim->setup((const isg::zd*)o_true, s);
+  if (!c->dbim->ok()) return fail(c, ISG_E_DEVICE, c->dbim->error());
+  fill_stats(s, st);
+  return ISG_OK;
+}
+
+int isg_dbim_setup_measured(isg_context* c, const double* mes, const double* o_true, int field_start,
+                            isg_dbim_stats* st) {
+  if (!check(c)) return ISG_E_STATE;
+  if (!mes || field_start < 0 || field_start > 1) return fail(c, ISG_E_ARG, "bad argument");
+  isg::DbimOptions opt;
+  opt.numtx = c->cfg.numtx;
+  opt.regular = c->cfg.regular;
+  opt.tol = c->cfg.toli;
+  opt.maxit = c->cfg.iti;
+  opt.field_start = field_start;
+  c->dbim.reset();
+  c->dbim.reset(new isg::Dbim(*c->eng, opt));
+  if (!c->dbim->ok()) return fail(c, ISG_E_DEVICE, c->dbim->error());
+  isg::DbimStats s;
+  c->dbim->setup_measured((const isg::zd*)mes, (const isg::zd*)o_true, s);
+  if (!c->dbim->ok()) return fail(c, ISG_E_DEVICE, c->dbim->error());
+  fill_stats(s, st);
+  return ISG_OK;
+}
+
+int isg_profile(isg_context* c, int enable) {
+  if (!check(c)) return ISG_E_STATE;
+  c->eng->profile(enable != 0);
+  return check(c) ? ISG_OK : ISG_E_DEVICE;
+}
+
+int isg_profile_read(isg_context* c, int kernel, double* total_ms, long long* launches) {
+  if (!check(c)) return ISG_E_STATE;
+  if (kernel < 0 || kernel >= isg::Engine::KT_COUNT || !total_ms || !launches) return fail(c, ISG_E_ARG, "bad argument");
+  c->eng->profile_read(kernel, *total_ms, *launches);
+  return check(c) ? ISG_OK : ISG_E_DEVICE;
+}
+
+int isg_dbim_iterate(isg_context* c, isg_dbim_stats* st) {
+  if (!check(c)) return ISG_E_STATE;
+  if (!c->dbim) return fail(c, ISG_E_STATE, "isg_dbim_setup has not been called");
+  isg::DbimStats s;
+  c->dbim->iterate(s);
+  if (!c->dbim->ok()) return fail(c, ISG_E_DEVICE, c->dbim->error());
+  fill_stats(s, st);
+  return ISG_OK;
+}
+
+int isg_dbim_get(isg_context* c, int which, double* out) {
+  if (!check(c)) return ISG_E_STATE;
+  if (!c->dbim) return fail(c, ISG_E_STATE, "isg_dbim_setup has not been called");
+  if (!out) return fail(c, ISG_E_ARG, "null pointer");
+  std::vector<isg::zd> v;
+  if (!c->dbim->get(which, v)) return fail(c, ISG_E_ARG, "bad field or device error");
+  std::memcpy(out, v.data(), v.size() * sizeof(isg::zd));
+  return ISG_OK;
+}
+
+int isg_dbim_set(isg_context* c, int which, const double* in) {
+  if (!check(c)) return ISG_E_STATE;
+  if (!c->dbim) return fail(c, ISG_E_STATE, "isg_dbim_setup has not been called");
+  if (!in) return fail(c, ISG_E_ARG, "null pointer");
+  const isg::Plan& P = c->eng->plan();
+  size_t n = (which == 4) ? (size_t)c->cfg.numtx * c->cfg.numrx : (which == 6 ? (size_t)c->cfg.numtx * P.N : (size_t)P.N);
+  std::vector<isg::zd> v(n);
+  std::memcpy(v.data(), in, n * sizeof(isg::zd));
+  if (!c->dbim->set(which, v)) return fail(c, ISG_E_ARG, "bad field or device error");
   return ISG_OK;
 }
 

@@ -1,0 +1,420 @@
+// isg_dbim.hip -- DBIM driver: vector algebra kernels and the iteration.
+#include "isg_dbim.hpp"
+
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+#include "isg_kernels.hpp"
+
+namespace isg {
+
+#define DB_HIP(call)                                                                   \
+  do {                                                                                 \
+    hipError_t e_ = (call);                                                            \
+    if (e_ != hipSuccess && err_.empty()) err_ = std::string("HIP: ") + hipGetErrorString(e_) + " at " #call; \
+  } while (0)
+
+#define DB_LOOP(N) for (long i = (long)blockIdx.x * kBlock + threadIdx.x; i < (N); i += (long)gridDim.x * kBlock)
+
+namespace {
+
+// inc[t][n] = exp(i k0 u_t . r_n): plane wave along transmitter t (setup_born.cpp:62-65)
+__global__ void __launch_bounds__(kBlock) k_incident(zd* inc, const zd* pos, const zd* tx, long N, double k0) {
+  int t = blockIdx.y;
+  zd u = tx[t];
+  DB_LOOP(N) {
+    zd p = pos[i];
+    double ph = k0 * (u.x * p.x + u.y * p.y);
+    double s, c;
+    sincos(ph, &s, &c);
+    inc[t * N + i] = mkc<double>(c, s);
+  }
+}
+
+// g = o k0 k0
+__global__ void __launch_bounds__(kBlock) k_weight(const zd* o, zd* g, long N, double k0) {
+  DB_LOOP(N) {
+    zd v = o[i];
+    g[i] = mkc<double>(v.x * k0 * k0, v.y * k0 * k0);
+  }
+}
+
+// out[t] = a[t] * w  (w shared)
+__global__ void __launch_bounds__(kBlock) k_rowmul(const zd* a, const zd* w, zd* out, long N) {
+  int t = blockIdx.y;
+  DB_LOOP(N) out[t * N + i] = cmul(a[t * N + i], w[i]);
+}
+
+// y[t] = tot[t] * ot * k0^2   (mom.cpp:284)
+__global__ void __launch_bounds__(kBlock) k_deriv_src(const zd* tot, const zd* ot, zd* y, long N, double k0) {
+  int t = blockIdx.y;
+  DB_LOOP(N) {
+    zd v = cmul(tot[t * N + i], ot[i]);
+    y[t * N + i] = mkc<double>(v.x * k0 * k0, v.y * k0 * k0);
+  }
+}
+
+// y[t] += g * x[t]   (mom.cpp:301)
+__global__ void __launch_bounds__(kBlock) k_add_gx(zd* y, const zd* g, const zd* x, long N) {
+  int t = blockIdx.y;
+  DB_LOOP(N) y[t * N + i] = cadd(y[t * N + i], cmul(g[i], x[t * N + i]));
+}
+
+// c[t] = conj(g) * y[t]   (mom.cpp:344)
+__global__ void __launch_bounds__(kBlock) k_adj_src(const zd* g, const zd* y, zd* c, long N) {
+  int t = blockIdx.y;
+  DB_LOOP(N) c[t * N + i] = cmul(cconj(g[i]), y[t * N + i]);
+}
+
+// y = conj(x)
+__global__ void __launch_bounds__(kBlock) k_conj(const zd* x, zd* y, long N) {
+  int t = blockIdx.y;
+  DB_LOOP(N) y[t * N + i] = cconj(x[t * N + i]);
+}
+
+// gradient accumulation over a batch, transmitters in order (mom.cpp:351-360):
+// for t: y_t += conj(Gc_t); buff += conj(tot_t) y_t; buff += reg^2 o
+__global__ void __launch_bounds__(kBlock) k_grad_acc(zd* buff, const zd* tot, const zd* y, const zd* gcx, const zd* o,
+                                                     long N, int nb, double reg2) {
+  DB_LOOP(N) {
+    zd acc = buff[i];
+    zd ov = o[i];
+    for (int t = 0; t < nb; t++) {
+      zd yv = cadd(y[t * N + i], cconj(gcx[t * N + i]));
+      acc = cadd(acc, cmul(cconj(tot[t * N + i]), yv));
+      acc = cadd(acc, mkc<double>(reg2 * ov.x, reg2 * ov.y));
+    }
+    buff[i] = acc;
+  }
+}
+
+// a = a + s * b  (complex s)
+__global__ void __launch_bounds__(kBlock) k_axpy(zd* a, const zd* b, zd s, long N) {
+  DB_LOOP(N) a[i] = cadd(a[i], cmul(s, b[i]));
+}
+
+// ot = od + beta ot
+__global__ void __launch_bounds__(kBlock) k_newdir(zd* ot, const zd* od, zd beta, long N) {
+  DB_LOOP(N) ot[i] = cadd(od[i], cmul(beta, ot[i]));
+}
+
+// partial sums for the metrics and the Polak-Ribiere coefficient:
+// 0 |od|^2, 1-2 od conj(od - save), 3 |save|^2, 4 |o|^2, 5 |o - omes|^2
+constexpr int kSums = 6;
+__global__ void __launch_bounds__(kBlock) k_sums(const zd* o, const zd* omes, const zd* od, const zd* save, long N,
+                                                 double* part) {
+  double v[kSums] = {0, 0, 0, 0, 0, 0};
+  DB_LOOP(N) {
+    zd a = od[i], s = save[i], ov = o[i], om = omes[i];
+    v[0] += a.x * a.x + a.y * a.y;
+    zd d = csub(a, s);
+    v[1] += a.x * d.x + a.y * d.y;
+    v[2] += a.y * d.x - a.x * d.y;
+    v[3] += s.x * s.x + s.y * s.y;
+    v[4] += ov.x * ov.x + ov.y * ov.y;
+    zd e = csub(ov, om);
+    v[5] += e.x * e.x + e.y * e.y;
+  }
+  __shared__ double sh[kSums][kBlock / 64];
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int s = 0; s < kSums; s++) {
+    double a = wave_sum(v[s]);
+    if (lane == 0) sh[s][w] = a;
+  }
+  __syncthreads();
+  if (threadIdx.x < kSums) {
+    double a = 0;
+    for (int j = 0; j < kBlock / 64; j++) a += sh[threadIdx.x][j];
+    part[threadIdx.x * gridDim.x + blockIdx.x] = a;
+  }
+}
+
+}  // namespace
+
+void* Dbim::dalloc(size_t bytes) {
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, bytes ? bytes : 16);
+  if (e != hipSuccess) {
+    if (err_.empty()) err_ = std::string("hipMalloc: ") + hipGetErrorString(e);
+    return nullptr;
+  }
+  allocs_.push_back(p);
+  return p;
+}
+
+Dbim::Dbim(Engine& eng, const DbimOptions& opt) : eng_(eng), opt_(opt) {
+  if (!eng_.ok()) return;
+  const Plan& P = eng_.plan();
+  N_ = P.N;
+  R_ = P.cfg.numrx;
+  T_ = opt_.numtx;
+  B_ = eng_.max_rhs();
+  k0_ = P.k0;
+  if (T_ < 1) {
+    err_ = "numtx must be >= 1";
+    return;
+  }
+  size_t vb = sizeof(zd) * (size_t)N_;
+  pos_ = (zd*)dalloc(vb);
+  tx_ = (zd*)dalloc(sizeof(zd) * T_);
+  o_ = (zd*)dalloc(vb);
+  od_ = (zd*)dalloc(vb);
+  ot_ = (zd*)dalloc(vb);
+  save_ = (zd*)dalloc(vb);
+  omes_ = (zd*)dalloc(vb);
+  g_ = (zd*)dalloc(vb);
+  buff_ = (zd*)dalloc(vb);
+  inc_ = (zd*)dalloc(vb * T_);
+  tot_ = (zd*)dalloc(vb * T_);
+  mes_ = (zd*)dalloc(sizeof(zd) * (size_t)T_ * R_);
+  mesd_ = (zd*)dalloc(sizeof(zd) * (size_t)T_ * R_);
+  rhs_ = (zd*)dalloc(sizeof(zd) * (size_t)T_ * R_);
+  wy_ = (zd*)dalloc(vb * B_);
+  wc_ = (zd*)dalloc(vb * B_);
+  wx_ = (zd*)dalloc(vb * B_);
+  nblk_ = (int)std::min<long>(1024, std::max<long>(1, (N_ + kBlock * 4 - 1) / (kBlock * 4)));
+  part_ = (double*)dalloc(sizeof(double) * kSums * nblk_);
+  DB_HIP(hipHostMalloc((void**)&part_h_, sizeof(double) * kSums * nblk_, hipHostMallocDefault));
+  if (!ok()) return;
+  DB_HIP(hipMemcpy(pos_, P.pos.data(), vb, hipMemcpyHostToDevice));
+  // transmitter directions equispaced on 360 degrees (mom.cpp:141-147)
+  std::vector<zd> tx(T_);
+  for (int i = 0; i < T_; i++) {
+    double ang = 360.0 / T_ * i;
+    tx[i] = mkc<double>(std::cos(ang / 360 * 2 * M_PI), std::sin(ang / 360 * 2 * M_PI));
+  }
+  DB_HIP(hipMemcpy(tx_, tx.data(), sizeof(zd) * T_, hipMemcpyHostToDevice));
+}
+
+Dbim::~Dbim() {
+  (void)hipStreamSynchronize(eng_.stream());
+  for (void* p : allocs_) (void)hipFree(p);
+  if (part_h_) (void)hipHostFree(part_h_);
+}
+
+void Dbim::solve(zd* x, const zd* b, bool herm, int nb, bool x_zero, DbimStats& st) {
+  std::vector<SolveStat> ss;
+  eng_.bicgs(x, g_, 0, b, herm, nb, x_zero, opt_.tol, opt_.maxit, ss);
+  for (auto& s : ss) {
+    st.solves++;
+    st.solve_iters += s.iters;
+    if (s.status != 0) st.not_converged++;
+  }
+}
+
+// mom.cpp:247-271 / :338-362: od = sum_t conj(tot_t) (A_t^H mesd_t) + reg^2 o
+void Dbim::gradient(DbimStats& st) {
+  hipStream_t s = eng_.stream();
+  const long N = N_;
+  DB_HIP(hipMemsetAsync(buff_, 0, sizeof(zd) * N, s));
+  for (int t0 = 0; t0 < T_; t0 += B_) {
+    int nb = std::min(B_, T_ - t0);
+    dim3 grid(nblk_, nb), blk(kBlock);
+    eng_.aggregateh(mesd_ + (size_t)t0 * R_, wy_, nb);
+    k_adj_src<<<grid, blk, 0, s>>>(g_, wy_, wc_, N);
+    DB_HIP(hipMemsetAsync(wx_, 0, sizeof(zd) * N * nb, s));
+    solve(wx_, wc_, true, nb, true, st);
+    // c = conj(G conj(x))  (mlfmah, solve.cpp:8-24); the conjugate of Gc is taken in k_grad_acc
+    k_conj<<<grid, blk, 0, s>>>(wx_, wc_, N);
+    eng_.mlfma(wc_, wx_, nb);
+    k_grad_acc<<<dim3(nblk_), blk, 0, s>>>(buff_, tot_ + (size_t)t0 * N, wy_, wx_, o_, N, nb,
+                                           opt_.regular * opt_.regular);
+  }
+  DB_HIP(hipMemcpyAsync(od_, buff_, sizeof(zd) * N, hipMemcpyDeviceToDevice, s));
+}
+
+// setup_born.cpp:171-242 metrics (hist.txt columns); all squared ratios
+void Dbim::metrics(DbimStats& st) {
+  hipStream_t s = eng_.stream();
+  k_sums<<<dim3(nblk_), dim3(kBlock), 0, s>>>(o_, omes_, od_, save_, N_, part_);
+  DB_HIP(hipMemcpyAsync(part_h_, part_, sizeof(double) * kSums * nblk_, hipMemcpyDeviceToHost, s));
+  std::vector<zd> md((size_t)T_ * R_);
+  DB_HIP(hipMemcpyAsync(md.data(), mesd_, sizeof(zd) * md.size(), hipMemcpyDeviceToHost, s));
+  DB_HIP(hipStreamSynchronize(s));
+  double sum[kSums] = {0, 0, 0, 0, 0, 0};
+  for (int k = 0; k < kSums; k++)
+    for (int j = 0; j < nblk_; j++) sum[k] += part_h_[k * nblk_ + j];
+  double err = 0;
+  for (auto& v : md) err += v.x * v.x + v.y * v.y;
+  double reg = opt_.regular;
+  st.hist[0] = err / mesnorm_;
+  st.hist[1] = sum[5] / objnorm_;
+  st.hist[2] = reg * reg * sum[4] / mesnorm_;
+  st.hist[3] = sum[0];
+  st.hist[4] = sum[4];
+}
+
+void Dbim::setup(const zd* otrue, DbimStats& st) {
+  if (!ok()) return;
+  auto t_start = std::chrono::steady_clock::now();
+  hipStream_t s = eng_.stream();
+  const long N = N_;
+  dim3 blk(kBlock);
+  k_incident<<<dim3(nblk_, T_), blk, 0, s>>>(inc_, pos_, tx_, N, k0_);
+  DB_HIP(hipMemcpyAsync(o_, otrue, sizeof(zd) * N, hipMemcpyHostToDevice, s));
+  k_weight<<<dim3(nblk_), blk, 0, s>>>(o_, g_, N, k0_);
+  // synthetic measurements: mes_t = aggregate(g * A^-1 inc_t)  (setup_born.cpp:139-149)
+  for (int t0 = 0; t0 < T_; t0 += B_) {
+    int nb = std::min(B_, T_ - t0);
+    DB_HIP(hipMemsetAsync(wx_, 0, sizeof(zd) * N * nb, s));
+    solve(wx_, inc_ + (size_t)t0 * N, false, nb, true, st);
+    k_rowmul<<<dim3(nblk_, nb), blk, 0, s>>>(wx_, g_, wy_, N);
+    eng_.aggregate(wy_, mes_ + (size_t)t0 * R_, nb);
+  }
+  std::vector<zd> mes((size_t)T_ * R_);
+  DB_HIP(hipMemcpyAsync(mes.data(), mes_, sizeof(zd) * mes.size(), hipMemcpyDeviceToHost, s));
+  DB_HIP(hipStreamSynchronize(s));
+  start(mes.data(), otrue, st);
+  st.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+}
+
+void Dbim::setup_measured(const zd* mes_host, const zd* otrue, DbimStats& st) {
+  if (!ok()) return;
+  auto t_start = std::chrono::steady_clock::now();
+  k_incident<<<dim3(nblk_, T_), dim3(kBlock), 0, eng_.stream()>>>(inc_, pos_, tx_, N_, k0_);
+  start(mes_host, otrue, st);
+  st.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+}
+
+// Inversion start from measurements (mom.cpp:222-276): zero object, total
+// field = incident field, mesd = -mes, first gradient and direction.
+void Dbim::start(const zd* mes_host, const zd* otrue, DbimStats& st) {
+  hipStream_t s = eng_.stream();
+  const long N = N_;
+  std::vector<zd> mes(mes_host, mes_host + (size_t)T_ * R_);
+  DB_HIP(hipMemcpyAsync(mes_, mes.data(), sizeof(zd) * mes.size(), hipMemcpyHostToDevice, s));
+  objnorm_ = 0;
+  if (otrue) {
+    DB_HIP(hipMemcpyAsync(omes_, otrue, sizeof(zd) * N, hipMemcpyHostToDevice, s));
+    for (long n = 0; n < N; n++) objnorm_ += otrue[n].x * otrue[n].x + otrue[n].y * otrue[n].y;
+  } else {
+    DB_HIP(hipMemsetAsync(omes_, 0, sizeof(zd) * N, s));
+  }
+  if (!(objnorm_ > 0)) objnorm_ = 1;
+  mesnorm_ = 0;
+  for (auto& v : mes) mesnorm_ += v.x * v.x + v.y * v.y;
+  // initial guess: no object, total field = incident field, mesd = -mes (mom.cpp:224-229)
+  for (auto& v : mes) v = mkc<double>(-v.x, -v.y);
+  DB_HIP(hipMemcpyAsync(mesd_, mes.data(), sizeof(zd) * mes.size(), hipMemcpyHostToDevice, s));
+  DB_HIP(hipMemsetAsync(o_, 0, sizeof(zd) * N, s));
+  DB_HIP(hipMemsetAsync(g_, 0, sizeof(zd) * N, s));
+  DB_HIP(hipMemsetAsync(save_, 0, sizeof(zd) * N, s));
+  DB_HIP(hipMemcpyAsync(tot_, inc_, sizeof(zd) * N * T_, hipMemcpyDeviceToDevice, s));
+  gradient(st);
+  metrics(st);
+  DB_HIP(hipMemcpyAsync(ot_, od_, sizeof(zd) * N, hipMemcpyDeviceToDevice, s));
+  DB_HIP(hipStreamSynchronize(s));
+  iter_ = 0;
+  st.matvecs = eng_.matvecs;
+}
+
+void Dbim::iterate(DbimStats& st) {
+  if (!ok()) return;
+  auto t_start = std::chrono::steady_clock::now();
+  long long mv0 = eng_.matvecs;
+  hipStream_t s = eng_.stream();
+  const long N = N_;
+  dim3 blk(kBlock);
+  // Gauss-Newton step length from the Frechet derivative along ot (mom.cpp:281-316)
+  for (int t0 = 0; t0 < T_; t0 += B_) {
+    int nb = std::min(B_, T_ - t0);
+    dim3 grid(nblk_, nb);
+    k_deriv_src<<<grid, blk, 0, s>>>(tot_ + (size_t)t0 * N, ot_, wy_, N, k0_);
+    eng_.mlfma(wy_, wc_, nb);
+    DB_HIP(hipMemsetAsync(wx_, 0, sizeof(zd) * N * nb, s));
+    solve(wx_, wc_, false, nb, true, st);
+    k_add_gx<<<grid, blk, 0, s>>>(wy_, g_, wx_, N);
+    eng_.aggregate(wy_, rhs_ + (size_t)t0 * R_, nb);
+  }
+  std::vector<zd> rhs((size_t)T_ * R_), mesd((size_t)T_ * R_);
+  DB_HIP(hipMemcpyAsync(rhs.data(), rhs_, sizeof(zd) * rhs.size(), hipMemcpyDeviceToHost, s));
+  DB_HIP(hipMemcpyAsync(mesd.data(), mesd_, sizeof(zd) * mesd.size(), hipMemcpyDeviceToHost, s));
+  DB_HIP(hipStreamSynchronize(s));
+  if (!ok()) return;
+  double num = 0, den = 0;
+  for (size_t m = 0; m < rhs.size(); m++) {
+    num += mesd[m].x * rhs[m].x + mesd[m].y * rhs[m].y;  // Re(conj(mesd) rhs)
+    den += rhs[m].x * rhs[m].x + rhs[m].y * rhs[m].y;
+  }
+  double alpha = num / den;
+  st.alpha = alpha;
+  k_axpy<<<dim3(nblk_), blk, 0, s>>>(o_, ot_, mkc<double>(-alpha, 0.0), N);
+  k_weight<<<dim3(nblk_), blk, 0, s>>>(o_, g_, N, k0_);
+  // total-field update for the new contrast (mom.cpp:322-336)
+  for (int t0 = 0; t0 < T_; t0 += B_) {
+    int nb = std::min(B_, T_ - t0);
+    zd* tot = tot_ + (size_t)t0 * N;
+    bool zero = opt_.field_start == 0;
+    if (zero) DB_HIP(hipMemsetAsync(tot, 0, sizeof(zd) * N * nb, s));
+    solve(tot, inc_ + (size_t)t0 * N, false, nb, zero, st);
+    k_rowmul<<<dim3(nblk_, nb), blk, 0, s>>>(tot, g_, wx_, N);
+    eng_.aggregate(wx_, rhs_ + (size_t)t0 * R_, nb);
+  }
+  DB_HIP(hipMemcpyAsync(rhs.data(), rhs_, sizeof(zd) * rhs.size(), hipMemcpyDeviceToHost, s));
+  std::vector<zd> mes((size_t)T_ * R_);
+  DB_HIP(hipMemcpyAsync(mes.data(), mes_, sizeof(zd) * mes.size(), hipMemcpyDeviceToHost, s));
+  DB_HIP(hipStreamSynchronize(s));
+  for (size_t m = 0; m < rhs.size(); m++) mesd[m] = csub(rhs[m], mes[m]);
+  DB_HIP(hipMemcpyAsync(mesd_, mesd.data(), sizeof(zd) * mesd.size(), hipMemcpyHostToDevice, s));
+  DB_HIP(hipMemcpyAsync(save_, od_, sizeof(zd) * N, hipMemcpyDeviceToDevice, s));
+  gradient(st);
+  metrics(st);
+  // Polak-Ribiere (mom.cpp:383-399): beta = sum od conj(od - save) / sum |save|^2
+  double s1r = 0, s1i = 0, s2 = 0;
+  for (int j = 0; j < nblk_; j++) {
+    s1r += part_h_[1 * nblk_ + j];
+    s1i += part_h_[2 * nblk_ + j];
+    s2 += part_h_[3 * nblk_ + j];
+  }
+  zd beta = mkc<double>(s1r / s2, s1i / s2);
+  st.beta_re = beta.x;
+  st.beta_im = beta.y;
+  k_newdir<<<dim3(nblk_), blk, 0, s>>>(ot_, od_, beta, N);
+  DB_HIP(hipStreamSynchronize(s));
+  iter_++;
+  st.matvecs = eng_.matvecs - mv0;
+  st.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+}
+
+bool Dbim::get(int which, std::vector<zd>& out) {
+  const zd* src = nullptr;
+  size_t n = 0;
+  switch (which) {
+    case 0: src = o_; n = N_; break;
+    case 1: src = od_; n = N_; break;
+    case 2: src = ot_; n = N_; break;
+    case 3: src = mes_; n = (size_t)T_ * R_; break;
+    case 4: src = mesd_; n = (size_t)T_ * R_; break;
+    case 5: src = inc_; n = (size_t)T_ * N_; break;
+    case 6: src = tot_; n = (size_t)T_ * N_; break;
+    default: return false;
+  }
+  out.resize(n);
+  DB_HIP(hipStreamSynchronize(eng_.stream()));
+  DB_HIP(hipMemcpy(out.data(), src, sizeof(zd) * n, hipMemcpyDeviceToHost));
+  return ok();
+}
+
+bool Dbim::set(int which, const std::vector<zd>& in) {
+  zd* dst = nullptr;
+  size_t n = 0;
+  switch (which) {
+    case 0: dst = o_; n = N_; break;
+    case 1: dst = od_; n = N_; break;
+    case 2: dst = ot_; n = N_; break;
+    case 4: dst = mesd_; n = (size_t)T_ * R_; break;
+    case 6: dst = tot_; n = (size_t)T_ * N_; break;
+    default: return false;
+  }
+  if (in.size() != n) return false;
+  DB_HIP(hipStreamSynchronize(eng_.stream()));
+  DB_HIP(hipMemcpy(dst, in.data(), sizeof(zd) * n, hipMemcpyHostToDevice));
+  if (which == 0) {
+    k_weight<<<dim3(nblk_), dim3(kBlock), 0, eng_.stream()>>>(o_, g_, N_, k0_);
+    DB_HIP(hipStreamSynchronize(eng_.stream()));
+  }
+  return ok();
+}
+
+}  // namespace isg

@@ -1,0 +1,78 @@
+// isg_dbim.hpp -- distorted-Born / nonlinear-CG inversion loop on one GPU.
+//
+// Device-resident restatement of the reference's DBIM driver (mom.cpp:222-400,
+// setup_born.cpp:8-242).  All per-transmitter fields live in HBM as
+// [numtx][N] arrays; transmitters are processed in batches of up to
+// Engine::max_rhs right-hand sides, so batch 1 is the reference's plain loop
+// and a batch of every transmitter is one multi-RHS solve per phase.
+#pragma once
+
+#include <vector>
+
+#include "isg_engine.hpp"
+
+namespace isg {
+
+struct DbimOptions {
+  int numtx = 16;
+  double regular = 0.0;   // gradient adds regular^2 * o per transmitter (mom.cpp:267-269)
+  double tol = 1e-4;      // BiCGSTAB relative residual (mom.cpp:21)
+  int maxit = 300;        // BiCGSTAB iteration limit (mom.cpp:22)
+  int field_start = 0;    // total-field update start: 0 zero (mom.cpp:324), 1 previous total field
+};
+
+struct DbimStats {
+  double alpha = 0;          // step length (mom.cpp:315)
+  double hist[5] = {0, 0, 0, 0, 0};  // hist.txt columns (setup_born.cpp:239)
+  double beta_re = 0, beta_im = 0;   // Polak-Ribiere coefficient (mom.cpp:394)
+  long long solves = 0, solve_iters = 0, matvecs = 0, not_converged = 0;
+  double seconds = 0;
+};
+
+class Dbim {
+ public:
+  Dbim(Engine& eng, const DbimOptions& opt);
+  ~Dbim();
+  bool ok() const { return eng_.ok() && err_.empty(); }
+  const std::string& error() const { return err_.empty() ? eng_.error() : err_; }
+
+  // Incident fields, synthetic measurements of the true contrast (tree order,
+  // host array of N complex), zero initial object and the first gradient.
+  void setup(const zd* otrue_host, DbimStats& st);
+  // Same start from given measurements mes [numtx][numrx] (host); otrue may be
+  // null (then the object-error metric is sum|o|^2).
+  void setup_measured(const zd* mes_host, const zd* otrue_host, DbimStats& st);
+  // One DBIM iteration (mom.cpp:277-400).
+  void iterate(DbimStats& st);
+  // which: 0 o, 1 od, 2 ot, 3 mes, 4 mesd, 5 inc, 6 tot ; copies to host
+  bool get(int which, std::vector<zd>& out);
+  bool set(int which, const std::vector<zd>& in);
+  int iteration() const { return iter_; }
+
+ private:
+  void start(const zd* mes_host, const zd* otrue_host, DbimStats& st);
+  void gradient(DbimStats& st);
+  void metrics(DbimStats& st);
+  void solve(zd* x, const zd* b, bool herm, int nb, bool x_zero, DbimStats& st);
+  void* dalloc(size_t bytes);
+
+  Engine& eng_;
+  DbimOptions opt_;
+  std::string err_;
+  int N_ = 0, R_ = 0, T_ = 0, B_ = 0;
+  double k0_ = 0;
+  zd *pos_ = nullptr, *tx_ = nullptr;
+  zd *o_ = nullptr, *od_ = nullptr, *ot_ = nullptr, *save_ = nullptr, *omes_ = nullptr, *g_ = nullptr,
+     *buff_ = nullptr;
+  zd *inc_ = nullptr, *tot_ = nullptr;
+  zd *mes_ = nullptr, *mesd_ = nullptr, *rhs_ = nullptr;
+  zd *wy_ = nullptr, *wc_ = nullptr, *wx_ = nullptr;
+  double* part_ = nullptr;
+  double* part_h_ = nullptr;
+  int nblk_ = 0;
+  double mesnorm_ = 0, objnorm_ = 0;
+  int iter_ = 0;
+  std::vector<void*> allocs_;
+};
+
+}  // namespace isg

@@ -24,7 +24,7 @@ __device__ __forceinline__ cplx<T> cadd(cplx<T> a, cplx<T> b) {
   return mkc<T>(a.x + b.x, a.y + b.y);
 }
 template <typename T>
-__device__ __forceinline__ cplx<T> csub(cplx<T> a, cplx<T> b) {
+__host__ __device__ __forceinline__ cplx<T> csub(cplx<T> a, cplx<T> b) {
   return mkc<T>(a.x - b.x, a.y - b.y);
 }
 template <typename T>

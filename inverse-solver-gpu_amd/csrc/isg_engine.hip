@@ -125,6 +125,10 @@ Engine::Engine(const EngineConfig& cfg) : cfg_(cfg) {
 
 Engine::~Engine() {
   if (stream_) (void)hipStreamSynchronize(stream_);
+  for (auto& k : kt_) {
+    for (auto e : k.ev0) (void)hipEventDestroy(e);
+    for (auto e : k.ev1) (void)hipEventDestroy(e);
+  }
   for (void* p : allocs_) (void)hipFree(p);
   if (active_h_) (void)hipHostFree(active_h_);
   if (state_h_) (void)hipHostFree(state_h_);
@@ -147,16 +151,92 @@ void Engine::apply_mlfma(const zd* y, zd* r, int nrhs, const int* act) {
     k_m2m<double><<<dim3(nblocks(sz(i)), nrhs), blk, 0, stream_>>>(M(i + 1), sz(i + 1), M(i), sz(i), d.interpT[i],
                                                                       interp0_[i], d.shiftM[i], P_.nsamp[i],
                                                                       P_.nsamp[i + 1], P_.nclus[i], cfg_.plan.ninter, act);
+  prof_begin(KT_M2L);
   for (int i = 2; i <= Lf; i++)
     k_m2l<double><<<dim3(nblocks(sz(i)), nrhs), blk, 0, stream_>>>(M(i), Lc(i), sz(i), d.trans[i], i, P_.nsamp[i], act);
+  prof_end(KT_M2L);
   for (int i = 2; i <= L - 2; i++)
     k_l2l<double><<<dim3(nblocks(sz(i + 1)), nrhs), blk, 0, stream_>>>(Lc(i), sz(i), Lc(i + 1), sz(i + 1), d.anterpT[i + 1],
                                                                           anterp0_[i + 1], d.shiftL[i], P_.nsamp[i],
                                                                           P_.nsamp[i + 1], P_.nclus[i],
                                                                           2 * cfg_.plan.ninter, 1, act);
-  k_l2p<double, double><<<dim3(nblocks((long)B * 64), nrhs), blk, 0, stream_>>>(Lc(Lf), sz(Lf), r, N, d.basisT, S, B, act);
-  k_p2p<double, double><<<dim3(nblocks((long)B * 64), nrhs), blk, 0, stream_>>>(y, N, r, N, d.near, 1 << Lf, B, act);
+  launch_near(y, Lc(Lf), r, nrhs, act, true);
   ISG_HIP(hipGetLastError());
+}
+
+// Near field + L2P on the matrix cores (k_near_mfma), or L2P alone when
+// with_near is false (receiver back-projection).
+void Engine::launch_near(const zd* y, const zd* loc, zd* r, int nrhs, const int* act, bool with_near) {
+  const int Lf = P_.L - 1, S = P_.nsamp[Lf], B = P_.B;
+  const long N = P_.N, ls = (long)lvl_size_[Lf];
+  const int nb = B >= 32 ? 32 : 16;
+  size_t lds = (size_t)nb * near_row(S) * sizeof(zd);
+  dim3 grid(B / nb, nrhs), blk(kBlock);
+  if (with_near) prof_begin(KT_NEAR);
+  if (nb == 32)
+    k_near_mfma<double, double, 32><<<grid, blk, lds, stream_>>>(y, N, loc, ls, r, N, op64_.near, op64_.basisT, S,
+                                                                 1 << Lf, with_near ? 1 : 0, act);
+  else
+    k_near_mfma<double, double, 16><<<grid, blk, lds, stream_>>>(y, N, loc, ls, r, N, op64_.near, op64_.basisT, S,
+                                                                 1 << Lf, with_near ? 1 : 0, act);
+  if (with_near) prof_end(KT_NEAR);
+}
+
+// ---- live kernel timing (HIP events on the engine stream)
+void Engine::profile(bool on) {
+  harvest();
+  prof_ = on;
+  for (auto& k : kt_) {
+    k.ms = 0;
+    k.launches = 0;
+  }
+}
+
+void Engine::prof_begin(int id) {
+  if (!prof_) return;
+  KTimer& k = kt_[id];
+  if (k.used == (int)k.ev0.size()) {
+    if (k.used >= 512) {
+      ISG_HIP(hipStreamSynchronize(stream_));
+      harvest();
+    } else {
+      hipEvent_t a, b;
+      ISG_HIP(hipEventCreate(&a));
+      ISG_HIP(hipEventCreate(&b));
+      k.ev0.push_back(a);
+      k.ev1.push_back(b);
+    }
+  }
+  ISG_HIP(hipEventRecord(k.ev0[k.used], stream_));
+}
+
+void Engine::prof_end(int id) {
+  if (!prof_) return;
+  KTimer& k = kt_[id];
+  ISG_HIP(hipEventRecord(k.ev1[k.used], stream_));
+  k.used++;
+}
+
+// Accumulate the elapsed time of every recorded launch; the stream must have
+// reached the recorded events (callers synchronize first).
+void Engine::harvest() {
+  for (auto& k : kt_) {
+    for (int i = 0; i < k.used; i++) {
+      float ms = 0;
+      if (hipEventSynchronize(k.ev1[i]) == hipSuccess && hipEventElapsedTime(&ms, k.ev0[i], k.ev1[i]) == hipSuccess) {
+        k.ms += ms;
+        k.launches++;
+      }
+    }
+    k.used = 0;
+  }
+}
+
+void Engine::profile_read(int id, double& ms, long long& launches) {
+  ISG_HIP(hipStreamSynchronize(stream_));
+  harvest();
+  ms = kt_[id].ms;
+  launches = kt_[id].launches;
 }
 
 void Engine::mlfma(const zd* x, zd* r, int nrhs, const int* act) {
@@ -203,8 +283,7 @@ void Engine::aggregateh(const zd* r, zd* x, int nrhs) {
                                                                           anterp0_[i + 1], d.shiftL[i], P_.nsamp[i],
                                                                           P_.nsamp[i + 1], P_.nclus[i],
                                                                           2 * cfg_.plan.ninter, 0, nullptr);
-  k_l2p<double, double><<<dim3(nblocks((long)B * 64), nrhs), blk, 0, stream_>>>(Lc(Lf), sz(Lf), x, N, d.basisT, S, B,
-                                                                                nullptr);
+  launch_near(nullptr, Lc(Lf), x, nrhs, nullptr, false);
   ISG_HIP(hipGetLastError());
 }
 
@@ -243,6 +322,7 @@ void Engine::bicgs(zd* x, const zd* g, long gs, const zd* b, bool herm, int nrhs
     ISG_HIP(hipMemcpyAsync(active_h_, active_, nrhs * sizeof(int), hipMemcpyDeviceToHost, stream_));
     ISG_HIP(hipStreamSynchronize(stream_));
     if (!ok()) return;
+    if (prof_) harvest();
     int nact = 0;
     for (int i = 0; i < nrhs; i++) nact += active_h_[i] != 0;
     if (nact == 0) break;
@@ -254,7 +334,9 @@ void Engine::bicgs(zd* x, const zd* g, long gs, const zd* b, bool herm, int nrhs
     apply_mlfma(y, op, nrhs, active_);
     k_bi_post2<<<grid, blk, 0, stream_>>>(s, op, g, gs, t, N, h, state_, part_);
     reduce_finalize(PH_OMEGA, nrhs, tol, maxit);
+    prof_begin(KT_VEC);
     k_bi_xrupd<<<grid, blk, 0, stream_>>>(x, p, s, t, r, rt, N, state_, part_);
+    prof_end(KT_VEC);
     reduce_finalize(PH_RESID, nrhs, tol, maxit);
     matvecs += 2 * nact;
   }

@@ -87,7 +87,25 @@ class Engine {
   // low-level launches used by the DBIM driver
   void launch_prep(const zd* x, const zd* g, long g_stride, zd* y, bool herm, int nrhs);
 
+  // Live per-kernel timing with HIP events on the engine stream.
+  enum { KT_NEAR = 0, KT_M2L = 1, KT_VEC = 2, KT_COUNT = 3 };
+  void profile(bool on);
+  void profile_read(int id, double& ms, long long& launches);
+
  private:
+  struct KTimer {
+    std::vector<hipEvent_t> ev0, ev1;
+    int used = 0;
+    double ms = 0;
+    long long launches = 0;
+  };
+  void prof_begin(int id);
+  void prof_end(int id);
+  void harvest();
+  void launch_near(const zd* y, const zd* loc, zd* r, int nrhs, const int* act, bool with_near);
+  bool prof_ = false;
+  KTimer kt_[KT_COUNT];
+
   template <typename T>
   void upload(DevOp<T>& d);
   void apply_mlfma(const zd* y, zd* r, int nrhs, const int* act);

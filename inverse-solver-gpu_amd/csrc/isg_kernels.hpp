@@ -6,8 +6,8 @@
 //   k_m2m     child -> parent interpolate+shift  (kernels.cu:83-136)
 //   k_m2l     diagonal translation               (kernels.cu:239-266)
 //   k_l2l     parent -> child shift+anterpolate  (kernels.cu:137-238)
-//   k_l2p     leaf receiving patterns            (kernels.cu:32-81)
-//   k_p2p     near field                         (kernels.cu:292-319)
+//   k_near_mfma  near field + leaf receiving patterns on the matrix cores
+//                                                (kernels.cu:32-81, :292-319)
 //   k_rx_*    receiver interpolation / anterpolation (matvec_gpu.cu:351-361, :395-401)
 // Grids are (work items / 256, right-hand sides); blockIdx.y selects the
 // right-hand side and an optional `act` mask skips converged ones.
@@ -126,46 +126,128 @@ __global__ void __launch_bounds__(kBlock) k_l2l(const cplx<T>* __restrict__ par,
   *dst = accumulate ? cadd(*dst, acc) : acc;
 }
 
-// r[b*64+n] = sum_k basisT[k][n] loc[b][k]   (assigns)
-template <typename T, typename TO>
-__global__ void __launch_bounds__(kBlock) k_l2p(const cplx<T>* __restrict__ agl, long as, cplx<TO>* __restrict__ r,
-                                                long rs, const cplx<T>* __restrict__ basisT, int S, int B,
-                                                const int* __restrict__ act) {
-  int rhs = blockIdx.y;
-  if (rhs_off(act, rhs)) return;
-  long idx = (long)blockIdx.x * kBlock + threadIdx.x;
-  if (idx >= (long)B * 64) return;
-  int b = (int)(idx >> 6), n = (int)(idx & 63);
-  const cplx<T>* lb = agl + rhs * as + (long)b * S;
-  cplx<T> acc = mkc<T>(0, 0);
-  for (int k = 0; k < S; k++) acc = cfma(basisT[k * 64 + n], lb[k], acc);
-  r[rhs * rs + idx] = ccast<TO>(acc);
-}
-
-// r[b*64+m] += sum over in-domain 3x3 neighbour boxes nb (slot s) of
-//              sum_n near[s][n][m] y[nb*64+n]
-template <typename T, typename TI>
-__global__ void __launch_bounds__(kBlock) k_p2p(const cplx<TI>* __restrict__ y, long ys, cplx<TI>* __restrict__ r,
-                                                long rs, const cplx<T>* __restrict__ near, int nbs, int B,
-                                                const int* __restrict__ act) {
-  int rhs = blockIdx.y;
-  if (rhs_off(act, rhs)) return;
-  long idx = (long)blockIdx.x * kBlock + threadIdx.x;
-  if (idx >= (long)B * 64) return;
-  int b = (int)(idx >> 6), m = (int)(idx & 63);
-  int X = (int)compact2((unsigned)b), Y = (int)compact2((unsigned)b >> 1);
-  cplx<T> acc = mkc<T>(0, 0);
-  for (int s = 0; s < 9; s++) {
-    int sx = X + s % 3 - 1, sy = Y + s / 3 - 1;
-    if (sx < 0 || sy < 0 || sx >= nbs || sy >= nbs) continue;
-    long nb = morton2((unsigned)sx, (unsigned)sy);
-    const cplx<TI>* yb = y + rhs * ys + nb * 64;
-    const cplx<T>* blk = near + s * 4096 + m;
-#pragma unroll 8
-    for (int n = 0; n < 64; n++) acc = cfma(blk[n * 64], ccast<T>(yb[n]), acc);
+// ---------------------------------------------------------------------------
+// Matrix-core (MFMA) near field.  For a tile of NB Morton-consecutive leaf
+// boxes the output r[box][m] (64 rows m = test pixel, NB columns) is one
+// complex GEMM over K = 9*64 (near blocks times the gathered neighbour
+// pixels) + S (leaf receiving pattern times the leaf locals, i.e. L2P), so
+// the near field and L2P are fused and r is written once.
+// Wave w owns rows 16w..16w+15; each 16x16 output block accumulates four real
+// products (re*re, -im*im, re*im, im*re) with v_mfma_*_16x16x4.  Operand
+// layout of the 16x16x4 instructions (checked on gfx950): A[i][k] in lane
+// i + 16k, B[k][j] in lane j + 16k; the 16x16 result D[i][j] is in lane
+// j + 16(i/4), register i%4 for f32 and in lane j + 16(i%4), register i/4
+// for f64 -- Mfma<T>::row gives the row of (lane group g, register q).
+template <typename T>
+struct Mfma;
+template <>
+struct Mfma<double> {
+  typedef double v4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ v4 mac(double a, double b, v4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
-  cplx<TI>* dst = r + rhs * rs + idx;
-  *dst = cadd(*dst, ccast<TI>(acc));
+  static __device__ __forceinline__ int row(int g, int q) { return g + 4 * q; }
+};
+template <>
+struct Mfma<float> {
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ v4 mac(float a, float b, v4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int g, int q) { return 4 * g + q; }
+};
+
+// LDS row length (complex elements) of the near/L2P tile: 64 pixels or S
+// locals, plus one element of padding so 16 lanes reading 16 different rows
+// hit 16 different banks.
+__host__ __device__ inline int near_row(int S) { return (S > 64 ? S : 64) + 1; }
+
+template <typename T, typename TI, int NB>
+__global__ void __launch_bounds__(kBlock) k_near_mfma(const cplx<TI>* __restrict__ y, long ys,
+                                                      const cplx<T>* __restrict__ loc, long ls,
+                                                      cplx<TI>* __restrict__ r, long rs,
+                                                      const cplx<T>* __restrict__ near,
+                                                      const cplx<T>* __restrict__ basisT, int S, int nbs,
+                                                      int with_near, const int* __restrict__ act) {
+  typedef typename Mfma<T>::v4 v4;
+  constexpr int NN = NB / 16;
+  extern __shared__ char smem_raw[];
+  cplx<T>* tile = reinterpret_cast<cplx<T>*>(smem_raw);
+  const int row = near_row(S);
+  int rhs = blockIdx.y;
+  if (rhs_off(act, rhs)) return;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  const long b0 = (long)blockIdx.x * NB;
+  v4 cr[NN], ci[NN];
+#pragma unroll
+  for (int j = 0; j < NN; j++) {
+    cr[j] = v4{0, 0, 0, 0};
+    ci[j] = v4{0, 0, 0, 0};
+  }
+  const cplx<TI>* yr = y + rhs * ys;
+  if (with_near) {
+    for (int s = 0; s < 9; s++) {
+      const int dX = s % 3 - 1, dY = s / 3 - 1;
+      __syncthreads();
+      for (int idx = tid; idx < NB * 64; idx += kBlock) {
+        int j = idx >> 6, n = idx & 63;
+        unsigned b = (unsigned)(b0 + j);
+        int sx = (int)compact2(b) + dX, sy = (int)compact2(b >> 1) + dY;
+        cplx<T> v = mkc<T>(0, 0);
+        if (sx >= 0 && sy >= 0 && sx < nbs && sy < nbs) v = ccast<T>(yr[(long)morton2(sx, sy) * 64 + n]);
+        tile[j * row + n] = v;
+      }
+      __syncthreads();
+      const cplx<T>* A = near + (long)s * 4096 + 16 * w + li;
+#pragma unroll 4
+      for (int kk = 0; kk < 16; kk++) {
+        const int k = 4 * kk + lk;
+        cplx<T> a = A[k * 64];
+        T nai = -a.y;
+#pragma unroll
+        for (int j = 0; j < NN; j++) {
+          cplx<T> bv = tile[(16 * j + li) * row + k];
+          cr[j] = Mfma<T>::mac(a.x, bv.x, cr[j]);
+          cr[j] = Mfma<T>::mac(nai, bv.y, cr[j]);
+          ci[j] = Mfma<T>::mac(a.x, bv.y, ci[j]);
+          ci[j] = Mfma<T>::mac(a.y, bv.x, ci[j]);
+        }
+      }
+    }
+  }
+  // leaf locals -> pixels (L2P) as the last K block
+  __syncthreads();
+  const cplx<T>* lr = loc + rhs * ls + b0 * S;
+  for (int idx = tid; idx < NB * S; idx += kBlock) {
+    int j = idx / S, k = idx - j * S;
+    tile[j * row + k] = lr[idx];
+  }
+  __syncthreads();
+  const cplx<T>* A = basisT + 16 * w + li;
+  for (int kk = 0; kk < S / 4; kk++) {
+    const int k = 4 * kk + lk;
+    cplx<T> a = A[k * 64];
+    T nai = -a.y;
+#pragma unroll
+    for (int j = 0; j < NN; j++) {
+      cplx<T> bv = tile[(16 * j + li) * row + k];
+      cr[j] = Mfma<T>::mac(a.x, bv.x, cr[j]);
+      cr[j] = Mfma<T>::mac(nai, bv.y, cr[j]);
+      ci[j] = Mfma<T>::mac(a.x, bv.y, ci[j]);
+      ci[j] = Mfma<T>::mac(a.y, bv.x, ci[j]);
+    }
+  }
+  cplx<TI>* out = r + rhs * rs;
+#pragma unroll
+  for (int j = 0; j < NN; j++) {
+    long box = b0 + 16 * j + li;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      int m = 16 * w + Mfma<T>::row(lk, q);
+      out[box * 64 + m] = mkc<TI>((TI)cr[j][q], (TI)ci[j][q]);
+    }
+  }
 }
 
 // receiver values: out[m] = sum_t wT[t][m] agm0[(i0[m]+t) mod S0]

@@ -92,6 +92,16 @@ _SIGS = {
                                      ctypes.c_int, _P(SolveInfo)]),
     "isg_stage": (ctypes.c_int, [_V, ctypes.c_int, ctypes.c_int, ctypes.c_int, _D]),
     "isg_counters": (ctypes.c_int, [_V, _P(ctypes.c_longlong)]),
+    "isg_dbim_setup": (ctypes.c_int, [_V, _D, ctypes.c_int, _P(DbimStats)]),
+    "isg_dbim_setup_measured": (ctypes.c_int, [_V, _D, _D, ctypes.c_int, _P(DbimStats)]),
+    "isg_profile": (ctypes.c_int, [_V, ctypes.c_int]),
+    "isg_profile_read": (ctypes.c_int, [_V, ctypes.c_int, _D, _P(ctypes.c_longlong)]),
+    "isg_dbim_iterate": (ctypes.c_int, [_V, _P(DbimStats)]),
+    "isg_dbim_get": (ctypes.c_int, [_V, ctypes.c_int, _D]),
+    "isg_dbim_set": (ctypes.c_int, [_V, ctypes.c_int, _D]),
+    "isg_phantom_blobs": (ctypes.c_int, [ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_ulonglong, _D]),
+    "isg_phantom_ellipses": (ctypes.c_int, [ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_ulonglong,
+                                            ctypes.c_double, ctypes.c_double, ctypes.c_double, _D]),
 }
 
 _lib = None
@@ -224,6 +234,19 @@ class Solver:
         self._err(self.lib.isg_stage(self.h, which, level, rhs, _dp(out)), "isg_stage")
         return out.reshape(4 ** level, self.numsamp[level])
 
+    KERNELS = {"near": 0, "m2l": 1, "bicgs_update": 2}
+
+    def profile(self, enable=True):
+        """Start (clearing totals) or stop live HIP-event timing of the hot kernels."""
+        self._err(self.lib.isg_profile(self.h, int(enable)), "isg_profile")
+
+    def profile_read(self, kernel):
+        ms = ctypes.c_double()
+        n = ctypes.c_longlong()
+        self._err(self.lib.isg_profile_read(self.h, self.KERNELS[kernel], ctypes.byref(ms), ctypes.byref(n)),
+                  "isg_profile_read")
+        return ms.value, n.value
+
     def counters(self):
         out = (ctypes.c_longlong * 3)()
         self._err(self.lib.isg_counters(self.h, out), "isg_counters")
@@ -244,3 +267,78 @@ class Solver:
         rc = self._err(self.lib.isg_bicgs_dev(self.h, _V(x_ptr), _V(o_ptr), o_stride, _V(b_ptr), int(herm),
                                               nrhs, int(x_zero), infos), "isg_bicgs_dev")
         return rc, [{"status": i.status, "iters": i.iters, "relres": i.relres} for i in infos]
+
+
+_DBIM_FIELDS = {"o": 0, "od": 1, "ot": 2, "mes": 3, "mesd": 4, "inc": 5, "tot": 6}
+
+
+def _stats_dict(st):
+    return {f: getattr(st, f) for f, _ in DbimStats._fields_}
+
+
+class Dbim:
+    """DBIM / nonlinear-CG inversion on the GPU (mom.cpp:222-400).
+
+    ``solver`` fixes the operator, the transmitter count (numtx), the batch
+    size (max_rhs) and the BiCGSTAB tolerance (toli, iti)."""
+
+    def __init__(self, solver, o_true_tree, field_start=0, measurements=None):
+        self.s = solver
+        st = DbimStats()
+        if measurements is None:
+            o = _c128(o_true_tree, solver.N)
+            solver._err(solver.lib.isg_dbim_setup(solver.h, _dp(o), int(field_start), ctypes.byref(st)),
+                        "isg_dbim_setup")
+        else:
+            m = _c128(measurements, solver.cfg.numtx * solver.cfg.numrx)
+            optr = None if o_true_tree is None else _dp(_c128(o_true_tree, solver.N))
+            solver._err(solver.lib.isg_dbim_setup_measured(solver.h, _dp(m), optr, int(field_start),
+                                                           ctypes.byref(st)), "isg_dbim_setup_measured")
+        self.setup_stats = _stats_dict(st)
+
+    def iterate(self):
+        st = DbimStats()
+        self.s._err(self.s.lib.isg_dbim_iterate(self.s.h, ctypes.byref(st)), "isg_dbim_iterate")
+        return _stats_dict(st)
+
+    def _size(self, name):
+        T, R, N = self.s.cfg.numtx, self.s.cfg.numrx, self.s.N
+        return {"o": N, "od": N, "ot": N, "mes": T * R, "mesd": T * R, "inc": T * N, "tot": T * N}[name]
+
+    def get(self, name):
+        out = np.empty(self._size(name), np.complex128)
+        self.s._err(self.s.lib.isg_dbim_get(self.s.h, _DBIM_FIELDS[name], _dp(out)), "isg_dbim_get")
+        return out
+
+    def set(self, name, value):
+        v = _c128(value, self._size(name))
+        self.s._err(self.s.lib.isg_dbim_set(self.s.h, _DBIM_FIELDS[name], _dp(v)), "isg_dbim_set")
+
+
+def phantom_blobs(side, n, seed, res=0.1):
+    """Raster (side x side) complex contrast: n seeded Gaussian blobs."""
+    out = np.empty(side * side, np.complex128)
+    if load().isg_phantom_blobs(side, res, n, seed, _dp(out)) != 0:
+        raise ValueError("bad phantom arguments")
+    return out.reshape(side, side)
+
+
+def phantom_ellipses(side, n, seed, res=0.1, vmin=0.05, vmax=0.8, imag=0.02):
+    """Raster (side x side) complex contrast: n seeded ellipses."""
+    out = np.empty(side * side, np.complex128)
+    if load().isg_phantom_ellipses(side, res, n, seed, vmin, vmax, imag, _dp(out)) != 0:
+        raise ValueError("bad phantom arguments")
+    return out.reshape(side, side)
+
+
+def raster_to_tree(raster, r2t):
+    """Reorder a raster image into the solver's tree order (setup_born.cpp:109)."""
+    flat = np.ascontiguousarray(raster).ravel()
+    out = np.empty_like(flat)
+    out[r2t] = flat
+    return out
+
+
+def tree_to_raster(tree, r2t, side):
+    """Inverse of raster_to_tree (setup_born.cpp:117, :214)."""
+    return np.asarray(tree)[r2t].reshape(side, side)

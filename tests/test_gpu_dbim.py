@@ -1,0 +1,67 @@
+"""GPU DBIM against the CPU oracle on the 64x64 configuration.
+
+64x64 pixels of lambda/10 (level 4), 16 plane-wave transmitters, 16 far-field
+receivers, 3 seeded Gaussian blobs (seed 1), BiCGSTAB tolerance 1e-6,
+regular = sqrt(1e-3) (Tikhonov alpha = 1e-3), 3 DBIM iterations.
+Bar: reconstructed contrast within 1e-4 relative of the oracle's after the
+fixed number of iterations; misfit history within 1e-4 relative.
+"""
+import numpy as np
+import pytest
+
+from conftest import oracle_for, relerr
+
+pytestmark = pytest.mark.gpu
+
+LEVEL, NTX, NRX, TOL, REG = 4, 16, 16, 1e-6, float(np.sqrt(1e-3))
+
+
+@pytest.fixture(scope="module")
+def case():
+    import isg
+    from oracle.oracle import Dbim as OrcDbim
+    o = oracle_for(LEVEL, numrx=NRX)
+    phantom = isg.phantom_blobs(o.side, 3, 1)
+    otrue = isg.raster_to_tree(phantom, o.unkmap())
+    ref = OrcDbim(o, NTX, otrue, regular=REG, tol=TOL, maxit=300)
+    hist_ref = [ref.hist()]
+    for _ in range(3):
+        ref.iterate()
+        hist_ref.append(ref.hist())
+    return isg, otrue, ref, np.array(hist_ref)
+
+
+@pytest.mark.parametrize("batch", [1, 16])
+def test_dbim_matches_oracle(case, batch):
+    isg, otrue, ref, hist_ref = case
+    print("dbim parity batch", batch, flush=True)
+    s = isg.Solver(level=LEVEL, numrx=NRX, numtx=NTX, toli=TOL, iti=300, regular=REG, max_rhs=batch)
+    d = isg.Dbim(s, otrue)
+    st0 = d.setup_stats
+    hist = [[st0["misfit"], st0["object_error"], st0["object_cost"], st0["gradient_norm2"], st0["object_norm2"]]]
+    for _ in range(3):
+        st = d.iterate()
+        assert st["not_converged"] == 0
+        hist.append([st["misfit"], st["object_error"], st["object_cost"], st["gradient_norm2"], st["object_norm2"]])
+    hist = np.array(hist)
+    assert relerr(d.get("mes"), ref.get("mes")) < 1e-5
+    assert relerr(d.get("o"), ref.get("o")) < 1e-4
+    np.testing.assert_allclose(hist[:, [0, 1, 3, 4]], hist_ref[:, [0, 1, 3, 4]], rtol=1e-4, atol=0)
+    # the misfit must fall every iteration (mom.cpp:277-400 is a descent method)
+    assert np.all(np.diff(hist[:, 0]) < 0)
+
+
+def test_dbim_checkpoint_resume(case):
+    """3 iterations == 2 iterations + checkpoint {o, od, ot, mesd, tot} + resume + 1."""
+    isg, otrue, ref, _ = case
+    kw = dict(level=LEVEL, numrx=NRX, numtx=NTX, toli=TOL, iti=300, regular=REG, max_rhs=16)
+    a = isg.Dbim(isg.Solver(**kw), otrue)
+    for _ in range(2):
+        a.iterate()
+    ck = {k: a.get(k) for k in ("o", "od", "ot", "mesd", "tot")}
+    a.iterate()
+    b = isg.Dbim(isg.Solver(**kw), otrue)
+    for k, v in ck.items():
+        b.set(k, v)
+    b.iterate()
+    assert relerr(b.get("o"), a.get("o")) < 1e-12
