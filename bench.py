@@ -14,11 +14,12 @@ the timed region; the timed region brackets K iterations with device
 synchronisation on both sides (each iteration ends with a stream sync).
 
 Two paths run from the same synthetic measurements:
-  plain   FP64, one right-hand side at a time, every solve from zero -- the
-          reference's loop on this library's kernels;
+  plain   FP64 operator, one right-hand side at a time, every solve from zero
+          -- the reference's loop on this library's kernels;
   fast    every transmitter of a phase in one multi-RHS BiCGSTAB with
-          deflation, total-field solves warm-started from the previous
-          DBIM iterate.
+          deflation, FP32 operator with FP64 solver scalars/vectors and an FP64
+          true-residual defect correction (every solve still reaches 1e-6 in
+          FP64), total-field solves warm-started from the previous DBIM iterate.
 `value` is the fast path's s/iteration; the plain figure is reported beside it.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1]
@@ -36,8 +37,7 @@ sys.path.insert(0, ROOT)
 import numpy as np  # noqa: E402
 
 METRIC = "s per DBIM iteration, 1024² px ×64 Tx, rel-res 1e-6: plain vs batched+learned"
-PEAK_FP64_TFLOPS = 78.6
-PEAK_FP32_TFLOPS = 157.3
+PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3}
 PEAK_HBM_GBS = 8000.0
 
 WORKLOADS = {
@@ -59,12 +59,12 @@ def near_flops_per_rhs(level, S_leaf):
     return 8 * 64 * 64 * pairs + 8 * 64 * S_leaf * nb * nb, pairs
 
 
-def m2l_bytes_per_rhs(level, numsamp, cbytes=16):
+def m2l_bytes_per_rhs(level, numsamp, cbytes):
     """Read every multipole once and write every local once, levels 2..leaf."""
     return sum(2 * cbytes * 4 ** i * numsamp[i] for i in range(2, level))
 
 
-def run_path(isg, solver, dbim, warmup, steps, label):
+def run_path(solver, dbim, warmup, steps, label):
     for w in range(warmup):
         st = dbim.iterate()
         log("[%s] warmup iteration %d: %.3fs, %d solves, %d BiCGSTAB iterations, misfit %.4e"
@@ -78,7 +78,7 @@ def run_path(isg, solver, dbim, warmup, steps, label):
         log("[%s] timed iteration %d: %.3fs, %d solves, %d BiCGSTAB iterations, %d not converged, misfit %.4e"
             % (label, k, st["seconds"], st["solves"], st["solve_iters"], st["not_converged"], st["misfit"]))
     t1 = time.perf_counter()
-    prof = {k: solver.profile_read(k) for k in ("near", "m2l", "bicgs_update")}
+    prof = {k: solver.profile_read(k) for k in solver.KERNELS}
     solver.profile(False)
     return (t1 - t0) / steps, stats, prof
 
@@ -98,7 +98,7 @@ def cpu_baseline(level, numrx, matvecs_per_iter, nsample):
     t_mv = (time.perf_counter() - t0) / nsample
     threads = o.L.orc_threads()
     return {
-        "value": t_mv * matvecs_per_iter,
+        "value": round(t_mv * matvecs_per_iter, 2),
         "unit": "s",
         "cores": threads,
         "kind": "port",
@@ -116,6 +116,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="dbim-1024", choices=sorted(WORKLOADS))
     ap.add_argument("--paths", default="fast,plain")
+    ap.add_argument("--fast-precision", default="mixed", choices=["mixed", "fp64"])
     ap.add_argument("--cpu-sample", type=int, default=2, help="oracle matvecs timed for cpu_baseline (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -129,40 +130,55 @@ def main():
     side = 8 << (level - 1)
     common = dict(level=level, numrx=nrx, numtx=ntx, toli=1e-6, iti=300, regular=float(np.sqrt(1e-3)))
     paths = [p for p in args.paths.split(",") if p]
+    fast_prec = 1 if args.fast_precision == "mixed" else 0
 
-    # synthetic measurements of the true contrast, made once by the batched path
+    # synthetic measurements of the true contrast, made once by the batched FP64 path
     t0 = time.perf_counter()
-    fast = isg.Solver(max_rhs=ntx, **common)
-    pos, r2t = fast.geometry()
+    gen = isg.Solver(max_rhs=ntx, **common)
+    pos, r2t = gen.geometry()
     otrue = isg.raster_to_tree(isg.phantom_ellipses(side, wl["nell"], wl["seed"]), r2t)
-    d_fast = isg.Dbim(fast, otrue, field_start=1)
-    mes = d_fast.get("mes")
-    log("setup (measurements + first gradient): %.2fs, %d solves, %d BiCGSTAB iterations, %d not converged"
-        % (time.perf_counter() - t0, d_fast.setup_stats["solves"], d_fast.setup_stats["solve_iters"],
-           d_fast.setup_stats["not_converged"]))
-    S_leaf = fast.numsamp[-1]
-    numsamp = list(fast.numsamp)
+    d_gen = isg.Dbim(gen, otrue, field_start=1)
+    mes = d_gen.get("mes")
+    log("measurements (FP64, batched): %.2fs, %d solves, %d BiCGSTAB iterations, %d not converged"
+        % (time.perf_counter() - t0, d_gen.setup_stats["solves"], d_gen.setup_stats["solve_iters"],
+           d_gen.setup_stats["not_converged"]))
+    S_leaf = gen.numsamp[-1]
+    numsamp = list(gen.numsamp)
+    del d_gen
+    gen.close()
     flops_mv, pairs = near_flops_per_rhs(level, S_leaf)
 
     res = {}
     if "fast" in paths:
-        res["fast"] = run_path(isg, fast, d_fast, args.warmup, args.steps, "fast")
-    fast.close()
-    del d_fast
+        fast = isg.Solver(max_rhs=ntx, precision=fast_prec, **common)
+        d_fast = isg.Dbim(fast, otrue, field_start=1, measurements=mes)
+        res["fast"] = run_path(fast, d_fast, args.warmup, args.steps, "fast")
+        del d_fast
+        fast.close()
     if "plain" in paths:
         plain = isg.Solver(max_rhs=1, **common)
         d_plain = isg.Dbim(plain, otrue, field_start=0, measurements=mes)
-        res["plain"] = run_path(isg, plain, d_plain, args.warmup, args.steps, "plain")
+        res["plain"] = run_path(plain, d_plain, args.warmup, args.steps, "plain")
+        del d_plain
         plain.close()
 
     main_path = "fast" if "fast" in res else "plain"
     s_iter, stats, prof = res[main_path]
+    mixed = main_path == "fast" and fast_prec == 1
     matvecs = sum(s["matvecs"] for s in stats)
-    near_ms, near_n = prof["near"]
-    m2l_ms, m2l_n = prof["m2l"]
+    mv32 = sum(s["matvecs_fp32"] for s in stats)
+    if mixed:
+        near_ms, near_n = prof["near_fp32"]
+        m2l_ms, m2l_n = prof["m2l_fp32"]
+        n_near, prec, cb = mv32, "f32", 8
+    else:
+        near_ms, near_n = prof["near"]
+        m2l_ms, m2l_n = prof["m2l"]
+        n_near, prec, cb = matvecs, "f64", 16
     vec_ms, vec_n = prof["bicgs_update"]
-    achieved = matvecs * flops_mv / (near_ms * 1e-3) / 1e12 if near_ms > 0 else 0.0
-    m2l_gbs = matvecs * m2l_bytes_per_rhs(level, numsamp) / (m2l_ms * 1e-3) / 1e9 if m2l_ms > 0 else 0.0
+    achieved = n_near * flops_mv / (near_ms * 1e-3) / 1e12 if near_ms > 0 else 0.0
+    m2l_gbs = n_near * m2l_bytes_per_rhs(level, numsamp, cb) / (m2l_ms * 1e-3) / 1e9 if m2l_ms > 0 else 0.0
+    iters = sum(s["solve_iters"] for s in stats)
 
     out = {
         "metric": METRIC,
@@ -175,44 +191,47 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic (seeded ellipse phantom, measurements synthesized by the forward solver)",
+        "dtype": "f32 operator + f64 solver" if mixed else "f64",
+        "data": "synthetic (seeded ellipse phantom, measurements synthesized by the FP64 forward solver)",
         "config": {
             "workload": "DBIM %dx%d px lambda/10, level %d, %d Tx, %d Rx, %d ellipses seed %d, rel-res 1e-6"
                         % (side, side, level, ntx, nrx, wl["nell"], wl["seed"]),
-            "path": {"fast": "batched %d RHS + deflation, FP64, previous-iterate total-field start" % ntx,
-                     "plain": "1 RHS, FP64, zero start"}[main_path],
+            "path": ("batched %d RHS + deflation, %s, previous-iterate total-field start"
+                     % (ntx, "FP32 operator / FP64 solver + FP64 defect correction" if mixed else "FP64")
+                     if main_path == "fast" else "1 RHS, FP64, zero start"),
         },
         "roofline": {
-            "kernel": "k_near_mfma (P2P + L2P, FP64 v_mfma_f64_16x16x4)",
+            "kernel": "k_near_mfma (P2P + L2P on v_mfma_%s_16x16x4)" % prec,
             "bound": "mfma",
             "achieved": round(achieved, 2),
-            "peak": PEAK_FP64_TFLOPS,
+            "peak": PEAK_TFLOPS[prec],
             "unit": "TFLOP/s",
-            "frac": round(achieved / PEAK_FP64_TFLOPS, 4),
+            "frac": round(achieved / PEAK_TFLOPS[prec], 4),
             "traffic": None,
             "flops_per_rhs_launch": flops_mv,
             "avg_launch_ms": round(near_ms / max(near_n, 1), 4),
             "launches": near_n,
         },
-        "m2l": {"achieved_GBs": round(m2l_gbs, 1), "peak_GBs": PEAK_HBM_GBS,
-                "frac": round(m2l_gbs / PEAK_HBM_GBS, 4), "ms_per_matvec_launchset": round(m2l_ms / max(m2l_n, 1), 4)},
+        "m2l": {"achieved_GBs": round(m2l_gbs, 1), "peak_GBs": PEAK_HBM_GBS, "frac": round(m2l_gbs / PEAK_HBM_GBS, 4),
+                "ms_per_matvec_launchset": round(m2l_ms / max(m2l_n, 1), 4)},
         # x/r update reads x, p, s, t, r~ and writes x, r: 7 complex FP64 vectors per RHS iteration
-        "bicgs_update": {"GBs": round(7 * 16 * side * side * sum(s["solve_iters"] for s in stats)
-                                      / max(vec_ms * 1e-3, 1e-12) / 1e9, 1),
+        "bicgs_update": {"GBs": round(7 * 16 * side * side * iters / max(vec_ms * 1e-3, 1e-12) / 1e9, 1),
                          "avg_launch_ms": round(vec_ms / max(vec_n, 1), 4)},
         "solves": sum(s["solves"] for s in stats),
-        "bicgstab_iterations": sum(s["solve_iters"] for s in stats),
+        "bicgstab_iterations": iters,
         "matvecs": matvecs,
+        "matvecs_fp32": mv32,
+        "misfit": stats[-1]["misfit"],
     }
     if "plain" in res and "fast" in res:
         out["plain_s_per_iter"] = round(res["plain"][0], 4)
         out["speedup_vs_plain"] = round(res["plain"][0] / s_iter, 3)
+        out["plain_misfit"] = res["plain"][1][-1]["misfit"]
     if args.cpu_sample > 0 and "plain" in res:
         plain_mv = int(round(sum(s["matvecs"] for s in res["plain"][1]) / len(res["plain"][1])))
         try:
             out["cpu_baseline"] = cpu_baseline(level, nrx, plain_mv, args.cpu_sample)
-        except Exception as e:  # the CPU oracle is optional on a box without a compiler
+        except Exception as e:  # the CPU oracle needs its .so; report instead of failing the bench
             out["cpu_baseline"] = {"error": str(e)}
     print(json.dumps(out), flush=True)
 

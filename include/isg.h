@@ -66,7 +66,10 @@ typedef struct isg_config {
   int iti;         /* BiCGSTAB iteration limit */
   double regular;  /* Tikhonov weight: the gradient adds regular^2 * o per transmitter */
   int max_rhs;     /* right-hand sides solved together (1 = the reference's plain loop) */
-  int precision;   /* 0: FP64 operator (plain).  Reserved for the mixed path. */
+  int precision;   /* 0: FP64 operator (plain).  1: mixed -- FP32 operator tables, multipoles and
+                      matrix-core arithmetic inside BiCGSTAB; FP64 vectors, dot products and
+                      scalars; FP64 true-residual check with defect correction until the FP64
+                      relative residual meets toli.  isg_mlfma/aggregate(h) stay FP64. */
 } isg_config;
 
 /* Per-solve result. */
@@ -105,6 +108,10 @@ int isg_aggregateh(isg_context* ctx, double* x, const double* r);
  * o is the weighted contrast o*k0^2 as passed by the reference's callers. */
 int isg_bicgs(isg_context* ctx, double* x, const double* o, const double* b, int herm, isg_solve_info* info);
 
+/* r[k] = G x[k] for nrhs host vectors ([nrhs][numunk], nrhs <= max_rhs) in one batched pass,
+ * in the context's precision (the FP32 operator when precision is 1) */
+int isg_mlfma_many(isg_context* ctx, const double* x, double* r, int nrhs);
+
 /* ---- device-pointer, batched surface (vectors [nrhs][numunk], receivers [nrhs][numrx]) ---- */
 int isg_mlfma_dev(isg_context* ctx, const void* x, void* r, int nrhs);
 int isg_aggregate_dev(isg_context* ctx, const void* x, void* r, int nrhs);
@@ -119,7 +126,8 @@ int isg_bicgs_dev(isg_context* ctx, void* x, const void* o, long o_stride, const
 int isg_stage(isg_context* ctx, int which, int level, int rhs, double* out);
 
 /* operator applications and BiCGSTAB counters since creation:
- * out[0] single-RHS matvecs (mom.cpp:61), out[1] solves, out[2] solver iterations */
+ * out[0] single-RHS matvecs (mom.cpp:61), out[1] solves, out[2] solver iterations,
+ * out[3] matvecs with the FP32 operator, out[4] defect-correction solves (5 entries) */
 int isg_counters(const isg_context* ctx, long long* out);
 
 /* ---- DBIM / nonlinear-CG inversion (mom.cpp:222-400, setup_born.cpp) ---- */
@@ -132,6 +140,7 @@ typedef struct isg_dbim_stats {
   double object_norm2;   /* sum|o|^2                            (col 5) */
   double beta_re, beta_im; /* Polak-Ribiere coefficient (mom.cpp:394) */
   long long solves, solve_iters, matvecs, not_converged;
+  long long matvecs_fp32;  /* of matvecs, those with the FP32 operator */
   double seconds;        /* wall time of the call, device work included */
 } isg_dbim_stats;
 
@@ -159,6 +168,7 @@ int isg_dbim_set(isg_context* ctx, int which, const double* in);
  * kernel 0: near field + L2P on the matrix cores, one launch per matvec
  * kernel 1: M2L over all levels of one matvec
  * kernel 2: BiCGSTAB x/r update with its two dot products
+ * kernel 3, 4: as 0, 1 for the FP32 operator of the mixed path
  * isg_profile(ctx, 1) clears the totals and starts recording. */
 int isg_profile(isg_context* ctx, int enable);
 int isg_profile_read(isg_context* ctx, int kernel, double* total_ms, long long* launches);

@@ -76,7 +76,7 @@ int isg_create(const isg_config* cfg, isg_context** out) {
   if (!cfg || !out) return ISG_E_ARG;
   *out = nullptr;
   if (cfg->max_rhs < 1) return ISG_E_ARG;
-  if (cfg->precision != 0) return ISG_E_ARG;
+  if (cfg->precision != 0 && cfg->precision != 1) return ISG_E_ARG;
   isg_context* c = new isg_context();
   c->cfg = *cfg;
   isg::EngineConfig ec;
@@ -147,6 +147,28 @@ int isg_mlfma(isg_context* c, const double* x, double* r) {
   return sync(c);
 }
 
+int isg_mlfma_many(isg_context* c, const double* x, double* r, int nrhs) {
+  if (!check(c)) return ISG_E_STATE;
+  if (!x || !r || nrhs < 1 || nrhs > c->cfg.max_rhs) return fail(c, ISG_E_ARG, "bad argument");
+  hipStream_t s = c->eng->stream();
+  size_t bytes = vbytes(c) * (size_t)nrhs;
+  isg::zd *dx = nullptr, *dr = nullptr;
+  if (!hip_ok(c, hipMalloc((void**)&dx, bytes), "hipMalloc") || !hip_ok(c, hipMalloc((void**)&dr, bytes), "hipMalloc")) {
+    if (dx) (void)hipFree(dx);
+    return ISG_E_DEVICE;
+  }
+  int rc = ISG_OK;
+  if (!hip_ok(c, hipMemcpyAsync(dx, x, bytes, hipMemcpyHostToDevice, s), "H2D")) rc = ISG_E_DEVICE;
+  if (rc == ISG_OK) {
+    c->eng->mlfma(dx, dr, nrhs, nullptr, c->cfg.precision);
+    if (!hip_ok(c, hipMemcpyAsync(r, dr, bytes, hipMemcpyDeviceToHost, s), "D2H")) rc = ISG_E_DEVICE;
+  }
+  if (rc == ISG_OK) rc = sync(c);
+  (void)hipFree(dx);
+  (void)hipFree(dr);
+  return rc;
+}
+
 int isg_mlfmah(isg_context* c, const double* x, double* r) {
   if (!check(c)) return ISG_E_STATE;
   if (!x || !r) return fail(c, ISG_E_ARG, "null pointer");
@@ -191,7 +213,7 @@ int isg_bicgs(isg_context* c, double* x, const double* o, const double* b, int h
       !hip_ok(c, hipMemcpyAsync(c->dr, b, vb, hipMemcpyHostToDevice, s), "H2D"))
     return ISG_E_DEVICE;
   std::vector<isg::SolveStat> st;
-  c->eng->bicgs(c->dx, c->dg, 0, c->dr, herm != 0, 1, false, c->cfg.toli, c->cfg.iti, st);
+  c->eng->bicgs(c->dx, c->dg, 0, c->dr, herm != 0, 1, false, c->cfg.toli, c->cfg.iti, st, c->cfg.precision);
   if (!check(c)) return ISG_E_DEVICE;
   if (!hip_ok(c, hipMemcpyAsync(x, c->dx, vb, hipMemcpyDeviceToHost, s), "D2H")) return ISG_E_DEVICE;
   int rc = sync(c);
@@ -232,7 +254,7 @@ int isg_bicgs_dev(isg_context* c, void* x, const void* o, long o_stride, const v
   if (o_stride != 0 && o_stride != c->eng->plan().N) return fail(c, ISG_E_ARG, "o_stride must be 0 or numunk");
   std::vector<isg::SolveStat> st;
   c->eng->bicgs((isg::zd*)x, (const isg::zd*)o, o_stride, (const isg::zd*)b, herm != 0, nrhs, x_zero != 0,
-                c->cfg.toli, c->cfg.iti, st);
+                c->cfg.toli, c->cfg.iti, st, c->cfg.precision);
   if (!check(c)) return ISG_E_DEVICE;
   int worst = 0;
   for (int i = 0; i < nrhs; i++) {
@@ -261,6 +283,8 @@ int isg_counters(const isg_context* c, long long* out) {
   out[0] = c->eng->matvecs;
   out[1] = c->eng->solves;
   out[2] = c->eng->solve_iters;
+  out[3] = c->eng->matvecs32;
+  out[4] = c->eng->refinements;
   return ISG_OK;
 }
 
@@ -277,6 +301,7 @@ static void fill_stats(const isg::DbimStats& s, isg_dbim_stats* o) {
   o->solves = s.solves;
   o->solve_iters = s.solve_iters;
   o->matvecs = s.matvecs;
+  o->matvecs_fp32 = s.matvecs32;
   o->not_converged = s.not_converged;
   o->seconds = s.seconds;
 }
@@ -290,6 +315,7 @@ int isg_dbim_setup(isg_context* c, const double* o_true, int field_start, isg_db
   opt.tol = c->cfg.toli;
   opt.maxit = c->cfg.iti;
   opt.field_start = field_start;
+  opt.precision = c->cfg.precision;
   c->dbim.reset();
   c->dbim.reset(new isg::Dbim(*c->eng, opt));
   if (!c->dbim->ok()) return fail(c, ISG_E_DEVICE, c->dbim->error());
@@ -310,6 +336,7 @@ int isg_dbim_setup_measured(isg_context* c, const double* mes, const double* o_t
   opt.tol = c->cfg.toli;
   opt.maxit = c->cfg.iti;
   opt.field_start = field_start;
+  opt.precision = c->cfg.precision;
   c->dbim.reset();
   c->dbim.reset(new isg::Dbim(*c->eng, opt));
   if (!c->dbim->ok()) return fail(c, ISG_E_DEVICE, c->dbim->error());

@@ -195,7 +195,7 @@ Dbim::~Dbim() {
 
 void Dbim::solve(zd* x, const zd* b, bool herm, int nb, bool x_zero, DbimStats& st) {
   std::vector<SolveStat> ss;
-  eng_.bicgs(x, g_, 0, b, herm, nb, x_zero, opt_.tol, opt_.maxit, ss);
+  eng_.bicgs(x, g_, 0, b, herm, nb, x_zero, opt_.tol, opt_.maxit, ss, opt_.precision);
   for (auto& s : ss) {
     st.solves++;
     st.solve_iters += s.iters;
@@ -312,7 +312,7 @@ void Dbim::start(const zd* mes_host, const zd* otrue, DbimStats& st) {
 void Dbim::iterate(DbimStats& st) {
   if (!ok()) return;
   auto t_start = std::chrono::steady_clock::now();
-  long long mv0 = eng_.matvecs;
+  long long mv0 = eng_.matvecs, mv32 = eng_.matvecs32;
   hipStream_t s = eng_.stream();
   const long N = N_;
   dim3 blk(kBlock);
@@ -374,6 +374,7 @@ void Dbim::iterate(DbimStats& st) {
   DB_HIP(hipStreamSynchronize(s));
   iter_++;
   st.matvecs = eng_.matvecs - mv0;
+  st.matvecs32 = eng_.matvecs32 - mv32;
   st.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
 }
 

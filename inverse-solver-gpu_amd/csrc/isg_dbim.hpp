@@ -19,13 +19,14 @@ struct DbimOptions {
   double tol = 1e-4;      // BiCGSTAB relative residual (mom.cpp:21)
   int maxit = 300;        // BiCGSTAB iteration limit (mom.cpp:22)
   int field_start = 0;    // total-field update start: 0 zero (mom.cpp:324), 1 previous total field
+  int precision = 0;      // solver operator: PREC_FP64 or PREC_MIXED
 };
 
 struct DbimStats {
   double alpha = 0;          // step length (mom.cpp:315)
   double hist[5] = {0, 0, 0, 0, 0};  // hist.txt columns (setup_born.cpp:239)
   double beta_re = 0, beta_im = 0;   // Polak-Ribiere coefficient (mom.cpp:394)
-  long long solves = 0, solve_iters = 0, matvecs = 0, not_converged = 0;
+  long long solves = 0, solve_iters = 0, matvecs = 0, not_converged = 0, matvecs32 = 0;
   double seconds = 0;
 };
 

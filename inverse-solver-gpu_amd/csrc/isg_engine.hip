@@ -3,6 +3,7 @@
 #include "isg_engine.hpp"
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 
@@ -21,8 +22,50 @@ namespace isg {
 
 namespace {
 inline unsigned nblocks(long work) { return (unsigned)((work + kBlock - 1) / kBlock); }
-enum { V_R = 0, V_RT, V_P, V_V, V_S, V_T, V_Y, V_OP, V_AUX, V_COUNT };
+enum { V_R = 0, V_RT, V_P, V_V, V_S, V_T, V_Y, V_OP, V_RES, V_COR, V_COUNT };
+// smallest power of two >= v (v >= 1)
+int pow2_ceil(int v) {
+  int p = 1;
+  while (p < v) p <<= 1;
+  return p;
+}
+// largest power of two <= v (v >= 1)
+int pow2_floor(int v) {
+  int p = 1;
+  while (p * 2 <= v) p <<= 1;
+  return p;
+}
+constexpr size_t kLdsLimit = 64 * 1024;
+constexpr int kDenseMax = 320;  // largest sample count interpolated as a dense GEMM
+constexpr int kRefineRounds = 3;
+
+// Largest LDS window (source samples) of a target chunk for the chunked
+// interpolation kernels, or -1 when the first-tap indices of some chunk are
+// not monotone (mod the source count) and the window formula would not hold.
+int chunk_window(const std::vector<int>& first, int ntgt, int nsrc, int ntap, int kch) {
+  int wmax = 0;
+  for (int k0 = 0; k0 < ntgt; k0 += kch) {
+    int kn = std::min(kch, ntgt - k0), prev = 0;
+    for (int k = k0; k < k0 + kn; k++) {
+      int d = first[k] - first[k0];
+      if (d < 0) d += nsrc;
+      if (d < prev) return -1;
+      prev = d;
+    }
+    wmax = std::max(wmax, prev + ntap);
+  }
+  return wmax;
+}
 }  // namespace
+
+template <>
+DevOp<double>& Engine::ops<double>() {
+  return op64_;
+}
+template <>
+DevOp<float>& Engine::ops<float>() {
+  return op32_;
+}
 
 void* Engine::dalloc(size_t bytes) {
   void* p = nullptr;
@@ -69,6 +112,9 @@ void Engine::upload(DevOp<T>& d) {
   }
   d.rxT = up_r(P_.rxT);
   d.rxaT = up_r(P_.rxaT);
+  size_t tot = lvl_off_[L - 1] + lvl_size_[L - 1] * cfg_.max_rhs;
+  d.agm = (cplx<T>*)dalloc(tot * sizeof(cplx<T>));
+  d.agl = (cplx<T>*)dalloc(tot * sizeof(cplx<T>));
 }
 
 Engine::Engine(const EngineConfig& cfg) : cfg_(cfg) {
@@ -83,21 +129,7 @@ Engine::Engine(const EngineConfig& cfg) : cfg_(cfg) {
     return;
   }
   ISG_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
-  upload(op64_);
-  auto up_i = [&](const std::vector<int>& src) -> int* {
-    int* p = (int*)dalloc(src.size() * sizeof(int));
-    if (p) ISG_HIP(hipMemcpy(p, src.data(), src.size() * sizeof(int), hipMemcpyHostToDevice));
-    return p;
-  };
   int L = P_.L, R = cfg_.max_rhs;
-  interp0_.assign(L, nullptr);
-  anterp0_.assign(L, nullptr);
-  for (int i = 0; i < L; i++) {
-    if (!P_.interp0[i].empty()) interp0_[i] = up_i(P_.interp0[i]);
-    if (!P_.anterp0[i].empty()) anterp0_[i] = up_i(P_.anterp0[i]);
-  }
-  rx0_ = up_i(P_.rx0);
-  rxaI_ = up_i(P_.rxaI);
   // multipole and local stores, all levels, [level][rhs][4^i S_i]
   lvl_off_.assign(L, 0);
   lvl_size_.assign(L, 0);
@@ -107,16 +139,35 @@ Engine::Engine(const EngineConfig& cfg) : cfg_(cfg) {
     lvl_off_[i] = tot;
     tot += lvl_size_[i] * R;
   }
-  agm_ = (zd*)dalloc(tot * sizeof(zd));
-  agl_ = (zd*)dalloc(tot * sizeof(zd));
+  upload(op64_);
+  has32_ = cfg_.precision == PREC_MIXED;
+  if (has32_) upload(op32_);
+  auto up_i = [&](const std::vector<int>& src) -> int* {
+    int* p = (int*)dalloc(src.size() * sizeof(int));
+    if (p) ISG_HIP(hipMemcpy(p, src.data(), src.size() * sizeof(int), hipMemcpyHostToDevice));
+    return p;
+  };
+  interp0_.assign(L, nullptr);
+  anterp0_.assign(L, nullptr);
+  for (int i = 0; i < L; i++) {
+    if (!P_.interp0[i].empty()) interp0_[i] = up_i(P_.interp0[i]);
+    if (!P_.anterp0[i].empty()) anterp0_[i] = up_i(P_.anterp0[i]);
+  }
+  rx0_ = up_i(P_.rx0);
+  rxaI_ = up_i(P_.rxaI);
+  plan_interp(op64_, m2m64_, l2l64_);
+  if (has32_) plan_interp(op32_, m2m32_, l2l32_);
   vec_.assign(V_COUNT, nullptr);
   for (int k = 0; k < V_COUNT; k++) vec_[k] = (zd*)dalloc((size_t)R * P_.N * sizeof(zd));
   nblk_ = (int)std::min<long>(1024, std::max<long>(1, (P_.N + kBlock * 4 - 1) / (kBlock * 4)));
   part_ = (double2*)dalloc((size_t)R * kSlots * nblk_ * sizeof(double2));
   state_ = (BiState*)dalloc((size_t)R * sizeof(BiState));
   active_ = (int*)dalloc((size_t)R * sizeof(int));
+  tolv_ = (double*)dalloc((size_t)R * sizeof(double));
   ISG_HIP(hipHostMalloc((void**)&active_h_, (size_t)R * sizeof(int), hipHostMallocDefault));
   ISG_HIP(hipHostMalloc((void**)&state_h_, (size_t)R * sizeof(BiState), hipHostMallocDefault));
+  ISG_HIP(hipHostMalloc((void**)&tol_h_, (size_t)R * sizeof(double), hipHostMallocDefault));
+  ISG_HIP(hipHostMalloc((void**)&part_h_, (size_t)R * kSlots * nblk_ * sizeof(double2), hipHostMallocDefault));
   if (ok()) {
     ISG_HIP(hipMemset(state_, 0, (size_t)R * sizeof(BiState)));
     ISG_HIP(hipMemset(active_, 0, (size_t)R * sizeof(int)));
@@ -132,58 +183,260 @@ Engine::~Engine() {
   for (void* p : allocs_) (void)hipFree(p);
   if (active_h_) (void)hipHostFree(active_h_);
   if (state_h_) (void)hipHostFree(state_h_);
+  if (tol_h_) (void)hipHostFree(tol_h_);
+  if (part_h_) (void)hipHostFree(part_h_);
   if (stream_) (void)hipStreamDestroy(stream_);
 }
 
-// r = G y (y already contrast-weighted).  Stage order of matvec_gpu.cu:103-264:
-// P2M, M2M up to level 2, M2L on levels 2..leaf, L2L down, L2P, then P2P.
-void Engine::apply_mlfma(const zd* y, zd* r, int nrhs, const int* act) {
-  if (!ok()) return;
-  const int L = P_.L, Lf = L - 1, N = P_.N, B = P_.B;
-  const DevOp<double>& d = op64_;
-  const int S = P_.nsamp[Lf];
+// ------------------------------------------------------- interpolation plan
+template <typename T>
+void Engine::plan_interp(DevOp<T>& d, std::vector<InterpShape>& m2m, std::vector<InterpShape>& l2l) {
+  const int L = P_.L, ni = cfg_.plan.ninter, na = 2 * ni;
+  m2m.assign(L, InterpShape());
+  l2l.assign(L, InterpShape());
+  d.dense_m2m.assign(L, nullptr);
+  d.dense_l2l.assign(L, nullptr);
+  const size_t cz = sizeof(cplx<T>);
+  auto upload_dense = [&](const std::vector<double>& A) -> T* {
+    std::vector<T> tmp(A.begin(), A.end());
+    T* p = (T*)dalloc(tmp.size() * sizeof(T));
+    if (p) ISG_HIP(hipMemcpy(p, tmp.data(), tmp.size() * sizeof(T), hipMemcpyHostToDevice));
+    return p;
+  };
+  for (int i = 0; i + 1 < L; i++) {
+    const int Sp = P_.nsamp[i], Sc = P_.nsamp[i + 1], np = P_.nclus[i];
+    // M2M: rows = parent samples, K = child samples
+    InterpShape& m = m2m[i];
+    if (Sp <= kDenseMax && np >= 16) {
+      m.ncb = (32 * (Sc + 1) * cz <= kLdsLimit) ? 2 : ((16 * (Sc + 1) * cz <= kLdsLimit) ? 1 : 0);
+      if (m.ncb > 0 && np * 4 % (16 * m.ncb) == 0) {
+        m.lds_dense = (size_t)16 * m.ncb * (Sc + 1) * cz;
+        std::vector<double> A((size_t)Sc * Sp, 0.0);
+        for (int k = 0; k < Sp; k++)
+          for (int t = 0; t < ni; t++) A[(size_t)((P_.interp0[i][k] + t) % Sc) * Sp + k] += P_.interpT[i][(size_t)t * Sp + k];
+        d.dense_m2m[i] = upload_dense(A);
+      } else {
+        m.ncb = 0;
+      }
+    }
+    m.g = (Sp <= 128) ? std::min(np, pow2_floor(std::max(1, kBlock / Sp))) : 1;
+    m.kch = (Sp <= 128) ? Sp : kBlock;
+    int Wm = chunk_window(P_.interp0[i], Sp, Sc, ni, m.kch);
+    m.lds_chunk = (size_t)4 * m.g * std::max(Wm, 0) * cz;
+    m.chunk_ok = Wm > 0 && m.lds_chunk <= kLdsLimit;
+    // L2L: rows = child samples, K = parent samples
+    InterpShape& l = l2l[i];
+    if (Sp <= kDenseMax && np >= 16) {
+      l.ncb = (32 * (Sp + 1) * cz <= kLdsLimit) ? 2 : ((16 * (Sp + 1) * cz <= kLdsLimit) ? 1 : 0);
+      if (l.ncb > 0 && np * 4 % (16 * l.ncb) == 0) {
+        l.lds_dense = (size_t)16 * l.ncb * (Sp + 1) * cz;
+        std::vector<double> A((size_t)Sp * Sc, 0.0);
+        for (int k = 0; k < Sc; k++)
+          for (int t = 0; t < na; t++)
+            A[(size_t)((P_.anterp0[i + 1][k] + t) % Sp) * Sc + k] += P_.anterpT[i + 1][(size_t)t * Sc + k];
+        d.dense_l2l[i] = upload_dense(A);
+      } else {
+        l.ncb = 0;
+      }
+    }
+    l.g = 1;
+    l.kch = std::min(Sc, 64);
+    int Wl = chunk_window(P_.anterp0[i + 1], Sc, Sp, na, l.kch);
+    l.lds_chunk = (size_t)4 * std::max(Wl, 0) * cz;
+    l.chunk_ok = Wl > 0 && l.lds_chunk <= kLdsLimit;
+  }
+}
+
+// dense GEMM when it yields at least one workgroup per CU, else chunked, else direct
+int Engine::interp_kind(const InterpShape& sh, int np, int nrhs) const {
+  if (sh.ncb > 0 && (long)(4 * np / (16 * sh.ncb)) * nrhs >= 256) return 0;
+  if (sh.chunk_ok) return 1;
+  return sh.ncb > 0 ? 0 : 2;
+}
+
+// --------------------------------------------------------------- launches
+// leaf radiation patterns on the matrix cores
+template <typename T>
+void Engine::launch_p2m(const zd* y, int nrhs, const int* act) {
+  const int Lf = P_.L - 1, S = P_.nsamp[Lf], B = P_.B;
+  const int nb = B >= 32 ? 32 : 16;
+  dim3 grid(B / nb, nrhs), blk(kBlock);
+  DevOp<T>& d = ops<T>();
+  cplx<T>* out = d.agm + lvl_off_[Lf];
+  long as = (long)lvl_size_[Lf];
+  if (nb == 32)
+    k_p2m_mfma<T, double, 32><<<grid, blk, 0, stream_>>>(y, P_.N, out, as, d.coeffT, S, act);
+  else
+    k_p2m_mfma<T, double, 16><<<grid, blk, 0, stream_>>>(y, P_.N, out, as, d.coeffT, S, act);
+}
+
+// child level i+1 -> parent level i
+template <typename T>
+void Engine::launch_m2m(int i, int nrhs, const int* act) {
+  const int Sp = P_.nsamp[i], Sc = P_.nsamp[i + 1], np = P_.nclus[i], nt = cfg_.plan.ninter;
+  DevOp<T>& d = ops<T>();
+  cplx<T>*child = d.agm + lvl_off_[i + 1], *par = d.agm + lvl_off_[i];
+  long cs = (long)lvl_size_[i + 1], ps = (long)lvl_size_[i];
+  const InterpShape& sh = (sizeof(T) == 4 ? m2m32_ : m2m64_)[i];
   dim3 blk(kBlock);
-  auto M = [&](int i) { return agm_ + lvl_off_[i]; };
-  auto Lc = [&](int i) { return agl_ + lvl_off_[i]; };
-  auto sz = [&](int i) { return (long)lvl_size_[i]; };
-  k_p2m<double, double><<<dim3(nblocks((long)B * S), nrhs), blk, 0, stream_>>>(y, N, M(Lf), sz(Lf), d.coeffT, S, B, act);
-  for (int i = L - 2; i >= 2; i--)
-    k_m2m<double><<<dim3(nblocks(sz(i)), nrhs), blk, 0, stream_>>>(M(i + 1), sz(i + 1), M(i), sz(i), d.interpT[i],
-                                                                      interp0_[i], d.shiftM[i], P_.nsamp[i],
-                                                                      P_.nsamp[i + 1], P_.nclus[i], cfg_.plan.ninter, act);
-  prof_begin(KT_M2L);
-  for (int i = 2; i <= Lf; i++)
-    k_m2l<double><<<dim3(nblocks(sz(i)), nrhs), blk, 0, stream_>>>(M(i), Lc(i), sz(i), d.trans[i], i, P_.nsamp[i], act);
-  prof_end(KT_M2L);
-  for (int i = 2; i <= L - 2; i++)
-    k_l2l<double><<<dim3(nblocks(sz(i + 1)), nrhs), blk, 0, stream_>>>(Lc(i), sz(i), Lc(i + 1), sz(i + 1), d.anterpT[i + 1],
-                                                                          anterp0_[i + 1], d.shiftL[i], P_.nsamp[i],
-                                                                          P_.nsamp[i + 1], P_.nclus[i],
-                                                                          2 * cfg_.plan.ninter, 1, act);
-  launch_near(y, Lc(Lf), r, nrhs, act, true);
-  ISG_HIP(hipGetLastError());
+  const int kind = interp_kind(sh, np, nrhs);
+  if (kind == 0) {
+    dim3 grid(4 * np / (16 * sh.ncb), nrhs);
+    if (sh.ncb == 2)
+      k_interp_mfma<T, 0, 2><<<grid, blk, sh.lds_dense, stream_>>>(child, cs, par, ps, d.dense_m2m[i], d.shiftM[i], Sp,
+                                                                   Sc, Sp, 0, act);
+    else
+      k_interp_mfma<T, 0, 1><<<grid, blk, sh.lds_dense, stream_>>>(child, cs, par, ps, d.dense_m2m[i], d.shiftM[i], Sp,
+                                                                   Sc, Sp, 0, act);
+  } else if (kind == 1) {
+    int nch = (Sp + sh.kch - 1) / sh.kch;
+    dim3 grid(np / sh.g * nch, nrhs);
+    k_m2m_chunk<T><<<grid, blk, sh.lds_chunk, stream_>>>(child, cs, par, ps, d.interpT[i], interp0_[i], d.shiftM[i],
+                                                         Sp, Sc, sh.g, sh.kch, nch, nt, act);
+  } else {
+    k_m2m<T><<<dim3(nblocks((long)np * Sp), nrhs), blk, 0, stream_>>>(child, cs, par, ps, d.interpT[i], interp0_[i],
+                                                                       d.shiftM[i], Sp, Sc, np, nt, act);
+  }
+}
+
+// parent level i -> child level i+1, accumulating (matvec) or assigning
+// (receiver back-projection).
+template <typename T>
+void Engine::launch_l2l(int i, bool accumulate, int nrhs, const int* act) {
+  const int Sp = P_.nsamp[i], Sc = P_.nsamp[i + 1], np = P_.nclus[i], nt = 2 * cfg_.plan.ninter;
+  DevOp<T>& d = ops<T>();
+  cplx<T>*par = d.agl + lvl_off_[i], *child = d.agl + lvl_off_[i + 1];
+  long ps = (long)lvl_size_[i], cs = (long)lvl_size_[i + 1];
+  const InterpShape& sh = (sizeof(T) == 4 ? l2l32_ : l2l64_)[i];
+  dim3 blk(kBlock);
+  int acc = accumulate ? 1 : 0;
+  const int kind = interp_kind(sh, np, nrhs);
+  if (kind == 0) {
+    dim3 grid(4 * np / (16 * sh.ncb), nrhs);
+    if (sh.ncb == 2)
+      k_interp_mfma<T, 1, 2><<<grid, blk, sh.lds_dense, stream_>>>(par, ps, child, cs, d.dense_l2l[i], d.shiftL[i], Sc,
+                                                                   Sp, Sp, acc, act);
+    else
+      k_interp_mfma<T, 1, 1><<<grid, blk, sh.lds_dense, stream_>>>(par, ps, child, cs, d.dense_l2l[i], d.shiftL[i], Sc,
+                                                                   Sp, Sp, acc, act);
+  } else if (kind == 1) {
+    int nch = (Sc + sh.kch - 1) / sh.kch;
+    dim3 grid(np * nch, nrhs);
+    k_l2l_chunk<T><<<grid, blk, sh.lds_chunk, stream_>>>(par, ps, child, cs, d.anterpT[i + 1], anterp0_[i + 1],
+                                                         d.shiftL[i], Sp, Sc, sh.kch, nch, nt, acc, act);
+  } else {
+    k_l2l<T><<<dim3(nblocks((long)4 * np * Sc), nrhs), blk, 0, stream_>>>(par, ps, child, cs, d.anterpT[i + 1],
+                                                                           anterp0_[i + 1], d.shiftL[i], Sp, Sc, np,
+                                                                           nt, acc, act);
+  }
+}
+
+// translations of level i: TS x TS target tiles, 8 samples per workgroup;
+// 4x4 tiles when 8x8 tiles would leave fewer than 1024 workgroups
+template <typename T>
+void Engine::launch_m2l(int i, int nrhs, const int* act) {
+  const int S = P_.nsamp[i], nb = 1 << i, KC = 8;
+  const int nchunk = (S + KC - 1) / KC;
+  const int TS = (nb >= 8 && (long)(nb * nb / 64) * nchunk * nrhs >= 1024) ? 8 : 4;
+  const int ntile = nb * nb / (TS * TS);
+  dim3 grid(ntile * nchunk, nrhs), blk(kBlock);
+  DevOp<T>& d = ops<T>();
+  const cplx<T>* m = d.agm + lvl_off_[i];
+  cplx<T>* l = d.agl + lvl_off_[i];
+  long st = (long)lvl_size_[i];
+  if (TS == 8)
+    k_m2l_tile<T, 8, 8><<<grid, blk, 0, stream_>>>(m, l, st, d.trans[i], i, S, act);
+  else
+    k_m2l_tile<T, 4, 8><<<grid, blk, 0, stream_>>>(m, l, st, d.trans[i], i, S, act);
 }
 
 // Near field + L2P on the matrix cores (k_near_mfma), or L2P alone when
 // with_near is false (receiver back-projection).
-void Engine::launch_near(const zd* y, const zd* loc, zd* r, int nrhs, const int* act, bool with_near) {
+template <typename T>
+void Engine::launch_near(const zd* y, zd* r, int nrhs, const int* act, bool with_near) {
   const int Lf = P_.L - 1, S = P_.nsamp[Lf], B = P_.B;
   const long N = P_.N, ls = (long)lvl_size_[Lf];
   const int nb = B >= 32 ? 32 : 16;
-  size_t lds = (size_t)nb * near_row(S) * sizeof(zd);
+  DevOp<T>& d = ops<T>();
+  const cplx<T>* loc = d.agl + lvl_off_[Lf];
+  size_t lds = (size_t)nb * near_row(S) * sizeof(cplx<T>);
   dim3 grid(B / nb, nrhs), blk(kBlock);
-  if (with_near) prof_begin(KT_NEAR);
+  const int kt = sizeof(T) == 4 ? KT_NEAR32 : KT_NEAR;
+  if (with_near) prof_begin(kt);
   if (nb == 32)
-    k_near_mfma<double, double, 32><<<grid, blk, lds, stream_>>>(y, N, loc, ls, r, N, op64_.near, op64_.basisT, S,
-                                                                 1 << Lf, with_near ? 1 : 0, act);
+    k_near_mfma<T, double, 32><<<grid, blk, lds, stream_>>>(y, N, loc, ls, r, N, d.near, d.basisT, S, 1 << Lf,
+                                                            with_near ? 1 : 0, act);
   else
-    k_near_mfma<double, double, 16><<<grid, blk, lds, stream_>>>(y, N, loc, ls, r, N, op64_.near, op64_.basisT, S,
-                                                                 1 << Lf, with_near ? 1 : 0, act);
-  if (with_near) prof_end(KT_NEAR);
+    k_near_mfma<T, double, 16><<<grid, blk, lds, stream_>>>(y, N, loc, ls, r, N, d.near, d.basisT, S, 1 << Lf,
+                                                            with_near ? 1 : 0, act);
+  if (with_near) prof_end(kt);
+}
+
+// r = G y (y already contrast-weighted).  Stage order of matvec_gpu.cu:103-264:
+// P2M, M2M up to level 2, M2L on levels 2..leaf, L2L down, then near field
+// and L2P together.
+template <typename T>
+void Engine::apply(const zd* y, zd* r, int nrhs, const int* act) {
+  const int L = P_.L, Lf = L - 1;
+  launch_p2m<T>(y, nrhs, act);
+  for (int i = L - 2; i >= 2; i--) launch_m2m<T>(i, nrhs, act);
+  const int kt = sizeof(T) == 4 ? KT_M2L32 : KT_M2L;
+  prof_begin(kt);
+  for (int i = 2; i <= Lf; i++) launch_m2l<T>(i, nrhs, act);
+  prof_end(kt);
+  for (int i = 2; i <= L - 2; i++) launch_l2l<T>(i, true, nrhs, act);
+  launch_near<T>(y, r, nrhs, act, true);
+}
+
+void Engine::apply_mlfma(const zd* y, zd* r, int nrhs, const int* act, int prec) {
+  if (!ok()) return;
+  if (prec == PREC_MIXED && has32_)
+    apply<float>(y, r, nrhs, act);
+  else
+    apply<double>(y, r, nrhs, act);
+  ISG_HIP(hipGetLastError());
+}
+
+void Engine::mlfma(const zd* x, zd* r, int nrhs, const int* act, int prec) {
+  apply_mlfma(x, r, nrhs, act, prec);
+  matvecs += nrhs;
+  if (prec == PREC_MIXED && has32_) matvecs32 += nrhs;
+}
+
+// matvec_gpu.cu:266-368 with the top levels and the receiver interpolation
+// on the device as well (FP64).
+void Engine::aggregate(const zd* x, zd* r, int nrhs) {
+  if (!ok()) return;
+  const int L = P_.L;
+  launch_p2m<double>(x, nrhs, nullptr);
+  for (int i = L - 2; i >= 0; i--) launch_m2m<double>(i, nrhs, nullptr);
+  int R = cfg_.plan.numrx;
+  k_rx_interp<double><<<dim3(nblocks(R), nrhs), dim3(kBlock), 0, stream_>>>(
+      op64_.agm + lvl_off_[0], (long)lvl_size_[0], r, R, op64_.rxT, rx0_, P_.nsamp[0], R, cfg_.plan.ninter);
+  ISG_HIP(hipGetLastError());
+}
+
+// matvec_gpu.cu:369-470 (argument order out x, in r as in the reference),
+// every level on the device (FP64).
+void Engine::aggregateh(const zd* r, zd* x, int nrhs) {
+  if (!ok()) return;
+  const int L = P_.L;
+  int R = cfg_.plan.numrx;
+  double sc = (double)R * cfg_.plan.res * cfg_.plan.res;
+  k_rx_anterp<double><<<dim3(nblocks(P_.nsamp[0]), nrhs), dim3(kBlock), 0, stream_>>>(
+      r, R, op64_.agl + lvl_off_[0], (long)lvl_size_[0], op64_.rxaT, rxaI_, P_.nsamp[0], 2 * cfg_.plan.ninter, sc);
+  for (int i = 0; i <= L - 2; i++) launch_l2l<double>(i, false, nrhs, nullptr);
+  launch_near<double>(nullptr, x, nrhs, nullptr, false);
+  ISG_HIP(hipGetLastError());
+}
+
+void Engine::launch_prep(const zd* x, const zd* g, long gs, zd* y, bool herm, int nrhs) {
+  k_prep<<<dim3(nblk_, nrhs), dim3(kBlock), 0, stream_>>>(x, g, gs, y, P_.N, herm ? 1 : 0);
 }
 
 // ---- live kernel timing (HIP events on the engine stream)
 void Engine::profile(bool on) {
+  if (stream_) ISG_HIP(hipStreamSynchronize(stream_));
   harvest();
   prof_ = on;
   for (auto& k : kt_) {
@@ -239,85 +492,27 @@ void Engine::profile_read(int id, double& ms, long long& launches) {
   launches = kt_[id].launches;
 }
 
-void Engine::mlfma(const zd* x, zd* r, int nrhs, const int* act) {
-  apply_mlfma(x, r, nrhs, act);
-  matvecs += nrhs;
-}
-
-// matvec_gpu.cu:266-368 with the top levels and the receiver interpolation
-// on the device as well.
-void Engine::aggregate(const zd* x, zd* r, int nrhs) {
-  if (!ok()) return;
-  const int L = P_.L, Lf = L - 1, N = P_.N, B = P_.B, S = P_.nsamp[Lf];
-  const DevOp<double>& d = op64_;
-  auto M = [&](int i) { return agm_ + lvl_off_[i]; };
-  auto sz = [&](int i) { return (long)lvl_size_[i]; };
-  dim3 blk(kBlock);
-  k_p2m<double, double><<<dim3(nblocks((long)B * S), nrhs), blk, 0, stream_>>>(x, N, M(Lf), sz(Lf), d.coeffT, S, B, nullptr);
-  for (int i = L - 2; i >= 0; i--)
-    k_m2m<double><<<dim3(nblocks(sz(i)), nrhs), blk, 0, stream_>>>(M(i + 1), sz(i + 1), M(i), sz(i), d.interpT[i],
-                                                                      interp0_[i], d.shiftM[i], P_.nsamp[i],
-                                                                      P_.nsamp[i + 1], P_.nclus[i], cfg_.plan.ninter,
-                                                                      nullptr);
-  int R = cfg_.plan.numrx;
-  k_rx_interp<double><<<dim3(nblocks(R), nrhs), blk, 0, stream_>>>(M(0), sz(0), r, R, d.rxT, rx0_, P_.nsamp[0], R,
-                                                                    cfg_.plan.ninter);
-  ISG_HIP(hipGetLastError());
-}
-
-// matvec_gpu.cu:369-470 (argument order out x, in r as in the reference),
-// every level on the device.
-void Engine::aggregateh(const zd* r, zd* x, int nrhs) {
-  if (!ok()) return;
-  const int L = P_.L, Lf = L - 1, N = P_.N, B = P_.B, S = P_.nsamp[Lf];
-  const DevOp<double>& d = op64_;
-  auto Lc = [&](int i) { return agl_ + lvl_off_[i]; };
-  auto sz = [&](int i) { return (long)lvl_size_[i]; };
-  dim3 blk(kBlock);
-  int R = cfg_.plan.numrx;
-  double sc = (double)R * cfg_.plan.res * cfg_.plan.res;
-  k_rx_anterp<double><<<dim3(nblocks(P_.nsamp[0]), nrhs), blk, 0, stream_>>>(r, R, Lc(0), sz(0), d.rxaT, rxaI_,
-                                                                              P_.nsamp[0], 2 * cfg_.plan.ninter, sc);
-  for (int i = 0; i <= L - 2; i++)
-    k_l2l<double><<<dim3(nblocks(sz(i + 1)), nrhs), blk, 0, stream_>>>(Lc(i), sz(i), Lc(i + 1), sz(i + 1), d.anterpT[i + 1],
-                                                                          anterp0_[i + 1], d.shiftL[i], P_.nsamp[i],
-                                                                          P_.nsamp[i + 1], P_.nclus[i],
-                                                                          2 * cfg_.plan.ninter, 0, nullptr);
-  launch_near(nullptr, Lc(Lf), x, nrhs, nullptr, false);
-  ISG_HIP(hipGetLastError());
-}
-
-void Engine::launch_prep(const zd* x, const zd* g, long gs, zd* y, bool herm, int nrhs) {
-  k_prep<<<dim3(nblk_, nrhs), dim3(kBlock), 0, stream_>>>(x, g, gs, y, P_.N, herm ? 1 : 0);
-}
-
-void Engine::reduce_finalize(int phase, int nrhs, double tol, int maxit) {
-  k_bi_fin<<<dim3(nrhs), dim3(kBlock), 0, stream_>>>(phase, state_, active_, part_, nblk_, tol, maxit);
-}
-
+// ---------------------------------------------------------------- solver
 // bicgs_gpu.cu:116-224, batched over right-hand sides with device-resident
-// scalars; the host reads only the per-right-hand-side activity flags once
-// per iteration to know when every system has stopped.
-void Engine::bicgs(zd* x, const zd* g, long gs, const zd* b, bool herm, int nrhs, bool x_zero, double tol,
-                   int maxit, std::vector<SolveStat>& out) {
-  out.assign(nrhs, SolveStat{-1, 0, 0.0});
-  if (!ok()) return;
-  if (nrhs > cfg_.max_rhs) {
-    set_error("bicgs: nrhs exceeds max_rhs");
-    return;
-  }
+// scalars and per-system tolerances (tol_h_ -> tolv_); the host reads only
+// the per-system activity flags once per iteration.
+void Engine::run_bicgs(zd* x, const zd* g, long gs, const zd* b, bool herm, int nrhs, bool x_zero, int maxit,
+                       int prec) {
   const long N = P_.N;
   zd *r = vec_[V_R], *rt = vec_[V_RT], *p = vec_[V_P], *v = vec_[V_V], *s = vec_[V_S], *t = vec_[V_T],
      *y = vec_[V_Y], *op = vec_[V_OP];
   const int h = herm ? 1 : 0;
   dim3 grid(nblk_, nrhs), blk(kBlock);
+  ISG_HIP(hipMemcpyAsync(tolv_, tol_h_, nrhs * sizeof(double), hipMemcpyHostToDevice, stream_));
+  const long long f32 = (prec == PREC_MIXED && has32_) ? 1 : 0;
   if (!x_zero) {
     k_prep<<<grid, blk, 0, stream_>>>(x, g, gs, y, N, h);
-    apply_mlfma(y, op, nrhs, nullptr);
+    apply_mlfma(y, op, nrhs, nullptr, prec);
     matvecs += nrhs;
+    matvecs32 += f32 * nrhs;
   }
   k_bi_init<<<grid, blk, 0, stream_>>>(x, op, g, gs, b, r, rt, N, h, x_zero ? 1 : 0, part_);
-  reduce_finalize(PH_INIT, nrhs, tol, maxit);
+  k_bi_fin<<<dim3(nrhs), blk, 0, stream_>>>(PH_INIT, state_, active_, part_, nblk_, tolv_, maxit);
   for (int it = 0; it <= maxit; it++) {
     ISG_HIP(hipMemcpyAsync(active_h_, active_, nrhs * sizeof(int), hipMemcpyDeviceToHost, stream_));
     ISG_HIP(hipStreamSynchronize(stream_));
@@ -327,34 +522,112 @@ void Engine::bicgs(zd* x, const zd* g, long gs, const zd* b, bool herm, int nrhs
     for (int i = 0; i < nrhs; i++) nact += active_h_[i] != 0;
     if (nact == 0) break;
     k_bi_pupd<<<grid, blk, 0, stream_>>>(r, p, v, g, gs, y, N, h, state_);
-    apply_mlfma(y, op, nrhs, active_);
+    apply_mlfma(y, op, nrhs, active_, prec);
     k_bi_post1<<<grid, blk, 0, stream_>>>(p, op, g, gs, v, rt, N, h, state_, part_);
-    reduce_finalize(PH_ALPHA, nrhs, tol, maxit);
+    k_bi_fin<<<dim3(nrhs), blk, 0, stream_>>>(PH_ALPHA, state_, active_, part_, nblk_, tolv_, maxit);
     k_bi_supd<<<grid, blk, 0, stream_>>>(r, v, g, gs, s, y, N, h, state_);
-    apply_mlfma(y, op, nrhs, active_);
+    apply_mlfma(y, op, nrhs, active_, prec);
     k_bi_post2<<<grid, blk, 0, stream_>>>(s, op, g, gs, t, N, h, state_, part_);
-    reduce_finalize(PH_OMEGA, nrhs, tol, maxit);
+    k_bi_fin<<<dim3(nrhs), blk, 0, stream_>>>(PH_OMEGA, state_, active_, part_, nblk_, tolv_, maxit);
     prof_begin(KT_VEC);
     k_bi_xrupd<<<grid, blk, 0, stream_>>>(x, p, s, t, r, rt, N, state_, part_);
     prof_end(KT_VEC);
-    reduce_finalize(PH_RESID, nrhs, tol, maxit);
+    k_bi_fin<<<dim3(nrhs), blk, 0, stream_>>>(PH_RESID, state_, active_, part_, nblk_, tolv_, maxit);
     matvecs += 2 * nact;
+    matvecs32 += f32 * 2 * nact;
   }
   ISG_HIP(hipMemcpyAsync(state_h_, state_, nrhs * sizeof(BiState), hipMemcpyDeviceToHost, stream_));
   ISG_HIP(hipStreamSynchronize(stream_));
+}
+
+// res = b - A x with the FP64 operator; rn, bn: per-system norms of res and b.
+void Engine::true_residual(const zd* x, const zd* g, long gs, const zd* b, bool herm, int nrhs, zd* res,
+                           std::vector<double>& rn, std::vector<double>& bn) {
+  const long N = P_.N;
+  const int h = herm ? 1 : 0;
+  dim3 grid(nblk_, nrhs), blk(kBlock);
+  zd *y = vec_[V_Y], *op = vec_[V_OP];
+  k_prep<<<grid, blk, 0, stream_>>>(x, g, gs, y, N, h);
+  apply_mlfma(y, op, nrhs, nullptr, PREC_FP64);
+  matvecs += nrhs;
+  k_bi_init<<<grid, blk, 0, stream_>>>(x, op, g, gs, b, res, vec_[V_RT], N, h, 0, part_);
+  ISG_HIP(hipMemcpyAsync(part_h_, part_, (size_t)nrhs * kSlots * nblk_ * sizeof(double2), hipMemcpyDeviceToHost,
+                         stream_));
+  ISG_HIP(hipStreamSynchronize(stream_));
+  rn.assign(nrhs, 0.0);
+  bn.assign(nrhs, 0.0);
+  for (int k = 0; k < nrhs; k++) {
+    const double2* pr = part_h_ + (size_t)k * kSlots * nblk_;
+    double sb = 0, sr = 0;
+    for (int j = 0; j < nblk_; j++) {
+      sb += pr[j].x;
+      sr += pr[nblk_ + j].x;
+    }
+    bn[k] = std::sqrt(sb);
+    rn[k] = std::sqrt(sr);
+  }
+}
+
+void Engine::bicgs(zd* x, const zd* g, long gs, const zd* b, bool herm, int nrhs, bool x_zero, double tol,
+                   int maxit, std::vector<SolveStat>& out, int prec) {
+  out.assign(nrhs, SolveStat{-1, 0, 0.0});
+  if (!ok()) return;
+  if (nrhs > cfg_.max_rhs) {
+    set_error("bicgs: nrhs exceeds max_rhs");
+    return;
+  }
+  const bool mixed = prec == PREC_MIXED && has32_;
+  for (int i = 0; i < nrhs; i++) tol_h_[i] = tol;
+  run_bicgs(x, g, gs, b, herm, nrhs, x_zero, maxit, mixed ? PREC_MIXED : PREC_FP64);
+  if (!ok()) return;
   for (int i = 0; i < nrhs; i++) {
     out[i].status = state_h_[i].status;
     out[i].iters = state_h_[i].iter;
     out[i].relres = state_h_[i].err;
+  }
+  if (mixed) {
+    // FP64 true residual; defect correction with the FP32 operator until the
+    // FP64 relative residual meets tol (or the rounds are used up)
+    zd *res = vec_[V_RES], *cor = vec_[V_COR];
+    std::vector<double> rn, bn;
+    for (int round = 0; round <= kRefineRounds; round++) {
+      true_residual(x, g, gs, b, herm, nrhs, res, rn, bn);
+      if (!ok()) return;
+      bool any = false;
+      for (int i = 0; i < nrhs; i++) {
+        double bnrm = bn[i] < 1e-20 ? 1.0 : bn[i];
+        double err = rn[i] / bnrm;
+        out[i].relres = err;
+        const bool broke = out[i].status == 2 || out[i].status == 3;
+        const bool need = !(err < tol) && !broke;
+        if (err < tol)
+          out[i].status = 0;
+        else if (!broke)
+          out[i].status = 1;
+        // correction target relative to |res|, with a factor-2 margin; systems
+        // that are done get an unreachable target and stay inactive
+        tol_h_[i] = need ? 0.5 * tol * bnrm / std::max(rn[i], 1e-300) : 1e300;
+        any = any || need;
+      }
+      if (!any || round == kRefineRounds) break;
+      refinements++;
+      ISG_HIP(hipMemsetAsync(cor, 0, (size_t)nrhs * P_.N * sizeof(zd), stream_));
+      run_bicgs(cor, g, gs, res, herm, nrhs, true, maxit, PREC_MIXED);
+      if (!ok()) return;
+      k_add<<<dim3(nblk_, nrhs), dim3(kBlock), 0, stream_>>>(x, cor, P_.N);
+      for (int i = 0; i < nrhs; i++) out[i].iters += state_h_[i].iter;
+    }
+  }
+  for (int i = 0; i < nrhs; i++) {
     solves++;
-    solve_iters += state_h_[i].iter;
+    solve_iters += out[i].iters;
   }
 }
 
 void Engine::stage(int which, int level, int rhs, std::vector<zd>& out) {
   out.clear();
   if (!ok() || level < 0 || level >= P_.L || rhs < 0 || rhs >= cfg_.max_rhs) return;
-  const zd* base = (which == 0 ? agm_ : agl_) + lvl_off_[level] + (size_t)rhs * lvl_size_[level];
+  const zd* base = (which == 0 ? op64_.agm : op64_.agl) + lvl_off_[level] + (size_t)rhs * lvl_size_[level];
   out.resize(lvl_size_[level]);
   ISG_HIP(hipStreamSynchronize(stream_));
   ISG_HIP(hipMemcpy(out.data(), base, out.size() * sizeof(zd), hipMemcpyDeviceToHost));

@@ -1,9 +1,15 @@
-// isg_engine.hpp -- single-GPU MLFMA / BiCGSTAB / DBIM engine (host side).
+// isg_engine.hpp -- single-GPU MLFMA / BiCGSTAB engine (host side).
 //
 // One Engine owns one operator (an isg_plan), its device tables, a HIP stream
 // and the device work buffers for up to `max_rhs` simultaneous right-hand
 // sides.  Every vector is complex, interleaved, in tree order, laid out
 // [rhs][N] in HBM; multipoles and locals of level i are [rhs][4^i][S_i].
+//
+// Precision: the solver vectors and scalars are always FP64.  The operator
+// runs either in FP64 (plain) or, in the mixed path, with FP32 tables,
+// multipoles and FP32 matrix-core arithmetic; a mixed solve is followed by
+// FP64 true-residual checks and defect-correction solves until the FP64
+// relative residual meets the tolerance.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -16,36 +22,41 @@
 
 namespace isg {
 
+enum { PREC_FP64 = 0, PREC_MIXED = 1 };
+
 struct EngineConfig {
   PlanConfig plan;
   int max_rhs = 1;        // right-hand sides solved together
-  int precision = 0;      // 0: FP64 operator; 1: also build the FP32 operator
+  int precision = 0;      // PREC_MIXED also builds the FP32 operator
 };
 
 // Result of one BiCGSTAB solve (one right-hand side).
 struct SolveStat {
   int status;   // 0 converged, 1 not converged, 2 rho breakdown, 3 omega breakdown
-  int iters;
-  double relres;
+  int iters;    // BiCGSTAB iterations, defect-correction solves included
+  double relres;  // final relative residual (FP64 true residual for mixed solves)
 };
 
 // Device-resident BiCGSTAB scalars of one right-hand side.
 struct BiState {
   zd rho, rho1, alpha, omega, beta;
-  double bnrm, err;
+  double bnrm, err, tol;
   int iter, status, active, first;
 };
 
-// Device copies of the operator tables in precision T.
+// Device copies of the operator tables and the multipole/local stores in
+// precision T.
 template <typename T>
 struct DevOp {
   cplx<T>* near = nullptr;     // [9][64][64]
   cplx<T>* coeffT = nullptr;   // [64][S_leaf]
   cplx<T>* basisT = nullptr;   // [S_leaf][64]
-  std::vector<T*> interpT, anterpT;
+  std::vector<T*> interpT, anterpT, dense_m2m, dense_l2l;
   std::vector<cplx<T>*> shiftM, shiftL, trans;
   T* rxT = nullptr;
   T* rxaT = nullptr;
+  cplx<T>* agm = nullptr;      // [level][rhs][4^i S_i]
+  cplx<T>* agl = nullptr;
 };
 
 class Engine {
@@ -58,37 +69,36 @@ class Engine {
   const Plan& plan() const { return P_; }
   hipStream_t stream() const { return stream_; }
   int max_rhs() const { return cfg_.max_rhs; }
+  bool has_mixed() const { return has32_; }
 
   // r = G x for nrhs right-hand sides (device pointers, FP64, [nrhs][N]).
   // `act` (device, may be null) masks right-hand sides out.
-  void mlfma(const zd* x, zd* r, int nrhs, const int* act = nullptr);
-  // far-field pattern at the receivers: r [nrhs][numrx]
+  void mlfma(const zd* x, zd* r, int nrhs, const int* act = nullptr, int prec = PREC_FP64);
+  // far-field pattern at the receivers: r [nrhs][numrx]  (FP64)
   void aggregate(const zd* x, zd* r, int nrhs);
-  // receiver back-projection: x [nrhs][N] from r [nrhs][numrx]
+  // receiver back-projection: x [nrhs][N] from r [nrhs][numrx]  (FP64)
   void aggregateh(const zd* r, zd* x, int nrhs);
 
   // BiCGSTAB on (I - G diag(g)) x = b (herm=false) or its adjoint
   // (I - diag(conj g) G^H) (herm=true), x in/out [nrhs][N].  g is [N] when
   // g_stride == 0 (shared contrast) or [nrhs][N] when g_stride == N.
   // x_zero tells the solver that x is all zeros (skips the first matvec).
-  void bicgs(zd* x, const zd* g, long g_stride, const zd* b, bool herm, int nrhs, bool x_zero,
-             double tol, int maxit, std::vector<SolveStat>& out);
+  void bicgs(zd* x, const zd* g, long g_stride, const zd* b, bool herm, int nrhs, bool x_zero, double tol,
+             int maxit, std::vector<SolveStat>& out, int prec = PREC_FP64);
 
-  // multipoles (which=0) or locals (1) of `level` for one rhs, FP64 host copy
+  // multipoles (which=0) or locals (1) of `level` for one rhs, FP64 store
   void stage(int which, int level, int rhs, std::vector<zd>& out);
 
   // counters
   long long matvecs = 0;       // single-rhs operator applications
-  long long solves = 0, solve_iters = 0;
+  long long matvecs32 = 0;     // ... of which with the FP32 operator
+  long long solves = 0, solve_iters = 0, refinements = 0;
 
-  // scratch vector owned by the engine ([max_rhs][N] FP64)
-  zd* scratch(int k) { return vec_[k]; }
-
-  // low-level launches used by the DBIM driver
   void launch_prep(const zd* x, const zd* g, long g_stride, zd* y, bool herm, int nrhs);
 
   // Live per-kernel timing with HIP events on the engine stream.
-  enum { KT_NEAR = 0, KT_M2L = 1, KT_VEC = 2, KT_COUNT = 3 };
+  // KT_NEAR/KT_M2L: FP64 operator launches; KT_NEAR32/KT_M2L32: FP32 ones.
+  enum { KT_NEAR = 0, KT_M2L = 1, KT_VEC = 2, KT_NEAR32 = 3, KT_M2L32 = 4, KT_COUNT = 5 };
   void profile(bool on);
   void profile_read(int id, double& ms, long long& launches);
 
@@ -99,39 +109,74 @@ class Engine {
     double ms = 0;
     long long launches = 0;
   };
+  // Launch options of one interpolation level, fixed at set-up; the dense
+  // GEMM is used when it gives enough workgroups, else the chunked kernel,
+  // else the direct-from-cache kernel.
+  struct InterpShape {
+    int ncb = 0;            // dense MFMA: column blocks of 16 (0: unavailable)
+    size_t lds_dense = 0;
+    bool chunk_ok = false;  // chunked LDS kernel usable
+    int g = 1;              // chunked: parents per workgroup
+    int kch = 256;          // chunked: target samples per workgroup
+    size_t lds_chunk = 0;
+  };
+
+  template <typename T>
+  DevOp<T>& ops();
+  template <typename T>
+  void upload(DevOp<T>& d);
+  template <typename T>
+  void plan_interp(DevOp<T>& d, std::vector<InterpShape>& m2m, std::vector<InterpShape>& l2l);
+  template <typename T>
+  void apply(const zd* y, zd* r, int nrhs, const int* act);
+  template <typename T>
+  void launch_p2m(const zd* y, int nrhs, const int* act);
+  template <typename T>
+  void launch_m2m(int i, int nrhs, const int* act);
+  template <typename T>
+  void launch_l2l(int i, bool accumulate, int nrhs, const int* act);
+  template <typename T>
+  void launch_m2l(int i, int nrhs, const int* act);
+  template <typename T>
+  void launch_near(const zd* y, zd* r, int nrhs, const int* act, bool with_near);
+  int interp_kind(const InterpShape& sh, int np, int nrhs) const;
+  std::vector<InterpShape>& m2m_shapes(bool fp32) { return fp32 ? m2m32_ : m2m64_; }
+
+  void apply_mlfma(const zd* y, zd* r, int nrhs, const int* act, int prec);
+  void run_bicgs(zd* x, const zd* g, long gs, const zd* b, bool herm, int nrhs, bool x_zero, int maxit, int prec);
+  void true_residual(const zd* x, const zd* g, long gs, const zd* b, bool herm, int nrhs, zd* res,
+                     std::vector<double>& rn, std::vector<double>& bn);
   void prof_begin(int id);
   void prof_end(int id);
   void harvest();
-  void launch_near(const zd* y, const zd* loc, zd* r, int nrhs, const int* act, bool with_near);
-  bool prof_ = false;
-  KTimer kt_[KT_COUNT];
-
-  template <typename T>
-  void upload(DevOp<T>& d);
-  void apply_mlfma(const zd* y, zd* r, int nrhs, const int* act);
-  void reduce_finalize(int phase, int nrhs, double tol, int maxit);
+  void* dalloc(size_t bytes);
 
   EngineConfig cfg_;
   Plan P_;
   std::string err_;
   hipStream_t stream_ = nullptr;
+  bool has32_ = false;
   DevOp<double> op64_;
+  DevOp<float> op32_;
+  std::vector<InterpShape> m2m64_, l2l64_, m2m32_, l2l32_;  // per parent level
   std::vector<int*> interp0_, anterp0_;
   int* rx0_ = nullptr;
   int* rxaI_ = nullptr;
-  std::vector<size_t> lvl_off_;   // offset of level i in the multipole store, per rhs block
+  std::vector<size_t> lvl_off_;   // offset of level i in the stores (rhs blocks included)
   std::vector<size_t> lvl_size_;  // 4^i * S_i
-  zd* agm_ = nullptr;             // [level][rhs][4^i S_i]
-  zd* agl_ = nullptr;
   std::vector<zd*> vec_;          // solver vectors, each [max_rhs][N]
   double2* part_ = nullptr;       // reduction partials [max_rhs][3][nblk]
+  double2* part_h_ = nullptr;     // pinned host mirror
   BiState* state_ = nullptr;      // [max_rhs]
   int* active_ = nullptr;         // [max_rhs]
-  int* active_h_ = nullptr;       // pinned host mirror
-  BiState* state_h_ = nullptr;    // pinned host mirror
+  double* tolv_ = nullptr;        // [max_rhs] per-system tolerance
+  int* active_h_ = nullptr;       // pinned host mirrors
+  BiState* state_h_ = nullptr;
+  double* tol_h_ = nullptr;
   int nblk_ = 0;
+  bool prof_ = false;
+  KTimer kt_[KT_COUNT];
   std::vector<void*> allocs_;
-  void* dalloc(size_t bytes);
 };
 
 }  // namespace isg

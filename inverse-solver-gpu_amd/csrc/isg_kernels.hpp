@@ -2,10 +2,10 @@
 // vector algebra, written for gfx950 (64-lane wavefronts, 256-thread blocks).
 //
 // Operator stages (reference counterparts in parentheses):
-//   k_p2m     leaf radiation patterns            (kernels.cu:6-30)
-//   k_m2m     child -> parent interpolate+shift  (kernels.cu:83-136)
-//   k_m2l     diagonal translation               (kernels.cu:239-266)
-//   k_l2l     parent -> child shift+anterpolate  (kernels.cu:137-238)
+//   k_p2m_mfma   leaf radiation patterns on the matrix cores (kernels.cu:6-30)
+//   k_m2m_lds    child -> parent interpolate+shift  (kernels.cu:83-136)
+//   k_m2l_tile   diagonal translation               (kernels.cu:239-266)
+//   k_l2l_lds    parent -> child shift+anterpolate  (kernels.cu:137-238)
 //   k_near_mfma  near field + leaf receiving patterns on the matrix cores
 //                                                (kernels.cu:32-81, :292-319)
 //   k_rx_*    receiver interpolation / anterpolation (matvec_gpu.cu:351-361, :395-401)
@@ -21,23 +21,8 @@ constexpr int kBlock = 256;
 
 __device__ __forceinline__ bool rhs_off(const int* act, int rhs) { return act != nullptr && act[rhs] == 0; }
 
-// agm[b][k] = sum_n coeffT[n][k] y[b*64+n]
-template <typename T, typename TI>
-__global__ void __launch_bounds__(kBlock) k_p2m(const cplx<TI>* __restrict__ y, long ys, cplx<T>* __restrict__ agm,
-                                                long as, const cplx<T>* __restrict__ coeffT, int S, int B,
-                                                const int* __restrict__ act) {
-  int rhs = blockIdx.y;
-  if (rhs_off(act, rhs)) return;
-  long idx = (long)blockIdx.x * kBlock + threadIdx.x;
-  if (idx >= (long)B * S) return;
-  int b = (int)(idx / S), k = (int)(idx - (long)b * S);
-  const cplx<TI>* yb = y + rhs * ys + (long)b * 64;
-  cplx<T> acc = mkc<T>(0, 0);
-#pragma unroll 8
-  for (int n = 0; n < 64; n++) acc = cfma(coeffT[n * S + k], ccast<T>(yb[n]), acc);
-  agm[rhs * as + idx] = acc;
-}
-
+// Direct-from-cache interpolation kernels, one thread per target sample; used
+// for root levels whose LDS staging would exceed 64 KB.
 // parent[p][k] = sum_c shift[c][k] * sum_t w[t][k] child[4p+c][(i0[k]+t) mod Sc]
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_m2m(const cplx<T>* __restrict__ child, long cs, cplx<T>* __restrict__ par,
@@ -65,38 +50,6 @@ __global__ void __launch_bounds__(kBlock) k_m2m(const cplx<T>* __restrict__ chil
     acc = cfma(red, shift[c * Sp + k], acc);
   }
   par[rhs * ps + idx] = acc;
-}
-
-// loc[m][k] = sum over the interaction list of m of T[offset][k] agm[n][k].
-// The list is enumerated from Morton coordinates: children of the parent's
-// 3x3 neighbours that are not themselves neighbours of m.
-template <typename T>
-__global__ void __launch_bounds__(kBlock) k_m2l(const cplx<T>* __restrict__ agm, cplx<T>* __restrict__ agl, long st,
-                                                const cplx<T>* __restrict__ trans, int lev, int S,
-                                                const int* __restrict__ act) {
-  int rhs = blockIdx.y;
-  if (rhs_off(act, rhs)) return;
-  int nb = 1 << lev;
-  long tot = (long)nb * nb * S;
-  long idx = (long)blockIdx.x * kBlock + threadIdx.x;
-  if (idx >= tot) return;
-  int m = (int)(idx / S), k = (int)(idx - (long)m * S);
-  int X = (int)compact2((unsigned)m), Y = (int)compact2((unsigned)m >> 1);
-  int x0 = (X >> 1) * 2 - 2, y0 = (Y >> 1) * 2 - 2;
-  const cplx<T>* src = agm + rhs * st;
-  cplx<T> acc = mkc<T>(0, 0);
-  for (int sy = y0; sy < y0 + 6; sy++) {
-    if (sy < 0 || sy >= nb) continue;
-    int dy = sy - Y;
-    for (int sx = x0; sx < x0 + 6; sx++) {
-      if (sx < 0 || sx >= nb) continue;
-      int dx = sx - X;
-      if (dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1) continue;
-      long n = morton2((unsigned)sx, (unsigned)sy);
-      acc = cfma(trans[(7 * (dy + 3) + dx + 3) * S + k], src[n * S + k], acc);
-    }
-  }
-  agl[rhs * st + idx] = acc;
 }
 
 // child[c][k] (+)= sum_t a[t][k] * shiftL[c%4][j] * par[c/4][j], j = (j0[k]+t) mod Sp
@@ -138,23 +91,41 @@ __global__ void __launch_bounds__(kBlock) k_l2l(const cplx<T>* __restrict__ par,
 // i + 16k, B[k][j] in lane j + 16k; the 16x16 result D[i][j] is in lane
 // j + 16(i/4), register i%4 for f32 and in lane j + 16(i%4), register i/4
 // for f64 -- Mfma<T>::row gives the row of (lane group g, register q).
+// Every kernel puts the data vectors (boxes, clusters) in the A operand and
+// the operator table in the B operand, so the 16 lanes of a group hold 16
+// consecutive samples / pixels of the result and stores coalesce.
 template <typename T>
 struct Mfma;
 template <>
 struct Mfma<double> {
   typedef double v4 __attribute__((ext_vector_type(4)));
+  static constexpr bool kRowsAcrossLanes = true;  // rows g + 4q: siblings across lane groups
   static __device__ __forceinline__ v4 mac(double a, double b, v4 c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
   static __device__ __forceinline__ int row(int g, int q) { return g + 4 * q; }
+  // cr + i ci += d * a   (d data, A operand; a table, B operand; nai = -a.y)
+  static __device__ __forceinline__ void cmac(cplx<double> d, cplx<double> a, double nai, v4& cr, v4& ci) {
+    cr = mac(d.x, a.x, cr);
+    cr = mac(d.y, nai, cr);
+    ci = mac(d.x, a.y, ci);
+    ci = mac(d.y, a.x, ci);
+  }
 };
 template <>
 struct Mfma<float> {
   typedef float v4 __attribute__((ext_vector_type(4)));
+  static constexpr bool kRowsAcrossLanes = false;  // rows 4g + q: siblings in one lane
   static __device__ __forceinline__ v4 mac(float a, float b, v4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
   static __device__ __forceinline__ int row(int g, int q) { return 4 * g + q; }
+  static __device__ __forceinline__ void cmac(cplx<float> d, cplx<float> a, float nai, v4& cr, v4& ci) {
+    cr = mac(d.x, a.x, cr);
+    cr = mac(d.y, nai, cr);
+    ci = mac(d.x, a.y, ci);
+    ci = mac(d.y, a.x, ci);
+  }
 };
 
 // LDS row length (complex elements) of the near/L2P tile: 64 pixels or S
@@ -208,10 +179,7 @@ __global__ void __launch_bounds__(kBlock) k_near_mfma(const cplx<TI>* __restrict
 #pragma unroll
         for (int j = 0; j < NN; j++) {
           cplx<T> bv = tile[(16 * j + li) * row + k];
-          cr[j] = Mfma<T>::mac(a.x, bv.x, cr[j]);
-          cr[j] = Mfma<T>::mac(nai, bv.y, cr[j]);
-          ci[j] = Mfma<T>::mac(a.x, bv.y, ci[j]);
-          ci[j] = Mfma<T>::mac(a.y, bv.x, ci[j]);
+          Mfma<T>::cmac(bv, a, nai, cr[j], ci[j]);
         }
       }
     }
@@ -232,20 +200,314 @@ __global__ void __launch_bounds__(kBlock) k_near_mfma(const cplx<TI>* __restrict
 #pragma unroll
     for (int j = 0; j < NN; j++) {
       cplx<T> bv = tile[(16 * j + li) * row + k];
-      cr[j] = Mfma<T>::mac(a.x, bv.x, cr[j]);
-      cr[j] = Mfma<T>::mac(nai, bv.y, cr[j]);
-      ci[j] = Mfma<T>::mac(a.x, bv.y, ci[j]);
-      ci[j] = Mfma<T>::mac(a.y, bv.x, ci[j]);
+      Mfma<T>::cmac(bv, a, nai, cr[j], ci[j]);
     }
   }
-  cplx<TI>* out = r + rhs * rs;
+  // D[box][m]: lane li holds pixel m = 16w + li, so each store writes 16
+  // consecutive pixels of 4 boxes.
+  cplx<TI>* out = r + rhs * rs + 16 * w + li;
 #pragma unroll
   for (int j = 0; j < NN; j++) {
-    long box = b0 + 16 * j + li;
 #pragma unroll
     for (int q = 0; q < 4; q++) {
-      int m = 16 * w + Mfma<T>::row(lk, q);
-      out[box * 64 + m] = mkc<TI>((TI)cr[j][q], (TI)ci[j][q]);
+      long box = b0 + 16 * j + Mfma<T>::row(lk, q);
+      out[box * 64] = mkc<TI>((TI)cr[j][q], (TI)ci[j][q]);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Sample-chunked LDS interpolation (upper levels: few clusters, many samples).
+// A workgroup owns one parent (or G small parents) and a chunk of <= 256
+// target samples.  Taps of consecutive targets advance monotonically (mod the
+// source count), so the chunk needs one contiguous window of each source
+// vector: window start = first[k0], offset of target k = (first[k] -
+// first[k0]) mod Ssrc.  All four windows (children, or shifted parent copies)
+// go to LDS once; each thread then forms one target from LDS.
+
+// M2M: parent[p][k] = sum_c shiftM[c][k] sum_t w[t][k] child[4p+c][first[k]+t]
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_m2m_chunk(const cplx<T>* __restrict__ child, long cs,
+                                                      cplx<T>* __restrict__ par, long ps, const T* __restrict__ wT,
+                                                      const int* __restrict__ i0, const cplx<T>* __restrict__ shift,
+                                                      int Sp, int Sc, int G, int kch, int nch, int ntap,
+                                                      const int* __restrict__ act) {
+  extern __shared__ char smem_raw[];
+  cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem_raw);
+  const int rhs = blockIdx.y;
+  if (rhs_off(act, rhs)) return;
+  const int pg = blockIdx.x / nch, ch = blockIdx.x - pg * nch;
+  const long p0 = (long)pg * G;
+  const int k0 = ch * kch, kn = min(kch, Sp - k0);
+  const int base = i0[k0];
+  int W = i0[k0 + kn - 1] - base;
+  if (W < 0) W += Sc;
+  W += ntap;
+  const cplx<T>* cr = child + rhs * cs + 4 * p0 * Sc;
+  for (int e = threadIdx.x; e < 4 * G * W; e += kBlock) {
+    int v = e / W, t = e - v * W;  // v = 4 g + c (consecutive children)
+    int j = base + t;
+    while (j >= Sc) j -= Sc;
+    lds[e] = cr[(long)v * Sc + j];
+  }
+  __syncthreads();
+  int o = threadIdx.x;
+  if (o >= G * kn) return;
+  int g = o / kn, k = k0 + (o - g * kn);
+  int d = i0[k] - base;
+  if (d < 0) d += Sc;
+  cplx<T> acc = mkc<T>(0, 0);
+  for (int c = 0; c < 4; c++) {
+    const cplx<T>* src = lds + (4 * g + c) * W + d;
+    cplx<T> red = mkc<T>(0, 0);
+    for (int t = 0; t < ntap; t++) {
+      T w = wT[t * Sp + k];
+      red.x += w * src[t].x;
+      red.y += w * src[t].y;
+    }
+    acc = cfma(red, shift[c * Sp + k], acc);
+  }
+  par[rhs * ps + (p0 + g) * Sp + k] = acc;
+}
+
+// L2L: child[4p+c][k] (+)= sum_t a[t][k] (shiftL[c] o par[p])[first[k]+t]
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_l2l_chunk(const cplx<T>* __restrict__ par, long ps,
+                                                      cplx<T>* __restrict__ child, long cs, const T* __restrict__ aT,
+                                                      const int* __restrict__ j0s, const cplx<T>* __restrict__ shift,
+                                                      int Sp, int Sc, int kch, int nch, int ntap, int accumulate,
+                                                      const int* __restrict__ act) {
+  extern __shared__ char smem_raw[];
+  cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem_raw);
+  const int rhs = blockIdx.y;
+  if (rhs_off(act, rhs)) return;
+  const int p = blockIdx.x / nch, ch = blockIdx.x - p * nch;
+  const int k0 = ch * kch, kn = min(kch, Sc - k0);
+  const int base = j0s[k0];
+  int W = j0s[k0 + kn - 1] - base;
+  if (W < 0) W += Sp;
+  W += ntap;
+  const cplx<T>* pv = par + rhs * ps + (long)p * Sp;
+  for (int e = threadIdx.x; e < 4 * W; e += kBlock) {
+    int c = e / W, t = e - c * W;
+    int j = base + t;
+    while (j >= Sp) j -= Sp;
+    lds[e] = cmul(shift[c * Sp + j], pv[j]);
+  }
+  __syncthreads();
+  for (int o = threadIdx.x; o < 4 * kn; o += kBlock) {
+    int c = o / kn, k = k0 + (o - c * kn);
+    int d = j0s[k] - base;
+    if (d < 0) d += Sp;
+    const cplx<T>* src = lds + c * W + d;
+    cplx<T> red = mkc<T>(0, 0);
+    for (int t = 0; t < ntap; t++) {
+      T w = aT[t * Sc + k];
+      red.x += w * src[t].x;
+      red.y += w * src[t].y;
+    }
+    cplx<T>* dst = child + rhs * cs + (4L * p + c) * Sc + k;
+    *dst = accumulate ? cadd(*dst, red) : red;
+  }
+}
+
+// Dense interpolation on the matrix cores (lower levels, where sample counts
+// are small and clusters many).  Out[k][col] = sum_j A[k][j] In[j][col] with A
+// the (banded) interpolator stored densely K-major (AT[j][k], real), and the
+// columns = the child vectors (4 per parent) of NC/4 consecutive parents.
+//   M2M (MODE 0): In = children (contiguous); the epilogue multiplies by
+//                 shiftM[c][k] and sums the 4 sibling columns (lanes j..j+3)
+//                 with two xor-shuffles, then writes parent[p][k].
+//   L2L (MODE 1): In = shiftL[c] o parent, staged per column; Out is added to
+//                 (or assigned to) child[4p+c][k].
+template <typename T, int MODE, int NCB>
+__global__ void __launch_bounds__(kBlock) k_interp_mfma(const cplx<T>* __restrict__ src, long ss,
+                                                        cplx<T>* __restrict__ dst, long ds, const T* __restrict__ AT,
+                                                        const cplx<T>* __restrict__ shift, int Mrows, int K,
+                                                        int Spar, int accumulate, const int* __restrict__ act) {
+  typedef typename Mfma<T>::v4 v4;
+  constexpr int NC = 16 * NCB;
+  extern __shared__ char smem_raw[];
+  cplx<T>* tile = reinterpret_cast<cplx<T>*>(smem_raw);
+  const int rhs = blockIdx.y;
+  if (rhs_off(act, rhs)) return;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  const int row = K + 1;
+  const long p0 = (long)blockIdx.x * (NC / 4);
+  if (MODE == 0) {
+    const cplx<T>* s = src + rhs * ss + 4 * p0 * K;  // NC consecutive children of length K
+    for (int e = tid; e < NC * K; e += kBlock) {
+      int col = e / K, j = e - col * K;
+      tile[col * row + j] = s[e];
+    }
+  } else {
+    const cplx<T>* s = src + rhs * ss + p0 * K;  // NC/4 parents of length K
+    for (int e = tid; e < NC * K; e += kBlock) {
+      int col = e / K, j = e - col * K;
+      tile[col * row + j] = cmul(shift[(col & 3) * K + j], s[(long)(col >> 2) * K + j]);
+    }
+  }
+  __syncthreads();
+  // (row block, column block) items spread evenly over the 4 waves
+  const int nrb = (Mrows + 15) / 16;
+  for (int item = w; item < nrb * NCB; item += 4) {
+    const int rb = item / NCB, cb = item - rb * NCB;
+    v4 cr0 = v4{0, 0, 0, 0}, ci0 = v4{0, 0, 0, 0};
+    const int krow = 16 * rb + li;
+    const cplx<T>* tcol = tile + (16 * cb + li) * row;
+    for (int kk = 0; kk < K / 4; kk++) {
+      const int j = 4 * kk + lk;
+      T a = (krow < Mrows) ? AT[(long)j * Mrows + krow] : (T)0;
+      cplx<T> bv = tcol[j];
+      cr0 = Mfma<T>::mac(bv.x, a, cr0);
+      ci0 = Mfma<T>::mac(bv.y, a, ci0);
+    }
+    // D[col][k]: lane li holds target sample k = krow; col = 16 cb + row(lk, q)
+    const int k = krow;
+    if (MODE == 0) {
+      // parent = sum over the 4 sibling columns of shiftM[c][k] * D[col][k]
+      if (Mfma<T>::kRowsAcrossLanes) {
+        // col = 16cb + lk + 4q: sibling c = lk, parent 4cb + q
+        const cplx<T> s1 = (k < Mrows) ? shift[lk * Mrows + k] : mkc<T>(0, 0);
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          T vr = cr0[q], vi = ci0[q];
+          T xr = vr * s1.x - vi * s1.y, xi = vr * s1.y + vi * s1.x;
+          xr += __shfl_xor(xr, 16, 64);
+          xi += __shfl_xor(xi, 16, 64);
+          xr += __shfl_xor(xr, 32, 64);
+          xi += __shfl_xor(xi, 32, 64);
+          if (lk == 0 && k < Mrows) dst[rhs * ds + (p0 + 4 * cb + q) * Mrows + k] = mkc<T>(xr, xi);
+        }
+      } else {
+        // col = 16cb + 4lk + q: sibling c = q, parent 4cb + lk
+        T xr = 0, xi = 0;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          const cplx<T> sq = (k < Mrows) ? shift[q * Mrows + k] : mkc<T>(0, 0);
+          T vr = cr0[q], vi = ci0[q];
+          xr += vr * sq.x - vi * sq.y;
+          xi += vr * sq.y + vi * sq.x;
+        }
+        if (k < Mrows) dst[rhs * ds + (p0 + 4 * cb + lk) * Mrows + k] = mkc<T>(xr, xi);
+      }
+    } else if (k < Mrows) {
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const int col = 16 * cb + Mfma<T>::row(lk, q);
+        cplx<T>* o = dst + rhs * ds + (4 * p0 + col) * (long)Mrows + k;
+        cplx<T> v = mkc<T>(cr0[q], ci0[q]);
+        *o = accumulate ? cadd(*o, v) : v;
+      }
+    }
+  }
+  (void)Spar;
+}
+
+// Tiled M2L.  A workgroup owns a TS x TS block of target boxes (Morton
+// consecutive) and KC consecutive samples.  The interaction lists of the
+// block lie in the (TS+4)^2 block of boxes around it; those multipoles and
+// the 49 translation operators for the KC samples are staged in LDS once, so
+// each multipole sample is read from memory (TS+4)^2/TS^2 times instead of
+// once per interaction (<= 27).
+template <typename T, int TS, int KC>
+__global__ void __launch_bounds__(kBlock) k_m2l_tile(const cplx<T>* __restrict__ agm, cplx<T>* __restrict__ agl,
+                                                     long st, const cplx<T>* __restrict__ trans, int lev, int S,
+                                                     const int* __restrict__ act) {
+  constexpr int SB = TS + 4;
+  __shared__ cplx<T> src[SB * SB][KC];
+  __shared__ cplx<T> tr[49][KC];
+  const int rhs = blockIdx.y;
+  if (rhs_off(act, rhs)) return;
+  const int nb = 1 << lev;
+  const int nchunk = (S + KC - 1) / KC;
+  const int tile = blockIdx.x / nchunk, chunk = blockIdx.x - tile * nchunk;
+  const int k0 = chunk * KC;
+  const unsigned m0 = (unsigned)tile * TS * TS;
+  const int X0 = (int)compact2(m0), Y0 = (int)compact2(m0 >> 1);
+  const cplx<T>* a = agm + rhs * st;
+  for (int e = threadIdx.x; e < SB * SB * KC; e += kBlock) {
+    int sb = e / KC, kk = e - sb * KC;
+    int sx = X0 - 2 + sb % SB, sy = Y0 - 2 + sb / SB;
+    cplx<T> v = mkc<T>(0, 0);
+    if (sx >= 0 && sy >= 0 && sx < nb && sy < nb && k0 + kk < S) v = a[(long)morton2(sx, sy) * S + k0 + kk];
+    src[sb][kk] = v;
+  }
+  for (int e = threadIdx.x; e < 49 * KC; e += kBlock) {
+    int q = e / KC, kk = e - q * KC;
+    tr[q][kk] = (k0 + kk < S) ? trans[(long)q * S + k0 + kk] : mkc<T>(0, 0);
+  }
+  __syncthreads();
+  for (int o = threadIdx.x; o < TS * TS * KC; o += kBlock) {
+    int tl = o / KC, kk = o - tl * KC;
+    if (k0 + kk >= S) continue;
+    unsigned m = m0 + tl;
+    int X = (int)compact2(m), Y = (int)compact2(m >> 1);
+    int bx = (X >> 1) * 2 - 2, by = (Y >> 1) * 2 - 2;
+    cplx<T> acc = mkc<T>(0, 0);
+    for (int sy = by; sy < by + 6; sy++) {
+      if (sy < 0 || sy >= nb) continue;
+      int dy = sy - Y;
+      for (int sx = bx; sx < bx + 6; sx++) {
+        if (sx < 0 || sx >= nb) continue;
+        int dx = sx - X;
+        if (dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1) continue;
+        acc = cfma(tr[7 * (dy + 3) + dx + 3][kk], src[(sy - Y0 + 2) * SB + (sx - X0 + 2)][kk], acc);
+      }
+    }
+    agl[rhs * st + (long)m * S + k0 + kk] = acc;
+  }
+}
+
+// P2M on the matrix cores: agm[box][k] = sum_n coeffT[n][k] y[box*64+n] for a
+// tile of NB boxes; rows k in blocks of 16 spread over the 4 waves.
+template <typename T, typename TI, int NB>
+__global__ void __launch_bounds__(kBlock) k_p2m_mfma(const cplx<TI>* __restrict__ y, long ys,
+                                                     cplx<T>* __restrict__ agm, long as,
+                                                     const cplx<T>* __restrict__ coeffT, int S,
+                                                     const int* __restrict__ act) {
+  typedef typename Mfma<T>::v4 v4;
+  constexpr int NN = NB / 16;
+  __shared__ cplx<T> tile[NB * 65];
+  const int rhs = blockIdx.y;
+  if (rhs_off(act, rhs)) return;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  const long b0 = (long)blockIdx.x * NB;
+  const cplx<TI>* yr = y + rhs * ys + b0 * 64;
+  for (int idx = tid; idx < NB * 64; idx += kBlock) tile[(idx >> 6) * 65 + (idx & 63)] = ccast<T>(yr[idx]);
+  __syncthreads();
+  const int nrb = (S + 15) / 16;
+  for (int rb = w; rb < nrb; rb += 4) {
+    v4 cr[NN], ci[NN];
+#pragma unroll
+    for (int j = 0; j < NN; j++) {
+      cr[j] = v4{0, 0, 0, 0};
+      ci[j] = v4{0, 0, 0, 0};
+    }
+    const int krow = 16 * rb + li;
+#pragma unroll 4
+    for (int kk = 0; kk < 16; kk++) {
+      const int n = 4 * kk + lk;
+      cplx<T> a = (krow < S) ? coeffT[n * S + krow] : mkc<T>(0, 0);
+      T nai = -a.y;
+#pragma unroll
+      for (int j = 0; j < NN; j++) {
+        cplx<T> bv = tile[(16 * j + li) * 65 + n];
+        Mfma<T>::cmac(bv, a, nai, cr[j], ci[j]);
+      }
+    }
+    cplx<T>* out = agm + rhs * as;
+    if (krow < S) {
+#pragma unroll
+      for (int j = 0; j < NN; j++) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          long box = b0 + 16 * j + Mfma<T>::row(lk, q);
+          out[box * S + krow] = mkc<T>(cr[j][q], ci[j][q]);
+        }
+      }
     }
   }
 }

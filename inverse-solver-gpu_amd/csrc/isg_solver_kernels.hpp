@@ -182,10 +182,12 @@ __global__ void __launch_bounds__(kBlock) k_bi_xrupd(zd* __restrict__ x, const z
 // One block per right-hand side: fixed-order reduction of the partials, then
 // the scalar step of bicgs_gpu.cu:148-213 for that right-hand side.
 __global__ void __launch_bounds__(kBlock) k_bi_fin(int phase, BiState* __restrict__ st, int* __restrict__ active,
-                                                   const double2* __restrict__ part, int nblk, double tol, int maxit) {
+                                                   const double2* __restrict__ part, int nblk,
+                                                   const double* __restrict__ tolv, int maxit) {
   int rhs = blockIdx.x;
   BiState& S = st[rhs];
   if (phase != PH_INIT && !S.active) return;
+  const double tol = (phase == PH_INIT) ? tolv[rhs] : S.tol;
   const double2* pr = part + (long)rhs * kSlots * nblk;
   int ns = (phase == PH_ALPHA) ? 1 : 2;
   __shared__ double sh[4][kBlock / 64];
@@ -210,6 +212,7 @@ __global__ void __launch_bounds__(kBlock) k_bi_fin(int phase, BiState* __restric
   if (phase == PH_INIT) {
     double bn = sqrt(s0.x);
     if (bn < 1e-20) bn = 1;
+    S.tol = tol;
     S.bnrm = bn;
     S.err = sqrt(s1.x) / bn;
     S.iter = 0;
@@ -262,6 +265,13 @@ __global__ void __launch_bounds__(kBlock) k_bi_fin(int phase, BiState* __restric
     }
   }
   active[rhs] = S.active;
+}
+
+// x += c  (defect correction of the mixed-precision solve)
+__global__ void __launch_bounds__(kBlock) k_add(zd* __restrict__ x, const zd* __restrict__ c, long N) {
+  int rhs = blockIdx.y;
+  long o = rhs * N;
+  ISG_VEC_LOOP(N) x[o + i] = cadd(x[o + i], c[o + i]);
 }
 
 // y = prep(x)

@@ -60,6 +60,7 @@ class DbimStats(ctypes.Structure):
         ("solve_iters", ctypes.c_longlong),
         ("matvecs", ctypes.c_longlong),
         ("not_converged", ctypes.c_longlong),
+        ("matvecs_fp32", ctypes.c_longlong),
         ("seconds", ctypes.c_double),
     ]
 
@@ -82,6 +83,7 @@ _SIGS = {
     "isg_stream": (_V, [_V]),
     "isg_mlfma": (ctypes.c_int, [_V, _D, _D]),
     "isg_mlfmah": (ctypes.c_int, [_V, _D, _D]),
+    "isg_mlfma_many": (ctypes.c_int, [_V, _D, _D, ctypes.c_int]),
     "isg_aggregate": (ctypes.c_int, [_V, _D, _D]),
     "isg_aggregateh": (ctypes.c_int, [_V, _D, _D]),
     "isg_bicgs": (ctypes.c_int, [_V, _D, _D, _D, ctypes.c_int, _P(SolveInfo)]),
@@ -201,6 +203,16 @@ class Solver:
         self._err(self.lib.isg_mlfma(self.h, _dp(x), _dp(r)), "isg_mlfma")
         return r
 
+    def mlfma_many(self, X):
+        """Batched r[k] = G x[k] for an (nrhs, N) array, one pass."""
+        X = np.ascontiguousarray(X, dtype=np.complex128)
+        nrhs = X.shape[0]
+        if X.shape != (nrhs, self.N):
+            raise ValueError("expected (nrhs, %d)" % self.N)
+        R = np.empty_like(X)
+        self._err(self.lib.isg_mlfma_many(self.h, _dp(X), _dp(R), nrhs), "isg_mlfma_many")
+        return R
+
     def mlfmah(self, x):
         x = _c128(x, self.N)
         r = np.empty(self.N, np.complex128)
@@ -234,7 +246,7 @@ class Solver:
         self._err(self.lib.isg_stage(self.h, which, level, rhs, _dp(out)), "isg_stage")
         return out.reshape(4 ** level, self.numsamp[level])
 
-    KERNELS = {"near": 0, "m2l": 1, "bicgs_update": 2}
+    KERNELS = {"near": 0, "m2l": 1, "bicgs_update": 2, "near_fp32": 3, "m2l_fp32": 4}
 
     def profile(self, enable=True):
         """Start (clearing totals) or stop live HIP-event timing of the hot kernels."""
@@ -248,9 +260,10 @@ class Solver:
         return ms.value, n.value
 
     def counters(self):
-        out = (ctypes.c_longlong * 3)()
+        out = (ctypes.c_longlong * 5)()
         self._err(self.lib.isg_counters(self.h, out), "isg_counters")
-        return {"matvecs": out[0], "solves": out[1], "solve_iters": out[2]}
+        return {"matvecs": out[0], "solves": out[1], "solve_iters": out[2], "matvecs_fp32": out[3],
+                "refinements": out[4]}
 
     # ---- device-pointer surface (pointers are integers, e.g. torch data_ptr())
     def mlfma_dev(self, x_ptr, r_ptr, nrhs=1):

@@ -1,0 +1,73 @@
+"""Mixed precision on the GPU: FP32 operator, FP64 solver scalars and vectors.
+
+Bars (stated here, checked against the CPU oracle on seeded inputs):
+  * FP32 MLFMA matvec within 1e-5 relative of the FP64 oracle matvec;
+  * mixed BiCGSTAB: FP64 true relative residual below the tolerance (the
+    solver's own defect correction), solution within 10x tolerance of the FP64
+    oracle solution;
+  * mixed + batched DBIM on the 64x64 configuration: reconstructed contrast
+    within 1e-4 relative of the oracle's after 3 iterations.
+"""
+import numpy as np
+import pytest
+
+from conftest import oracle_for, rand_c, relerr
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def isg():
+    import isg as m
+    return m
+
+
+@pytest.mark.parametrize("level,nrhs", [(4, 1), (6, 4), (7, 2)])
+def test_fp32_matvec_within_1e5(isg, level, nrhs):
+    print("fp32 matvec level", level, flush=True)
+    o = oracle_for(level)
+    s = isg.Solver(level=level, numrx=16, precision=1, max_rhs=nrhs)
+    rng = np.random.default_rng(500 + level)
+    X = rand_c(rng, (nrhs, o.N))
+    R = s.mlfma_many(X)
+    for k in range(nrhs):
+        e = relerr(R[k], o.mlfma(X[k]))
+        assert e < 1e-5, e
+        assert e > 1e-12  # really computed in FP32
+
+
+@pytest.mark.parametrize("herm", [False, True])
+def test_mixed_bicgs_reaches_fp64_residual(isg, herm):
+    level, tol = 5, 1e-6
+    o = oracle_for(level)
+    s = isg.Solver(level=level, numrx=16, toli=tol, iti=300, precision=1)
+    pos = o.pos()
+    contrast = 0.4 * np.exp(-np.abs(pos) ** 2 / (2 * 2.5 ** 2)) * (1 + 0.05j)
+    g = contrast * o.k0 ** 2
+    b = np.exp(1j * o.k0 * (0.6 * pos.real + 0.8 * pos.imag))
+    x, info = s.bicgs(g, b, herm=herm)
+    assert info["status"] == 0, info
+    # independent FP64 check of the residual with the oracle operator
+    if herm:
+        ax = x - np.conj(g * o.mlfma(np.conj(x)))
+    else:
+        ax = x - o.mlfma(g * x)
+    true_res = np.linalg.norm(b - ax) / np.linalg.norm(b)
+    assert true_res < tol * 1.05, true_res
+    x_ref, _ = o.bicgs(g, b, herm=herm, tol=tol, maxit=300)
+    assert relerr(x, x_ref) < 10 * tol * 10
+
+
+def test_mixed_batched_dbim_matches_oracle(isg):
+    from oracle.oracle import Dbim as OrcDbim
+    level, ntx, nrx, tol, reg = 4, 16, 16, 1e-6, float(np.sqrt(1e-3))
+    o = oracle_for(level, numrx=nrx)
+    otrue = isg.raster_to_tree(isg.phantom_blobs(o.side, 3, 1), o.unkmap())
+    ref = OrcDbim(o, ntx, otrue, regular=reg, tol=tol, maxit=300)
+    s = isg.Solver(level=level, numrx=nrx, numtx=ntx, toli=tol, iti=300, regular=reg, max_rhs=ntx, precision=1)
+    d = isg.Dbim(s, otrue, field_start=1)
+    for _ in range(3):
+        ref.iterate()
+        st = d.iterate()
+        assert st["not_converged"] == 0
+    assert relerr(d.get("o"), ref.get("o")) < 1e-4

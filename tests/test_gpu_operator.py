@@ -36,6 +36,30 @@ def test_mlfma_matches_oracle(isg, level):
         assert relerr(s.stage(1, i), o.stage(1, i)) < TOL, ("locals", i)
 
 
+def test_mlfma_level7_matches_oracle(isg):
+    """512x512: the two leaf levels take the dense matrix-core interpolation path."""
+    o = oracle_for(7)
+    s = isg.Solver(level=7, numrx=16)
+    rng = np.random.default_rng(7)
+    x = rand_c(rng, o.N)
+    assert relerr(s.mlfma(x), o.mlfma(x)) < TOL
+    for i in range(2, 7):
+        assert relerr(s.stage(1, i), o.stage(1, i)) < TOL, ("locals", i)
+
+
+@pytest.mark.parametrize("level,nrhs", [(5, 4), (6, 16)])
+def test_mlfma_batched_matches_oracle(isg, level, nrhs):
+    """Batched passes (every launch shape that depends on the RHS count)."""
+    print("batched mlfma parity", level, nrhs, flush=True)
+    o = oracle_for(level)
+    s = isg.Solver(level=level, numrx=16, max_rhs=nrhs)
+    rng = np.random.default_rng(300 + level)
+    X = rand_c(rng, (nrhs, o.N))
+    R = s.mlfma_many(X)
+    for k in (0, nrhs // 2, nrhs - 1):
+        assert relerr(R[k], o.mlfma(X[k])) < TOL, k
+
+
 def test_geometry_matches_oracle(isg):
     o = oracle_for(5)
     s = isg.Solver(level=5, numrx=16)
