@@ -59,6 +59,23 @@ def near_flops_per_rhs(level, S_leaf):
     return 8 * 64 * 64 * pairs + 8 * 64 * S_leaf * nb * nb, pairs
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_fp32_operator_1024x64rhs.txt")
+
+
+def measured_traffic():
+    """HBM bytes (FETCH_SIZE + WRITE_SIZE, copy-calibrated) of one 64-RHS
+    launch of the FP32 near-field kernel at level 8, from the committed
+    rocprofv3 counter summary; None when that summary is absent."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch_64rhs="):
+                    return int(line.split("=")[1])
+    except OSError:
+        pass
+    return None
+
+
 def m2l_bytes_per_rhs(level, numsamp, cbytes):
     """Read every multipole once and write every local once, levels 2..leaf."""
     return sum(2 * cbytes * 4 ** i * numsamp[i] for i in range(2, level))
@@ -179,6 +196,7 @@ def main():
     achieved = n_near * flops_mv / (near_ms * 1e-3) / 1e12 if near_ms > 0 else 0.0
     m2l_gbs = n_near * m2l_bytes_per_rhs(level, numsamp, cb) / (m2l_ms * 1e-3) / 1e9 if m2l_ms > 0 else 0.0
     iters = sum(s["solve_iters"] for s in stats)
+    traffic = measured_traffic() if (mixed and level == 8 and ntx == 64) else None
 
     out = {
         "metric": METRIC,
@@ -207,7 +225,9 @@ def main():
             "peak": PEAK_TFLOPS[prec],
             "unit": "TFLOP/s",
             "frac": round(achieved / PEAK_TFLOPS[prec], 4),
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_launch": "64 RHS, level 8 (rocprofv3 FETCH_SIZE x2 + WRITE_SIZE x1, copy-calibrated)"
+                              if traffic else None,
             "flops_per_rhs_launch": flops_mv,
             "avg_launch_ms": round(near_ms / max(near_n, 1), 4),
             "launches": near_n,

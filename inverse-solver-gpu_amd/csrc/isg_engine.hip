@@ -359,7 +359,7 @@ void Engine::launch_near(const zd* y, zd* r, int nrhs, const int* act, bool with
   const int nb = B >= 32 ? 32 : 16;
   DevOp<T>& d = ops<T>();
   const cplx<T>* loc = d.agl + lvl_off_[Lf];
-  size_t lds = (size_t)nb * near_row(S) * sizeof(cplx<T>);
+  size_t lds = (size_t)near_lds_elems(nb, S) * sizeof(cplx<T>);
   dim3 grid(B / nb, nrhs), blk(kBlock);
   const int kt = sizeof(T) == 4 ? KT_NEAR32 : KT_NEAR;
   if (with_near) prof_begin(kt);

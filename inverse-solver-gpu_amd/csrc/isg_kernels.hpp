@@ -131,7 +131,13 @@ struct Mfma<float> {
 // LDS row length (complex elements) of the near/L2P tile: 64 pixels or S
 // locals, plus one element of padding so 16 lanes reading 16 different rows
 // hit 16 different banks.
-__host__ __device__ inline int near_row(int S) { return (S > 64 ? S : 64) + 1; }
+// LDS elements of k_near_mfma<NB>: the (TX+2) x (TY+2) halo boxes of 65
+// elements, later reused for NB rows of S + 1 leaf locals.
+__host__ __device__ inline int near_lds_elems(int NB, int S) {
+  int tx = (NB == 16) ? 4 : 8, ty = NB / tx;
+  int halo = (tx + 2) * (ty + 2) * 65, l2p = NB * (S + 1);
+  return halo > l2p ? halo : l2p;
+}
 
 template <typename T, typename TI, int NB>
 __global__ void __launch_bounds__(kBlock) k_near_mfma(const cplx<TI>* __restrict__ y, long ys,
@@ -142,9 +148,14 @@ __global__ void __launch_bounds__(kBlock) k_near_mfma(const cplx<TI>* __restrict
                                                       int with_near, const int* __restrict__ act) {
   typedef typename Mfma<T>::v4 v4;
   constexpr int NN = NB / 16;
+  // Morton-consecutive NB boxes form a TX x TY block; with its one-box halo
+  // the nine near slots of every box in the tile lie in HX x HY boxes, staged
+  // once (each input pixel enters LDS once per tile instead of once per slot).
+  constexpr int TX = (NB == 16) ? 4 : 8, TY = NB / TX, HX = TX + 2, HY = TY + 2;
+  constexpr int HROW = 65;  // 64 pixels + 1 padding element
   extern __shared__ char smem_raw[];
   cplx<T>* tile = reinterpret_cast<cplx<T>*>(smem_raw);
-  const int row = near_row(S);
+  const int row = S + 1;    // L2P rows: S locals + 1 padding element
   int rhs = blockIdx.y;
   if (rhs_off(act, rhs)) return;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -158,18 +169,24 @@ __global__ void __launch_bounds__(kBlock) k_near_mfma(const cplx<TI>* __restrict
   }
   const cplx<TI>* yr = y + rhs * ys;
   if (with_near) {
+    const int X0 = (int)compact2((unsigned)b0), Y0 = (int)compact2((unsigned)b0 >> 1);
+    for (int idx = tid; idx < HX * HY * 64; idx += kBlock) {
+      int h = idx >> 6, n = idx & 63;
+      int sx = X0 - 1 + h % HX, sy = Y0 - 1 + h / HX;
+      cplx<T> v = mkc<T>(0, 0);
+      if (sx >= 0 && sy >= 0 && sx < nbs && sy < nbs) v = ccast<T>(yr[(long)morton2(sx, sy) * 64 + n]);
+      tile[h * HROW + n] = v;
+    }
+    // halo row of column 16 j + li for the centre slot
+    int hc[NN];
+#pragma unroll
+    for (int j = 0; j < NN; j++) {
+      unsigned c = (unsigned)(16 * j + li);
+      hc[j] = ((int)compact2(c >> 1) + 1) * HX + (int)compact2(c) + 1;
+    }
+    __syncthreads();
     for (int s = 0; s < 9; s++) {
-      const int dX = s % 3 - 1, dY = s / 3 - 1;
-      __syncthreads();
-      for (int idx = tid; idx < NB * 64; idx += kBlock) {
-        int j = idx >> 6, n = idx & 63;
-        unsigned b = (unsigned)(b0 + j);
-        int sx = (int)compact2(b) + dX, sy = (int)compact2(b >> 1) + dY;
-        cplx<T> v = mkc<T>(0, 0);
-        if (sx >= 0 && sy >= 0 && sx < nbs && sy < nbs) v = ccast<T>(yr[(long)morton2(sx, sy) * 64 + n]);
-        tile[j * row + n] = v;
-      }
-      __syncthreads();
+      const int off = (s / 3 - 1) * HX + (s % 3 - 1);
       const cplx<T>* A = near + (long)s * 4096 + 16 * w + li;
 #pragma unroll 4
       for (int kk = 0; kk < 16; kk++) {
@@ -178,7 +195,7 @@ __global__ void __launch_bounds__(kBlock) k_near_mfma(const cplx<TI>* __restrict
         T nai = -a.y;
 #pragma unroll
         for (int j = 0; j < NN; j++) {
-          cplx<T> bv = tile[(16 * j + li) * row + k];
+          cplx<T> bv = tile[(hc[j] + off) * HROW + k];
           Mfma<T>::cmac(bv, a, nai, cr[j], ci[j]);
         }
       }
