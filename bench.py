@@ -19,7 +19,8 @@ Two paths run from the same synthetic measurements:
   fast    every transmitter of a phase in one multi-RHS BiCGSTAB with
           deflation, FP32 operator with FP64 solver scalars/vectors and an FP64
           true-residual defect correction (every solve still reaches 1e-6 in
-          FP64), total-field solves warm-started from the previous DBIM iterate.
+          FP64), total-field solves started from the previous DBIM total field
+          moved to first order along the step (--fast-start).
 `value` is the fast path's s/iteration; the plain figure is reported beside it.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1]
@@ -39,6 +40,9 @@ import numpy as np  # noqa: E402
 METRIC = "s per DBIM iteration, 1024² px ×64 Tx, rel-res 1e-6: plain vs batched+learned"
 PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3}
 PEAK_HBM_GBS = 8000.0
+
+# total-field start of the fast path (DbimOptions::field_start)
+START_KINDS = {"zero": 0, "previous": 1, "first-order": 2}
 
 WORKLOADS = {
     # name: level, transmitters, receivers, ellipses, seed
@@ -87,6 +91,7 @@ def run_path(solver, dbim, warmup, steps, label):
         log("[%s] warmup iteration %d: %.3fs, %d solves, %d BiCGSTAB iterations, misfit %.4e"
             % (label, w, st["seconds"], st["solves"], st["solve_iters"], st["misfit"]))
     solver.profile(True)
+    c0 = solver.counters()
     t0 = time.perf_counter()
     stats = []
     for k in range(steps):
@@ -95,6 +100,8 @@ def run_path(solver, dbim, warmup, steps, label):
         log("[%s] timed iteration %d: %.3fs, %d solves, %d BiCGSTAB iterations, %d not converged, misfit %.4e"
             % (label, k, st["seconds"], st["solves"], st["solve_iters"], st["not_converged"], st["misfit"]))
     t1 = time.perf_counter()
+    c1 = solver.counters()
+    stats[-1]["refinements_timed"] = c1["refinements"] - c0["refinements"]
     prof = {k: solver.profile_read(k) for k in solver.KERNELS}
     solver.profile(False)
     return (t1 - t0) / steps, stats, prof
@@ -134,6 +141,10 @@ def main():
     ap.add_argument("--workload", default="dbim-1024", choices=sorted(WORKLOADS))
     ap.add_argument("--paths", default="fast,plain")
     ap.add_argument("--fast-precision", default="mixed", choices=["mixed", "fp64"])
+    ap.add_argument("--fast-start", default="first-order", choices=sorted(START_KINDS),
+                    help="total-field start of the fast path")
+    ap.add_argument("--mixed-margin", type=float, default=0.5,
+                    help="FP32 solve target as a fraction of the tolerance (mixed precision)")
     ap.add_argument("--cpu-sample", type=int, default=2, help="oracle matvecs timed for cpu_baseline (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -167,8 +178,8 @@ def main():
 
     res = {}
     if "fast" in paths:
-        fast = isg.Solver(max_rhs=ntx, precision=fast_prec, **common)
-        d_fast = isg.Dbim(fast, otrue, field_start=1, measurements=mes)
+        fast = isg.Solver(max_rhs=ntx, precision=fast_prec, mixed_margin=args.mixed_margin, **common)
+        d_fast = isg.Dbim(fast, otrue, field_start=START_KINDS[args.fast_start], measurements=mes)
         res["fast"] = run_path(fast, d_fast, args.warmup, args.steps, "fast")
         del d_fast
         fast.close()
@@ -214,8 +225,9 @@ def main():
         "config": {
             "workload": "DBIM %dx%d px lambda/10, level %d, %d Tx, %d Rx, %d ellipses seed %d, rel-res 1e-6"
                         % (side, side, level, ntx, nrx, wl["nell"], wl["seed"]),
-            "path": ("batched %d RHS + deflation, %s, previous-iterate total-field start"
-                     % (ntx, "FP32 operator / FP64 solver + FP64 defect correction" if mixed else "FP64")
+            "path": ("batched %d RHS + deflation, %s, %s total-field start"
+                     % (ntx, "FP32 operator / FP64 solver + FP64 defect correction" if mixed else "FP64",
+                        args.fast_start)
                      if main_path == "fast" else "1 RHS, FP64, zero start"),
         },
         "roofline": {
@@ -241,6 +253,7 @@ def main():
         "bicgstab_iterations": iters,
         "matvecs": matvecs,
         "matvecs_fp32": mv32,
+        "defect_corrections": stats[-1].get("refinements_timed"),
         "misfit": stats[-1]["misfit"],
     }
     if "plain" in res and "fast" in res:

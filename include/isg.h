@@ -70,6 +70,9 @@ typedef struct isg_config {
                       matrix-core arithmetic inside BiCGSTAB; FP64 vectors, dot products and
                       scalars; FP64 true-residual check with defect correction until the FP64
                       relative residual meets toli.  isg_mlfma/aggregate(h) stay FP64. */
+  double mixed_margin; /* mixed precision only: the FP32 solve runs to mixed_margin * toli on its
+                          recursive residual before the FP64 check, so that the check rarely
+                          needs a defect-correction round; 0 < mixed_margin <= 1 (default 0.5) */
 } isg_config;
 
 /* Per-solve result. */
@@ -148,7 +151,9 @@ typedef struct isg_dbim_stats {
  * measurements of the true contrast o_true (tree order, numunk complex; the
  * reference's phantom file), zero initial object and the first gradient
  * (mom.cpp:222-276).  field_start: 0 = every total-field solve starts from zero
- * as in the reference (mom.cpp:324), 1 = from the previous total field. */
+ * as in the reference (mom.cpp:324), 1 = from the previous total field, 2 = from
+ * the previous total field moved to first order along the step: tot - alpha dtot,
+ * dtot being the derivative field of the step-length solve (mom.cpp:284-290). */
 int isg_dbim_setup(isg_context* ctx, const double* o_true, int field_start, isg_dbim_stats* st);
 /* Same start from given measurements mes (numtx x numrx complex, the reference's
  * mes array after setup_born.cpp:148) instead of synthesizing them; o_true may be

@@ -70,6 +70,7 @@ void isg_config_default(isg_config* c) {
   c->regular = 0.0;
   c->max_rhs = 1;
   c->precision = 0;
+  c->mixed_margin = 0.5;
 }
 
 int isg_create(const isg_config* cfg, isg_context** out) {
@@ -77,6 +78,7 @@ int isg_create(const isg_config* cfg, isg_context** out) {
   *out = nullptr;
   if (cfg->max_rhs < 1) return ISG_E_ARG;
   if (cfg->precision != 0 && cfg->precision != 1) return ISG_E_ARG;
+  if (!(cfg->mixed_margin > 0 && cfg->mixed_margin <= 1)) return ISG_E_ARG;
   isg_context* c = new isg_context();
   c->cfg = *cfg;
   isg::EngineConfig ec;
@@ -89,6 +91,7 @@ int isg_create(const isg_config* cfg, isg_context** out) {
   ec.plan.numrx = cfg->numrx;
   ec.max_rhs = cfg->max_rhs;
   ec.precision = cfg->precision;
+  ec.mixed_margin = cfg->mixed_margin;
   c->eng.reset(new isg::Engine(ec));
   if (!check(c)) {
     *out = c;  // keep the context so the caller can read the error
@@ -308,7 +311,7 @@ static void fill_stats(const isg::DbimStats& s, isg_dbim_stats* o) {
 
 int isg_dbim_setup(isg_context* c, const double* o_true, int field_start, isg_dbim_stats* st) {
   if (!check(c)) return ISG_E_STATE;
-  if (!o_true || field_start < 0 || field_start > 1) return fail(c, ISG_E_ARG, "bad argument");
+  if (!o_true || field_start < 0 || field_start > 2) return fail(c, ISG_E_ARG, "bad argument");
   isg::DbimOptions opt;
   opt.numtx = c->cfg.numtx;
   opt.regular = c->cfg.regular;
@@ -329,7 +332,7 @@ int isg_dbim_setup(isg_context* c, const double* o_true, int field_start, isg_db
 int isg_dbim_setup_measured(isg_context* c, const double* mes, const double* o_true, int field_start,
                             isg_dbim_stats* st) {
   if (!check(c)) return ISG_E_STATE;
-  if (!mes || field_start < 0 || field_start > 1) return fail(c, ISG_E_ARG, "bad argument");
+  if (!mes || field_start < 0 || field_start > 2) return fail(c, ISG_E_ARG, "bad argument");
   isg::DbimOptions opt;
   opt.numtx = c->cfg.numtx;
   opt.regular = c->cfg.regular;

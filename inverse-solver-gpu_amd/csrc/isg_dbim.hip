@@ -94,6 +94,12 @@ __global__ void __launch_bounds__(kBlock) k_axpy(zd* a, const zd* b, zd s, long 
   DB_LOOP(N) a[i] = cadd(a[i], cmul(s, b[i]));
 }
 
+// a[t] += s b[t] for every row t of a batch
+__global__ void __launch_bounds__(kBlock) k_axpy_rows(zd* a, const zd* b, zd s, long N) {
+  int t = blockIdx.y;
+  DB_LOOP(N) a[t * N + i] = cadd(a[t * N + i], cmul(s, b[t * N + i]));
+}
+
 // ot = od + beta ot
 __global__ void __launch_bounds__(kBlock) k_newdir(zd* ot, const zd* od, zd beta, long N) {
   DB_LOOP(N) ot[i] = cadd(od[i], cmul(beta, ot[i]));
@@ -173,6 +179,7 @@ Dbim::Dbim(Engine& eng, const DbimOptions& opt) : eng_(eng), opt_(opt) {
   wy_ = (zd*)dalloc(vb * B_);
   wc_ = (zd*)dalloc(vb * B_);
   wx_ = (zd*)dalloc(vb * B_);
+  if (opt_.field_start == 2) dtot_ = (zd*)dalloc(vb * T_);
   nblk_ = (int)std::min<long>(1024, std::max<long>(1, (N_ + kBlock * 4 - 1) / (kBlock * 4)));
   part_ = (double*)dalloc(sizeof(double) * kSums * nblk_);
   DB_HIP(hipHostMalloc((void**)&part_h_, sizeof(double) * kSums * nblk_, hipHostMallocDefault));
@@ -322,9 +329,10 @@ void Dbim::iterate(DbimStats& st) {
     dim3 grid(nblk_, nb);
     k_deriv_src<<<grid, blk, 0, s>>>(tot_ + (size_t)t0 * N, ot_, wy_, N, k0_);
     eng_.mlfma(wy_, wc_, nb);
-    DB_HIP(hipMemsetAsync(wx_, 0, sizeof(zd) * N * nb, s));
-    solve(wx_, wc_, false, nb, true, st);
-    k_add_gx<<<grid, blk, 0, s>>>(wy_, g_, wx_, N);
+    zd* dx = dtot_ ? dtot_ + (size_t)t0 * N : wx_;  // kept for the first-order field start
+    DB_HIP(hipMemsetAsync(dx, 0, sizeof(zd) * N * nb, s));
+    solve(dx, wc_, false, nb, true, st);
+    k_add_gx<<<grid, blk, 0, s>>>(wy_, g_, dx, N);
     eng_.aggregate(wy_, rhs_ + (size_t)t0 * R_, nb);
   }
   std::vector<zd> rhs((size_t)T_ * R_), mesd((size_t)T_ * R_);
@@ -347,6 +355,8 @@ void Dbim::iterate(DbimStats& st) {
     zd* tot = tot_ + (size_t)t0 * N;
     bool zero = opt_.field_start == 0;
     if (zero) DB_HIP(hipMemsetAsync(tot, 0, sizeof(zd) * N * nb, s));
+    // o moved by -alpha ot, so to first order tot moves by -alpha dtot
+    if (dtot_) k_axpy_rows<<<dim3(nblk_, nb), blk, 0, s>>>(tot, dtot_ + (size_t)t0 * N, mkc<double>(-alpha, 0.0), N);
     solve(tot, inc_ + (size_t)t0 * N, false, nb, zero, st);
     k_rowmul<<<dim3(nblk_, nb), blk, 0, s>>>(tot, g_, wx_, N);
     eng_.aggregate(wx_, rhs_ + (size_t)t0 * R_, nb);

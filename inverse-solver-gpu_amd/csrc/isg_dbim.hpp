@@ -18,7 +18,11 @@ struct DbimOptions {
   double regular = 0.0;   // gradient adds regular^2 * o per transmitter (mom.cpp:267-269)
   double tol = 1e-4;      // BiCGSTAB relative residual (mom.cpp:21)
   int maxit = 300;        // BiCGSTAB iteration limit (mom.cpp:22)
-  int field_start = 0;    // total-field update start: 0 zero (mom.cpp:324), 1 previous total field
+  // total-field update start: 0 zero (mom.cpp:324), 1 previous total field,
+  // 2 previous total field + first-order update -alpha * dtot, where dtot is the
+  // Frechet derivative field the step-length phase already solved for
+  // ((I - G g) dtot = G(k0^2 ot tot), mom.cpp:284-290)
+  int field_start = 0;
   int precision = 0;      // solver operator: PREC_FP64 or PREC_MIXED
 };
 
@@ -68,6 +72,7 @@ class Dbim {
   zd *inc_ = nullptr, *tot_ = nullptr;
   zd *mes_ = nullptr, *mesd_ = nullptr, *rhs_ = nullptr;
   zd *wy_ = nullptr, *wc_ = nullptr, *wx_ = nullptr;
+  zd* dtot_ = nullptr;  // [numtx][N] derivative fields, field_start 2 only
   double* part_ = nullptr;
   double* part_h_ = nullptr;
   int nblk_ = 0;

@@ -577,7 +577,7 @@ void Engine::bicgs(zd* x, const zd* g, long gs, const zd* b, bool herm, int nrhs
     return;
   }
   const bool mixed = prec == PREC_MIXED && has32_;
-  for (int i = 0; i < nrhs; i++) tol_h_[i] = tol;
+  for (int i = 0; i < nrhs; i++) tol_h_[i] = mixed ? cfg_.mixed_margin * tol : tol;
   run_bicgs(x, g, gs, b, herm, nrhs, x_zero, maxit, mixed ? PREC_MIXED : PREC_FP64);
   if (!ok()) return;
   for (int i = 0; i < nrhs; i++) {
@@ -604,9 +604,9 @@ void Engine::bicgs(zd* x, const zd* g, long gs, const zd* b, bool herm, int nrhs
           out[i].status = 0;
         else if (!broke)
           out[i].status = 1;
-        // correction target relative to |res|, with a factor-2 margin; systems
+        // correction target relative to |res|, with the same margin; systems
         // that are done get an unreachable target and stay inactive
-        tol_h_[i] = need ? 0.5 * tol * bnrm / std::max(rn[i], 1e-300) : 1e300;
+        tol_h_[i] = need ? cfg_.mixed_margin * tol * bnrm / std::max(rn[i], 1e-300) : 1e300;
         any = any || need;
       }
       if (!any || round == kRefineRounds) break;

@@ -28,6 +28,7 @@ struct EngineConfig {
   PlanConfig plan;
   int max_rhs = 1;        // right-hand sides solved together
   int precision = 0;      // PREC_MIXED also builds the FP32 operator
+  double mixed_margin = 0.5;  // FP32 solve target = mixed_margin * tol before the FP64 check
 };
 
 // Result of one BiCGSTAB solve (one right-hand side).

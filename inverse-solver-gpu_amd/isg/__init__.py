@@ -39,6 +39,7 @@ class Config(ctypes.Structure):
         ("regular", ctypes.c_double),
         ("max_rhs", ctypes.c_int),
         ("precision", ctypes.c_int),
+        ("mixed_margin", ctypes.c_double),
     ]
 
 

@@ -33,7 +33,7 @@ def test_defaults_are_reference_constants():
     c = isg.default_config()
     assert (c.res, c.box, c.level, c.beta, c.ninter) == (0.1, 8, 7, 5, 30)
     assert (c.toli, c.iti, c.regular) == (1e-4, 300, 0.0)
-    assert c.max_rhs == 1 and c.precision == 0
+    assert c.max_rhs == 1 and c.precision == 0 and c.mixed_margin == 0.5
 
 
 @pytest.mark.skipif(gpu_present(), reason="checks the no-device error path")
@@ -48,6 +48,8 @@ def test_bad_config_rejected():
         isg.Solver(level=4, box=4)
     with pytest.raises(isg.IsgError):
         isg.Solver(level=4, precision=7)
+    with pytest.raises(isg.IsgError):
+        isg.Solver(level=4, precision=1, mixed_margin=0.0)
 
 
 class SplitMix64:
