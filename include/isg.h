@@ -145,6 +145,10 @@ typedef struct isg_dbim_stats {
   long long solves, solve_iters, matvecs, not_converged;
   long long matvecs_fp32;  /* of matvecs, those with the FP32 operator */
   double seconds;        /* wall time of the call, device work included */
+  long long field_iters; /* BiCGSTAB iterations of the total-field update phase (mom.cpp:322-336) */
+  double field_seconds;  /* wall time of that phase (network and blend included) */
+  double lambda_abs;     /* learned start: mean |lambda| of the blend */
+  double net_seconds;    /* learned start: network inference time */
 } isg_dbim_stats;
 
 /* Incident plane waves for numtx transmitters (setup_born.cpp:62-65), synthetic
@@ -153,7 +157,10 @@ typedef struct isg_dbim_stats {
  * (mom.cpp:222-276).  field_start: 0 = every total-field solve starts from zero
  * as in the reference (mom.cpp:324), 1 = from the previous total field, 2 = from
  * the previous total field moved to first order along the step: tot - alpha dtot,
- * dtot being the derivative field of the step-length solve (mom.cpp:284-290). */
+ * dtot being the derivative field of the step-length solve (mom.cpp:284-290),
+ * 3 = learned: tot_prev + lambda d with d the warm-start network's correction and
+ * lambda minimising the initial residual per transmitter (needs
+ * isg_dbim_set_warmnet before isg_dbim_iterate). */
 int isg_dbim_setup(isg_context* ctx, const double* o_true, int field_start, isg_dbim_stats* st);
 /* Same start from given measurements mes (numtx x numrx complex, the reference's
  * mes array after setup_born.cpp:148) instead of synthesizing them; o_true may be
@@ -177,6 +184,41 @@ int isg_dbim_set(isg_context* ctx, int which, const double* in);
  * isg_profile(ctx, 1) clears the totals and starts recording. */
 int isg_profile(isg_context* ctx, int enable);
 int isg_profile_read(isg_context* ctx, int kernel, double* total_ms, long long* launches);
+
+/* ---- learned warm start of the total-field solves (no counterpart in the
+ * reference, whose total-field solves restart from zero, mom.cpp:324) ----
+ * A 4-level convolutional encoder-decoder (32 base channels, BF16 weights,
+ * FP32 accumulation) mapping per transmitter (Re/Im contrast, Re/Im incident
+ * field, Re/Im previous total field) to a 2-channel correction of the previous
+ * total field.  Images are raster order, NHWC FP32, H and W multiples of 64.
+ * Calls without a context run on the null stream and return after the work. */
+typedef struct isg_warmnet isg_warmnet;
+/* Workspace for max_batch images of max_pixels pixels (multiple of 4096);
+ * He-normal weights from seed. */
+int isg_warmnet_create(int max_batch, long max_pixels, unsigned long long seed, isg_warmnet** out);
+void isg_warmnet_destroy(isg_warmnet* net);
+const char* isg_warmnet_last_error(const isg_warmnet* net);
+/* Flat FP32 parameters: per layer W[ks*ks][cout][cin] then bias[cout]. */
+long isg_warmnet_num_params(void);
+int isg_warmnet_get_params(isg_warmnet* net, float* out);
+int isg_warmnet_set_params(isg_warmnet* net, const float* in);  /* also resets Adam */
+int isg_warmnet_get_grads(isg_warmnet* net, float* out);
+/* 7 scales: input channel c is multiplied by s[c] (c < 6), the output by s[6]. */
+int isg_warmnet_set_scales(isg_warmnet* net, const float* s);
+int isg_warmnet_get_scales(isg_warmnet* net, float* s);
+/* in [batch][H][W][6], out [batch][H][W][2], device pointers */
+int isg_warmnet_forward_dev(isg_warmnet* net, const void* in, void* out, int batch, int H, int W);
+/* forward + backward of mean((out - target)^2) over the 2 output channels;
+ * gradients kept for isg_warmnet_adam / isg_warmnet_get_grads */
+int isg_warmnet_loss_grad_dev(isg_warmnet* net, const void* in, const void* target, int batch, int H, int W,
+                              double* loss);
+/* Adam step (beta1 0.9, beta2 0.999, eps 1e-8) with learning rate lr */
+int isg_warmnet_adam(isg_warmnet* net, float lr);
+/* weight file: "ISGWNET1", uint32 count, 7 float scales, float parameters */
+int isg_warmnet_save(isg_warmnet* net, const char* path);
+int isg_warmnet_load(isg_warmnet* net, const char* path);
+/* Network used by field_start 3 of this context's DBIM (not owned). */
+int isg_dbim_set_warmnet(isg_context* ctx, isg_warmnet* net);
 
 /* ---- seeded phantoms (raster order, 2*side*side doubles) ---- */
 /* n Gaussian blobs, centres in the inner 80%, sigma U(0.3,1), amplitude U(0.1,0.5)+iU(0.01,0.05) */

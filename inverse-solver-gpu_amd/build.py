@@ -30,6 +30,7 @@ SOURCES = [
     ("isg_phantom.cpp", False),
     ("isg_engine.hip", True),
     ("isg_dbim.hip", True),
+    ("isg_warmnet.hip", True),
     ("isg_capi.cpp", True),
 ]
 

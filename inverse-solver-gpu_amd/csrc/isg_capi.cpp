@@ -9,6 +9,12 @@
 #include "../../include/isg.h"
 #include "isg_dbim.hpp"
 #include "isg_engine.hpp"
+#include "isg_warmnet.hpp"
+
+struct isg_warmnet {
+  std::unique_ptr<isg::WarmNet> net;
+  std::string err;
+};
 
 struct isg_context {
   isg_config cfg;
@@ -307,11 +313,15 @@ static void fill_stats(const isg::DbimStats& s, isg_dbim_stats* o) {
   o->matvecs_fp32 = s.matvecs32;
   o->not_converged = s.not_converged;
   o->seconds = s.seconds;
+  o->field_iters = s.field_iters;
+  o->field_seconds = s.field_seconds;
+  o->lambda_abs = s.lambda_abs;
+  o->net_seconds = s.net_seconds;
 }
 
 int isg_dbim_setup(isg_context* c, const double* o_true, int field_start, isg_dbim_stats* st) {
   if (!check(c)) return ISG_E_STATE;
-  if (!o_true || field_start < 0 || field_start > 2) return fail(c, ISG_E_ARG, "bad argument");
+  if (!o_true || field_start < 0 || field_start > 3) return fail(c, ISG_E_ARG, "bad argument");
   isg::DbimOptions opt;
   opt.numtx = c->cfg.numtx;
   opt.regular = c->cfg.regular;
@@ -332,7 +342,7 @@ int isg_dbim_setup(isg_context* c, const double* o_true, int field_start, isg_db
 int isg_dbim_setup_measured(isg_context* c, const double* mes, const double* o_true, int field_start,
                             isg_dbim_stats* st) {
   if (!check(c)) return ISG_E_STATE;
-  if (!mes || field_start < 0 || field_start > 2) return fail(c, ISG_E_ARG, "bad argument");
+  if (!mes || field_start < 0 || field_start > 3) return fail(c, ISG_E_ARG, "bad argument");
   isg::DbimOptions opt;
   opt.numtx = c->cfg.numtx;
   opt.regular = c->cfg.regular;
@@ -371,6 +381,122 @@ int isg_dbim_iterate(isg_context* c, isg_dbim_stats* st) {
   if (!c->dbim->ok()) return fail(c, ISG_E_DEVICE, c->dbim->error());
   fill_stats(s, st);
   return ISG_OK;
+}
+
+int isg_dbim_set_warmnet(isg_context* c, isg_warmnet* n) {
+  if (!check(c)) return ISG_E_STATE;
+  if (!c->dbim) return fail(c, ISG_E_STATE, "isg_dbim_setup has not been called");
+  if (!n || !n->net || !n->net->ok()) return fail(c, ISG_E_ARG, "no usable warm-start network");
+  c->dbim->set_warmnet(n->net.get());
+  if (!c->dbim->ok()) return fail(c, ISG_E_ARG, c->dbim->error());
+  return ISG_OK;
+}
+
+// ---------------------------------------------------------------- warm-start network
+static int net_fail(isg_warmnet* n, int code, const std::string& msg) {
+  if (n) n->err = msg;
+  return code;
+}
+
+static int net_check(isg_warmnet* n) {
+  if (!n || !n->net) return ISG_E_STATE;
+  if (!n->net->ok()) return net_fail(n, ISG_E_DEVICE, n->net->error());
+  return ISG_OK;
+}
+
+int isg_warmnet_create(int max_batch, long max_pixels, unsigned long long seed, isg_warmnet** out) {
+  if (!out) return ISG_E_ARG;
+  *out = nullptr;
+  isg_warmnet* n = new isg_warmnet();
+  n->net.reset(new isg::WarmNet(max_batch, max_pixels, seed));
+  *out = n;
+  if (!n->net->ok()) {
+    n->err = n->net->error();
+    return n->err.find("no HIP device") != std::string::npos ? ISG_E_DEVICE : ISG_E_ARG;
+  }
+  return ISG_OK;
+}
+
+void isg_warmnet_destroy(isg_warmnet* n) { delete n; }
+
+const char* isg_warmnet_last_error(const isg_warmnet* n) { return n ? n->err.c_str() : "null network"; }
+
+long isg_warmnet_num_params(void) { return (long)isg::WarmNet::num_params(); }
+
+int isg_warmnet_get_params(isg_warmnet* n, float* out) {
+  if (net_check(n) != ISG_OK) return ISG_E_STATE;
+  if (!out) return net_fail(n, ISG_E_ARG, "null pointer");
+  std::vector<float> w;
+  n->net->get_params(w);
+  std::memcpy(out, w.data(), w.size() * sizeof(float));
+  return net_check(n);
+}
+
+int isg_warmnet_set_params(isg_warmnet* n, const float* in) {
+  if (net_check(n) != ISG_OK) return ISG_E_STATE;
+  if (!in) return net_fail(n, ISG_E_ARG, "null pointer");
+  std::vector<float> w(in, in + isg::WarmNet::num_params());
+  n->net->set_params(w);
+  return net_check(n);
+}
+
+int isg_warmnet_get_grads(isg_warmnet* n, float* out) {
+  if (net_check(n) != ISG_OK) return ISG_E_STATE;
+  if (!out) return net_fail(n, ISG_E_ARG, "null pointer");
+  std::vector<float> g;
+  n->net->get_grads(g);
+  std::memcpy(out, g.data(), g.size() * sizeof(float));
+  return net_check(n);
+}
+
+int isg_warmnet_set_scales(isg_warmnet* n, const float* s) {
+  if (net_check(n) != ISG_OK) return ISG_E_STATE;
+  if (!s) return net_fail(n, ISG_E_ARG, "null pointer");
+  n->net->set_scales(s);
+  return ISG_OK;
+}
+
+int isg_warmnet_get_scales(isg_warmnet* n, float* s) {
+  if (net_check(n) != ISG_OK) return ISG_E_STATE;
+  if (!s) return net_fail(n, ISG_E_ARG, "null pointer");
+  std::memcpy(s, n->net->scales(), sizeof(float) * isg::WarmNet::kScales);
+  return ISG_OK;
+}
+
+int isg_warmnet_forward_dev(isg_warmnet* n, const void* in, void* out, int batch, int H, int W) {
+  if (net_check(n) != ISG_OK) return ISG_E_STATE;
+  if (!in || !out) return net_fail(n, ISG_E_ARG, "null pointer");
+  n->net->forward((const float*)in, (float*)out, batch, H, W, nullptr);
+  if (net_check(n) != ISG_OK) return ISG_E_ARG;
+  if (hipStreamSynchronize(nullptr) != hipSuccess) return net_fail(n, ISG_E_DEVICE, "stream sync");
+  return net_check(n);
+}
+
+int isg_warmnet_loss_grad_dev(isg_warmnet* n, const void* in, const void* target, int batch, int H, int W,
+                              double* loss) {
+  if (net_check(n) != ISG_OK) return ISG_E_STATE;
+  if (!in || !target || !loss) return net_fail(n, ISG_E_ARG, "null pointer");
+  *loss = n->net->loss_grad((const float*)in, (const float*)target, batch, H, W, nullptr);
+  return net_check(n) == ISG_OK ? ISG_OK : ISG_E_ARG;
+}
+
+int isg_warmnet_adam(isg_warmnet* n, float lr) {
+  if (net_check(n) != ISG_OK) return ISG_E_STATE;
+  n->net->adam(lr, nullptr);
+  if (hipStreamSynchronize(nullptr) != hipSuccess) return net_fail(n, ISG_E_DEVICE, "stream sync");
+  return net_check(n);
+}
+
+int isg_warmnet_save(isg_warmnet* n, const char* path) {
+  if (net_check(n) != ISG_OK) return ISG_E_STATE;
+  if (!path) return net_fail(n, ISG_E_ARG, "null pointer");
+  return n->net->save(path) ? ISG_OK : net_fail(n, ISG_E_ARG, n->net->error());
+}
+
+int isg_warmnet_load(isg_warmnet* n, const char* path) {
+  if (net_check(n) != ISG_OK) return ISG_E_STATE;
+  if (!path) return net_fail(n, ISG_E_ARG, "null pointer");
+  return n->net->load(path) ? ISG_OK : net_fail(n, ISG_E_ARG, n->net->error());
 }
 
 int isg_dbim_get(isg_context* c, int which, double* out) {

@@ -100,6 +100,33 @@ __global__ void __launch_bounds__(kBlock) k_axpy_rows(zd* a, const zd* b, zd s, 
   DB_LOOP(N) a[t * N + i] = cadd(a[t * N + i], cmul(s, b[t * N + i]));
 }
 
+// Network input of transmitters t0..t0+nc-1 in raster order:
+// in[t][q] = (o, inc_t, tot_t)[r2t[q]] as 6 floats.
+__global__ void __launch_bounds__(kBlock) k_net_in(const zd* o, const zd* inc, const zd* tot, const int* r2t,
+                                                   float* in, long N) {
+  int t = blockIdx.y;
+  DB_LOOP(N) {
+    int n = r2t[i];
+    zd a = o[n], b = inc[t * N + n], c = tot[t * N + n];
+    float* p = in + (t * N + i) * 6;
+    p[0] = (float)a.x;
+    p[1] = (float)a.y;
+    p[2] = (float)b.x;
+    p[3] = (float)b.y;
+    p[4] = (float)c.x;
+    p[5] = (float)c.y;
+  }
+}
+
+// d[t][r2t[q]] = out[t][q] (complex from 2 floats)
+__global__ void __launch_bounds__(kBlock) k_net_dir(const float* out, const int* r2t, zd* d, long N) {
+  int t = blockIdx.y;
+  DB_LOOP(N) {
+    const float* p = out + (t * N + i) * 2;
+    d[t * N + r2t[i]] = mkc<double>(p[0], p[1]);
+  }
+}
+
 // ot = od + beta ot
 __global__ void __launch_bounds__(kBlock) k_newdir(zd* ot, const zd* od, zd beta, long N) {
   DB_LOOP(N) ot[i] = cadd(od[i], cmul(beta, ot[i]));
@@ -350,6 +377,9 @@ void Dbim::iterate(DbimStats& st) {
   k_axpy<<<dim3(nblk_), blk, 0, s>>>(o_, ot_, mkc<double>(-alpha, 0.0), N);
   k_weight<<<dim3(nblk_), blk, 0, s>>>(o_, g_, N, k0_);
   // total-field update for the new contrast (mom.cpp:322-336)
+  DB_HIP(hipStreamSynchronize(s));
+  const auto t_field = std::chrono::steady_clock::now();
+  const long long it_field = st.solve_iters;
   for (int t0 = 0; t0 < T_; t0 += B_) {
     int nb = std::min(B_, T_ - t0);
     zd* tot = tot_ + (size_t)t0 * N;
@@ -357,10 +387,17 @@ void Dbim::iterate(DbimStats& st) {
     if (zero) DB_HIP(hipMemsetAsync(tot, 0, sizeof(zd) * N * nb, s));
     // o moved by -alpha ot, so to first order tot moves by -alpha dtot
     if (dtot_) k_axpy_rows<<<dim3(nblk_, nb), blk, 0, s>>>(tot, dtot_ + (size_t)t0 * N, mkc<double>(-alpha, 0.0), N);
-    solve(tot, inc_ + (size_t)t0 * N, false, nb, zero, st);
+    if (opt_.field_start == 3) {
+      learned_solve(tot, inc_ + (size_t)t0 * N, nb, st);
+    } else {
+      solve(tot, inc_ + (size_t)t0 * N, false, nb, zero, st);
+    }
     k_rowmul<<<dim3(nblk_, nb), blk, 0, s>>>(tot, g_, wx_, N);
     eng_.aggregate(wx_, rhs_ + (size_t)t0 * R_, nb);
   }
+  DB_HIP(hipStreamSynchronize(s));
+  st.field_iters = st.solve_iters - it_field;
+  st.field_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_field).count();
   DB_HIP(hipMemcpyAsync(rhs.data(), rhs_, sizeof(zd) * rhs.size(), hipMemcpyDeviceToHost, s));
   std::vector<zd> mes((size_t)T_ * R_);
   DB_HIP(hipMemcpyAsync(mes.data(), mes_, sizeof(zd) * mes.size(), hipMemcpyDeviceToHost, s));
@@ -386,6 +423,66 @@ void Dbim::iterate(DbimStats& st) {
   st.matvecs = eng_.matvecs - mv0;
   st.matvecs32 = eng_.matvecs32 - mv32;
   st.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+}
+
+void Dbim::set_warmnet(WarmNet* net) {
+  net_ = net;
+  if (!net || !ok()) return;
+  const Plan& P = eng_.plan();
+  const long side = (long)P.side;
+  if (side % 64 != 0 || side * side > net->max_pixels()) {
+    err_ = "warm-start network workspace does not fit this domain (side multiple of 64 needed)";
+    return;
+  }
+  if (!r2t_) {
+    r2t_ = (int*)dalloc(sizeof(int) * N_);
+    wdir_ = (zd*)dalloc(sizeof(zd) * (size_t)N_ * B_);
+    if (!ok()) return;
+    DB_HIP(hipMemcpy(r2t_, P.raster2tree.data(), sizeof(int) * N_, hipMemcpyHostToDevice));
+  }
+  if (net->max_batch() > net_chunk_) {
+    net_chunk_ = std::min(net->max_batch(), B_);
+    net_in_ = (float*)dalloc(sizeof(float) * 6 * (size_t)N_ * net_chunk_);
+    net_out_ = (float*)dalloc(sizeof(float) * 2 * (size_t)N_ * net_chunk_);
+  }
+}
+
+// Learned start (field_start 3): per chunk of transmitters, pack
+// (o, inc_t, tot_t) into raster images, run the network, scatter its
+// correction back to tree order, then one blended solve for the batch.
+void Dbim::learned_solve(zd* tot, const zd* inc, int nb, DbimStats& st) {
+  if (!net_ || !net_in_) {
+    if (err_.empty()) err_ = "field_start 3 needs a warm-start network (isg_dbim_set_warmnet)";
+    return;
+  }
+  hipStream_t s = eng_.stream();
+  const long N = N_;
+  const int side = eng_.plan().side;
+  auto t0 = std::chrono::steady_clock::now();
+  for (int c0 = 0; c0 < nb; c0 += net_chunk_) {
+    int nc = std::min(net_chunk_, nb - c0);
+    k_net_in<<<dim3(nblk_, nc), dim3(kBlock), 0, s>>>(o_, inc + (size_t)c0 * N, tot + (size_t)c0 * N, r2t_,
+                                                       net_in_, N);
+    net_->forward(net_in_, net_out_, nc, side, side, s);
+    if (!net_->ok()) {
+      if (err_.empty()) err_ = net_->error();
+      return;
+    }
+    k_net_dir<<<dim3(nblk_, nc), dim3(kBlock), 0, s>>>(net_out_, r2t_, wdir_ + (size_t)c0 * N, N);
+  }
+  DB_HIP(hipStreamSynchronize(s));
+  st.net_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  std::vector<SolveStat> ss;
+  std::vector<zd> lam;
+  eng_.bicgs_blend(tot, wdir_, g_, 0, inc, false, nb, opt_.tol, opt_.maxit, ss, opt_.precision, &lam);
+  double la = 0;
+  for (auto& l : lam) la += std::sqrt(l.x * l.x + l.y * l.y);
+  st.lambda_abs = lam.empty() ? 0 : la / lam.size();
+  for (auto& q : ss) {
+    st.solves++;
+    st.solve_iters += q.iters;
+    if (q.status != 0) st.not_converged++;
+  }
 }
 
 bool Dbim::get(int which, std::vector<zd>& out) {

@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "isg_engine.hpp"
+#include "isg_warmnet.hpp"
 
 namespace isg {
 
@@ -21,7 +22,9 @@ struct DbimOptions {
   // total-field update start: 0 zero (mom.cpp:324), 1 previous total field,
   // 2 previous total field + first-order update -alpha * dtot, where dtot is the
   // Frechet derivative field the step-length phase already solved for
-  // ((I - G g) dtot = G(k0^2 ot tot), mom.cpp:284-290)
+  // ((I - G g) dtot = G(k0^2 ot tot), mom.cpp:284-290), 3 learned: the
+  // warm-start network's correction d of the previous total field, started
+  // from tot_prev + lambda d with the residual-minimising lambda
   int field_start = 0;
   int precision = 0;      // solver operator: PREC_FP64 or PREC_MIXED
 };
@@ -32,6 +35,11 @@ struct DbimStats {
   double beta_re = 0, beta_im = 0;   // Polak-Ribiere coefficient (mom.cpp:394)
   long long solves = 0, solve_iters = 0, matvecs = 0, not_converged = 0, matvecs32 = 0;
   double seconds = 0;
+  // total-field update phase (mom.cpp:322-336) alone: BiCGSTAB iterations,
+  // wall time (network inference and blend included) and, for the learned
+  // start, the mean |lambda| and the network time
+  long long field_iters = 0;
+  double field_seconds = 0, lambda_abs = 0, net_seconds = 0;
 };
 
 class Dbim {
@@ -53,12 +61,15 @@ class Dbim {
   bool get(int which, std::vector<zd>& out);
   bool set(int which, const std::vector<zd>& in);
   int iteration() const { return iter_; }
+  // Network used by field_start 3 (not owned; must outlive the iterations).
+  void set_warmnet(WarmNet* net);
 
  private:
   void start(const zd* mes_host, const zd* otrue_host, DbimStats& st);
   void gradient(DbimStats& st);
   void metrics(DbimStats& st);
   void solve(zd* x, const zd* b, bool herm, int nb, bool x_zero, DbimStats& st);
+  void learned_solve(zd* tot, const zd* inc, int nb, DbimStats& st);
   void* dalloc(size_t bytes);
 
   Engine& eng_;
@@ -73,6 +84,11 @@ class Dbim {
   zd *mes_ = nullptr, *mesd_ = nullptr, *rhs_ = nullptr;
   zd *wy_ = nullptr, *wc_ = nullptr, *wx_ = nullptr;
   zd* dtot_ = nullptr;  // [numtx][N] derivative fields, field_start 2 only
+  WarmNet* net_ = nullptr;
+  int net_chunk_ = 0;
+  float *net_in_ = nullptr, *net_out_ = nullptr;  // [chunk][side][side][6], [..][2]
+  zd* wdir_ = nullptr;                             // [max_rhs][N] learned corrections
+  int* r2t_ = nullptr;                             // raster -> tree index
   double* part_ = nullptr;
   double* part_h_ = nullptr;
   int nblk_ = 0;

@@ -164,6 +164,7 @@ Engine::Engine(const EngineConfig& cfg) : cfg_(cfg) {
   state_ = (BiState*)dalloc((size_t)R * sizeof(BiState));
   active_ = (int*)dalloc((size_t)R * sizeof(int));
   tolv_ = (double*)dalloc((size_t)R * sizeof(double));
+  lam_ = (zd*)dalloc((size_t)R * sizeof(zd));
   ISG_HIP(hipHostMalloc((void**)&active_h_, (size_t)R * sizeof(int), hipHostMallocDefault));
   ISG_HIP(hipHostMalloc((void**)&state_h_, (size_t)R * sizeof(BiState), hipHostMallocDefault));
   ISG_HIP(hipHostMalloc((void**)&tol_h_, (size_t)R * sizeof(double), hipHostMallocDefault));
@@ -497,7 +498,7 @@ void Engine::profile_read(int id, double& ms, long long& launches) {
 // scalars and per-system tolerances (tol_h_ -> tolv_); the host reads only
 // the per-system activity flags once per iteration.
 void Engine::run_bicgs(zd* x, const zd* g, long gs, const zd* b, bool herm, int nrhs, bool x_zero, int maxit,
-                       int prec) {
+                       int prec, const zd* r0) {
   const long N = P_.N;
   zd *r = vec_[V_R], *rt = vec_[V_RT], *p = vec_[V_P], *v = vec_[V_V], *s = vec_[V_S], *t = vec_[V_T],
      *y = vec_[V_Y], *op = vec_[V_OP];
@@ -505,13 +506,13 @@ void Engine::run_bicgs(zd* x, const zd* g, long gs, const zd* b, bool herm, int 
   dim3 grid(nblk_, nrhs), blk(kBlock);
   ISG_HIP(hipMemcpyAsync(tolv_, tol_h_, nrhs * sizeof(double), hipMemcpyHostToDevice, stream_));
   const long long f32 = (prec == PREC_MIXED && has32_) ? 1 : 0;
-  if (!x_zero) {
+  if (!x_zero && !r0) {
     k_prep<<<grid, blk, 0, stream_>>>(x, g, gs, y, N, h);
     apply_mlfma(y, op, nrhs, nullptr, prec);
     matvecs += nrhs;
     matvecs32 += f32 * nrhs;
   }
-  k_bi_init<<<grid, blk, 0, stream_>>>(x, op, g, gs, b, r, rt, N, h, x_zero ? 1 : 0, part_);
+  k_bi_init<<<grid, blk, 0, stream_>>>(x, op, g, gs, b, r0, r, rt, N, h, x_zero ? 1 : 0, part_);
   k_bi_fin<<<dim3(nrhs), blk, 0, stream_>>>(PH_INIT, state_, active_, part_, nblk_, tolv_, maxit);
   for (int it = 0; it <= maxit; it++) {
     ISG_HIP(hipMemcpyAsync(active_h_, active_, nrhs * sizeof(int), hipMemcpyDeviceToHost, stream_));
@@ -550,7 +551,7 @@ void Engine::true_residual(const zd* x, const zd* g, long gs, const zd* b, bool 
   k_prep<<<grid, blk, 0, stream_>>>(x, g, gs, y, N, h);
   apply_mlfma(y, op, nrhs, nullptr, PREC_FP64);
   matvecs += nrhs;
-  k_bi_init<<<grid, blk, 0, stream_>>>(x, op, g, gs, b, res, vec_[V_RT], N, h, 0, part_);
+  k_bi_init<<<grid, blk, 0, stream_>>>(x, op, g, gs, b, nullptr, res, vec_[V_RT], N, h, 0, part_);
   ISG_HIP(hipMemcpyAsync(part_h_, part_, (size_t)nrhs * kSlots * nblk_ * sizeof(double2), hipMemcpyDeviceToHost,
                          stream_));
   ISG_HIP(hipStreamSynchronize(stream_));
@@ -570,6 +571,44 @@ void Engine::true_residual(const zd* x, const zd* g, long gs, const zd* b, bool 
 
 void Engine::bicgs(zd* x, const zd* g, long gs, const zd* b, bool herm, int nrhs, bool x_zero, double tol,
                    int maxit, std::vector<SolveStat>& out, int prec) {
+  solve(x, g, gs, b, herm, nrhs, x_zero, tol, maxit, out, prec, nullptr);
+}
+
+// r_x = b - A x and A d with the solver's operator, lambda = <Ad, r_x> / <Ad, Ad>,
+// x += lambda d, r0 = r_x - lambda Ad; BiCGSTAB then starts from (x, r0).
+void Engine::bicgs_blend(zd* x, const zd* d, const zd* g, long gs, const zd* b, bool herm, int nrhs, double tol,
+                         int maxit, std::vector<SolveStat>& out, int prec, std::vector<zd>* lam) {
+  out.assign(nrhs, SolveStat{-1, 0, 0.0});
+  if (!ok()) return;
+  if (nrhs > cfg_.max_rhs) {
+    set_error("bicgs: nrhs exceeds max_rhs");
+    return;
+  }
+  const long N = P_.N;
+  const int h = herm ? 1 : 0;
+  const int p = (prec == PREC_MIXED && has32_) ? PREC_MIXED : PREC_FP64;
+  dim3 grid(nblk_, nrhs), blk(kBlock);
+  zd *y = vec_[V_Y], *op = vec_[V_OP], *res = vec_[V_RES], *ad = vec_[V_COR];
+  k_prep<<<grid, blk, 0, stream_>>>(x, g, gs, y, N, h);
+  apply_mlfma(y, op, nrhs, nullptr, p);
+  k_bi_init<<<grid, blk, 0, stream_>>>(x, op, g, gs, b, nullptr, res, vec_[V_RT], N, h, 0, part_);
+  k_prep<<<grid, blk, 0, stream_>>>(d, g, gs, y, N, h);
+  apply_mlfma(y, op, nrhs, nullptr, p);
+  k_bl_dots<<<grid, blk, 0, stream_>>>(d, op, g, gs, res, ad, N, h, part_);
+  k_bl_fin<<<dim3(nrhs), blk, 0, stream_>>>(part_, nblk_, lam_);
+  k_bl_apply<<<grid, blk, 0, stream_>>>(x, d, res, ad, lam_, N);
+  matvecs += 2 * nrhs;
+  if (p == PREC_MIXED) matvecs32 += 2 * nrhs;
+  if (lam) {
+    lam->resize(nrhs);
+    ISG_HIP(hipMemcpyAsync(lam->data(), lam_, nrhs * sizeof(zd), hipMemcpyDeviceToHost, stream_));
+    ISG_HIP(hipStreamSynchronize(stream_));
+  }
+  solve(x, g, gs, b, herm, nrhs, false, tol, maxit, out, prec, res);
+}
+
+void Engine::solve(zd* x, const zd* g, long gs, const zd* b, bool herm, int nrhs, bool x_zero, double tol,
+                   int maxit, std::vector<SolveStat>& out, int prec, const zd* r0) {
   out.assign(nrhs, SolveStat{-1, 0, 0.0});
   if (!ok()) return;
   if (nrhs > cfg_.max_rhs) {
@@ -578,7 +617,7 @@ void Engine::bicgs(zd* x, const zd* g, long gs, const zd* b, bool herm, int nrhs
   }
   const bool mixed = prec == PREC_MIXED && has32_;
   for (int i = 0; i < nrhs; i++) tol_h_[i] = mixed ? cfg_.mixed_margin * tol : tol;
-  run_bicgs(x, g, gs, b, herm, nrhs, x_zero, maxit, mixed ? PREC_MIXED : PREC_FP64);
+  run_bicgs(x, g, gs, b, herm, nrhs, x_zero, maxit, mixed ? PREC_MIXED : PREC_FP64, r0);
   if (!ok()) return;
   for (int i = 0; i < nrhs; i++) {
     out[i].status = state_h_[i].status;

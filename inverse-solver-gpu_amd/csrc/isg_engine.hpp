@@ -86,6 +86,11 @@ class Engine {
   // x_zero tells the solver that x is all zeros (skips the first matvec).
   void bicgs(zd* x, const zd* g, long g_stride, const zd* b, bool herm, int nrhs, bool x_zero, double tol,
              int maxit, std::vector<SolveStat>& out, int prec = PREC_FP64);
+  // The same solve started from x + lambda d, lambda chosen per system to
+  // minimise |b - A (x + lambda d)| (so the start is never worse than x); two
+  // operator applications replace the usual one.  lam (host) receives lambda.
+  void bicgs_blend(zd* x, const zd* d, const zd* g, long g_stride, const zd* b, bool herm, int nrhs, double tol,
+                   int maxit, std::vector<SolveStat>& out, int prec, std::vector<zd>* lam);
 
   // multipoles (which=0) or locals (1) of `level` for one rhs, FP64 store
   void stage(int which, int level, int rhs, std::vector<zd>& out);
@@ -144,7 +149,10 @@ class Engine {
   std::vector<InterpShape>& m2m_shapes(bool fp32) { return fp32 ? m2m32_ : m2m64_; }
 
   void apply_mlfma(const zd* y, zd* r, int nrhs, const int* act, int prec);
-  void run_bicgs(zd* x, const zd* g, long gs, const zd* b, bool herm, int nrhs, bool x_zero, int maxit, int prec);
+  void solve(zd* x, const zd* g, long gs, const zd* b, bool herm, int nrhs, bool x_zero, double tol, int maxit,
+             std::vector<SolveStat>& out, int prec, const zd* r0);
+  void run_bicgs(zd* x, const zd* g, long gs, const zd* b, bool herm, int nrhs, bool x_zero, int maxit, int prec,
+                 const zd* r0 = nullptr);
   void true_residual(const zd* x, const zd* g, long gs, const zd* b, bool herm, int nrhs, zd* res,
                      std::vector<double>& rn, std::vector<double>& bn);
   void prof_begin(int id);
@@ -171,6 +179,7 @@ class Engine {
   BiState* state_ = nullptr;      // [max_rhs]
   int* active_ = nullptr;         // [max_rhs]
   double* tolv_ = nullptr;        // [max_rhs] per-system tolerance
+  zd* lam_ = nullptr;             // [max_rhs] blend coefficients
   int* active_h_ = nullptr;       // pinned host mirrors
   BiState* state_h_ = nullptr;
   double* tol_h_ = nullptr;

@@ -59,17 +59,24 @@ __device__ __forceinline__ void block_store(zd (&v)[NS], double2* part, int nblk
 
 #define ISG_VEC_LOOP(N) for (long i = (long)blockIdx.x * kBlock + threadIdx.x; i < (N); i += (long)gridDim.x * kBlock)
 
+// r0 (may be null) supplies a residual computed beforehand (blended start).
 __global__ void __launch_bounds__(kBlock) k_bi_init(const zd* __restrict__ x, const zd* __restrict__ gy,
                                                     const zd* __restrict__ g, long gs, const zd* __restrict__ b,
-                                                    zd* __restrict__ r, zd* __restrict__ rt, long N, int herm,
-                                                    int xzero, double2* __restrict__ part) {
+                                                    const zd* __restrict__ r0, zd* __restrict__ r,
+                                                    zd* __restrict__ rt, long N, int herm, int xzero,
+                                                    double2* __restrict__ part) {
   int rhs = blockIdx.y;
   long o = rhs * N;
   zd acc[2] = {mkc<double>(0, 0), mkc<double>(0, 0)};
   ISG_VEC_LOOP(N) {
     zd bv = b[o + i];
-    zd ax = xzero ? mkc<double>(0, 0) : post_val(x[o + i], gy[o + i], g[rhs * gs + i], herm);
-    zd rv = csub(bv, ax);
+    zd rv;
+    if (r0) {
+      rv = r0[o + i];
+    } else {
+      zd ax = xzero ? mkc<double>(0, 0) : post_val(x[o + i], gy[o + i], g[rhs * gs + i], herm);
+      rv = csub(bv, ax);
+    }
     r[o + i] = rv;
     rt[o + i] = rv;
     acc[0].x += bv.x * bv.x + bv.y * bv.y;
@@ -265,6 +272,63 @@ __global__ void __launch_bounds__(kBlock) k_bi_fin(int phase, BiState* __restric
     }
   }
   active[rhs] = S.active;
+}
+
+// Blended start: ad = A d (post of the operator product), partials
+// <ad, r> (= sum conj(ad) r) and <ad, ad>.
+__global__ void __launch_bounds__(kBlock) k_bl_dots(const zd* __restrict__ d, const zd* __restrict__ gy,
+                                                    const zd* __restrict__ g, long gs, const zd* __restrict__ r,
+                                                    zd* __restrict__ ad, long N, int herm,
+                                                    double2* __restrict__ part) {
+  int rhs = blockIdx.y;
+  long o = rhs * N;
+  zd acc[2] = {mkc<double>(0, 0), mkc<double>(0, 0)};
+  ISG_VEC_LOOP(N) {
+    zd a = post_val(d[o + i], gy[o + i], g[rhs * gs + i], herm);
+    ad[o + i] = a;
+    zd rv = r[o + i];
+    acc[0].x += a.x * rv.x + a.y * rv.y;
+    acc[0].y += a.x * rv.y - a.y * rv.x;
+    acc[1].x += a.x * a.x + a.y * a.y;
+  }
+  block_store<2>(acc, part + (long)rhs * kSlots * gridDim.x, gridDim.x, blockIdx.x);
+}
+
+// lambda = <ad, r> / <ad, ad> per right-hand side (0 when ad = 0), fixed-order sums
+__global__ void __launch_bounds__(kBlock) k_bl_fin(const double2* __restrict__ part, int nblk, zd* __restrict__ lam) {
+  int rhs = blockIdx.x;
+  const double2* pr = part + (long)rhs * kSlots * nblk;
+  __shared__ double sh[3][kBlock / 64];
+  double v[3] = {0, 0, 0};
+  for (int j = threadIdx.x; j < nblk; j += kBlock) {
+    v[0] += pr[j].x;
+    v[1] += pr[j].y;
+    v[2] += pr[nblk + j].x;
+  }
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int s = 0; s < 3; s++) {
+    double a = wave_sum(v[s]);
+    if (lane == 0) sh[s][w] = a;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  double t[3] = {0, 0, 0};
+  for (int s = 0; s < 3; s++)
+    for (int i = 0; i < kBlock / 64; i++) t[s] += sh[s][i];
+  lam[rhs] = t[2] > 0 ? mkc<double>(t[0] / t[2], t[1] / t[2]) : mkc<double>(0, 0);
+}
+
+// x += lambda d, r -= lambda ad
+__global__ void __launch_bounds__(kBlock) k_bl_apply(zd* __restrict__ x, const zd* __restrict__ d,
+                                                     zd* __restrict__ r, const zd* __restrict__ ad,
+                                                     const zd* __restrict__ lam, long N) {
+  int rhs = blockIdx.y;
+  long o = rhs * N;
+  zd l = lam[rhs], ml = mkc<double>(-l.x, -l.y);
+  ISG_VEC_LOOP(N) {
+    x[o + i] = cfma(l, d[o + i], x[o + i]);
+    r[o + i] = cfma(ml, ad[o + i], r[o + i]);
+  }
 }
 
 // x += c  (defect correction of the mixed-precision solve)

@@ -63,6 +63,10 @@ class DbimStats(ctypes.Structure):
         ("not_converged", ctypes.c_longlong),
         ("matvecs_fp32", ctypes.c_longlong),
         ("seconds", ctypes.c_double),
+        ("field_iters", ctypes.c_longlong),
+        ("field_seconds", ctypes.c_double),
+        ("lambda_abs", ctypes.c_double),
+        ("net_seconds", ctypes.c_double),
     ]
 
 
@@ -102,6 +106,21 @@ _SIGS = {
     "isg_dbim_iterate": (ctypes.c_int, [_V, _P(DbimStats)]),
     "isg_dbim_get": (ctypes.c_int, [_V, ctypes.c_int, _D]),
     "isg_dbim_set": (ctypes.c_int, [_V, ctypes.c_int, _D]),
+    "isg_warmnet_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_long, ctypes.c_ulonglong, _P(_V)]),
+    "isg_warmnet_destroy": (None, [_V]),
+    "isg_warmnet_last_error": (ctypes.c_char_p, [_V]),
+    "isg_warmnet_num_params": (ctypes.c_long, []),
+    "isg_warmnet_get_params": (ctypes.c_int, [_V, _P(ctypes.c_float)]),
+    "isg_warmnet_set_params": (ctypes.c_int, [_V, _P(ctypes.c_float)]),
+    "isg_warmnet_get_grads": (ctypes.c_int, [_V, _P(ctypes.c_float)]),
+    "isg_warmnet_set_scales": (ctypes.c_int, [_V, _P(ctypes.c_float)]),
+    "isg_warmnet_get_scales": (ctypes.c_int, [_V, _P(ctypes.c_float)]),
+    "isg_warmnet_forward_dev": (ctypes.c_int, [_V, _V, _V, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "isg_warmnet_loss_grad_dev": (ctypes.c_int, [_V, _V, _V, ctypes.c_int, ctypes.c_int, ctypes.c_int, _D]),
+    "isg_warmnet_adam": (ctypes.c_int, [_V, ctypes.c_float]),
+    "isg_warmnet_save": (ctypes.c_int, [_V, ctypes.c_char_p]),
+    "isg_warmnet_load": (ctypes.c_int, [_V, ctypes.c_char_p]),
+    "isg_dbim_set_warmnet": (ctypes.c_int, [_V, _V]),
     "isg_phantom_blobs": (ctypes.c_int, [ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_ulonglong, _D]),
     "isg_phantom_ellipses": (ctypes.c_int, [ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_ulonglong,
                                             ctypes.c_double, ctypes.c_double, ctypes.c_double, _D]),
@@ -327,6 +346,124 @@ class Dbim:
     def set(self, name, value):
         v = _c128(value, self._size(name))
         self.s._err(self.s.lib.isg_dbim_set(self.s.h, _DBIM_FIELDS[name], _dp(v)), "isg_dbim_set")
+
+    def set_warmnet(self, net):
+        """Network for field_start 3 (kept referenced by this object)."""
+        self._net = net
+        self.s._err(self.s.lib.isg_dbim_set_warmnet(self.s.h, net.h), "isg_dbim_set_warmnet")
+
+
+def _fp(a):
+    return a.ctypes.data_as(_P(ctypes.c_float))
+
+
+class WarmNet:
+    """Warm-start network on the GPU (csrc/isg_warmnet.hip): a 4-level conv
+    encoder-decoder from (Re/Im contrast, Re/Im incident, Re/Im previous total
+    field) to a 2-channel correction of the previous total field.  Images are
+    raster NHWC FP32 with H, W multiples of 64."""
+
+    NUM_SCALES = 7
+
+    def __init__(self, max_batch=4, max_pixels=512 * 512, seed=1, path=None):
+        self.lib = load()
+        self.h = _V()
+        rc = self.lib.isg_warmnet_create(int(max_batch), int(max_pixels), int(seed), ctypes.byref(self.h))
+        if rc != 0:
+            msg = self.lib.isg_warmnet_last_error(self.h).decode() if self.h else "isg_warmnet_create failed"
+            if self.h:
+                self.lib.isg_warmnet_destroy(self.h)
+                self.h = None
+            raise IsgError(msg)
+        self.max_batch, self.max_pixels = max_batch, max_pixels
+        if path is not None:
+            self.load(path)
+
+    def close(self):
+        if self.h:
+            self.lib.isg_warmnet_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _err(self, rc, what):
+        if rc != 0:
+            raise IsgError("%s: %s" % (what, self.lib.isg_warmnet_last_error(self.h).decode()))
+
+    @staticmethod
+    def num_params():
+        return int(load().isg_warmnet_num_params())
+
+    @property
+    def params(self):
+        out = np.empty(self.num_params(), np.float32)
+        self._err(self.lib.isg_warmnet_get_params(self.h, _fp(out)), "isg_warmnet_get_params")
+        return out
+
+    @params.setter
+    def params(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        if w.size != self.num_params():
+            raise ValueError("expected %d parameters" % self.num_params())
+        self._err(self.lib.isg_warmnet_set_params(self.h, _fp(w)), "isg_warmnet_set_params")
+
+    def grads(self):
+        out = np.empty(self.num_params(), np.float32)
+        self._err(self.lib.isg_warmnet_get_grads(self.h, _fp(out)), "isg_warmnet_get_grads")
+        return out
+
+    @property
+    def scales(self):
+        out = np.empty(self.NUM_SCALES, np.float32)
+        self._err(self.lib.isg_warmnet_get_scales(self.h, _fp(out)), "isg_warmnet_get_scales")
+        return out
+
+    @scales.setter
+    def scales(self, s):
+        s = np.ascontiguousarray(s, dtype=np.float32)
+        if s.size != self.NUM_SCALES:
+            raise ValueError("expected 7 scales")
+        self._err(self.lib.isg_warmnet_set_scales(self.h, _fp(s)), "isg_warmnet_set_scales")
+
+    def forward_dev(self, in_ptr, out_ptr, batch, H, W):
+        self._err(self.lib.isg_warmnet_forward_dev(self.h, _V(in_ptr), _V(out_ptr), batch, H, W),
+                  "isg_warmnet_forward_dev")
+
+    def loss_grad_dev(self, in_ptr, target_ptr, batch, H, W):
+        loss = ctypes.c_double()
+        self._err(self.lib.isg_warmnet_loss_grad_dev(self.h, _V(in_ptr), _V(target_ptr), batch, H, W,
+                                                     ctypes.byref(loss)), "isg_warmnet_loss_grad_dev")
+        return loss.value
+
+    def adam(self, lr):
+        self._err(self.lib.isg_warmnet_adam(self.h, float(lr)), "isg_warmnet_adam")
+
+    def save(self, path):
+        self._err(self.lib.isg_warmnet_save(self.h, str(path).encode()), "isg_warmnet_save")
+
+    def load(self, path):
+        self._err(self.lib.isg_warmnet_load(self.h, str(path).encode()), "isg_warmnet_load")
+
+    # host-array conveniences (device staging through torch tensors)
+    def forward(self, x):
+        """x: (batch, H, W, 6) float32 host array -> (batch, H, W, 2)."""
+        import torch
+        xt = torch.from_numpy(np.ascontiguousarray(x, dtype=np.float32)).cuda()
+        out = torch.empty(xt.shape[:3] + (2,), dtype=torch.float32, device=xt.device)
+        torch.cuda.synchronize()
+        self.forward_dev(xt.data_ptr(), out.data_ptr(), xt.shape[0], xt.shape[1], xt.shape[2])
+        return out.cpu().numpy()
+
+    def loss_grad(self, x, target):
+        import torch
+        xt = torch.from_numpy(np.ascontiguousarray(x, dtype=np.float32)).cuda()
+        tt = torch.from_numpy(np.ascontiguousarray(target, dtype=np.float32)).cuda()
+        torch.cuda.synchronize()
+        return self.loss_grad_dev(xt.data_ptr(), tt.data_ptr(), xt.shape[0], xt.shape[1], xt.shape[2])
 
 
 def phantom_blobs(side, n, seed, res=0.1):

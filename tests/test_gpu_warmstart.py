@@ -28,15 +28,17 @@ def case():
     return isg, otrue, ref.get("o")
 
 
-def run(isg, otrue, field_start, precision=0):
+def run(isg, otrue, field_start, precision=0, net=None, key="solve_iters"):
     s = isg.Solver(level=LEVEL, numrx=NRX, numtx=NTX, toli=TOL, iti=300, regular=REG, max_rhs=NTX,
                    precision=precision)
     d = isg.Dbim(s, otrue, field_start=field_start)
+    if net is not None:
+        d.set_warmnet(net)
     iters = 0
     for _ in range(3):
         st = d.iterate()
         assert st["not_converged"] == 0
-        iters += st["solve_iters"]
+        iters += st[key]
     return d.get("o"), iters
 
 
@@ -49,3 +51,25 @@ def test_first_order_start_matches_oracle_and_saves_iterations(case, precision):
     assert relerr(o_prev, o_ref) < 1e-4
     assert relerr(o_lin, o_ref) < 1e-4
     assert it_lin < it_prev, (it_lin, it_prev)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_learned_start_falls_back_with_untrained_network(case, precision):
+    """An untrained network gives a useless correction; the residual-minimising
+    blend must keep the solve at (about) the previous-iterate start's cost and
+    the inversion on the oracle's trajectory."""
+    isg, otrue, o_ref = case
+    print("learned start, untrained network, precision", precision, flush=True)
+    net = isg.WarmNet(max_batch=8, max_pixels=64 * 64, seed=5)
+    o_l, it_l = run(isg, otrue, 3, precision, net=net, key="field_iters")
+    _, it_p = run(isg, otrue, 1, precision, key="field_iters")
+    assert relerr(o_l, o_ref) < 1e-4
+    assert it_l <= it_p + 3 * NTX, (it_l, it_p)
+
+
+def test_learned_start_needs_a_network(case):
+    isg, otrue, _ = case
+    s = isg.Solver(level=LEVEL, numrx=NRX, numtx=NTX, toli=TOL, iti=300, regular=REG, max_rhs=NTX)
+    d = isg.Dbim(s, otrue, field_start=3)
+    with pytest.raises(isg.IsgError, match="warm-start network"):
+        d.iterate()
