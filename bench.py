@@ -42,7 +42,8 @@ PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3}
 PEAK_HBM_GBS = 8000.0
 
 # total-field start of the fast path (DbimOptions::field_start)
-START_KINDS = {"zero": 0, "previous": 1, "first-order": 2}
+START_KINDS = {"zero": 0, "previous": 1, "first-order": 2, "learned": 3}
+WARMNET_FILE = os.path.join(ROOT, "inverse-solver-gpu_amd", "weights", "warmnet_512.bin")
 
 WORKLOADS = {
     # name: level, transmitters, receivers, ellipses, seed
@@ -180,6 +181,8 @@ def main():
     if "fast" in paths:
         fast = isg.Solver(max_rhs=ntx, precision=fast_prec, mixed_margin=args.mixed_margin, **common)
         d_fast = isg.Dbim(fast, otrue, field_start=START_KINDS[args.fast_start], measurements=mes)
+        if args.fast_start == "learned":
+            d_fast.set_warmnet(isg.WarmNet(max_batch=8, max_pixels=side * side, path=WARMNET_FILE))
         res["fast"] = run_path(fast, d_fast, args.warmup, args.steps, "fast")
         del d_fast
         fast.close()
@@ -251,6 +254,7 @@ def main():
                          "avg_launch_ms": round(vec_ms / max(vec_n, 1), 4)},
         "solves": sum(s["solves"] for s in stats),
         "bicgstab_iterations": iters,
+        "field_bicgstab_iterations": sum(s["field_iters"] for s in stats),
         "matvecs": matvecs,
         "matvecs_fp32": mv32,
         "defect_corrections": stats[-1].get("refinements_timed"),
