@@ -169,11 +169,30 @@ int isg_dbim_setup_measured(isg_context* ctx, const double* mes, const double* o
                             isg_dbim_stats* st);
 /* One DBIM iteration (mom.cpp:277-400). */
 int isg_dbim_iterate(isg_context* ctx, isg_dbim_stats* st);
-/* which: 0 o, 1 od (gradient), 2 ot (direction), 3 mes, 4 mesd, 5 inc, 6 tot.
+/* which: 0 o, 1 od (gradient), 2 ot (direction), 3 mes, 4 mesd, 5 inc, 6 tot,
+ * 7 true contrast (get only).
  * Vectors are numunk (x numtx for inc/tot) complex; mes/mesd numtx x numrx.
  * isg_dbim_set accepts 0, 1, 2, 4 and 6 (checkpoint / resume). */
 int isg_dbim_get(isg_context* ctx, int which, double* out);
 int isg_dbim_set(isg_context* ctx, int which, const double* in);
+
+/* ---- the reference's files (setup_born.cpp:105-121, 171-242; mom.cpp:220) ---- */
+/* cg_<k>.bin (which 0: current contrast) or cg_ref.bin (which 7: true
+ * contrast): numunk complex doubles in raster order (replaces the unkmap
+ * gather + fwrite of write_error, setup_born.cpp:209-218). */
+int isg_dbim_write_cg(isg_context* ctx, const char* path, int which);
+/* contrast from a raster cg_<k>.bin (restart, mom.cpp:230-235 commented out upstream) */
+int isg_dbim_read_cg(isg_context* ctx, const char* path);
+/* hist.txt line "%e %e %e %e %e\n" (setup_born.cpp:239); create = 1 truncates first */
+int isg_hist_append(const char* path, const isg_dbim_stats* st, int create);
+/* phantom text file: numunk lines "%f", raster order; out = val * scale + 0i
+ * (the reference reads with scale 0.1, setup_born.cpp:105-112) */
+int isg_phantom_read_txt(const char* path, int numunk, double scale, double* raster_out);
+/* writes Re(raster[n]) / scale per line (the inverse of the reader) */
+int isg_phantom_write_txt(const char* path, int numunk, double scale, const double* raster_in);
+/* raw raster files of numunk complex doubles */
+int isg_raster_bin_write(const char* path, int numunk, const double* raster);
+int isg_raster_bin_read(const char* path, int numunk, double* raster);
 
 /* ---- live kernel timing (HIP events on the context's stream; the reference
  * times whole matvecs with cudaEvents, matvec_gpu.cu:117-122) ----

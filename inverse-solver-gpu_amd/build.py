@@ -28,6 +28,7 @@ HIPFLAGS = ["-x", "hip", "--offload-arch=" + ARCH, "-Xclang", "-target-feature",
 SOURCES = [
     ("isg_plan.cpp", False),
     ("isg_phantom.cpp", False),
+    ("isg_io.cpp", False),
     ("isg_engine.hip", True),
     ("isg_dbim.hip", True),
     ("isg_warmnet.hip", True),

@@ -509,6 +509,36 @@ int isg_dbim_get(isg_context* c, int which, double* out) {
   return ISG_OK;
 }
 
+// cg_<k>.bin / cg_ref.bin (setup_born.cpp:115-121, 209-218): the contrast in
+// raster order, n-th entry = tree entry unkmap[n]
+int isg_dbim_write_cg(isg_context* c, const char* path, int which) {
+  if (!check(c)) return ISG_E_STATE;
+  if (!c->dbim) return fail(c, ISG_E_STATE, "isg_dbim_setup has not been called");
+  if (!path || (which != 0 && which != 7)) return fail(c, ISG_E_ARG, "bad argument");
+  std::vector<isg::zd> v;
+  if (!c->dbim->get(which, v)) return fail(c, ISG_E_DEVICE, c->dbim->error());
+  const std::vector<int>& r2t = c->eng->plan().raster2tree;
+  std::vector<isg::zd> ras(v.size());
+  for (size_t q = 0; q < ras.size(); q++) ras[q] = v[r2t[q]];
+  if (isg_raster_bin_write(path, (int)ras.size(), (const double*)ras.data()) != ISG_OK)
+    return fail(c, ISG_E_ARG, std::string("cannot write ") + path);
+  return ISG_OK;
+}
+
+// The reference's commented-out restart path (mom.cpp:230-235): contrast read
+// from a raster cg_<k>.bin as the current object.
+int isg_dbim_read_cg(isg_context* c, const char* path) {
+  if (!check(c)) return ISG_E_STATE;
+  if (!c->dbim) return fail(c, ISG_E_STATE, "isg_dbim_setup has not been called");
+  const isg::Plan& P = c->eng->plan();
+  std::vector<isg::zd> ras(P.N), v(P.N);
+  if (!path || isg_raster_bin_read(path, P.N, (double*)ras.data()) != ISG_OK)
+    return fail(c, ISG_E_ARG, std::string("cannot read ") + (path ? path : "(null)"));
+  for (int q = 0; q < P.N; q++) v[P.raster2tree[q]] = ras[q];
+  if (!c->dbim->set(0, v)) return fail(c, ISG_E_DEVICE, c->dbim->error());
+  return ISG_OK;
+}
+
 int isg_dbim_set(isg_context* c, int which, const double* in) {
   if (!check(c)) return ISG_E_STATE;
   if (!c->dbim) return fail(c, ISG_E_STATE, "isg_dbim_setup has not been called");

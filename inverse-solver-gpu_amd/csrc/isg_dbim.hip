@@ -496,6 +496,7 @@ bool Dbim::get(int which, std::vector<zd>& out) {
     case 4: src = mesd_; n = (size_t)T_ * R_; break;
     case 5: src = inc_; n = (size_t)T_ * N_; break;
     case 6: src = tot_; n = (size_t)T_ * N_; break;
+    case 7: src = omes_; n = N_; break;
     default: return false;
   }
   out.resize(n);

@@ -57,7 +57,7 @@ class Dbim {
   void setup_measured(const zd* mes_host, const zd* otrue_host, DbimStats& st);
   // One DBIM iteration (mom.cpp:277-400).
   void iterate(DbimStats& st);
-  // which: 0 o, 1 od, 2 ot, 3 mes, 4 mesd, 5 inc, 6 tot ; copies to host
+  // which: 0 o, 1 od, 2 ot, 3 mes, 4 mesd, 5 inc, 6 tot, 7 true contrast; copies to host
   bool get(int which, std::vector<zd>& out);
   bool set(int which, const std::vector<zd>& in);
   int iteration() const { return iter_; }

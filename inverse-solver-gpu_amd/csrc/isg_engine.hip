@@ -36,7 +36,32 @@ int pow2_floor(int v) {
   return p;
 }
 constexpr size_t kLdsLimit = 64 * 1024;
-constexpr int kDenseMax = 320;  // largest sample count interpolated as a dense GEMM
+constexpr int kQM2M = 12, kQL2L = 24;  // band windows up to 48 (M2M) / 96 (L2L) samples
+
+// Band blocks of a banded M x K matrix whose row k has taps t < nt on input
+// (first[k] + t) mod K with weight wt[t * M + k]: for each block of 16 rows,
+// the window start s0 and the dense 16 x KB block AB[rb][kk][16].  Returns KB
+// (multiple of 4) or 0 when some block's taps do not fit in kbmax <= K inputs.
+int build_band(int M, int K, int nt, const std::vector<int>& first, const std::vector<double>& wt, int kbmax,
+               std::vector<double>& AB, std::vector<int>& s0) {
+  const int nrb = (M + 15) / 16;
+  s0.assign(nrb, 0);
+  int need = 0;
+  auto md = [K](long v) { return (int)(((v % K) + K) % K); };
+  for (int rb = 0; rb < nrb; rb++) {
+    s0[rb] = md(first[16 * rb]);
+    for (int i = 0; i < 16 && 16 * rb + i < M; i++) need = std::max(need, md(first[16 * rb + i] - s0[rb]) + nt);
+  }
+  const int KB = (need + 3) / 4 * 4;
+  if (KB > std::min(K, kbmax)) return 0;
+  AB.assign((size_t)nrb * KB * 16, 0.0);
+  for (int rb = 0; rb < nrb; rb++)
+    for (int i = 0; i < 16 && 16 * rb + i < M; i++) {
+      const int k = 16 * rb + i, off = md(first[k] - s0[rb]);
+      for (int t = 0; t < nt; t++) AB[((size_t)rb * KB + off + t) * 16 + i] += wt[(size_t)t * M + k];
+    }
+  return KB;
+}
 constexpr int kRefineRounds = 3;
 
 // Largest LDS window (source samples) of a target chunk for the chunked
@@ -195,30 +220,37 @@ void Engine::plan_interp(DevOp<T>& d, std::vector<InterpShape>& m2m, std::vector
   const int L = P_.L, ni = cfg_.plan.ninter, na = 2 * ni;
   m2m.assign(L, InterpShape());
   l2l.assign(L, InterpShape());
-  d.dense_m2m.assign(L, nullptr);
-  d.dense_l2l.assign(L, nullptr);
+  d.band_m2m.assign(L, nullptr);
+  d.band_l2l.assign(L, nullptr);
+  d.bs0_m2m.assign(L, nullptr);
+  d.bs0_l2l.assign(L, nullptr);
   const size_t cz = sizeof(cplx<T>);
-  auto upload_dense = [&](const std::vector<double>& A) -> T* {
+  auto upload_band = [&](const std::vector<double>& A, const std::vector<int>& s0, T*& pa, int*& ps) {
     std::vector<T> tmp(A.begin(), A.end());
-    T* p = (T*)dalloc(tmp.size() * sizeof(T));
-    if (p) ISG_HIP(hipMemcpy(p, tmp.data(), tmp.size() * sizeof(T), hipMemcpyHostToDevice));
-    return p;
+    pa = (T*)dalloc(tmp.size() * sizeof(T));
+    ps = (int*)dalloc(s0.size() * sizeof(int));
+    if (pa) ISG_HIP(hipMemcpy(pa, tmp.data(), tmp.size() * sizeof(T), hipMemcpyHostToDevice));
+    if (ps) ISG_HIP(hipMemcpy(ps, s0.data(), s0.size() * sizeof(int), hipMemcpyHostToDevice));
   };
+  // widest column count (4, 2 or 1 blocks of 16) whose staged tile fits in LDS
+  auto pick_ncb = [&](int K, int np) {
+    for (int c = 4; c >= 1; c /= 2)
+      if ((size_t)16 * c * (K + 1) * cz <= kLdsLimit && (4 * np) % (16 * c) == 0) return c;
+    return 0;
+  };
+  std::vector<double> AB;
+  std::vector<int> s0;
   for (int i = 0; i + 1 < L; i++) {
     const int Sp = P_.nsamp[i], Sc = P_.nsamp[i + 1], np = P_.nclus[i];
     // M2M: rows = parent samples, K = child samples
     InterpShape& m = m2m[i];
-    if (Sp <= kDenseMax && np >= 16) {
-      m.ncb = (32 * (Sc + 1) * cz <= kLdsLimit) ? 2 : ((16 * (Sc + 1) * cz <= kLdsLimit) ? 1 : 0);
-      if (m.ncb > 0 && np * 4 % (16 * m.ncb) == 0) {
-        m.lds_dense = (size_t)16 * m.ncb * (Sc + 1) * cz;
-        std::vector<double> A((size_t)Sc * Sp, 0.0);
-        for (int k = 0; k < Sp; k++)
-          for (int t = 0; t < ni; t++) A[(size_t)((P_.interp0[i][k] + t) % Sc) * Sp + k] += P_.interpT[i][(size_t)t * Sp + k];
-        d.dense_m2m[i] = upload_dense(A);
-      } else {
-        m.ncb = 0;
-      }
+    m.ncb = pick_ncb(Sc, np);
+    if (m.ncb > 0) m.kb = build_band(Sp, Sc, ni, P_.interp0[i], P_.interpT[i], 4 * kQM2M, AB, s0);
+    if (m.ncb > 0 && m.kb > 0) {
+      m.lds_band = (size_t)16 * m.ncb * (Sc + 1) * cz;
+      upload_band(AB, s0, d.band_m2m[i], d.bs0_m2m[i]);
+    } else {
+      m.ncb = 0;
     }
     m.g = (Sp <= 128) ? std::min(np, pow2_floor(std::max(1, kBlock / Sp))) : 1;
     m.kch = (Sp <= 128) ? Sp : kBlock;
@@ -227,18 +259,13 @@ void Engine::plan_interp(DevOp<T>& d, std::vector<InterpShape>& m2m, std::vector
     m.chunk_ok = Wm > 0 && m.lds_chunk <= kLdsLimit;
     // L2L: rows = child samples, K = parent samples
     InterpShape& l = l2l[i];
-    if (Sp <= kDenseMax && np >= 16) {
-      l.ncb = (32 * (Sp + 1) * cz <= kLdsLimit) ? 2 : ((16 * (Sp + 1) * cz <= kLdsLimit) ? 1 : 0);
-      if (l.ncb > 0 && np * 4 % (16 * l.ncb) == 0) {
-        l.lds_dense = (size_t)16 * l.ncb * (Sp + 1) * cz;
-        std::vector<double> A((size_t)Sp * Sc, 0.0);
-        for (int k = 0; k < Sc; k++)
-          for (int t = 0; t < na; t++)
-            A[(size_t)((P_.anterp0[i + 1][k] + t) % Sp) * Sc + k] += P_.anterpT[i + 1][(size_t)t * Sc + k];
-        d.dense_l2l[i] = upload_dense(A);
-      } else {
-        l.ncb = 0;
-      }
+    l.ncb = pick_ncb(Sp, np);
+    if (l.ncb > 0) l.kb = build_band(Sc, Sp, na, P_.anterp0[i + 1], P_.anterpT[i + 1], 4 * kQL2L, AB, s0);
+    if (l.ncb > 0 && l.kb > 0) {
+      l.lds_band = (size_t)16 * l.ncb * (Sp + 1) * cz;
+      upload_band(AB, s0, d.band_l2l[i], d.bs0_l2l[i]);
+    } else {
+      l.ncb = 0;
     }
     l.g = 1;
     l.kch = std::min(Sc, 64);
@@ -248,7 +275,7 @@ void Engine::plan_interp(DevOp<T>& d, std::vector<InterpShape>& m2m, std::vector
   }
 }
 
-// dense GEMM when it yields at least one workgroup per CU, else chunked, else direct
+// band GEMM when it yields at least one workgroup per CU, else chunked, else direct
 int Engine::interp_kind(const InterpShape& sh, int np, int nrhs) const {
   if (sh.ncb > 0 && (long)(4 * np / (16 * sh.ncb)) * nrhs >= 256) return 0;
   if (sh.chunk_ok) return 1;
@@ -283,12 +310,17 @@ void Engine::launch_m2m(int i, int nrhs, const int* act) {
   const int kind = interp_kind(sh, np, nrhs);
   if (kind == 0) {
     dim3 grid(4 * np / (16 * sh.ncb), nrhs);
-    if (sh.ncb == 2)
-      k_interp_mfma<T, 0, 2><<<grid, blk, sh.lds_dense, stream_>>>(child, cs, par, ps, d.dense_m2m[i], d.shiftM[i], Sp,
-                                                                   Sc, Sp, 0, act);
+    const T* ab = d.band_m2m[i];
+    const int* s0 = d.bs0_m2m[i];
+    if (sh.ncb == 4)
+      k_interp_band<T, 0, 4, kQM2M><<<grid, blk, sh.lds_band, stream_>>>(child, cs, par, ps, ab, s0, sh.kb,
+                                                                         d.shiftM[i], Sp, Sc, 0, act);
+    else if (sh.ncb == 2)
+      k_interp_band<T, 0, 2, kQM2M><<<grid, blk, sh.lds_band, stream_>>>(child, cs, par, ps, ab, s0, sh.kb,
+                                                                         d.shiftM[i], Sp, Sc, 0, act);
     else
-      k_interp_mfma<T, 0, 1><<<grid, blk, sh.lds_dense, stream_>>>(child, cs, par, ps, d.dense_m2m[i], d.shiftM[i], Sp,
-                                                                   Sc, Sp, 0, act);
+      k_interp_band<T, 0, 1, kQM2M><<<grid, blk, sh.lds_band, stream_>>>(child, cs, par, ps, ab, s0, sh.kb,
+                                                                         d.shiftM[i], Sp, Sc, 0, act);
   } else if (kind == 1) {
     int nch = (Sp + sh.kch - 1) / sh.kch;
     dim3 grid(np / sh.g * nch, nrhs);
@@ -314,12 +346,17 @@ void Engine::launch_l2l(int i, bool accumulate, int nrhs, const int* act) {
   const int kind = interp_kind(sh, np, nrhs);
   if (kind == 0) {
     dim3 grid(4 * np / (16 * sh.ncb), nrhs);
-    if (sh.ncb == 2)
-      k_interp_mfma<T, 1, 2><<<grid, blk, sh.lds_dense, stream_>>>(par, ps, child, cs, d.dense_l2l[i], d.shiftL[i], Sc,
-                                                                   Sp, Sp, acc, act);
+    const T* ab = d.band_l2l[i];
+    const int* s0 = d.bs0_l2l[i];
+    if (sh.ncb == 4)
+      k_interp_band<T, 1, 4, kQL2L><<<grid, blk, sh.lds_band, stream_>>>(par, ps, child, cs, ab, s0, sh.kb,
+                                                                         d.shiftL[i], Sc, Sp, acc, act);
+    else if (sh.ncb == 2)
+      k_interp_band<T, 1, 2, kQL2L><<<grid, blk, sh.lds_band, stream_>>>(par, ps, child, cs, ab, s0, sh.kb,
+                                                                         d.shiftL[i], Sc, Sp, acc, act);
     else
-      k_interp_mfma<T, 1, 1><<<grid, blk, sh.lds_dense, stream_>>>(par, ps, child, cs, d.dense_l2l[i], d.shiftL[i], Sc,
-                                                                   Sp, Sp, acc, act);
+      k_interp_band<T, 1, 1, kQL2L><<<grid, blk, sh.lds_band, stream_>>>(par, ps, child, cs, ab, s0, sh.kb,
+                                                                         d.shiftL[i], Sc, Sp, acc, act);
   } else if (kind == 1) {
     int nch = (Sc + sh.kch - 1) / sh.kch;
     dim3 grid(np * nch, nrhs);

@@ -52,7 +52,8 @@ struct DevOp {
   cplx<T>* near = nullptr;     // [9][64][64]
   cplx<T>* coeffT = nullptr;   // [64][S_leaf]
   cplx<T>* basisT = nullptr;   // [S_leaf][64]
-  std::vector<T*> interpT, anterpT, dense_m2m, dense_l2l;
+  std::vector<T*> interpT, anterpT, band_m2m, band_l2l;  // band blocks [nrb][KB][16]
+  std::vector<int*> bs0_m2m, bs0_l2l;                    // window start of each band block
   std::vector<cplx<T>*> shiftM, shiftL, trans;
   T* rxT = nullptr;
   T* rxaT = nullptr;
@@ -119,8 +120,9 @@ class Engine {
   // GEMM is used when it gives enough workgroups, else the chunked kernel,
   // else the direct-from-cache kernel.
   struct InterpShape {
-    int ncb = 0;            // dense MFMA: column blocks of 16 (0: unavailable)
-    size_t lds_dense = 0;
+    int ncb = 0;            // band MFMA: column blocks of 16 (0: unavailable)
+    int kb = 0;             // band MFMA: window length of a 16-row block
+    size_t lds_band = 0;
     bool chunk_ok = false;  // chunked LDS kernel usable
     int g = 1;              // chunked: parents per workgroup
     int kch = 256;          // chunked: target samples per workgroup

@@ -328,20 +328,26 @@ __global__ void __launch_bounds__(kBlock) k_l2l_chunk(const cplx<T>* __restrict_
   }
 }
 
-// Dense interpolation on the matrix cores (lower levels, where sample counts
-// are small and clusters many).  Out[k][col] = sum_j A[k][j] In[j][col] with A
-// the (banded) interpolator stored densely K-major (AT[j][k], real), and the
+// Interpolation (M2M) and anterpolation (L2L) on the matrix cores, for the
+// levels with many clusters.  Out[k][col] = sum_j A[k][j] In[j][col], A the
+// real interpolator (30 taps per row, M2M) or anterpolator (60 taps, L2L),
 // columns = the child vectors (4 per parent) of NC/4 consecutive parents.
+// A is banded: the taps of 16 consecutive rows k all fall in one window of
+// KB <= K inputs starting at s0[rb] (modulo K), so each 16-row block is a
+// 16 x KB dense block ("band block"), stored K-major as AB[rb][kk][16].
 //   M2M (MODE 0): In = children (contiguous); the epilogue multiplies by
-//                 shiftM[c][k] and sums the 4 sibling columns (lanes j..j+3)
-//                 with two xor-shuffles, then writes parent[p][k].
+//                 shiftM[c][k] and sums the 4 sibling columns, then writes
+//                 parent[p][k].
 //   L2L (MODE 1): In = shiftL[c] o parent, staged per column; Out is added to
 //                 (or assigned to) child[4p+c][k].
-template <typename T, int MODE, int NCB>
-__global__ void __launch_bounds__(kBlock) k_interp_mfma(const cplx<T>* __restrict__ src, long ss,
-                                                        cplx<T>* __restrict__ dst, long ds, const T* __restrict__ AT,
+// The KB/4 table fragments of a block are loaded into registers before its
+// MFMA chain; the data columns come from LDS (one padding element per column).
+template <typename T, int MODE, int NCB, int KQ>
+__global__ void __launch_bounds__(kBlock) k_interp_band(const cplx<T>* __restrict__ src, long ss,
+                                                        cplx<T>* __restrict__ dst, long ds, const T* __restrict__ AB,
+                                                        const int* __restrict__ s0v, int KB,
                                                         const cplx<T>* __restrict__ shift, int Mrows, int K,
-                                                        int Spar, int accumulate, const int* __restrict__ act) {
+                                                        int accumulate, const int* __restrict__ act) {
   typedef typename Mfma<T>::v4 v4;
   constexpr int NC = 16 * NCB;
   extern __shared__ char smem_raw[];
@@ -366,22 +372,30 @@ __global__ void __launch_bounds__(kBlock) k_interp_mfma(const cplx<T>* __restric
     }
   }
   __syncthreads();
-  // (row block, column block) items spread evenly over the 4 waves
   const int nrb = (Mrows + 15) / 16;
+  const int nkq = KB >> 2;
+  // (row block, column block) items spread evenly over the 4 waves
   for (int item = w; item < nrb * NCB; item += 4) {
     const int rb = item / NCB, cb = item - rb * NCB;
-    v4 cr0 = v4{0, 0, 0, 0}, ci0 = v4{0, 0, 0, 0};
-    const int krow = 16 * rb + li;
+    const T* ab = AB + (long)rb * KB * 16;
+    T a[KQ];
+#pragma unroll
+    for (int q = 0; q < KQ; q++) a[q] = (q < nkq) ? ab[(4 * q + lk) * 16 + li] : (T)0;
+    const int s0 = s0v[rb];
     const cplx<T>* tcol = tile + (16 * cb + li) * row;
-    for (int kk = 0; kk < K / 4; kk++) {
-      const int j = 4 * kk + lk;
-      T a = (krow < Mrows) ? AT[(long)j * Mrows + krow] : (T)0;
-      cplx<T> bv = tcol[j];
-      cr0 = Mfma<T>::mac(bv.x, a, cr0);
-      ci0 = Mfma<T>::mac(bv.y, a, ci0);
+    v4 cr0 = v4{0, 0, 0, 0}, ci0 = v4{0, 0, 0, 0};
+#pragma unroll
+    for (int q = 0; q < KQ; q++) {
+      if (q < nkq) {
+        int j = s0 + 4 * q + lk;
+        if (j >= K) j -= K;
+        const cplx<T> bv = tcol[j];
+        cr0 = Mfma<T>::mac(bv.x, a[q], cr0);
+        ci0 = Mfma<T>::mac(bv.y, a[q], ci0);
+      }
     }
-    // D[col][k]: lane li holds target sample k = krow; col = 16 cb + row(lk, q)
-    const int k = krow;
+    // D[col][k]: lane li holds target sample k = 16 rb + li; col = 16 cb + row(lk, q)
+    const int k = 16 * rb + li;
     if (MODE == 0) {
       // parent = sum over the 4 sibling columns of shiftM[c][k] * D[col][k]
       if (Mfma<T>::kRowsAcrossLanes) {
@@ -419,7 +433,6 @@ __global__ void __launch_bounds__(kBlock) k_interp_mfma(const cplx<T>* __restric
       }
     }
   }
-  (void)Spar;
 }
 
 // Tiled M2L.  A workgroup owns a TS x TS block of target boxes (Morton

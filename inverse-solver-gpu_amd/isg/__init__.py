@@ -121,6 +121,13 @@ _SIGS = {
     "isg_warmnet_save": (ctypes.c_int, [_V, ctypes.c_char_p]),
     "isg_warmnet_load": (ctypes.c_int, [_V, ctypes.c_char_p]),
     "isg_dbim_set_warmnet": (ctypes.c_int, [_V, _V]),
+    "isg_dbim_write_cg": (ctypes.c_int, [_V, ctypes.c_char_p, ctypes.c_int]),
+    "isg_dbim_read_cg": (ctypes.c_int, [_V, ctypes.c_char_p]),
+    "isg_hist_append": (ctypes.c_int, [ctypes.c_char_p, _P(DbimStats), ctypes.c_int]),
+    "isg_phantom_read_txt": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.c_double, _D]),
+    "isg_phantom_write_txt": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.c_double, _D]),
+    "isg_raster_bin_write": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, _D]),
+    "isg_raster_bin_read": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, _D]),
     "isg_phantom_blobs": (ctypes.c_int, [ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_ulonglong, _D]),
     "isg_phantom_ellipses": (ctypes.c_int, [ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_ulonglong,
                                             ctypes.c_double, ctypes.c_double, ctypes.c_double, _D]),
@@ -347,6 +354,15 @@ class Dbim:
         v = _c128(value, self._size(name))
         self.s._err(self.s.lib.isg_dbim_set(self.s.h, _DBIM_FIELDS[name], _dp(v)), "isg_dbim_set")
 
+    def write_cg(self, path, which="o"):
+        """cg_<k>.bin (which 'o') or cg_ref.bin (which 'true'): raster complex doubles."""
+        code = {"o": 0, "true": 7}[which]
+        self.s._err(self.s.lib.isg_dbim_write_cg(self.s.h, str(path).encode(), code), "isg_dbim_write_cg")
+
+    def read_cg(self, path):
+        """Set the current contrast from a raster cg_<k>.bin."""
+        self.s._err(self.s.lib.isg_dbim_read_cg(self.s.h, str(path).encode()), "isg_dbim_read_cg")
+
     def set_warmnet(self, net):
         """Network for field_start 3 (kept referenced by this object)."""
         self._net = net
@@ -464,6 +480,39 @@ class WarmNet:
         tt = torch.from_numpy(np.ascontiguousarray(target, dtype=np.float32)).cuda()
         torch.cuda.synchronize()
         return self.loss_grad_dev(xt.data_ptr(), tt.data_ptr(), xt.shape[0], xt.shape[1], xt.shape[2])
+
+
+def hist_append(path, stats, create=False):
+    """Append the hist.txt line of a stats dict (setup_born.cpp:239)."""
+    st = DbimStats()
+    for f, _ in DbimStats._fields_:
+        if f in stats:
+            setattr(st, f, stats[f])
+    if load().isg_hist_append(str(path).encode(), ctypes.byref(st), 1 if create else 0) != 0:
+        raise IsgError("cannot write %s" % path)
+
+
+def read_phantom_txt(path, side, scale=0.1):
+    """Reference phantom text file -> raster (side x side) complex contrast."""
+    out = np.empty(side * side, np.complex128)
+    if load().isg_phantom_read_txt(str(path).encode(), side * side, scale, _dp(out)) != 0:
+        raise IsgError("cannot read phantom %s" % path)
+    return out.reshape(side, side)
+
+
+def write_phantom_txt(path, raster, scale=0.1):
+    """Raster contrast -> reference phantom text file (real part / scale)."""
+    r = _c128(raster)
+    if load().isg_phantom_write_txt(str(path).encode(), r.size, scale, _dp(r)) != 0:
+        raise IsgError("cannot write phantom %s" % path)
+
+
+def read_raster_bin(path, side):
+    """cg_<k>.bin / cg_ref.bin -> raster (side x side) complex."""
+    out = np.empty(side * side, np.complex128)
+    if load().isg_raster_bin_read(str(path).encode(), side * side, _dp(out)) != 0:
+        raise IsgError("cannot read %s" % path)
+    return out.reshape(side, side)
 
 
 def phantom_blobs(side, n, seed, res=0.1):

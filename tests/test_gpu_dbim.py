@@ -65,3 +65,26 @@ def test_dbim_checkpoint_resume(case):
         b.set(k, v)
     b.iterate()
     assert relerr(b.get("o"), a.get("o")) < 1e-12
+
+
+def test_cg_files(case, tmp_path):
+    """cg_<k>.bin / cg_ref.bin are the raster-order gathers of the contrast
+    (setup_born.cpp:115-121, 209-218) and read back as a restart."""
+    isg, otrue, _, _ = case
+    kw = dict(level=LEVEL, numrx=NRX, numtx=NTX, toli=TOL, iti=300, regular=REG, max_rhs=16)
+    s = isg.Solver(**kw)
+    _, r2t = s.geometry()
+    side = 8 << (LEVEL - 1)
+    d = isg.Dbim(s, otrue)
+    d.write_cg(tmp_path / "cg_ref.bin", "true")
+    st = d.iterate()
+    isg.hist_append(tmp_path / "hist.txt", st, create=True)
+    d.write_cg(tmp_path / "cg_1.bin")
+    np.testing.assert_array_equal(isg.read_raster_bin(tmp_path / "cg_ref.bin", side),
+                                  isg.tree_to_raster(otrue, r2t, side))
+    np.testing.assert_array_equal(isg.read_raster_bin(tmp_path / "cg_1.bin", side),
+                                  isg.tree_to_raster(d.get("o"), r2t, side))
+    e = isg.Dbim(isg.Solver(**kw), otrue)
+    e.read_cg(tmp_path / "cg_1.bin")
+    np.testing.assert_array_equal(e.get("o"), d.get("o"))
+    assert len((tmp_path / "hist.txt").read_text().splitlines()) == 1
