@@ -146,6 +146,8 @@ def main():
                     help="total-field start of the fast path")
     ap.add_argument("--mixed-margin", type=float, default=0.5,
                     help="FP32 solve target as a fraction of the tolerance (mixed precision)")
+    ap.add_argument("--tuning", default="committed",
+                    help="kernel-variant cache: 'committed' (inverse-solver-gpu_amd/tuning), 'none' or a path")
     ap.add_argument("--cpu-sample", type=int, default=2, help="oracle matvecs timed for cpu_baseline (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -157,7 +159,9 @@ def main():
     wl = WORKLOADS[args.workload]
     level, ntx, nrx = wl["level"], wl["numtx"], wl["numrx"]
     side = 8 << (level - 1)
-    common = dict(level=level, numrx=nrx, numtx=ntx, toli=1e-6, iti=300, regular=float(np.sqrt(1e-3)))
+    tuning = {"committed": True, "none": False}.get(args.tuning, args.tuning)
+    common = dict(level=level, numrx=nrx, numtx=ntx, toli=1e-6, iti=300, regular=float(np.sqrt(1e-3)),
+                  tuning=tuning)
     paths = [p for p in args.paths.split(",") if p]
     fast_prec = 1 if args.fast_precision == "mixed" else 0
 
@@ -232,6 +236,7 @@ def main():
                      % (ntx, "FP32 operator / FP64 solver + FP64 defect correction" if mixed else "FP64",
                         args.fast_start)
                      if main_path == "fast" else "1 RHS, FP64, zero start"),
+            "kernel_tuning": args.tuning,
         },
         "roofline": {
             "kernel": "k_near_mfma (P2P + L2P on v_mfma_%s_16x16x4)" % prec,

@@ -194,6 +194,23 @@ int isg_phantom_write_txt(const char* path, int numunk, double scale, const doub
 int isg_raster_bin_write(const char* path, int numunk, const double* raster);
 int isg_raster_bin_read(const char* path, int numunk, double* raster);
 
+/* ---- autotuner (kernel variants per stage and level, measured on the device) ----
+ * stage 0 near-field tile width (16 or 32 boxes), 1 P2M tile width, 2 M2L tile
+ * variant of `level` (0: 4x4 boxes x 8 samples, 1: 8x8 x 8, 2: 8x8 x 16,
+ * 3: 16x16 x 8, 4: 8x8 x 32, 5: 16x16 x 16; 4 and 5 FP32 only), 3 / 4 M2M / L2L
+ * kernel of `level` (0 band MFMA, 1 chunked LDS, 2 direct); value -1 = built-in
+ * rule.  precision 0 = FP64 operator, 1 = FP32 operator of the mixed path;
+ * batch_class 0 for fewer than 16 right-hand sides, 1 otherwise.
+ * isg_autotune times every admissible variant (reps launches each) and keeps
+ * the fastest for batch_class(nrhs) and every operator precision the context has.
+ * The file holds lines "tree-depth precision batch_class stage level value";
+ * entries for other tree depths are ignored on load. */
+int isg_autotune(isg_context* ctx, int nrhs, int reps);
+int isg_tuning_set(isg_context* ctx, int precision, int batch_class, int stage, int level, int value);
+int isg_tuning_get(isg_context* ctx, int precision, int batch_class, int stage, int level, int* value);
+int isg_tuning_save(isg_context* ctx, const char* path);
+int isg_tuning_load(isg_context* ctx, const char* path);
+
 /* ---- live kernel timing (HIP events on the context's stream; the reference
  * times whole matvecs with cudaEvents, matvec_gpu.cu:117-122) ----
  * kernel 0: near field + L2P on the matrix cores, one launch per matvec

@@ -360,6 +360,38 @@ int isg_dbim_setup_measured(isg_context* c, const double* mes, const double* o_t
   return ISG_OK;
 }
 
+int isg_autotune(isg_context* c, int nrhs, int reps) {
+  if (!check(c)) return ISG_E_STATE;
+  if (nrhs < 1 || nrhs > c->cfg.max_rhs || reps < 1) return fail(c, ISG_E_ARG, "bad argument");
+  c->eng->autotune(nrhs, reps);
+  return check(c) ? ISG_OK : ISG_E_DEVICE;
+}
+
+int isg_tuning_set(isg_context* c, int precision, int batch_class, int stage, int level, int value) {
+  if (!check(c)) return ISG_E_STATE;
+  return c->eng->tuning_set(precision, batch_class, stage, level, value) ? ISG_OK
+                                                                       : fail(c, ISG_E_ARG, "bad tuning entry");
+}
+
+int isg_tuning_get(isg_context* c, int precision, int batch_class, int stage, int level, int* value) {
+  if (!check(c)) return ISG_E_STATE;
+  if (!value) return fail(c, ISG_E_ARG, "null pointer");
+  *value = c->eng->tuning_get(precision, batch_class, stage, level);
+  return ISG_OK;
+}
+
+int isg_tuning_save(isg_context* c, const char* path) {
+  if (!check(c)) return ISG_E_STATE;
+  if (!path || !c->eng->tuning_save(path)) return fail(c, ISG_E_ARG, "cannot write tuning file");
+  return ISG_OK;
+}
+
+int isg_tuning_load(isg_context* c, const char* path) {
+  if (!check(c)) return ISG_E_STATE;
+  if (!path || !c->eng->tuning_load(path)) return fail(c, ISG_E_ARG, "cannot read tuning file");
+  return ISG_OK;
+}
+
 int isg_profile(isg_context* c, int enable) {
   if (!check(c)) return ISG_E_STATE;
   c->eng->profile(enable != 0);

@@ -6,6 +6,8 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <fstream>
+#include <sstream>
 
 #include "isg_kernels.hpp"
 #include "isg_solver_kernels.hpp"
@@ -182,6 +184,12 @@ Engine::Engine(const EngineConfig& cfg) : cfg_(cfg) {
   rxaI_ = up_i(P_.rxaI);
   plan_interp(op64_, m2m64_, l2l64_);
   if (has32_) plan_interp(op32_, m2m32_, l2l32_);
+  for (auto& row : tune_)
+    for (auto& t : row) {
+      t.m2l.assign(P_.L, -1);
+      t.m2m.assign(P_.L, -1);
+      t.l2l.assign(P_.L, -1);
+    }
   vec_.assign(V_COUNT, nullptr);
   for (int k = 0; k < V_COUNT; k++) vec_[k] = (zd*)dalloc((size_t)R * P_.N * sizeof(zd));
   nblk_ = (int)std::min<long>(1024, std::max<long>(1, (P_.N + kBlock * 4 - 1) / (kBlock * 4)));
@@ -275,6 +283,10 @@ void Engine::plan_interp(DevOp<T>& d, std::vector<InterpShape>& m2m, std::vector
   }
 }
 
+bool Engine::kind_ok(const InterpShape& sh, int kind) const {
+  return (kind == 0 && sh.ncb > 0) || (kind == 1 && sh.chunk_ok) || kind == 2;
+}
+
 // band GEMM when it yields at least one workgroup per CU, else chunked, else direct
 int Engine::interp_kind(const InterpShape& sh, int np, int nrhs) const {
   if (sh.ncb > 0 && (long)(4 * np / (16 * sh.ncb)) * nrhs >= 256) return 0;
@@ -285,9 +297,11 @@ int Engine::interp_kind(const InterpShape& sh, int np, int nrhs) const {
 // --------------------------------------------------------------- launches
 // leaf radiation patterns on the matrix cores
 template <typename T>
-void Engine::launch_p2m(const zd* y, int nrhs, const int* act) {
+void Engine::launch_p2m(const zd* y, int nrhs, const int* act, int force) {
   const int Lf = P_.L - 1, S = P_.nsamp[Lf], B = P_.B;
-  const int nb = B >= 32 ? 32 : 16;
+  const int tuned = tune(sizeof(T) == 4 ? PREC_MIXED : PREC_FP64, nrhs).p2m_nb;
+  int nb = force > 0 ? force : (tuned > 0 ? tuned : (B >= 32 ? 32 : 16));
+  if (B % nb != 0) nb = 16;
   dim3 grid(B / nb, nrhs), blk(kBlock);
   DevOp<T>& d = ops<T>();
   cplx<T>* out = d.agm + lvl_off_[Lf];
@@ -300,14 +314,15 @@ void Engine::launch_p2m(const zd* y, int nrhs, const int* act) {
 
 // child level i+1 -> parent level i
 template <typename T>
-void Engine::launch_m2m(int i, int nrhs, const int* act) {
+void Engine::launch_m2m(int i, int nrhs, const int* act, int force) {
   const int Sp = P_.nsamp[i], Sc = P_.nsamp[i + 1], np = P_.nclus[i], nt = cfg_.plan.ninter;
   DevOp<T>& d = ops<T>();
   cplx<T>*child = d.agm + lvl_off_[i + 1], *par = d.agm + lvl_off_[i];
   long cs = (long)lvl_size_[i + 1], ps = (long)lvl_size_[i];
   const InterpShape& sh = (sizeof(T) == 4 ? m2m32_ : m2m64_)[i];
   dim3 blk(kBlock);
-  const int kind = interp_kind(sh, np, nrhs);
+  const int tuned = tune(sizeof(T) == 4 ? PREC_MIXED : PREC_FP64, nrhs).m2m[i];
+  const int kind = kind_ok(sh, force) ? force : (kind_ok(sh, tuned) ? tuned : interp_kind(sh, np, nrhs));
   if (kind == 0) {
     dim3 grid(4 * np / (16 * sh.ncb), nrhs);
     const T* ab = d.band_m2m[i];
@@ -335,7 +350,7 @@ void Engine::launch_m2m(int i, int nrhs, const int* act) {
 // parent level i -> child level i+1, accumulating (matvec) or assigning
 // (receiver back-projection).
 template <typename T>
-void Engine::launch_l2l(int i, bool accumulate, int nrhs, const int* act) {
+void Engine::launch_l2l(int i, bool accumulate, int nrhs, const int* act, int force) {
   const int Sp = P_.nsamp[i], Sc = P_.nsamp[i + 1], np = P_.nclus[i], nt = 2 * cfg_.plan.ninter;
   DevOp<T>& d = ops<T>();
   cplx<T>*par = d.agl + lvl_off_[i], *child = d.agl + lvl_off_[i + 1];
@@ -343,7 +358,8 @@ void Engine::launch_l2l(int i, bool accumulate, int nrhs, const int* act) {
   const InterpShape& sh = (sizeof(T) == 4 ? l2l32_ : l2l64_)[i];
   dim3 blk(kBlock);
   int acc = accumulate ? 1 : 0;
-  const int kind = interp_kind(sh, np, nrhs);
+  const int tuned = tune(sizeof(T) == 4 ? PREC_MIXED : PREC_FP64, nrhs).l2l[i];
+  const int kind = kind_ok(sh, force) ? force : (kind_ok(sh, tuned) ? tuned : interp_kind(sh, np, nrhs));
   if (kind == 0) {
     dim3 grid(4 * np / (16 * sh.ncb), nrhs);
     const T* ab = d.band_l2l[i];
@@ -369,32 +385,59 @@ void Engine::launch_l2l(int i, bool accumulate, int nrhs, const int* act) {
   }
 }
 
-// translations of level i: TS x TS target tiles, 8 samples per workgroup;
-// 4x4 tiles when 8x8 tiles would leave fewer than 1024 workgroups
+// M2L tile variants: TS x TS target boxes and KC samples per workgroup
+constexpr int kM2LVariants = 6;
+constexpr int kM2LShape[kM2LVariants][2] = {{4, 8}, {8, 8}, {8, 16}, {16, 8}, {8, 32}, {16, 16}};
+
 template <typename T>
-void Engine::launch_m2l(int i, int nrhs, const int* act) {
-  const int S = P_.nsamp[i], nb = 1 << i, KC = 8;
+bool Engine::m2l_ok(int v, int i) const {
+  if (v < 0 || v >= kM2LVariants) return false;
+  const int TS = kM2LShape[v][0], KC = kM2LShape[v][1];
+  const size_t lds = (size_t)((TS + 4) * (TS + 4) + 49) * KC * sizeof(cplx<T>);
+  return (1 << i) >= TS && lds <= kLdsLimit;
+}
+
+// translations of level i.  Built-in rule: 8x8 tiles of 8 samples, 4x4 when
+// 8x8 tiles would leave fewer than 1024 workgroups.
+template <typename T>
+void Engine::launch_m2l(int i, int nrhs, const int* act, int force) {
+  const int S = P_.nsamp[i], nb = 1 << i;
+  int v = force;
+  if (!m2l_ok<T>(v, i)) v = tune(sizeof(T) == 4 ? PREC_MIXED : PREC_FP64, nrhs).m2l[i];
+  if (!m2l_ok<T>(v, i)) v = (nb >= 8 && (long)(nb * nb / 64) * ((S + 7) / 8) * nrhs >= 1024) ? 1 : 0;
+  const int TS = kM2LShape[v][0], KC = kM2LShape[v][1];
   const int nchunk = (S + KC - 1) / KC;
-  const int TS = (nb >= 8 && (long)(nb * nb / 64) * nchunk * nrhs >= 1024) ? 8 : 4;
   const int ntile = nb * nb / (TS * TS);
   dim3 grid(ntile * nchunk, nrhs), blk(kBlock);
   DevOp<T>& d = ops<T>();
   const cplx<T>* m = d.agm + lvl_off_[i];
   cplx<T>* l = d.agl + lvl_off_[i];
   long st = (long)lvl_size_[i];
-  if (TS == 8)
-    k_m2l_tile<T, 8, 8><<<grid, blk, 0, stream_>>>(m, l, st, d.trans[i], i, S, act);
-  else
-    k_m2l_tile<T, 4, 8><<<grid, blk, 0, stream_>>>(m, l, st, d.trans[i], i, S, act);
+  switch (v) {
+    case 0: k_m2l_tile<T, 4, 8><<<grid, blk, 0, stream_>>>(m, l, st, d.trans[i], i, S, act); break;
+    case 1: k_m2l_tile<T, 8, 8><<<grid, blk, 0, stream_>>>(m, l, st, d.trans[i], i, S, act); break;
+    case 2: k_m2l_tile<T, 8, 16><<<grid, blk, 0, stream_>>>(m, l, st, d.trans[i], i, S, act); break;
+    case 3: k_m2l_tile<T, 16, 8><<<grid, blk, 0, stream_>>>(m, l, st, d.trans[i], i, S, act); break;
+    default:
+      if constexpr (sizeof(T) == 4) {
+        if (v == 4)
+          k_m2l_tile<T, 8, 32><<<grid, blk, 0, stream_>>>(m, l, st, d.trans[i], i, S, act);
+        else
+          k_m2l_tile<T, 16, 16><<<grid, blk, 0, stream_>>>(m, l, st, d.trans[i], i, S, act);
+      }
+      break;
+  }
 }
 
 // Near field + L2P on the matrix cores (k_near_mfma), or L2P alone when
 // with_near is false (receiver back-projection).
 template <typename T>
-void Engine::launch_near(const zd* y, zd* r, int nrhs, const int* act, bool with_near) {
+void Engine::launch_near(const zd* y, zd* r, int nrhs, const int* act, bool with_near, int force) {
   const int Lf = P_.L - 1, S = P_.nsamp[Lf], B = P_.B;
   const long N = P_.N, ls = (long)lvl_size_[Lf];
-  const int nb = B >= 32 ? 32 : 16;
+  const int tuned = tune(sizeof(T) == 4 ? PREC_MIXED : PREC_FP64, nrhs).near_nb;
+  int nb = force > 0 ? force : (tuned > 0 ? tuned : (B >= 32 ? 32 : 16));
+  if (B % nb != 0) nb = 16;
   DevOp<T>& d = ops<T>();
   const cplx<T>* loc = d.agl + lvl_off_[Lf];
   size_t lds = (size_t)near_lds_elems(nb, S) * sizeof(cplx<T>);
@@ -698,6 +741,138 @@ void Engine::solve(zd* x, const zd* g, long gs, const zd* b, bool herm, int nrhs
     solves++;
     solve_iters += out[i].iters;
   }
+}
+
+// ---------------------------------------------------------------- autotuner
+template <typename T>
+void Engine::autotune_prec(int nrhs, int reps) {
+  const int L = P_.L, Lf = L - 1;
+  Tuning& tu = tune(sizeof(T) == 4 ? PREC_MIXED : PREC_FP64, nrhs);
+  zd *y = vec_[V_Y], *op = vec_[V_OP];
+  ISG_HIP(hipMemsetAsync(y, 0, (size_t)nrhs * P_.N * sizeof(zd), stream_));
+  apply<T>(y, op, nrhs, nullptr);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  ISG_HIP(hipEventCreate(&e0));
+  ISG_HIP(hipEventCreate(&e1));
+  if (!ok()) return;
+  // average time of one launch sequence (after one untimed run)
+  auto timed = [&](auto&& run) -> float {
+    run();
+    ISG_HIP(hipEventRecord(e0, stream_));
+    for (int r = 0; r < reps; r++) run();
+    ISG_HIP(hipEventRecord(e1, stream_));
+    ISG_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    ISG_HIP(hipEventElapsedTime(&ms, e0, e1));
+    return ms / reps;
+  };
+  auto pick = [&](int ncand, auto&& admissible, auto&& run) -> int {
+    int best = -1;
+    float tbest = 0;
+    for (int c = 0; c < ncand; c++) {
+      if (!admissible(c)) continue;
+      float t = timed([&] { run(c); });
+      if (best < 0 || t < tbest) {
+        best = c;
+        tbest = t;
+      }
+    }
+    return best;
+  };
+  const int widths[2] = {16, 32};
+  auto width_ok = [&](int c) { return P_.B % widths[c] == 0; };
+  int c = pick(2, width_ok, [&](int c) { launch_near<T>(y, op, nrhs, nullptr, true, widths[c]); });
+  tu.near_nb = c < 0 ? -1 : widths[c];
+  c = pick(2, width_ok, [&](int c) { launch_p2m<T>(y, nrhs, nullptr, widths[c]); });
+  tu.p2m_nb = c < 0 ? -1 : widths[c];
+  for (int i = 2; i <= Lf; i++)
+    tu.m2l[i] = pick(
+        kM2LVariants, [&](int v) { return m2l_ok<T>(v, i); }, [&](int v) { launch_m2l<T>(i, nrhs, nullptr, v); });
+  const std::vector<InterpShape>& sm = sizeof(T) == 4 ? m2m32_ : m2m64_;
+  const std::vector<InterpShape>& sl = sizeof(T) == 4 ? l2l32_ : l2l64_;
+  for (int i = 0; i + 1 < L; i++) {
+    tu.m2m[i] = pick(
+        3, [&](int k) { return kind_ok(sm[i], k); }, [&](int k) { launch_m2m<T>(i, nrhs, nullptr, k); });
+    tu.l2l[i] = pick(
+        3, [&](int k) { return kind_ok(sl[i], k); }, [&](int k) { launch_l2l<T>(i, true, nrhs, nullptr, k); });
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  ISG_HIP(hipGetLastError());
+}
+
+void Engine::autotune(int nrhs, int reps) {
+  if (!ok()) return;
+  if (nrhs < 1 || nrhs > cfg_.max_rhs || reps < 1) {
+    set_error("autotune: nrhs must be in 1..max_rhs and reps >= 1");
+    return;
+  }
+  autotune_prec<double>(nrhs, reps);
+  if (has32_) autotune_prec<float>(nrhs, reps);
+}
+
+static const char* kStageName[Engine::ST_COUNT] = {"near", "p2m", "m2l", "m2m", "l2l"};
+
+bool Engine::tuning_set(int prec, int cls, int stage, int level, int value) {
+  if ((prec != PREC_FP64 && prec != PREC_MIXED) || cls < 0 || cls > 1 || stage < 0 || stage >= ST_COUNT) return false;
+  Tuning& t = tune_[prec == PREC_MIXED ? 1 : 0][cls];
+  if (stage == ST_NEAR || stage == ST_P2M) {
+    if (value != -1 && value != 16 && value != 32) return false;
+    (stage == ST_NEAR ? t.near_nb : t.p2m_nb) = value;
+    return true;
+  }
+  if (level < 0 || level >= P_.L || value < -1) return false;
+  if (stage == ST_M2L) {
+    if (value >= kM2LVariants) return false;
+    t.m2l[level] = value;
+  } else {
+    if (value > 2) return false;
+    (stage == ST_M2M ? t.m2m : t.l2l)[level] = value;
+  }
+  return true;
+}
+
+int Engine::tuning_get(int prec, int cls, int stage, int level) const {
+  if ((prec != PREC_FP64 && prec != PREC_MIXED) || cls < 0 || cls > 1 || stage < 0 || stage >= ST_COUNT) return -1;
+  const Tuning& t = tune_[prec == PREC_MIXED ? 1 : 0][cls];
+  if (stage == ST_NEAR) return t.near_nb;
+  if (stage == ST_P2M) return t.p2m_nb;
+  if (level < 0 || level >= P_.L) return -1;
+  return stage == ST_M2L ? t.m2l[level] : (stage == ST_M2M ? t.m2m[level] : t.l2l[level]);
+}
+
+// Text cache, one choice per line: "depth precision class stage level value".
+bool Engine::tuning_save(const std::string& path) {
+  std::ofstream f(path);
+  if (!f) return false;
+  f << "# isg tuning: tree-depth precision(0 fp64, 1 mixed) batch-class(0 nrhs<16, 1 nrhs>=16) stage level value\n";
+  for (int p = 0; p < 2; p++)
+    for (int c = 0; c < 2; c++)
+      for (int s = 0; s < ST_COUNT; s++)
+        for (int i = 0; i < ((s == ST_NEAR || s == ST_P2M) ? 1 : P_.L); i++) {
+          int v = tuning_get(p, c, s, i);
+          if (v >= 0) f << P_.L << ' ' << p << ' ' << c << ' ' << kStageName[s] << ' ' << i << ' ' << v << '\n';
+        }
+  return (bool)f;
+}
+
+bool Engine::tuning_load(const std::string& path) {
+  std::ifstream f(path);
+  if (!f) return false;
+  std::string line;
+  while (std::getline(f, line)) {
+    if (line.empty() || line[0] == '#') continue;
+    std::istringstream is(line);
+    int depth, p, c, i, v;
+    std::string name;
+    if (!(is >> depth >> p >> c >> name >> i >> v)) return false;
+    if (depth != P_.L) continue;
+    int s = -1;
+    for (int k = 0; k < ST_COUNT; k++)
+      if (name == kStageName[k]) s = k;
+    if (s < 0 || !tuning_set(p, c, s, i, v)) return false;
+  }
+  return true;
 }
 
 void Engine::stage(int which, int level, int rhs, std::vector<zd>& out) {

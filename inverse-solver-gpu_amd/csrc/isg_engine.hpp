@@ -109,6 +109,25 @@ class Engine {
   void profile(bool on);
   void profile_read(int id, double& ms, long long& launches);
 
+  // Kernel-variant choices per operator precision and batch class (nrhs < 16
+  // or >= 16); -1 = the built-in rule.  Stages: near/P2M tile width (16 or 32
+  // boxes), M2L tile variant per level (kM2LVariants), M2M/L2L kernel kind per
+  // level (0 band MFMA, 1 chunked LDS, 2 direct).
+  enum { ST_NEAR = 0, ST_P2M = 1, ST_M2L = 2, ST_M2M = 3, ST_L2L = 4, ST_COUNT = 5 };
+  struct Tuning {
+    int near_nb = -1, p2m_nb = -1;
+    std::vector<int> m2l, m2m, l2l;
+  };
+  static int batch_class(int nrhs) { return nrhs >= 16 ? 1 : 0; }
+  // Times every admissible variant of every stage with HIP events (reps
+  // launches each, on zero data) and records the fastest for (each operator
+  // precision, batch_class(nrhs)).
+  void autotune(int nrhs, int reps);
+  bool tuning_set(int prec, int cls, int stage, int level, int value);
+  int tuning_get(int prec, int cls, int stage, int level) const;
+  bool tuning_save(const std::string& path);
+  bool tuning_load(const std::string& path);  // entries for other tree depths are skipped
+
  private:
   struct KTimer {
     std::vector<hipEvent_t> ev0, ev1;
@@ -137,17 +156,24 @@ class Engine {
   void plan_interp(DevOp<T>& d, std::vector<InterpShape>& m2m, std::vector<InterpShape>& l2l);
   template <typename T>
   void apply(const zd* y, zd* r, int nrhs, const int* act);
+  // `force` >= 0 overrides the tuned / built-in choice (autotuner)
   template <typename T>
-  void launch_p2m(const zd* y, int nrhs, const int* act);
+  void launch_p2m(const zd* y, int nrhs, const int* act, int force = -1);
   template <typename T>
-  void launch_m2m(int i, int nrhs, const int* act);
+  void launch_m2m(int i, int nrhs, const int* act, int force = -1);
   template <typename T>
-  void launch_l2l(int i, bool accumulate, int nrhs, const int* act);
+  void launch_l2l(int i, bool accumulate, int nrhs, const int* act, int force = -1);
   template <typename T>
-  void launch_m2l(int i, int nrhs, const int* act);
+  void launch_m2l(int i, int nrhs, const int* act, int force = -1);
   template <typename T>
-  void launch_near(const zd* y, zd* r, int nrhs, const int* act, bool with_near);
+  void launch_near(const zd* y, zd* r, int nrhs, const int* act, bool with_near, int force = -1);
+  template <typename T>
+  void autotune_prec(int nrhs, int reps);
+  template <typename T>
+  bool m2l_ok(int variant, int level) const;
   int interp_kind(const InterpShape& sh, int np, int nrhs) const;
+  bool kind_ok(const InterpShape& sh, int kind) const;
+  Tuning& tune(int prec, int nrhs) { return tune_[prec == PREC_MIXED ? 1 : 0][batch_class(nrhs)]; }
   std::vector<InterpShape>& m2m_shapes(bool fp32) { return fp32 ? m2m32_ : m2m64_; }
 
   void apply_mlfma(const zd* y, zd* r, int nrhs, const int* act, int prec);
@@ -188,6 +214,7 @@ class Engine {
   int nblk_ = 0;
   bool prof_ = false;
   KTimer kt_[KT_COUNT];
+  Tuning tune_[2][2];  // [FP64, FP32 operator][batch class]
   std::vector<void*> allocs_;
 };
 

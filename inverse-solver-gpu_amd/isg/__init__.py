@@ -121,6 +121,11 @@ _SIGS = {
     "isg_warmnet_save": (ctypes.c_int, [_V, ctypes.c_char_p]),
     "isg_warmnet_load": (ctypes.c_int, [_V, ctypes.c_char_p]),
     "isg_dbim_set_warmnet": (ctypes.c_int, [_V, _V]),
+    "isg_autotune": (ctypes.c_int, [_V, ctypes.c_int, ctypes.c_int]),
+    "isg_tuning_set": (ctypes.c_int, [_V, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "isg_tuning_get": (ctypes.c_int, [_V, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _I]),
+    "isg_tuning_save": (ctypes.c_int, [_V, ctypes.c_char_p]),
+    "isg_tuning_load": (ctypes.c_int, [_V, ctypes.c_char_p]),
     "isg_dbim_write_cg": (ctypes.c_int, [_V, ctypes.c_char_p, ctypes.c_int]),
     "isg_dbim_read_cg": (ctypes.c_int, [_V, ctypes.c_char_p]),
     "isg_hist_append": (ctypes.c_int, [ctypes.c_char_p, _P(DbimStats), ctypes.c_int]),
@@ -186,7 +191,9 @@ class Solver:
     """One operator + solver context on the GPU (setup_mlfma + setup_gpu +
     setup_bicgs of the reference, mom.cpp:184-190)."""
 
-    def __init__(self, **kw):
+    def __init__(self, tuning=True, **kw):
+        """tuning: load the committed kernel-variant cache (when present) or a
+        given cache path; False keeps the built-in rules."""
         self.lib = load()
         self.cfg = default_config(**kw)
         h = _V()
@@ -201,6 +208,11 @@ class Solver:
         self.numsamp = [self.lib.isg_numsamp(h, i) for i in range(self.level)]
         self.k0 = self.lib.isg_k0(h)
         self.numrx = self.cfg.numrx
+        if tuning is True:
+            if os.path.exists(self.TUNING_FILE):
+                self.tuning_load(self.TUNING_FILE)
+        elif tuning:
+            self.tuning_load(tuning)
 
     def close(self):
         if getattr(self, "h", None):
@@ -285,6 +297,30 @@ class Solver:
         self._err(self.lib.isg_profile_read(self.h, self.KERNELS[kernel], ctypes.byref(ms), ctypes.byref(n)),
                   "isg_profile_read")
         return ms.value, n.value
+
+    # ---- autotuner (stage: 0 near, 1 p2m, 2 m2l, 3 m2m, 4 l2l)
+    TUNE_STAGES = {"near": 0, "p2m": 1, "m2l": 2, "m2m": 3, "l2l": 4}
+    TUNING_FILE = os.path.normpath(os.path.join(_HERE, "..", "tuning", "mi355x.txt"))
+
+    def autotune(self, nrhs=1, reps=3):
+        self._err(self.lib.isg_autotune(self.h, int(nrhs), int(reps)), "isg_autotune")
+
+    def tuning_set(self, precision, batch_class, stage, level, value):
+        self._err(self.lib.isg_tuning_set(self.h, precision, batch_class, self.TUNE_STAGES[stage], level, value),
+                  "isg_tuning_set")
+
+    def tuning_get(self, precision, batch_class, stage, level=0):
+        v = ctypes.c_int()
+        self._err(self.lib.isg_tuning_get(self.h, precision, batch_class, self.TUNE_STAGES[stage], level,
+                                          ctypes.byref(v)), "isg_tuning_get")
+        return v.value
+
+    def tuning_save(self, path):
+        self._err(self.lib.isg_tuning_save(self.h, str(path).encode()), "isg_tuning_save")
+
+    def tuning_load(self, path=None):
+        """Load a tuning cache (default: the committed MI355X cache)."""
+        self._err(self.lib.isg_tuning_load(self.h, str(path or self.TUNING_FILE).encode()), "isg_tuning_load")
 
     def counters(self):
         out = (ctypes.c_longlong * 5)()

@@ -1,0 +1,82 @@
+"""Autotuner: every kernel variant it can choose is an exact restatement of
+the same operator, so each forced variant must reproduce the oracle's FP64
+matvec (1e-12 relative, as the built-in choices do); the tuned choices
+survive a save / load round trip.
+
+Workload: 128x128 (level 5) and 256x256 (level 6) pixels, 4 right-hand sides
+(batch class 0) and 16 (class 1), seeded random inputs.
+"""
+import numpy as np
+import pytest
+
+from conftest import oracle_for, rand_c, relerr
+
+pytestmark = pytest.mark.gpu
+
+M2L_VARIANTS_F64 = [0, 1, 2, 3]
+
+
+@pytest.fixture(scope="module")
+def isg():
+    import isg as m
+    return m
+
+
+def check(s, o, nrhs, tol=1e-12):
+    X = np.stack([rand_c(o.N, 10 + r) for r in range(nrhs)])
+    Y = s.mlfma_many(X)
+    for r in range(nrhs):
+        assert relerr(Y[r], o.mlfma(X[r])) < tol
+
+
+@pytest.mark.parametrize("level", [5, 6])
+def test_forced_m2l_variants_match_oracle(isg, level):
+    o = oracle_for(level)
+    s = isg.Solver(level=level, max_rhs=4)
+    for v in M2L_VARIANTS_F64:
+        print("m2l variant", v, "level", level, flush=True)
+        for i in range(2, level):
+            if (1 << i) >= [4, 8, 8, 16][v]:
+                s.tuning_set(0, 0, "m2l", i, v)
+        check(s, o, 4)
+
+
+@pytest.mark.parametrize("kind", [0, 1, 2])
+def test_forced_interp_kinds_match_oracle(isg, kind):
+    level = 6
+    o = oracle_for(level)
+    s = isg.Solver(level=level, max_rhs=4)
+    print("interp kind", kind, flush=True)
+    for i in range(0, level - 1):
+        s.tuning_set(0, 0, "m2m", i, kind)  # kinds a level lacks fall back to the built-in rule
+        s.tuning_set(0, 0, "l2l", i, kind)
+    for w in (16, 32):
+        s.tuning_set(0, 0, "near", 0, w)
+        s.tuning_set(0, 0, "p2m", 0, w)
+        check(s, o, 4)
+
+
+def test_autotune_save_load(isg, tmp_path):
+    level = 6
+    o = oracle_for(level)
+    s = isg.Solver(level=level, max_rhs=16, precision=1)
+    print("autotune level 6, 16 rhs", flush=True)
+    s.autotune(16, reps=2)
+    p = tmp_path / "tune.txt"
+    s.tuning_save(p)
+    # the context's batched matvec runs the FP32 operator with the tuned choices
+    X = np.stack([rand_c(o.N, 30 + r) for r in range(16)])
+    Y = s.mlfma_many(X)
+    for r in range(16):
+        e = relerr(Y[r], o.mlfma(X[r]))
+        assert 1e-12 < e < 1e-5, e
+    # an FP64 context loading the same file runs the tuned FP64 choices
+    t = isg.Solver(level=level, max_rhs=16)
+    t.tuning_load(p)
+    for prec in (0, 1):
+        for st in ("near", "p2m"):
+            assert t.tuning_get(prec, 1, st) == s.tuning_get(prec, 1, st)
+        for st in ("m2l", "m2m", "l2l"):
+            for i in range(level):
+                assert t.tuning_get(prec, 1, st, i) == s.tuning_get(prec, 1, st, i)
+    check(t, o, 16)
