@@ -73,6 +73,8 @@ typedef struct isg_config {
   double mixed_margin; /* mixed precision only: the FP32 solve runs to mixed_margin * toli on its
                           recursive residual before the FP64 check, so that the check rarely
                           needs a defect-correction round; 0 < mixed_margin <= 1 (default 0.5) */
+  int graph;       /* 1 (default): each BiCGSTAB iteration is captured once per solve as a HIP graph
+                      and replayed; 0: launched kernel by kernel.  Results are identical. */
 } isg_config;
 
 /* Per-solve result. */

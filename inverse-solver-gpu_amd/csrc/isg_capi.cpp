@@ -77,6 +77,7 @@ void isg_config_default(isg_config* c) {
   c->max_rhs = 1;
   c->precision = 0;
   c->mixed_margin = 0.5;
+  c->graph = 1;
 }
 
 int isg_create(const isg_config* cfg, isg_context** out) {
@@ -98,6 +99,7 @@ int isg_create(const isg_config* cfg, isg_context** out) {
   ec.max_rhs = cfg->max_rhs;
   ec.precision = cfg->precision;
   ec.mixed_margin = cfg->mixed_margin;
+  ec.graph = cfg->graph != 0;
   c->eng.reset(new isg::Engine(ec));
   if (!check(c)) {
     *out = c;  // keep the context so the caller can read the error

@@ -528,6 +528,17 @@ void Engine::profile(bool on) {
 
 void Engine::prof_begin(int id) {
   if (!prof_) return;
+  if (capturing_) {
+    if (gev_used_ == (int)gev_.size()) {
+      GraphEvent g{id, nullptr, nullptr};
+      ISG_HIP(hipEventCreate(&g.a));
+      ISG_HIP(hipEventCreate(&g.b));
+      gev_.push_back(g);
+    }
+    gev_[gev_used_].id = id;
+    ISG_HIP(hipEventRecord(gev_[gev_used_].a, stream_));
+    return;
+  }
   KTimer& k = kt_[id];
   if (k.used == (int)k.ev0.size()) {
     if (k.used >= 512) {
@@ -546,6 +557,11 @@ void Engine::prof_begin(int id) {
 
 void Engine::prof_end(int id) {
   if (!prof_) return;
+  if (capturing_) {
+    ISG_HIP(hipEventRecord(gev_[gev_used_].b, stream_));
+    gev_used_++;
+    return;
+  }
   KTimer& k = kt_[id];
   ISG_HIP(hipEventRecord(k.ev1[k.used], stream_));
   k.used++;
@@ -566,6 +582,17 @@ void Engine::harvest() {
   }
 }
 
+// After a graph replay has completed: the kernel times of that replay.
+void Engine::harvest_graph() {
+  for (int i = 0; i < gev_used_; i++) {
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, gev_[i].a, gev_[i].b) == hipSuccess) {
+      kt_[gev_[i].id].ms += ms;
+      kt_[gev_[i].id].launches++;
+    }
+  }
+}
+
 void Engine::profile_read(int id, double& ms, long long& launches) {
   ISG_HIP(hipStreamSynchronize(stream_));
   harvest();
@@ -580,8 +607,7 @@ void Engine::profile_read(int id, double& ms, long long& launches) {
 void Engine::run_bicgs(zd* x, const zd* g, long gs, const zd* b, bool herm, int nrhs, bool x_zero, int maxit,
                        int prec, const zd* r0) {
   const long N = P_.N;
-  zd *r = vec_[V_R], *rt = vec_[V_RT], *p = vec_[V_P], *v = vec_[V_V], *s = vec_[V_S], *t = vec_[V_T],
-     *y = vec_[V_Y], *op = vec_[V_OP];
+  zd *r = vec_[V_R], *rt = vec_[V_RT], *y = vec_[V_Y], *op = vec_[V_OP];
   const int h = herm ? 1 : 0;
   dim3 grid(nblk_, nrhs), blk(kBlock);
   ISG_HIP(hipMemcpyAsync(tolv_, tol_h_, nrhs * sizeof(double), hipMemcpyHostToDevice, stream_));
@@ -594,31 +620,70 @@ void Engine::run_bicgs(zd* x, const zd* g, long gs, const zd* b, bool herm, int 
   }
   k_bi_init<<<grid, blk, 0, stream_>>>(x, op, g, gs, b, r0, r, rt, N, h, x_zero ? 1 : 0, part_);
   k_bi_fin<<<dim3(nrhs), blk, 0, stream_>>>(PH_INIT, state_, active_, part_, nblk_, tolv_, maxit);
-  for (int it = 0; it <= maxit; it++) {
-    ISG_HIP(hipMemcpyAsync(active_h_, active_, nrhs * sizeof(int), hipMemcpyDeviceToHost, stream_));
-    ISG_HIP(hipStreamSynchronize(stream_));
+  maxit_ = maxit;
+  // one iteration = bicgs_body; with graphs it is captured once per call and
+  // replayed, the host reading the activity flags after each replay
+  hipGraphExec_t exec = nullptr;
+  if (cfg_.graph) {
+    hipGraph_t gr = nullptr;
+    capturing_ = true;
+    gev_used_ = 0;
+    ISG_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+    bicgs_body(x, g, gs, herm, nrhs, prec);
+    ISG_HIP(hipStreamEndCapture(stream_, &gr));
+    capturing_ = false;
+    if (gr) {
+      ISG_HIP(hipGraphInstantiate(&exec, gr, nullptr, nullptr, 0));
+      (void)hipGraphDestroy(gr);
+    }
     if (!ok()) return;
-    if (prof_) harvest();
+  }
+  ISG_HIP(hipMemcpyAsync(active_h_, active_, nrhs * sizeof(int), hipMemcpyDeviceToHost, stream_));
+  for (int it = 0; it <= maxit; it++) {
+    ISG_HIP(hipStreamSynchronize(stream_));
+    if (!ok()) break;
+    if (prof_) {
+      harvest();
+      if (exec && it > 0) harvest_graph();
+    }
     int nact = 0;
     for (int i = 0; i < nrhs; i++) nact += active_h_[i] != 0;
     if (nact == 0) break;
-    k_bi_pupd<<<grid, blk, 0, stream_>>>(r, p, v, g, gs, y, N, h, state_);
-    apply_mlfma(y, op, nrhs, active_, prec);
-    k_bi_post1<<<grid, blk, 0, stream_>>>(p, op, g, gs, v, rt, N, h, state_, part_);
-    k_bi_fin<<<dim3(nrhs), blk, 0, stream_>>>(PH_ALPHA, state_, active_, part_, nblk_, tolv_, maxit);
-    k_bi_supd<<<grid, blk, 0, stream_>>>(r, v, g, gs, s, y, N, h, state_);
-    apply_mlfma(y, op, nrhs, active_, prec);
-    k_bi_post2<<<grid, blk, 0, stream_>>>(s, op, g, gs, t, N, h, state_, part_);
-    k_bi_fin<<<dim3(nrhs), blk, 0, stream_>>>(PH_OMEGA, state_, active_, part_, nblk_, tolv_, maxit);
-    prof_begin(KT_VEC);
-    k_bi_xrupd<<<grid, blk, 0, stream_>>>(x, p, s, t, r, rt, N, state_, part_);
-    prof_end(KT_VEC);
-    k_bi_fin<<<dim3(nrhs), blk, 0, stream_>>>(PH_RESID, state_, active_, part_, nblk_, tolv_, maxit);
+    if (exec)
+      ISG_HIP(hipGraphLaunch(exec, stream_));
+    else
+      bicgs_body(x, g, gs, herm, nrhs, prec);
     matvecs += 2 * nact;
     matvecs32 += f32 * 2 * nact;
   }
+  if (exec) (void)hipGraphExecDestroy(exec);
   ISG_HIP(hipMemcpyAsync(state_h_, state_, nrhs * sizeof(BiState), hipMemcpyDeviceToHost, stream_));
   ISG_HIP(hipStreamSynchronize(stream_));
+}
+
+// One BiCGSTAB iteration for every active system (bicgs_gpu.cu:146-213),
+// ending with the copy of the activity flags to the host.  No memset: the
+// sequence is captured into a graph.
+void Engine::bicgs_body(zd* x, const zd* g, long gs, bool herm, int nrhs, int prec) {
+  const long N = P_.N;
+  zd *r = vec_[V_R], *rt = vec_[V_RT], *p = vec_[V_P], *v = vec_[V_V], *s = vec_[V_S], *t = vec_[V_T],
+     *y = vec_[V_Y], *op = vec_[V_OP];
+  const int h = herm ? 1 : 0;
+  const int maxit = maxit_;
+  dim3 grid(nblk_, nrhs), blk(kBlock);
+  k_bi_pupd<<<grid, blk, 0, stream_>>>(r, p, v, g, gs, y, N, h, state_);
+  apply_mlfma(y, op, nrhs, active_, prec);
+  k_bi_post1<<<grid, blk, 0, stream_>>>(p, op, g, gs, v, rt, N, h, state_, part_);
+  k_bi_fin<<<dim3(nrhs), blk, 0, stream_>>>(PH_ALPHA, state_, active_, part_, nblk_, tolv_, maxit);
+  k_bi_supd<<<grid, blk, 0, stream_>>>(r, v, g, gs, s, y, N, h, state_);
+  apply_mlfma(y, op, nrhs, active_, prec);
+  k_bi_post2<<<grid, blk, 0, stream_>>>(s, op, g, gs, t, N, h, state_, part_);
+  k_bi_fin<<<dim3(nrhs), blk, 0, stream_>>>(PH_OMEGA, state_, active_, part_, nblk_, tolv_, maxit);
+  prof_begin(KT_VEC);
+  k_bi_xrupd<<<grid, blk, 0, stream_>>>(x, p, s, t, r, rt, N, state_, part_);
+  prof_end(KT_VEC);
+  k_bi_fin<<<dim3(nrhs), blk, 0, stream_>>>(PH_RESID, state_, active_, part_, nblk_, tolv_, maxit);
+  ISG_HIP(hipMemcpyAsync(active_h_, active_, nrhs * sizeof(int), hipMemcpyDeviceToHost, stream_));
 }
 
 // res = b - A x with the FP64 operator; rn, bn: per-system norms of res and b.

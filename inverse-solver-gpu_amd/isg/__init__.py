@@ -40,6 +40,7 @@ class Config(ctypes.Structure):
         ("max_rhs", ctypes.c_int),
         ("precision", ctypes.c_int),
         ("mixed_margin", ctypes.c_double),
+        ("graph", ctypes.c_int),
     ]
 
 
