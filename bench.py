@@ -86,12 +86,12 @@ def m2l_bytes_per_rhs(level, numsamp, cbytes):
     return sum(2 * cbytes * 4 ** i * numsamp[i] for i in range(2, level))
 
 
-def run_path(solver, dbim, warmup, steps, label):
+def run_path(solver, dbim, warmup, steps, label, timing=True):
     for w in range(warmup):
         st = dbim.iterate()
         log("[%s] warmup iteration %d: %.3fs, %d solves, %d BiCGSTAB iterations, misfit %.4e"
             % (label, w, st["seconds"], st["solves"], st["solve_iters"], st["misfit"]))
-    solver.profile(True)
+    solver.profile(timing)
     c0 = solver.counters()
     t0 = time.perf_counter()
     stats = []
@@ -148,6 +148,9 @@ def main():
                     help="FP32 solve target as a fraction of the tolerance (mixed precision)")
     ap.add_argument("--tuning", default="committed",
                     help="kernel-variant cache: 'committed' (inverse-solver-gpu_amd/tuning), 'none' or a path")
+    ap.add_argument("--kernel-timing", default="on", choices=["on", "off"],
+                    help="live HIP-event kernel timing in the timed region (off: BiCGSTAB iterations "
+                         "replay as HIP graphs, no roofline figures)")
     ap.add_argument("--cpu-sample", type=int, default=2, help="oracle matvecs timed for cpu_baseline (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -187,13 +190,13 @@ def main():
         d_fast = isg.Dbim(fast, otrue, field_start=START_KINDS[args.fast_start], measurements=mes)
         if args.fast_start == "learned":
             d_fast.set_warmnet(isg.WarmNet(max_batch=8, max_pixels=side * side, path=WARMNET_FILE))
-        res["fast"] = run_path(fast, d_fast, args.warmup, args.steps, "fast")
+        res["fast"] = run_path(fast, d_fast, args.warmup, args.steps, "fast", args.kernel_timing == "on")
         del d_fast
         fast.close()
     if "plain" in paths:
         plain = isg.Solver(max_rhs=1, **common)
         d_plain = isg.Dbim(plain, otrue, field_start=0, measurements=mes)
-        res["plain"] = run_path(plain, d_plain, args.warmup, args.steps, "plain")
+        res["plain"] = run_path(plain, d_plain, args.warmup, args.steps, "plain", args.kernel_timing == "on")
         del d_plain
         plain.close()
 

@@ -250,6 +250,10 @@ int isg_warmnet_forward_dev(isg_warmnet* net, const void* in, void* out, int bat
  * gradients kept for isg_warmnet_adam / isg_warmnet_get_grads */
 int isg_warmnet_loss_grad_dev(isg_warmnet* net, const void* in, const void* target, int batch, int H, int W,
                               double* loss);
+/* the same two calls on host arrays (staged through device buffers the handle keeps) */
+int isg_warmnet_forward(isg_warmnet* net, const float* in, float* out, int batch, int H, int W);
+int isg_warmnet_loss_grad(isg_warmnet* net, const float* in, const float* target, int batch, int H, int W,
+                          double* loss);
 /* Adam step (beta1 0.9, beta2 0.999, eps 1e-8) with learning rate lr */
 int isg_warmnet_adam(isg_warmnet* net, float lr);
 /* weight file: "ISGWNET1", uint32 count, 7 float scales, float parameters */

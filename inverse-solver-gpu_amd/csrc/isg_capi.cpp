@@ -14,6 +14,14 @@
 struct isg_warmnet {
   std::unique_ptr<isg::WarmNet> net;
   std::string err;
+  // device staging of the host-array entry points
+  float* din = nullptr;
+  float* dout = nullptr;
+  size_t cap = 0;  // pixels
+  ~isg_warmnet() {
+    if (din) (void)hipFree(din);
+    if (dout) (void)hipFree(dout);
+  }
 };
 
 struct isg_context {
@@ -512,6 +520,45 @@ int isg_warmnet_loss_grad_dev(isg_warmnet* n, const void* in, const void* target
   if (!in || !target || !loss) return net_fail(n, ISG_E_ARG, "null pointer");
   *loss = n->net->loss_grad((const float*)in, (const float*)target, batch, H, W, nullptr);
   return net_check(n) == ISG_OK ? ISG_OK : ISG_E_ARG;
+}
+
+static bool net_stage(isg_warmnet* n, size_t pix) {
+  if (pix <= n->cap) return true;
+  if (n->din) (void)hipFree(n->din);
+  if (n->dout) (void)hipFree(n->dout);
+  n->din = n->dout = nullptr;
+  n->cap = 0;
+  if (hipMalloc((void**)&n->din, pix * 6 * sizeof(float)) != hipSuccess ||
+      hipMalloc((void**)&n->dout, pix * 2 * sizeof(float)) != hipSuccess)
+    return false;
+  n->cap = pix;
+  return true;
+}
+
+int isg_warmnet_forward(isg_warmnet* n, const float* in, float* out, int batch, int H, int W) {
+  if (net_check(n) != ISG_OK) return ISG_E_STATE;
+  if (!in || !out || batch < 1 || H < 1 || W < 1) return net_fail(n, ISG_E_ARG, "bad argument");
+  const size_t pix = (size_t)batch * H * W;
+  if (!net_stage(n, pix)) return net_fail(n, ISG_E_DEVICE, "hipMalloc failed");
+  if (hipMemcpy(n->din, in, pix * 6 * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+    return net_fail(n, ISG_E_DEVICE, "copy to device failed");
+  int rc = isg_warmnet_forward_dev(n, n->din, n->dout, batch, H, W);
+  if (rc != ISG_OK) return rc;
+  if (hipMemcpy(out, n->dout, pix * 2 * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
+    return net_fail(n, ISG_E_DEVICE, "copy to host failed");
+  return ISG_OK;
+}
+
+int isg_warmnet_loss_grad(isg_warmnet* n, const float* in, const float* target, int batch, int H, int W,
+                          double* loss) {
+  if (net_check(n) != ISG_OK) return ISG_E_STATE;
+  if (!in || !target || !loss || batch < 1 || H < 1 || W < 1) return net_fail(n, ISG_E_ARG, "bad argument");
+  const size_t pix = (size_t)batch * H * W;
+  if (!net_stage(n, pix)) return net_fail(n, ISG_E_DEVICE, "hipMalloc failed");
+  if (hipMemcpy(n->din, in, pix * 6 * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(n->dout, target, pix * 2 * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+    return net_fail(n, ISG_E_DEVICE, "copy to device failed");
+  return isg_warmnet_loss_grad_dev(n, n->din, n->dout, batch, H, W, loss);
 }
 
 int isg_warmnet_adam(isg_warmnet* n, float lr) {

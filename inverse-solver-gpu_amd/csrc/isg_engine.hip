@@ -528,17 +528,6 @@ void Engine::profile(bool on) {
 
 void Engine::prof_begin(int id) {
   if (!prof_) return;
-  if (capturing_) {
-    if (gev_used_ == (int)gev_.size()) {
-      GraphEvent g{id, nullptr, nullptr};
-      ISG_HIP(hipEventCreate(&g.a));
-      ISG_HIP(hipEventCreate(&g.b));
-      gev_.push_back(g);
-    }
-    gev_[gev_used_].id = id;
-    ISG_HIP(hipEventRecord(gev_[gev_used_].a, stream_));
-    return;
-  }
   KTimer& k = kt_[id];
   if (k.used == (int)k.ev0.size()) {
     if (k.used >= 512) {
@@ -557,11 +546,6 @@ void Engine::prof_begin(int id) {
 
 void Engine::prof_end(int id) {
   if (!prof_) return;
-  if (capturing_) {
-    ISG_HIP(hipEventRecord(gev_[gev_used_].b, stream_));
-    gev_used_++;
-    return;
-  }
   KTimer& k = kt_[id];
   ISG_HIP(hipEventRecord(k.ev1[k.used], stream_));
   k.used++;
@@ -579,17 +563,6 @@ void Engine::harvest() {
       }
     }
     k.used = 0;
-  }
-}
-
-// After a graph replay has completed: the kernel times of that replay.
-void Engine::harvest_graph() {
-  for (int i = 0; i < gev_used_; i++) {
-    float ms = 0;
-    if (hipEventElapsedTime(&ms, gev_[i].a, gev_[i].b) == hipSuccess) {
-      kt_[gev_[i].id].ms += ms;
-      kt_[gev_[i].id].launches++;
-    }
   }
 }
 
@@ -624,14 +597,11 @@ void Engine::run_bicgs(zd* x, const zd* g, long gs, const zd* b, bool herm, int 
   // one iteration = bicgs_body; with graphs it is captured once per call and
   // replayed, the host reading the activity flags after each replay
   hipGraphExec_t exec = nullptr;
-  if (cfg_.graph) {
+  if (cfg_.graph && !prof_) {
     hipGraph_t gr = nullptr;
-    capturing_ = true;
-    gev_used_ = 0;
     ISG_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
     bicgs_body(x, g, gs, herm, nrhs, prec);
     ISG_HIP(hipStreamEndCapture(stream_, &gr));
-    capturing_ = false;
     if (gr) {
       ISG_HIP(hipGraphInstantiate(&exec, gr, nullptr, nullptr, 0));
       (void)hipGraphDestroy(gr);
@@ -642,10 +612,7 @@ void Engine::run_bicgs(zd* x, const zd* g, long gs, const zd* b, bool herm, int 
   for (int it = 0; it <= maxit; it++) {
     ISG_HIP(hipStreamSynchronize(stream_));
     if (!ok()) break;
-    if (prof_) {
-      harvest();
-      if (exec && it > 0) harvest_graph();
-    }
+    if (prof_) harvest();
     int nact = 0;
     for (int i = 0; i < nrhs; i++) nact += active_h_[i] != 0;
     if (nact == 0) break;

@@ -30,6 +30,8 @@ struct EngineConfig {
   int precision = 0;      // PREC_MIXED also builds the FP32 operator
   double mixed_margin = 0.5;  // FP32 solve target = mixed_margin * tol before the FP64 check
   bool graph = true;          // replay each BiCGSTAB iteration as one captured HIP graph
+                              // (not while live kernel timing is on: HIP events recorded
+                              // by graph nodes carry no timestamps)
 };
 
 // Result of one BiCGSTAB solve (one right-hand side).
@@ -187,7 +189,6 @@ class Engine {
   void prof_begin(int id);
   void prof_end(int id);
   void harvest();
-  void harvest_graph();
   void bicgs_body(zd* x, const zd* g, long gs, bool herm, int nrhs, int prec);
   void* dalloc(size_t bytes);
 
@@ -218,14 +219,6 @@ class Engine {
   int maxit_ = 0;
   bool prof_ = false;
   KTimer kt_[KT_COUNT];
-  // timing events recorded inside a captured graph (re-recorded by every replay)
-  struct GraphEvent {
-    int id;
-    hipEvent_t a, b;
-  };
-  std::vector<GraphEvent> gev_;
-  int gev_used_ = 0;
-  bool capturing_ = false;
   Tuning tune_[2][2];  // [FP64, FP32 operator][batch class]
   std::vector<void*> allocs_;
 };

@@ -118,6 +118,10 @@ _SIGS = {
     "isg_warmnet_get_scales": (ctypes.c_int, [_V, _P(ctypes.c_float)]),
     "isg_warmnet_forward_dev": (ctypes.c_int, [_V, _V, _V, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "isg_warmnet_loss_grad_dev": (ctypes.c_int, [_V, _V, _V, ctypes.c_int, ctypes.c_int, ctypes.c_int, _D]),
+    "isg_warmnet_forward": (ctypes.c_int, [_V, _P(ctypes.c_float), _P(ctypes.c_float), ctypes.c_int, ctypes.c_int,
+                                           ctypes.c_int]),
+    "isg_warmnet_loss_grad": (ctypes.c_int, [_V, _P(ctypes.c_float), _P(ctypes.c_float), ctypes.c_int, ctypes.c_int,
+                                             ctypes.c_int, _D]),
     "isg_warmnet_adam": (ctypes.c_int, [_V, ctypes.c_float]),
     "isg_warmnet_save": (ctypes.c_int, [_V, ctypes.c_char_p]),
     "isg_warmnet_load": (ctypes.c_int, [_V, ctypes.c_char_p]),
@@ -501,22 +505,21 @@ class WarmNet:
     def load(self, path):
         self._err(self.lib.isg_warmnet_load(self.h, str(path).encode()), "isg_warmnet_load")
 
-    # host-array conveniences (device staging through torch tensors)
+    # host arrays (staged through device buffers the library keeps)
     def forward(self, x):
         """x: (batch, H, W, 6) float32 host array -> (batch, H, W, 2)."""
-        import torch
-        xt = torch.from_numpy(np.ascontiguousarray(x, dtype=np.float32)).cuda()
-        out = torch.empty(xt.shape[:3] + (2,), dtype=torch.float32, device=xt.device)
-        torch.cuda.synchronize()
-        self.forward_dev(xt.data_ptr(), out.data_ptr(), xt.shape[0], xt.shape[1], xt.shape[2])
-        return out.cpu().numpy()
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        out = np.empty(x.shape[:3] + (2,), np.float32)
+        self._err(self.lib.isg_warmnet_forward(self.h, _fp(x), _fp(out), *x.shape[:3]), "isg_warmnet_forward")
+        return out
 
     def loss_grad(self, x, target):
-        import torch
-        xt = torch.from_numpy(np.ascontiguousarray(x, dtype=np.float32)).cuda()
-        tt = torch.from_numpy(np.ascontiguousarray(target, dtype=np.float32)).cuda()
-        torch.cuda.synchronize()
-        return self.loss_grad_dev(xt.data_ptr(), tt.data_ptr(), xt.shape[0], xt.shape[1], xt.shape[2])
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(target, dtype=np.float32)
+        loss = ctypes.c_double()
+        self._err(self.lib.isg_warmnet_loss_grad(self.h, _fp(x), _fp(t), *x.shape[:3], ctypes.byref(loss)),
+                  "isg_warmnet_loss_grad")
+        return loss.value
 
 
 def hist_append(path, stats, create=False):

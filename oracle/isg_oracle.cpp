@@ -57,6 +57,10 @@ struct Oracle {
   // setup_mlfma.cpp:3-4 (beta, ninter).
   int L = 0, box = 8, beta = 5, ninter = 30, numrx = 0;
   double res = 0.1, k0 = 0.0;
+  // 0: plane-wave transmitters and far-field receivers (the reference);
+  // > 0: line-source transmitters and point receivers on a circle of this
+  // radius (wavelengths), equispaced from angle 0
+  double radius = 0.0;
   int N = 0, len = 0;
   std::vector<int> nclus, nsamp, nterm;
   std::vector<double> csize;
@@ -504,8 +508,29 @@ void mlfmah(Oracle& o, cd* x, cd* r) {
   }
 }
 
-// solve.cpp:249-355: far-field pattern at the receiver angles.
+// Finite-radius mode: receiver m at radius (cos, sin)(2 pi m / numrx) and the
+// midpoint-rule Green's function weight (i/4) H0(k0 |rho_m - r_n|) res^2.
+cd circle_point(double radius, int m, int count) {
+  double a = 2 * M_PI * m / count;
+  return cd(radius * std::cos(a), radius * std::sin(a));
+}
+cd rx_weight(const Oracle& o, int m, int n) {
+  double d = std::abs(circle_point(o.radius, m, o.numrx) - o.pos[n]);
+  return cd(0, 0.25) * cd(j0(o.k0 * d), y0(o.k0 * d)) * (o.res * o.res);
+}
+
+// solve.cpp:249-355: far-field pattern at the receiver angles (radius 0), or
+// the scattered field at the receiver circle (radius > 0, direct sums).
 void aggregate(Oracle& o, const cd* x, cd* r) {
+  if (o.radius > 0) {
+#pragma omp parallel for
+    for (int m = 0; m < o.numrx; m++) {
+      cd s = 0;
+      for (int n = 0; n < o.N; n++) s += rx_weight(o, m, n) * x[n];
+      r[m] = s;
+    }
+    return;
+  }
   o.matvec_out++;
   p2m(o, x);
   for (int i = o.L - 2; i > 1; i--) m2m(o, i);
@@ -519,8 +544,18 @@ void aggregate(Oracle& o, const cd* x, cd* r) {
   }
 }
 
-// solve.cpp:356-456: receiver back-projection; x is the output.
+// solve.cpp:356-456: receiver back-projection; x is the output.  Radius > 0:
+// the exact adjoint of the direct-sum aggregate.
 void aggregateh(Oracle& o, cd* x, const cd* r) {
+  if (o.radius > 0) {
+#pragma omp parallel for
+    for (int n = 0; n < o.N; n++) {
+      cd s = 0;
+      for (int m = 0; m < o.numrx; m++) s += std::conj(rx_weight(o, m, n)) * r[m];
+      x[n] = s;
+    }
+    return;
+  }
   o.matvec_out++;
   int na = 2 * o.ninter, S0 = o.nsamp[0];
   for (int m = 0; m < S0; m++) {
@@ -697,9 +732,15 @@ void dbim_setup(Oracle& o, Dbim& d, const cd* otrue) {
   }
   d.inc.assign((size_t)T * N, cd(0, 0));
   for (int i = 0; i < T; i++)
-    for (int n = 0; n < N; n++)
-      d.inc[(size_t)i * N + n] =
-          std::exp(cd(0, kk * (d.tx[i].real() * o.pos[n].real() + d.tx[i].imag() * o.pos[n].imag())));
+    for (int n = 0; n < N; n++) {
+      if (o.radius > 0) {  // line source at the transmitter circle: (i/4) H0(k0 |r - rho_t|)
+        double dd = std::abs(o.pos[n] - circle_point(o.radius, i, T));
+        d.inc[(size_t)i * N + n] = cd(0, 0.25) * cd(j0(kk * dd), y0(kk * dd));
+      } else {
+        d.inc[(size_t)i * N + n] =
+            std::exp(cd(0, kk * (d.tx[i].real() * o.pos[n].real() + d.tx[i].imag() * o.pos[n].imag())));
+      }
+    }
   d.o.assign(otrue, otrue + N);
   d.mes.assign((size_t)T * R, cd(0, 0));
   std::vector<cd> x(N), g(N);
@@ -798,6 +839,7 @@ void* orc_create(int level, double res, double scale, int beta, int ninter, int 
   return o;
 }
 void orc_destroy(void* h) { delete (Oracle*)h; }
+void orc_set_radius(void* h, double radius) { ((Oracle*)h)->radius = radius; }
 
 int orc_numunk(void* h) { return ((Oracle*)h)->N; }
 int orc_numsamp(void* h, int i) { return ((Oracle*)h)->nsamp[i]; }

@@ -43,6 +43,7 @@ def lib():
             "orc_create": (_V, [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int,
                                 ctypes.c_int, ctypes.c_int]),
             "orc_destroy": (None, [_V]),
+            "orc_set_radius": (None, [_V, ctypes.c_double]),
             "orc_numunk": (ctypes.c_int, [_V]),
             "orc_numsamp": (ctypes.c_int, [_V, ctypes.c_int]),
             "orc_numterm": (ctypes.c_int, [_V, ctypes.c_int]),
@@ -111,12 +112,17 @@ class Oracle:
     (8 * 2**(level-1))**2 pixels of `res` wavelengths.  k0 = 2 pi scale.
     """
 
-    def __init__(self, level, numrx=16, res=0.1, scale=1.0, beta=5, ninter=30):
+    def __init__(self, level, numrx=16, res=0.1, scale=1.0, beta=5, ninter=30, radius=0.0):
+        """radius > 0: line-source transmitters and receivers on a circle of that
+        radius (wavelengths) instead of plane waves and far-field receivers."""
         self.L = lib()
         self.level = level
         self.numrx = numrx
         self.res = res
+        self.radius = radius
         self.h = self.L.orc_create(level, res, scale, beta, ninter, numrx)
+        if radius > 0:
+            self.L.orc_set_radius(self.h, radius)
         self.N = self.L.orc_numunk(self.h)
         self.k0 = self.L.orc_k0(self.h)
         self.numsamp = [self.L.orc_numsamp(self.h, i) for i in range(level)]

@@ -24,8 +24,8 @@ def isg():
 def test_graph_solve_bit_identical(isg, herm):
     print("graph vs eager bicgs, herm", herm, flush=True)
     o = oracle_for(4)
-    g = 0.3 * rand_c(o.N, 1) * o.k0 ** 2
-    b = rand_c(o.N, 2)
+    g = 0.3 * rand_c(np.random.default_rng(1), o.N) * o.k0 ** 2
+    b = rand_c(np.random.default_rng(2), o.N)
     out = []
     for graph in (0, 1):
         s = isg.Solver(level=4, toli=1e-10, iti=300, graph=graph)
@@ -35,7 +35,7 @@ def test_graph_solve_bit_identical(isg, herm):
     np.testing.assert_array_equal(out[0][0], out[1][0])
 
 
-def test_graph_dbim_bit_identical_and_timed(isg):
+def test_graph_dbim_bit_identical(isg):
     print("graph vs eager batched DBIM", flush=True)
     o = oracle_for(4, numrx=16)
     otrue = isg.raster_to_tree(isg.phantom_blobs(o.side, 3, 1), o.unkmap())
@@ -44,12 +44,23 @@ def test_graph_dbim_bit_identical_and_timed(isg):
         s = isg.Solver(level=4, numrx=16, numtx=16, toli=1e-6, iti=300, regular=float(np.sqrt(1e-3)), max_rhs=16,
                        precision=1, graph=graph)
         d = isg.Dbim(s, otrue, field_start=2)
-        s.profile(True)
         its = [d.iterate()["solve_iters"] for _ in range(2)]
-        ms, n = s.profile_read("near_fp32")
-        assert n > 0 and ms > 0
-        s.profile(False)
-        res.append((d.get("o"), its, n))
+        res.append((d.get("o"), its))
     assert res[0][1] == res[1][1]
-    assert res[0][2] == res[1][2]
     np.testing.assert_array_equal(res[0][0], res[1][0])
+
+
+def test_kernel_timing_with_graph_config(isg):
+    """Live kernel timing needs eager launches (events recorded by graph
+    nodes carry no timestamps): with timing on, a graph-enabled context
+    launches eagerly and every near-field launch is timed."""
+    o = oracle_for(4, numrx=16)
+    otrue = isg.raster_to_tree(isg.phantom_blobs(o.side, 3, 1), o.unkmap())
+    s = isg.Solver(level=4, numrx=16, numtx=16, toli=1e-6, iti=300, regular=float(np.sqrt(1e-3)), max_rhs=16,
+                   precision=1, graph=1)
+    d = isg.Dbim(s, otrue, field_start=2)
+    s.profile(True)
+    d.iterate()
+    ms, n = s.profile_read("near_fp32")
+    s.profile(False)
+    assert n > 0 and ms > 0

@@ -23,7 +23,7 @@ def isg():
 
 
 def check(s, o, nrhs, tol=1e-12):
-    X = np.stack([rand_c(o.N, 10 + r) for r in range(nrhs)])
+    X = np.stack([rand_c(np.random.default_rng(10 + r), o.N) for r in range(nrhs)])
     Y = s.mlfma_many(X)
     for r in range(nrhs):
         assert relerr(Y[r], o.mlfma(X[r])) < tol
@@ -65,7 +65,7 @@ def test_autotune_save_load(isg, tmp_path):
     p = tmp_path / "tune.txt"
     s.tuning_save(p)
     # the context's batched matvec runs the FP32 operator with the tuned choices
-    X = np.stack([rand_c(o.N, 30 + r) for r in range(16)])
+    X = np.stack([rand_c(np.random.default_rng(30 + r), o.N) for r in range(16)])
     Y = s.mlfma_many(X)
     for r in range(16):
         e = relerr(Y[r], o.mlfma(X[r]))
