@@ -257,8 +257,9 @@ def main():
         },
         "m2l": {"achieved_GBs": round(m2l_gbs, 1), "peak_GBs": PEAK_HBM_GBS, "frac": round(m2l_gbs / PEAK_HBM_GBS, 4),
                 "ms_per_matvec_launchset": round(m2l_ms / max(m2l_n, 1), 4)},
-        # x/r update reads x, p, s, t, r~ and writes x, r: 7 complex FP64 vectors per RHS iteration
-        "bicgs_update": {"GBs": round(7 * 16 * side * side * iters / max(vec_ms * 1e-3, 1e-12) / 1e9, 1),
+        # x/r update reads x, p, s, t, r~ and writes x, r: x is FP64 (16 B), the
+        # work vectors FP64 (plain) or FP32 (mixed): 7 x 16 B or 2 x 16 + 5 x 8 B per unknown
+        "bicgs_update": {"GBs": round((72 if mixed else 112) * side * side * iters / max(vec_ms * 1e-3, 1e-12) / 1e9, 1),
                          "avg_launch_ms": round(vec_ms / max(vec_n, 1), 4)},
         "solves": sum(s["solves"] for s in stats),
         "bicgstab_iterations": iters,

@@ -119,6 +119,9 @@ int isg_mlfma_many(isg_context* ctx, const double* x, double* r, int nrhs);
 
 /* ---- device-pointer, batched surface (vectors [nrhs][numunk], receivers [nrhs][numrx]) ---- */
 int isg_mlfma_dev(isg_context* ctx, const void* x, void* r, int nrhs);
+/* the same on interleaved FP32 complex vectors with the FP32 operator (precision 1
+ * contexts): the matrix-vector product the mixed-precision solver runs */
+int isg_mlfma_f32_dev(isg_context* ctx, const void* x, void* r, int nrhs);
 int isg_aggregate_dev(isg_context* ctx, const void* x, void* r, int nrhs);
 int isg_aggregateh_dev(isg_context* ctx, void* x, const void* r, int nrhs);
 /* o_stride 0: one contrast shared by every right-hand side; numunk: one per right-hand side.
@@ -199,8 +202,10 @@ int isg_raster_bin_read(const char* path, int numunk, double* raster);
 /* ---- autotuner (kernel variants per stage and level, measured on the device) ----
  * stage 0 near-field tile width (16 or 32 boxes), 1 P2M tile width, 2 M2L tile
  * variant of `level` (0: 4x4 boxes x 8 samples, 1: 8x8 x 8, 2: 8x8 x 16,
- * 3: 16x16 x 8, 4: 8x8 x 32, 5: 16x16 x 16; 4 and 5 FP32 only), 3 / 4 M2M / L2L
- * kernel of `level` (0 band MFMA, 1 chunked LDS, 2 direct); value -1 = built-in
+ * 3: 16x16 x 8, 4: 8x8 x 32, 5: 16x16 x 16; 6, 7, 8: 8x8 x 16, 32, 8 with the
+ * operators held in registers; 4, 5 and 7 FP32 only), 3 / 4 M2M / L2L
+ * kernel of `level` (0 band MFMA, 1 chunked LDS, 2 direct, 3 / 4 band MFMA with
+ * 16 / 32 columns per workgroup); value -1 = built-in
  * rule.  precision 0 = FP64 operator, 1 = FP32 operator of the mixed path;
  * batch_class 0 for fewer than 16 right-hand sides, 1 otherwise.
  * isg_autotune times every admissible variant (reps launches each) and keeps

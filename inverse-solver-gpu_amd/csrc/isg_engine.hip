@@ -192,6 +192,9 @@ Engine::Engine(const EngineConfig& cfg) : cfg_(cfg) {
     }
   vec_.assign(V_COUNT, nullptr);
   for (int k = 0; k < V_COUNT; k++) vec_[k] = (zd*)dalloc((size_t)R * P_.N * sizeof(zd));
+  vec32_.assign(V_COUNT, nullptr);
+  if (has32_)
+    for (int k = 0; k < V_COUNT; k++) vec32_[k] = (cplx<float>*)dalloc((size_t)R * P_.N * sizeof(cplx<float>));
   nblk_ = (int)std::min<long>(1024, std::max<long>(1, (P_.N + kBlock * 4 - 1) / (kBlock * 4)));
   part_ = (double2*)dalloc((size_t)R * kSlots * nblk_ * sizeof(double2));
   state_ = (BiState*)dalloc((size_t)R * sizeof(BiState));
@@ -283,8 +286,11 @@ void Engine::plan_interp(DevOp<T>& d, std::vector<InterpShape>& m2m, std::vector
   }
 }
 
+// kinds: 0 band MFMA with the widest column count, 1 chunked LDS, 2 direct,
+// 3 / 4 band MFMA with 16 / 32 columns per workgroup (less LDS, more waves)
 bool Engine::kind_ok(const InterpShape& sh, int kind) const {
-  return (kind == 0 && sh.ncb > 0) || (kind == 1 && sh.chunk_ok) || kind == 2;
+  return (kind == 0 && sh.ncb > 0) || (kind == 1 && sh.chunk_ok) || kind == 2 || (kind == 3 && sh.ncb >= 1) ||
+         (kind == 4 && sh.ncb >= 2);
 }
 
 // band GEMM when it yields at least one workgroup per CU, else chunked, else direct
@@ -296,8 +302,8 @@ int Engine::interp_kind(const InterpShape& sh, int np, int nrhs) const {
 
 // --------------------------------------------------------------- launches
 // leaf radiation patterns on the matrix cores
-template <typename T>
-void Engine::launch_p2m(const zd* y, int nrhs, const int* act, int force) {
+template <typename T, typename TI>
+void Engine::launch_p2m(const cplx<TI>* y, int nrhs, const int* act, int force) {
   const int Lf = P_.L - 1, S = P_.nsamp[Lf], B = P_.B;
   const int tuned = tune(sizeof(T) == 4 ? PREC_MIXED : PREC_FP64, nrhs).p2m_nb;
   int nb = force > 0 ? force : (tuned > 0 ? tuned : (B >= 32 ? 32 : 16));
@@ -307,9 +313,9 @@ void Engine::launch_p2m(const zd* y, int nrhs, const int* act, int force) {
   cplx<T>* out = d.agm + lvl_off_[Lf];
   long as = (long)lvl_size_[Lf];
   if (nb == 32)
-    k_p2m_mfma<T, double, 32><<<grid, blk, 0, stream_>>>(y, P_.N, out, as, d.coeffT, S, act);
+    k_p2m_mfma<T, TI, 32><<<grid, blk, 0, stream_>>>(y, P_.N, out, as, d.coeffT, S, act);
   else
-    k_p2m_mfma<T, double, 16><<<grid, blk, 0, stream_>>>(y, P_.N, out, as, d.coeffT, S, act);
+    k_p2m_mfma<T, TI, 16><<<grid, blk, 0, stream_>>>(y, P_.N, out, as, d.coeffT, S, act);
 }
 
 // child level i+1 -> parent level i
@@ -323,19 +329,21 @@ void Engine::launch_m2m(int i, int nrhs, const int* act, int force) {
   dim3 blk(kBlock);
   const int tuned = tune(sizeof(T) == 4 ? PREC_MIXED : PREC_FP64, nrhs).m2m[i];
   const int kind = kind_ok(sh, force) ? force : (kind_ok(sh, tuned) ? tuned : interp_kind(sh, np, nrhs));
-  if (kind == 0) {
-    dim3 grid(4 * np / (16 * sh.ncb), nrhs);
+  if (kind == 0 || kind >= 3) {
+    const int ncb = kind == 3 ? 1 : (kind == 4 ? 2 : sh.ncb);
+    const size_t lds = (size_t)16 * ncb * (Sc + 1) * sizeof(cplx<T>);
+    dim3 grid(4 * np / (16 * ncb), nrhs);
     const T* ab = d.band_m2m[i];
     const int* s0 = d.bs0_m2m[i];
-    if (sh.ncb == 4)
-      k_interp_band<T, 0, 4, kQM2M><<<grid, blk, sh.lds_band, stream_>>>(child, cs, par, ps, ab, s0, sh.kb,
-                                                                         d.shiftM[i], Sp, Sc, 0, act);
-    else if (sh.ncb == 2)
-      k_interp_band<T, 0, 2, kQM2M><<<grid, blk, sh.lds_band, stream_>>>(child, cs, par, ps, ab, s0, sh.kb,
-                                                                         d.shiftM[i], Sp, Sc, 0, act);
+    if (ncb == 4)
+      k_interp_band<T, 0, 4, kQM2M><<<grid, blk, lds, stream_>>>(child, cs, par, ps, ab, s0, sh.kb, d.shiftM[i], Sp,
+                                                                 Sc, 0, act);
+    else if (ncb == 2)
+      k_interp_band<T, 0, 2, kQM2M><<<grid, blk, lds, stream_>>>(child, cs, par, ps, ab, s0, sh.kb, d.shiftM[i], Sp,
+                                                                 Sc, 0, act);
     else
-      k_interp_band<T, 0, 1, kQM2M><<<grid, blk, sh.lds_band, stream_>>>(child, cs, par, ps, ab, s0, sh.kb,
-                                                                         d.shiftM[i], Sp, Sc, 0, act);
+      k_interp_band<T, 0, 1, kQM2M><<<grid, blk, lds, stream_>>>(child, cs, par, ps, ab, s0, sh.kb, d.shiftM[i], Sp,
+                                                                 Sc, 0, act);
   } else if (kind == 1) {
     int nch = (Sp + sh.kch - 1) / sh.kch;
     dim3 grid(np / sh.g * nch, nrhs);
@@ -360,19 +368,21 @@ void Engine::launch_l2l(int i, bool accumulate, int nrhs, const int* act, int fo
   int acc = accumulate ? 1 : 0;
   const int tuned = tune(sizeof(T) == 4 ? PREC_MIXED : PREC_FP64, nrhs).l2l[i];
   const int kind = kind_ok(sh, force) ? force : (kind_ok(sh, tuned) ? tuned : interp_kind(sh, np, nrhs));
-  if (kind == 0) {
-    dim3 grid(4 * np / (16 * sh.ncb), nrhs);
+  if (kind == 0 || kind >= 3) {
+    const int ncb = kind == 3 ? 1 : (kind == 4 ? 2 : sh.ncb);
+    const size_t lds = (size_t)16 * ncb * (Sp + 1) * sizeof(cplx<T>);
+    dim3 grid(4 * np / (16 * ncb), nrhs);
     const T* ab = d.band_l2l[i];
     const int* s0 = d.bs0_l2l[i];
-    if (sh.ncb == 4)
-      k_interp_band<T, 1, 4, kQL2L><<<grid, blk, sh.lds_band, stream_>>>(par, ps, child, cs, ab, s0, sh.kb,
-                                                                         d.shiftL[i], Sc, Sp, acc, act);
-    else if (sh.ncb == 2)
-      k_interp_band<T, 1, 2, kQL2L><<<grid, blk, sh.lds_band, stream_>>>(par, ps, child, cs, ab, s0, sh.kb,
-                                                                         d.shiftL[i], Sc, Sp, acc, act);
+    if (ncb == 4)
+      k_interp_band<T, 1, 4, kQL2L><<<grid, blk, lds, stream_>>>(par, ps, child, cs, ab, s0, sh.kb, d.shiftL[i], Sc,
+                                                                 Sp, acc, act);
+    else if (ncb == 2)
+      k_interp_band<T, 1, 2, kQL2L><<<grid, blk, lds, stream_>>>(par, ps, child, cs, ab, s0, sh.kb, d.shiftL[i], Sc,
+                                                                 Sp, acc, act);
     else
-      k_interp_band<T, 1, 1, kQL2L><<<grid, blk, sh.lds_band, stream_>>>(par, ps, child, cs, ab, s0, sh.kb,
-                                                                         d.shiftL[i], Sc, Sp, acc, act);
+      k_interp_band<T, 1, 1, kQL2L><<<grid, blk, lds, stream_>>>(par, ps, child, cs, ab, s0, sh.kb, d.shiftL[i], Sc,
+                                                                 Sp, acc, act);
   } else if (kind == 1) {
     int nch = (Sc + sh.kch - 1) / sh.kch;
     dim3 grid(np * nch, nrhs);
@@ -385,9 +395,11 @@ void Engine::launch_l2l(int i, bool accumulate, int nrhs, const int* act, int fo
   }
 }
 
-// M2L tile variants: TS x TS target boxes and KC samples per workgroup
-constexpr int kM2LVariants = 6;
-constexpr int kM2LShape[kM2LVariants][2] = {{4, 8}, {8, 8}, {8, 16}, {16, 8}, {8, 32}, {16, 16}};
+// M2L variants: TS x TS target boxes and KC samples per workgroup; 0-5
+// k_m2l_tile, 6-8 k_m2l_reg (operators in registers, 8x8 tiles)
+constexpr int kM2LVariants = 9;
+constexpr int kM2LShape[kM2LVariants][2] = {{4, 8}, {8, 8}, {8, 16}, {16, 8}, {8, 32}, {16, 16},
+                                            {8, 16}, {8, 32}, {8, 8}};
 
 template <typename T>
 bool Engine::m2l_ok(int v, int i) const {
@@ -418,6 +430,11 @@ void Engine::launch_m2l(int i, int nrhs, const int* act, int force) {
     case 1: k_m2l_tile<T, 8, 8><<<grid, blk, 0, stream_>>>(m, l, st, d.trans[i], i, S, act); break;
     case 2: k_m2l_tile<T, 8, 16><<<grid, blk, 0, stream_>>>(m, l, st, d.trans[i], i, S, act); break;
     case 3: k_m2l_tile<T, 16, 8><<<grid, blk, 0, stream_>>>(m, l, st, d.trans[i], i, S, act); break;
+    case 6: k_m2l_reg<T, 16><<<grid, blk, 0, stream_>>>(m, l, st, d.trans[i], i, S, act); break;
+    case 8: k_m2l_reg<T, 8><<<grid, blk, 0, stream_>>>(m, l, st, d.trans[i], i, S, act); break;
+    case 7:
+      if constexpr (sizeof(T) == 4) k_m2l_reg<T, 32><<<grid, blk, 0, stream_>>>(m, l, st, d.trans[i], i, S, act);
+      break;
     default:
       if constexpr (sizeof(T) == 4) {
         if (v == 4)
@@ -431,8 +448,8 @@ void Engine::launch_m2l(int i, int nrhs, const int* act, int force) {
 
 // Near field + L2P on the matrix cores (k_near_mfma), or L2P alone when
 // with_near is false (receiver back-projection).
-template <typename T>
-void Engine::launch_near(const zd* y, zd* r, int nrhs, const int* act, bool with_near, int force) {
+template <typename T, typename TI>
+void Engine::launch_near(const cplx<TI>* y, cplx<TI>* r, int nrhs, const int* act, bool with_near, int force) {
   const int Lf = P_.L - 1, S = P_.nsamp[Lf], B = P_.B;
   const long N = P_.N, ls = (long)lvl_size_[Lf];
   const int tuned = tune(sizeof(T) == 4 ? PREC_MIXED : PREC_FP64, nrhs).near_nb;
@@ -445,37 +462,63 @@ void Engine::launch_near(const zd* y, zd* r, int nrhs, const int* act, bool with
   const int kt = sizeof(T) == 4 ? KT_NEAR32 : KT_NEAR;
   if (with_near) prof_begin(kt);
   if (nb == 32)
-    k_near_mfma<T, double, 32><<<grid, blk, lds, stream_>>>(y, N, loc, ls, r, N, d.near, d.basisT, S, 1 << Lf,
-                                                            with_near ? 1 : 0, act);
+    k_near_mfma<T, TI, 32><<<grid, blk, lds, stream_>>>(y, N, loc, ls, r, N, d.near, d.basisT, S, 1 << Lf,
+                                                        with_near ? 1 : 0, act);
   else
-    k_near_mfma<T, double, 16><<<grid, blk, lds, stream_>>>(y, N, loc, ls, r, N, d.near, d.basisT, S, 1 << Lf,
-                                                            with_near ? 1 : 0, act);
+    k_near_mfma<T, TI, 16><<<grid, blk, lds, stream_>>>(y, N, loc, ls, r, N, d.near, d.basisT, S, 1 << Lf,
+                                                        with_near ? 1 : 0, act);
   if (with_near) prof_end(kt);
 }
 
 // r = G y (y already contrast-weighted).  Stage order of matvec_gpu.cu:103-264:
 // P2M, M2M up to level 2, M2L on levels 2..leaf, L2L down, then near field
 // and L2P together.
-template <typename T>
-void Engine::apply(const zd* y, zd* r, int nrhs, const int* act) {
+template <typename T, typename TI>
+void Engine::apply(const cplx<TI>* y, cplx<TI>* r, int nrhs, const int* act) {
   const int L = P_.L, Lf = L - 1;
-  launch_p2m<T>(y, nrhs, act);
+  launch_p2m<T, TI>(y, nrhs, act);
   for (int i = L - 2; i >= 2; i--) launch_m2m<T>(i, nrhs, act);
   const int kt = sizeof(T) == 4 ? KT_M2L32 : KT_M2L;
   prof_begin(kt);
   for (int i = 2; i <= Lf; i++) launch_m2l<T>(i, nrhs, act);
   prof_end(kt);
   for (int i = 2; i <= L - 2; i++) launch_l2l<T>(i, true, nrhs, act);
-  launch_near<T>(y, r, nrhs, act, true);
+  launch_near<T, TI>(y, r, nrhs, act, true);
 }
 
-void Engine::apply_mlfma(const zd* y, zd* r, int nrhs, const int* act, int prec) {
-  if (!ok()) return;
-  if (prec == PREC_MIXED && has32_)
-    apply<float>(y, r, nrhs, act);
+void Engine::apply_mlfma(const zd* y, zd* r, int nrhs, const int* act, int prec) { apply_vec<double>(y, r, nrhs, act, prec); }
+
+template <typename V>
+cplx<V>* Engine::vb(int k) {
+  if constexpr (sizeof(V) == 4)
+    return vec32_[k];
   else
-    apply<double>(y, r, nrhs, act);
+    return vec_[k];
+}
+
+// FP32 vectors always go with the FP32 operator; FP64 vectors with either
+template <typename V>
+void Engine::apply_vec(const cplx<V>* y, cplx<V>* r, int nrhs, const int* act, int prec) {
+  if (!ok()) return;
+  if constexpr (sizeof(V) == 4) {
+    apply<float, float>(y, r, nrhs, act);
+  } else {
+    if (prec == PREC_MIXED && has32_)
+      apply<float, double>(y, r, nrhs, act);
+    else
+      apply<double, double>(y, r, nrhs, act);
+  }
   ISG_HIP(hipGetLastError());
+}
+
+void Engine::mlfma_f32(const cplx<float>* x, cplx<float>* r, int nrhs) {
+  if (!has32_) {
+    set_error("mlfma_f32: the context has no FP32 operator (precision 1 needed)");
+    return;
+  }
+  apply_vec<float>(x, r, nrhs, nullptr, PREC_MIXED);
+  matvecs += nrhs;
+  matvecs32 += nrhs;
 }
 
 void Engine::mlfma(const zd* x, zd* r, int nrhs, const int* act, int prec) {
@@ -489,7 +532,7 @@ void Engine::mlfma(const zd* x, zd* r, int nrhs, const int* act, int prec) {
 void Engine::aggregate(const zd* x, zd* r, int nrhs) {
   if (!ok()) return;
   const int L = P_.L;
-  launch_p2m<double>(x, nrhs, nullptr);
+  launch_p2m<double, double>(x, nrhs, nullptr);
   for (int i = L - 2; i >= 0; i--) launch_m2m<double>(i, nrhs, nullptr);
   int R = cfg_.plan.numrx;
   k_rx_interp<double><<<dim3(nblocks(R), nrhs), dim3(kBlock), 0, stream_>>>(
@@ -507,7 +550,7 @@ void Engine::aggregateh(const zd* r, zd* x, int nrhs) {
   k_rx_anterp<double><<<dim3(nblocks(P_.nsamp[0]), nrhs), dim3(kBlock), 0, stream_>>>(
       r, R, op64_.agl + lvl_off_[0], (long)lvl_size_[0], op64_.rxaT, rxaI_, P_.nsamp[0], 2 * cfg_.plan.ninter, sc);
   for (int i = 0; i <= L - 2; i++) launch_l2l<double>(i, false, nrhs, nullptr);
-  launch_near<double>(nullptr, x, nrhs, nullptr, false);
+  launch_near<double, double>(nullptr, x, nrhs, nullptr, false);
   ISG_HIP(hipGetLastError());
 }
 
@@ -577,21 +620,22 @@ void Engine::profile_read(int id, double& ms, long long& launches) {
 // bicgs_gpu.cu:116-224, batched over right-hand sides with device-resident
 // scalars and per-system tolerances (tol_h_ -> tolv_); the host reads only
 // the per-system activity flags once per iteration.
+template <typename V>
 void Engine::run_bicgs(zd* x, const zd* g, long gs, const zd* b, bool herm, int nrhs, bool x_zero, int maxit,
-                       int prec, const zd* r0) {
+                       int prec, const cplx<V>* r0) {
   const long N = P_.N;
-  zd *r = vec_[V_R], *rt = vec_[V_RT], *y = vec_[V_Y], *op = vec_[V_OP];
+  cplx<V>*r = vb<V>(V_R), *rt = vb<V>(V_RT), *y = vb<V>(V_Y), *op = vb<V>(V_OP);
   const int h = herm ? 1 : 0;
   dim3 grid(nblk_, nrhs), blk(kBlock);
   ISG_HIP(hipMemcpyAsync(tolv_, tol_h_, nrhs * sizeof(double), hipMemcpyHostToDevice, stream_));
   const long long f32 = (prec == PREC_MIXED && has32_) ? 1 : 0;
   if (!x_zero && !r0) {
-    k_prep<<<grid, blk, 0, stream_>>>(x, g, gs, y, N, h);
-    apply_mlfma(y, op, nrhs, nullptr, prec);
+    k_prep<V><<<grid, blk, 0, stream_>>>(x, g, gs, y, N, h);
+    apply_vec<V>(y, op, nrhs, nullptr, prec);
     matvecs += nrhs;
     matvecs32 += f32 * nrhs;
   }
-  k_bi_init<<<grid, blk, 0, stream_>>>(x, op, g, gs, b, r0, r, rt, N, h, x_zero ? 1 : 0, part_);
+  k_bi_init<V><<<grid, blk, 0, stream_>>>(x, op, g, gs, b, r0, r, rt, N, h, x_zero ? 1 : 0, part_);
   k_bi_fin<<<dim3(nrhs), blk, 0, stream_>>>(PH_INIT, state_, active_, part_, nblk_, tolv_, maxit);
   maxit_ = maxit;
   // one iteration = bicgs_body; with graphs it is captured once per call and
@@ -600,7 +644,7 @@ void Engine::run_bicgs(zd* x, const zd* g, long gs, const zd* b, bool herm, int 
   if (cfg_.graph && !prof_) {
     hipGraph_t gr = nullptr;
     ISG_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
-    bicgs_body(x, g, gs, herm, nrhs, prec);
+    bicgs_body<V>(x, g, gs, herm, nrhs, prec);
     ISG_HIP(hipStreamEndCapture(stream_, &gr));
     if (gr) {
       ISG_HIP(hipGraphInstantiate(&exec, gr, nullptr, nullptr, 0));
@@ -619,7 +663,7 @@ void Engine::run_bicgs(zd* x, const zd* g, long gs, const zd* b, bool herm, int 
     if (exec)
       ISG_HIP(hipGraphLaunch(exec, stream_));
     else
-      bicgs_body(x, g, gs, herm, nrhs, prec);
+      bicgs_body<V>(x, g, gs, herm, nrhs, prec);
     matvecs += 2 * nact;
     matvecs32 += f32 * 2 * nact;
   }
@@ -631,23 +675,24 @@ void Engine::run_bicgs(zd* x, const zd* g, long gs, const zd* b, bool herm, int 
 // One BiCGSTAB iteration for every active system (bicgs_gpu.cu:146-213),
 // ending with the copy of the activity flags to the host.  No memset: the
 // sequence is captured into a graph.
+template <typename V>
 void Engine::bicgs_body(zd* x, const zd* g, long gs, bool herm, int nrhs, int prec) {
   const long N = P_.N;
-  zd *r = vec_[V_R], *rt = vec_[V_RT], *p = vec_[V_P], *v = vec_[V_V], *s = vec_[V_S], *t = vec_[V_T],
-     *y = vec_[V_Y], *op = vec_[V_OP];
+  cplx<V>*r = vb<V>(V_R), *rt = vb<V>(V_RT), *p = vb<V>(V_P), *v = vb<V>(V_V), *s = vb<V>(V_S), *t = vb<V>(V_T),
+     *y = vb<V>(V_Y), *op = vb<V>(V_OP);
   const int h = herm ? 1 : 0;
   const int maxit = maxit_;
   dim3 grid(nblk_, nrhs), blk(kBlock);
-  k_bi_pupd<<<grid, blk, 0, stream_>>>(r, p, v, g, gs, y, N, h, state_);
-  apply_mlfma(y, op, nrhs, active_, prec);
-  k_bi_post1<<<grid, blk, 0, stream_>>>(p, op, g, gs, v, rt, N, h, state_, part_);
+  k_bi_pupd<V><<<grid, blk, 0, stream_>>>(r, p, v, g, gs, y, N, h, state_);
+  apply_vec<V>(y, op, nrhs, active_, prec);
+  k_bi_post1<V><<<grid, blk, 0, stream_>>>(p, op, g, gs, v, rt, N, h, state_, part_);
   k_bi_fin<<<dim3(nrhs), blk, 0, stream_>>>(PH_ALPHA, state_, active_, part_, nblk_, tolv_, maxit);
-  k_bi_supd<<<grid, blk, 0, stream_>>>(r, v, g, gs, s, y, N, h, state_);
-  apply_mlfma(y, op, nrhs, active_, prec);
-  k_bi_post2<<<grid, blk, 0, stream_>>>(s, op, g, gs, t, N, h, state_, part_);
+  k_bi_supd<V><<<grid, blk, 0, stream_>>>(r, v, g, gs, s, y, N, h, state_);
+  apply_vec<V>(y, op, nrhs, active_, prec);
+  k_bi_post2<V><<<grid, blk, 0, stream_>>>(s, op, g, gs, t, N, h, state_, part_);
   k_bi_fin<<<dim3(nrhs), blk, 0, stream_>>>(PH_OMEGA, state_, active_, part_, nblk_, tolv_, maxit);
   prof_begin(KT_VEC);
-  k_bi_xrupd<<<grid, blk, 0, stream_>>>(x, p, s, t, r, rt, N, state_, part_);
+  k_bi_xrupd<V><<<grid, blk, 0, stream_>>>(x, p, s, t, r, rt, N, state_, part_);
   prof_end(KT_VEC);
   k_bi_fin<<<dim3(nrhs), blk, 0, stream_>>>(PH_RESID, state_, active_, part_, nblk_, tolv_, maxit);
   ISG_HIP(hipMemcpyAsync(active_h_, active_, nrhs * sizeof(int), hipMemcpyDeviceToHost, stream_));
@@ -660,10 +705,10 @@ void Engine::true_residual(const zd* x, const zd* g, long gs, const zd* b, bool 
   const int h = herm ? 1 : 0;
   dim3 grid(nblk_, nrhs), blk(kBlock);
   zd *y = vec_[V_Y], *op = vec_[V_OP];
-  k_prep<<<grid, blk, 0, stream_>>>(x, g, gs, y, N, h);
+  k_prep<double><<<grid, blk, 0, stream_>>>(x, g, gs, y, N, h);
   apply_mlfma(y, op, nrhs, nullptr, PREC_FP64);
   matvecs += nrhs;
-  k_bi_init<<<grid, blk, 0, stream_>>>(x, op, g, gs, b, nullptr, res, vec_[V_RT], N, h, 0, part_);
+  k_bi_init<double><<<grid, blk, 0, stream_>>>(x, op, g, gs, b, nullptr, res, vec_[V_RT], N, h, 0, part_);
   ISG_HIP(hipMemcpyAsync(part_h_, part_, (size_t)nrhs * kSlots * nblk_ * sizeof(double2), hipMemcpyDeviceToHost,
                          stream_));
   ISG_HIP(hipStreamSynchronize(stream_));
@@ -683,7 +728,7 @@ void Engine::true_residual(const zd* x, const zd* g, long gs, const zd* b, bool 
 
 void Engine::bicgs(zd* x, const zd* g, long gs, const zd* b, bool herm, int nrhs, bool x_zero, double tol,
                    int maxit, std::vector<SolveStat>& out, int prec) {
-  solve(x, g, gs, b, herm, nrhs, x_zero, tol, maxit, out, prec, nullptr);
+  solve(x, g, gs, b, herm, nrhs, x_zero, tol, maxit, out, prec, false);
 }
 
 // r_x = b - A x and A d with the solver's operator, lambda = <Ad, r_x> / <Ad, Ad>,
@@ -696,31 +741,39 @@ void Engine::bicgs_blend(zd* x, const zd* d, const zd* g, long gs, const zd* b, 
     set_error("bicgs: nrhs exceeds max_rhs");
     return;
   }
-  const long N = P_.N;
-  const int h = herm ? 1 : 0;
-  const int p = (prec == PREC_MIXED && has32_) ? PREC_MIXED : PREC_FP64;
-  dim3 grid(nblk_, nrhs), blk(kBlock);
-  zd *y = vec_[V_Y], *op = vec_[V_OP], *res = vec_[V_RES], *ad = vec_[V_COR];
-  k_prep<<<grid, blk, 0, stream_>>>(x, g, gs, y, N, h);
-  apply_mlfma(y, op, nrhs, nullptr, p);
-  k_bi_init<<<grid, blk, 0, stream_>>>(x, op, g, gs, b, nullptr, res, vec_[V_RT], N, h, 0, part_);
-  k_prep<<<grid, blk, 0, stream_>>>(d, g, gs, y, N, h);
-  apply_mlfma(y, op, nrhs, nullptr, p);
-  k_bl_dots<<<grid, blk, 0, stream_>>>(d, op, g, gs, res, ad, N, h, part_);
-  k_bl_fin<<<dim3(nrhs), blk, 0, stream_>>>(part_, nblk_, lam_);
-  k_bl_apply<<<grid, blk, 0, stream_>>>(x, d, res, ad, lam_, N);
+  const bool mixed = prec == PREC_MIXED && has32_;
+  if (mixed)
+    blend<float>(x, d, g, gs, b, herm, nrhs, PREC_MIXED);
+  else
+    blend<double>(x, d, g, gs, b, herm, nrhs, PREC_FP64);
   matvecs += 2 * nrhs;
-  if (p == PREC_MIXED) matvecs32 += 2 * nrhs;
+  if (mixed) matvecs32 += 2 * nrhs;
   if (lam) {
     lam->resize(nrhs);
     ISG_HIP(hipMemcpyAsync(lam->data(), lam_, nrhs * sizeof(zd), hipMemcpyDeviceToHost, stream_));
     ISG_HIP(hipStreamSynchronize(stream_));
   }
-  solve(x, g, gs, b, herm, nrhs, false, tol, maxit, out, prec, res);
+  solve(x, g, gs, b, herm, nrhs, false, tol, maxit, out, prec, true);
+}
+
+template <typename V>
+void Engine::blend(zd* x, const zd* d, const zd* g, long gs, const zd* b, bool herm, int nrhs, int prec) {
+  const long N = P_.N;
+  const int h = herm ? 1 : 0;
+  dim3 grid(nblk_, nrhs), blk(kBlock);
+  cplx<V>*y = vb<V>(V_Y), *op = vb<V>(V_OP), *res = vb<V>(V_RES), *ad = vb<V>(V_COR);
+  k_prep<V><<<grid, blk, 0, stream_>>>(x, g, gs, y, N, h);
+  apply_vec<V>(y, op, nrhs, nullptr, prec);
+  k_bi_init<V><<<grid, blk, 0, stream_>>>(x, op, g, gs, b, nullptr, res, vb<V>(V_RT), N, h, 0, part_);
+  k_prep<V><<<grid, blk, 0, stream_>>>(d, g, gs, y, N, h);
+  apply_vec<V>(y, op, nrhs, nullptr, prec);
+  k_bl_dots<V><<<grid, blk, 0, stream_>>>(d, op, g, gs, res, ad, N, h, part_);
+  k_bl_fin<<<dim3(nrhs), blk, 0, stream_>>>(part_, nblk_, lam_);
+  k_bl_apply<V><<<grid, blk, 0, stream_>>>(x, d, res, ad, lam_, N);
 }
 
 void Engine::solve(zd* x, const zd* g, long gs, const zd* b, bool herm, int nrhs, bool x_zero, double tol,
-                   int maxit, std::vector<SolveStat>& out, int prec, const zd* r0) {
+                   int maxit, std::vector<SolveStat>& out, int prec, bool r0_in_res) {
   out.assign(nrhs, SolveStat{-1, 0, 0.0});
   if (!ok()) return;
   if (nrhs > cfg_.max_rhs) {
@@ -729,7 +782,10 @@ void Engine::solve(zd* x, const zd* g, long gs, const zd* b, bool herm, int nrhs
   }
   const bool mixed = prec == PREC_MIXED && has32_;
   for (int i = 0; i < nrhs; i++) tol_h_[i] = mixed ? cfg_.mixed_margin * tol : tol;
-  run_bicgs(x, g, gs, b, herm, nrhs, x_zero, maxit, mixed ? PREC_MIXED : PREC_FP64, r0);
+  if (mixed)
+    run_bicgs<float>(x, g, gs, b, herm, nrhs, x_zero, maxit, PREC_MIXED, r0_in_res ? vb<float>(V_RES) : nullptr);
+  else
+    run_bicgs<double>(x, g, gs, b, herm, nrhs, x_zero, maxit, PREC_FP64, r0_in_res ? vb<double>(V_RES) : nullptr);
   if (!ok()) return;
   for (int i = 0; i < nrhs; i++) {
     out[i].status = state_h_[i].status;
@@ -763,7 +819,7 @@ void Engine::solve(zd* x, const zd* g, long gs, const zd* b, bool herm, int nrhs
       if (!any || round == kRefineRounds) break;
       refinements++;
       ISG_HIP(hipMemsetAsync(cor, 0, (size_t)nrhs * P_.N * sizeof(zd), stream_));
-      run_bicgs(cor, g, gs, res, herm, nrhs, true, maxit, PREC_MIXED);
+      run_bicgs<float>(cor, g, gs, res, herm, nrhs, true, maxit, PREC_MIXED, nullptr);
       if (!ok()) return;
       k_add<<<dim3(nblk_, nrhs), dim3(kBlock), 0, stream_>>>(x, cor, P_.N);
       for (int i = 0; i < nrhs; i++) out[i].iters += state_h_[i].iter;
@@ -782,7 +838,7 @@ void Engine::autotune_prec(int nrhs, int reps) {
   Tuning& tu = tune(sizeof(T) == 4 ? PREC_MIXED : PREC_FP64, nrhs);
   zd *y = vec_[V_Y], *op = vec_[V_OP];
   ISG_HIP(hipMemsetAsync(y, 0, (size_t)nrhs * P_.N * sizeof(zd), stream_));
-  apply<T>(y, op, nrhs, nullptr);
+  apply<T, double>(y, op, nrhs, nullptr);
   hipEvent_t e0 = nullptr, e1 = nullptr;
   ISG_HIP(hipEventCreate(&e0));
   ISG_HIP(hipEventCreate(&e1));
@@ -813,9 +869,9 @@ void Engine::autotune_prec(int nrhs, int reps) {
   };
   const int widths[2] = {16, 32};
   auto width_ok = [&](int c) { return P_.B % widths[c] == 0; };
-  int c = pick(2, width_ok, [&](int c) { launch_near<T>(y, op, nrhs, nullptr, true, widths[c]); });
+  int c = pick(2, width_ok, [&](int c) { launch_near<T, double>(y, op, nrhs, nullptr, true, widths[c]); });
   tu.near_nb = c < 0 ? -1 : widths[c];
-  c = pick(2, width_ok, [&](int c) { launch_p2m<T>(y, nrhs, nullptr, widths[c]); });
+  c = pick(2, width_ok, [&](int c) { launch_p2m<T, double>(y, nrhs, nullptr, widths[c]); });
   tu.p2m_nb = c < 0 ? -1 : widths[c];
   for (int i = 2; i <= Lf; i++)
     tu.m2l[i] = pick(
@@ -824,9 +880,9 @@ void Engine::autotune_prec(int nrhs, int reps) {
   const std::vector<InterpShape>& sl = sizeof(T) == 4 ? l2l32_ : l2l64_;
   for (int i = 0; i + 1 < L; i++) {
     tu.m2m[i] = pick(
-        3, [&](int k) { return kind_ok(sm[i], k); }, [&](int k) { launch_m2m<T>(i, nrhs, nullptr, k); });
+        5, [&](int k) { return kind_ok(sm[i], k); }, [&](int k) { launch_m2m<T>(i, nrhs, nullptr, k); });
     tu.l2l[i] = pick(
-        3, [&](int k) { return kind_ok(sl[i], k); }, [&](int k) { launch_l2l<T>(i, true, nrhs, nullptr, k); });
+        5, [&](int k) { return kind_ok(sl[i], k); }, [&](int k) { launch_l2l<T>(i, true, nrhs, nullptr, k); });
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
@@ -858,7 +914,7 @@ bool Engine::tuning_set(int prec, int cls, int stage, int level, int value) {
     if (value >= kM2LVariants) return false;
     t.m2l[level] = value;
   } else {
-    if (value > 2) return false;
+    if (value > 4) return false;
     (stage == ST_M2M ? t.m2m : t.l2l)[level] = value;
   }
   return true;

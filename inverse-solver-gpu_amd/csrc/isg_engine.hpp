@@ -79,6 +79,8 @@ class Engine {
   // r = G x for nrhs right-hand sides (device pointers, FP64, [nrhs][N]).
   // `act` (device, may be null) masks right-hand sides out.
   void mlfma(const zd* x, zd* r, int nrhs, const int* act = nullptr, int prec = PREC_FP64);
+  // r = G x with the FP32 operator on FP32 vectors (the mixed solver's matvec)
+  void mlfma_f32(const cplx<float>* x, cplx<float>* r, int nrhs);
   // far-field pattern at the receivers: r [nrhs][numrx]  (FP64)
   void aggregate(const zd* x, zd* r, int nrhs);
   // receiver back-projection: x [nrhs][N] from r [nrhs][numrx]  (FP64)
@@ -115,7 +117,8 @@ class Engine {
   // Kernel-variant choices per operator precision and batch class (nrhs < 16
   // or >= 16); -1 = the built-in rule.  Stages: near/P2M tile width (16 or 32
   // boxes), M2L tile variant per level (kM2LVariants), M2M/L2L kernel kind per
-  // level (0 band MFMA, 1 chunked LDS, 2 direct).
+  // level (0 band MFMA, 1 chunked LDS, 2 direct, 3 / 4 band MFMA with 16 / 32
+  // columns per workgroup).
   enum { ST_NEAR = 0, ST_P2M = 1, ST_M2L = 2, ST_M2M = 3, ST_L2L = 4, ST_COUNT = 5 };
   struct Tuning {
     int near_nb = -1, p2m_nb = -1;
@@ -157,19 +160,20 @@ class Engine {
   void upload(DevOp<T>& d);
   template <typename T>
   void plan_interp(DevOp<T>& d, std::vector<InterpShape>& m2m, std::vector<InterpShape>& l2l);
-  template <typename T>
-  void apply(const zd* y, zd* r, int nrhs, const int* act);
+  // operator with tables / multipoles of type T on vectors of type TI
+  template <typename T, typename TI>
+  void apply(const cplx<TI>* y, cplx<TI>* r, int nrhs, const int* act);
   // `force` >= 0 overrides the tuned / built-in choice (autotuner)
-  template <typename T>
-  void launch_p2m(const zd* y, int nrhs, const int* act, int force = -1);
+  template <typename T, typename TI>
+  void launch_p2m(const cplx<TI>* y, int nrhs, const int* act, int force = -1);
   template <typename T>
   void launch_m2m(int i, int nrhs, const int* act, int force = -1);
   template <typename T>
   void launch_l2l(int i, bool accumulate, int nrhs, const int* act, int force = -1);
   template <typename T>
   void launch_m2l(int i, int nrhs, const int* act, int force = -1);
-  template <typename T>
-  void launch_near(const zd* y, zd* r, int nrhs, const int* act, bool with_near, int force = -1);
+  template <typename T, typename TI>
+  void launch_near(const cplx<TI>* y, cplx<TI>* r, int nrhs, const int* act, bool with_near, int force = -1);
   template <typename T>
   void autotune_prec(int nrhs, int reps);
   template <typename T>
@@ -180,15 +184,25 @@ class Engine {
   std::vector<InterpShape>& m2m_shapes(bool fp32) { return fp32 ? m2m32_ : m2m64_; }
 
   void apply_mlfma(const zd* y, zd* r, int nrhs, const int* act, int prec);
+  // solver work vectors of storage type V (FP32 for the mixed solver)
+  template <typename V>
+  cplx<V>* vb(int k);
+  template <typename V>
+  void apply_vec(const cplx<V>* y, cplx<V>* r, int nrhs, const int* act, int prec);
+  // r0_in_res: the initial residual is already in vb<V>(V_RES) (blended start)
   void solve(zd* x, const zd* g, long gs, const zd* b, bool herm, int nrhs, bool x_zero, double tol, int maxit,
-             std::vector<SolveStat>& out, int prec, const zd* r0);
+             std::vector<SolveStat>& out, int prec, bool r0_in_res);
+  template <typename V>
   void run_bicgs(zd* x, const zd* g, long gs, const zd* b, bool herm, int nrhs, bool x_zero, int maxit, int prec,
-                 const zd* r0 = nullptr);
+                 const cplx<V>* r0);
+  template <typename V>
+  void blend(zd* x, const zd* d, const zd* g, long gs, const zd* b, bool herm, int nrhs, int prec);
   void true_residual(const zd* x, const zd* g, long gs, const zd* b, bool herm, int nrhs, zd* res,
                      std::vector<double>& rn, std::vector<double>& bn);
   void prof_begin(int id);
   void prof_end(int id);
   void harvest();
+  template <typename V>
   void bicgs_body(zd* x, const zd* g, long gs, bool herm, int nrhs, int prec);
   void* dalloc(size_t bytes);
 
@@ -206,6 +220,7 @@ class Engine {
   std::vector<size_t> lvl_off_;   // offset of level i in the stores (rhs blocks included)
   std::vector<size_t> lvl_size_;  // 4^i * S_i
   std::vector<zd*> vec_;          // solver vectors, each [max_rhs][N]
+  std::vector<cplx<float>*> vec32_;  // FP32 copies of the work vectors (mixed solver)
   double2* part_ = nullptr;       // reduction partials [max_rhs][3][nblk]
   double2* part_h_ = nullptr;     // pinned host mirror
   BiState* state_ = nullptr;      // [max_rhs]

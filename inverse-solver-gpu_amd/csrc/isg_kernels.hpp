@@ -21,6 +21,17 @@ constexpr int kBlock = 256;
 
 __device__ __forceinline__ bool rhs_off(const int* act, int rhs) { return act != nullptr && act[rhs] == 0; }
 
+// XCD-aware workgroup order.  Workgroups are dealt to the 8 XCDs (each with
+// its own L2) round-robin by linear id, so consecutive blockIdx.x run on
+// different L2s.  When gridDim.x is a multiple of 8 (then x mod 8 is the
+// XCD for every blockIdx.y), this bijection gives each XCD a contiguous
+// range of logical ids: neighbouring tiles and the sample chunks of one tile,
+// which read the same cache lines, then share one L2.
+__device__ __forceinline__ int xcd_block() {
+  const int x = blockIdx.x, n = gridDim.x;
+  return (n & 7) ? x : (x & 7) * (n >> 3) + (x >> 3);
+}
+
 // Direct-from-cache interpolation kernels, one thread per target sample; used
 // for root levels whose LDS staging would exceed 64 KB.
 // parent[p][k] = sum_c shift[c][k] * sum_t w[t][k] child[4p+c][(i0[k]+t) mod Sc]
@@ -160,7 +171,7 @@ __global__ void __launch_bounds__(kBlock) k_near_mfma(const cplx<TI>* __restrict
   if (rhs_off(act, rhs)) return;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int li = lane & 15, lk = lane >> 4;
-  const long b0 = (long)blockIdx.x * NB;
+  const long b0 = (long)xcd_block() * NB;
   v4 cr[NN], ci[NN];
 #pragma unroll
   for (int j = 0; j < NN; j++) {
@@ -357,7 +368,7 @@ __global__ void __launch_bounds__(kBlock) k_interp_band(const cplx<T>* __restric
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int li = lane & 15, lk = lane >> 4;
   const int row = K + 1;
-  const long p0 = (long)blockIdx.x * (NC / 4);
+  const long p0 = (long)xcd_block() * (NC / 4);
   if (MODE == 0) {
     const cplx<T>* s = src + rhs * ss + 4 * p0 * K;  // NC consecutive children of length K
     for (int e = tid; e < NC * K; e += kBlock) {
@@ -452,7 +463,8 @@ __global__ void __launch_bounds__(kBlock) k_m2l_tile(const cplx<T>* __restrict__
   if (rhs_off(act, rhs)) return;
   const int nb = 1 << lev;
   const int nchunk = (S + KC - 1) / KC;
-  const int tile = blockIdx.x / nchunk, chunk = blockIdx.x - tile * nchunk;
+  const int lb = xcd_block();
+  const int tile = lb / nchunk, chunk = lb - tile * nchunk;
   const int k0 = chunk * KC;
   const unsigned m0 = (unsigned)tile * TS * TS;
   const int X0 = (int)compact2(m0), Y0 = (int)compact2(m0 >> 1);
@@ -490,6 +502,89 @@ __global__ void __launch_bounds__(kBlock) k_m2l_tile(const cplx<T>* __restrict__
   }
 }
 
+// M2L with the translation operators in registers.  Staging as k_m2l_tile
+// (8x8 target boxes, their 12x12 source halo, KC samples).  A target box's
+// far list depends only on its position class (X & 1, Y & 1): the 27 boxes
+// of the parent's 6x6 neighbourhood outside its own 3x3, at offsets fixed
+// per class.  Each thread owns one sample kk and one class, loads that
+// class's 27 operators once and sweeps 64 / (256 / KC) targets of the class
+// with compile-time LDS offsets; source boxes outside the domain are staged
+// as zeros, so no bounds tests remain.
+template <typename T, int PX, int PY, int KC, int NT>
+__device__ __forceinline__ void m2l_class(const cplx<T> (*src)[KC], const cplx<T> (*tr)[KC], int kk, int part,
+                                          cplx<T>* __restrict__ out, unsigned m0, int S, int k0) {
+  constexpr int SB = 12;
+  cplx<T> t[27];
+  {
+    int j = 0;
+#pragma unroll
+    for (int dy = -2 - PY; dy <= 3 - PY; dy++)
+#pragma unroll
+      for (int dx = -2 - PX; dx <= 3 - PX; dx++)
+        if (dx < -1 || dx > 1 || dy < -1 || dy > 1) t[j++] = tr[7 * (dy + 3) + dx + 3][kk];
+  }
+  // the class's 16 targets (2i + PX, 2j + PY), i, j = 0..3, numbered 4j + i;
+  // this thread takes numbers part * NT .. part * NT + NT - 1
+#pragma unroll 1
+  for (int q = 0; q < NT; q++) {
+    const int tt = part * NT + q;
+    const int i = tt & 3, jj = tt >> 2;
+    const int lx = 2 * i + PX, ly = 2 * jj + PY;
+    const int base = (ly + 2) * SB + lx + 2;
+    cplx<T> acc = mkc<T>(0, 0);
+    int j = 0;
+#pragma unroll
+    for (int dy = -2 - PY; dy <= 3 - PY; dy++)
+#pragma unroll
+      for (int dx = -2 - PX; dx <= 3 - PX; dx++)
+        if (dx < -1 || dx > 1 || dy < -1 || dy > 1) acc = cfma(t[j++], src[base + dy * SB + dx][kk], acc);
+    if (k0 + kk < S) out[(long)(m0 + morton2((unsigned)lx, (unsigned)ly)) * S + k0 + kk] = acc;
+  }
+}
+
+template <typename T, int KC>
+__global__ void __launch_bounds__(kBlock) k_m2l_reg(const cplx<T>* __restrict__ agm, cplx<T>* __restrict__ agl,
+                                                    long st, const cplx<T>* __restrict__ trans, int lev, int S,
+                                                    const int* __restrict__ act) {
+  constexpr int TS = 8, SB = TS + 4;
+  constexpr int G = kBlock / KC;      // thread groups: 4 classes x G/4 parts
+  constexpr int NT = 16 / (G / 4);    // targets per thread (16 per class per tile)
+  static_assert(G % 4 == 0 && NT >= 1 && 16 % (G / 4) == 0, "KC must divide the tile");
+  __shared__ cplx<T> src[SB * SB][KC];
+  __shared__ cplx<T> tr[49][KC];
+  const int rhs = blockIdx.y;
+  if (rhs_off(act, rhs)) return;
+  const int nb = 1 << lev;
+  const int nchunk = (S + KC - 1) / KC;
+  const int lb = xcd_block();
+  const int tile = lb / nchunk, chunk = lb - tile * nchunk;
+  const int k0 = chunk * KC;
+  const unsigned m0 = (unsigned)tile * TS * TS;
+  const int X0 = (int)compact2(m0), Y0 = (int)compact2(m0 >> 1);
+  const cplx<T>* a = agm + rhs * st;
+  for (int e = threadIdx.x; e < SB * SB * KC; e += kBlock) {
+    int sb = e / KC, kk = e - sb * KC;
+    int sx = X0 - 2 + sb % SB, sy = Y0 - 2 + sb / SB;
+    cplx<T> v = mkc<T>(0, 0);
+    if (sx >= 0 && sy >= 0 && sx < nb && sy < nb && k0 + kk < S) v = a[(long)morton2(sx, sy) * S + k0 + kk];
+    src[sb][kk] = v;
+  }
+  for (int e = threadIdx.x; e < 49 * KC; e += kBlock) {
+    int q = e / KC, kk = e - q * KC;
+    tr[q][kk] = (k0 + kk < S) ? trans[(long)q * S + k0 + kk] : mkc<T>(0, 0);
+  }
+  __syncthreads();
+  const int kk = threadIdx.x % KC, grp = threadIdx.x / KC;
+  const int cls = grp & 3, part = grp >> 2;
+  cplx<T>* out = agl + rhs * st;
+  switch (cls) {
+    case 0: m2l_class<T, 0, 0, KC, NT>(src, tr, kk, part, out, m0, S, k0); break;
+    case 1: m2l_class<T, 1, 0, KC, NT>(src, tr, kk, part, out, m0, S, k0); break;
+    case 2: m2l_class<T, 0, 1, KC, NT>(src, tr, kk, part, out, m0, S, k0); break;
+    default: m2l_class<T, 1, 1, KC, NT>(src, tr, kk, part, out, m0, S, k0); break;
+  }
+}
+
 // P2M on the matrix cores: agm[box][k] = sum_n coeffT[n][k] y[box*64+n] for a
 // tile of NB boxes; rows k in blocks of 16 spread over the 4 waves.
 template <typename T, typename TI, int NB>
@@ -504,7 +599,7 @@ __global__ void __launch_bounds__(kBlock) k_p2m_mfma(const cplx<TI>* __restrict_
   if (rhs_off(act, rhs)) return;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int li = lane & 15, lk = lane >> 4;
-  const long b0 = (long)blockIdx.x * NB;
+  const long b0 = (long)xcd_block() * NB;
   const cplx<TI>* yr = y + rhs * ys + b0 * 64;
   for (int idx = tid; idx < NB * 64; idx += kBlock) tile[(idx >> 6) * 65 + (idx & 63)] = ccast<T>(yr[idx]);
   __syncthreads();

@@ -59,11 +59,25 @@ __device__ __forceinline__ void block_store(zd (&v)[NS], double2* part, int nblk
 
 #define ISG_VEC_LOOP(N) for (long i = (long)blockIdx.x * kBlock + threadIdx.x; i < (N); i += (long)gridDim.x * kBlock)
 
+// Vector kernels are templated on the storage type V of the work vectors
+// (double, or float for the mixed-precision solver); arithmetic and the dot
+// product partials are FP64 either way, the solution x and the right-hand
+// side b are always FP64.
+template <typename V>
+__device__ __forceinline__ zd ldv(const cplx<V>* a, long i) { return ccast<double>(a[i]); }
+template <typename V>
+__device__ __forceinline__ zd stv(cplx<V>* a, long i, zd v) {
+  cplx<V> w = ccast<V>(v);
+  a[i] = w;
+  return ccast<double>(w);  // the value as stored
+}
+
 // r0 (may be null) supplies a residual computed beforehand (blended start).
-__global__ void __launch_bounds__(kBlock) k_bi_init(const zd* __restrict__ x, const zd* __restrict__ gy,
+template <typename V>
+__global__ void __launch_bounds__(kBlock) k_bi_init(const zd* __restrict__ x, const cplx<V>* __restrict__ gy,
                                                     const zd* __restrict__ g, long gs, const zd* __restrict__ b,
-                                                    const zd* __restrict__ r0, zd* __restrict__ r,
-                                                    zd* __restrict__ rt, long N, int herm, int xzero,
+                                                    const cplx<V>* __restrict__ r0, cplx<V>* __restrict__ r,
+                                                    cplx<V>* __restrict__ rt, long N, int herm, int xzero,
                                                     double2* __restrict__ part) {
   int rhs = blockIdx.y;
   long o = rhs * N;
@@ -72,22 +86,23 @@ __global__ void __launch_bounds__(kBlock) k_bi_init(const zd* __restrict__ x, co
     zd bv = b[o + i];
     zd rv;
     if (r0) {
-      rv = r0[o + i];
+      rv = ldv(r0, o + i);
     } else {
-      zd ax = xzero ? mkc<double>(0, 0) : post_val(x[o + i], gy[o + i], g[rhs * gs + i], herm);
+      zd ax = xzero ? mkc<double>(0, 0) : post_val(x[o + i], ldv(gy, o + i), g[rhs * gs + i], herm);
       rv = csub(bv, ax);
     }
-    r[o + i] = rv;
-    rt[o + i] = rv;
+    rv = stv(r, o + i, rv);
+    stv(rt, o + i, rv);
     acc[0].x += bv.x * bv.x + bv.y * bv.y;
     acc[1].x += rv.x * rv.x + rv.y * rv.y;
   }
   block_store<2>(acc, part + (long)rhs * kSlots * gridDim.x, gridDim.x, blockIdx.x);
 }
 
-__global__ void __launch_bounds__(kBlock) k_bi_pupd(const zd* __restrict__ r, zd* __restrict__ p,
-                                                    const zd* __restrict__ v, const zd* __restrict__ g, long gs,
-                                                    zd* __restrict__ y, long N, int herm,
+template <typename V>
+__global__ void __launch_bounds__(kBlock) k_bi_pupd(const cplx<V>* __restrict__ r, cplx<V>* __restrict__ p,
+                                                    const cplx<V>* __restrict__ v, const zd* __restrict__ g, long gs,
+                                                    cplx<V>* __restrict__ y, long N, int herm,
                                                     const BiState* __restrict__ st) {
   int rhs = blockIdx.y;
   const BiState& S = st[rhs];
@@ -98,37 +113,38 @@ __global__ void __launch_bounds__(kBlock) k_bi_pupd(const zd* __restrict__ r, zd
   ISG_VEC_LOOP(N) {
     zd pv;
     if (first) {
-      pv = r[o + i];
+      pv = ldv(r, o + i);
     } else {
-      zd tmp = cfma(mom, v[o + i], p[o + i]);
-      pv = cfma(beta, tmp, r[o + i]);
+      zd tmp = cfma(mom, ldv(v, o + i), ldv(p, o + i));
+      pv = cfma(beta, tmp, ldv(r, o + i));
     }
-    p[o + i] = pv;
-    y[o + i] = prep_val(pv, g[rhs * gs + i], herm);
+    pv = stv(p, o + i, pv);
+    stv(y, o + i, prep_val(pv, g[rhs * gs + i], herm));
   }
 }
 
-__global__ void __launch_bounds__(kBlock) k_bi_post1(const zd* __restrict__ p, const zd* __restrict__ gy,
-                                                     const zd* __restrict__ g, long gs, zd* __restrict__ v,
-                                                     const zd* __restrict__ rt, long N, int herm,
+template <typename V>
+__global__ void __launch_bounds__(kBlock) k_bi_post1(const cplx<V>* __restrict__ p, const cplx<V>* __restrict__ gy,
+                                                     const zd* __restrict__ g, long gs, cplx<V>* __restrict__ v,
+                                                     const cplx<V>* __restrict__ rt, long N, int herm,
                                                      const BiState* __restrict__ st, double2* __restrict__ part) {
   int rhs = blockIdx.y;
   if (!st[rhs].active) return;
   long o = rhs * N;
   zd acc[1] = {mkc<double>(0, 0)};
   ISG_VEC_LOOP(N) {
-    zd vv = post_val(p[o + i], gy[o + i], g[rhs * gs + i], herm);
-    v[o + i] = vv;
-    zd a = rt[o + i];
+    zd vv = stv(v, o + i, post_val(ldv(p, o + i), ldv(gy, o + i), g[rhs * gs + i], herm));
+    zd a = ldv(rt, o + i);
     acc[0].x += a.x * vv.x + a.y * vv.y;
     acc[0].y += a.x * vv.y - a.y * vv.x;
   }
   block_store<1>(acc, part + (long)rhs * kSlots * gridDim.x, gridDim.x, blockIdx.x);
 }
 
-__global__ void __launch_bounds__(kBlock) k_bi_supd(const zd* __restrict__ r, const zd* __restrict__ v,
-                                                    const zd* __restrict__ g, long gs, zd* __restrict__ s,
-                                                    zd* __restrict__ y, long N, int herm,
+template <typename V>
+__global__ void __launch_bounds__(kBlock) k_bi_supd(const cplx<V>* __restrict__ r, const cplx<V>* __restrict__ v,
+                                                    const zd* __restrict__ g, long gs, cplx<V>* __restrict__ s,
+                                                    cplx<V>* __restrict__ y, long N, int herm,
                                                     const BiState* __restrict__ st) {
   int rhs = blockIdx.y;
   const BiState& S = st[rhs];
@@ -136,24 +152,23 @@ __global__ void __launch_bounds__(kBlock) k_bi_supd(const zd* __restrict__ r, co
   long o = rhs * N;
   zd mal = mkc<double>(-S.alpha.x, -S.alpha.y);
   ISG_VEC_LOOP(N) {
-    zd sv = cfma(mal, v[o + i], r[o + i]);
-    s[o + i] = sv;
-    y[o + i] = prep_val(sv, g[rhs * gs + i], herm);
+    zd sv = stv(s, o + i, cfma(mal, ldv(v, o + i), ldv(r, o + i)));
+    stv(y, o + i, prep_val(sv, g[rhs * gs + i], herm));
   }
 }
 
-__global__ void __launch_bounds__(kBlock) k_bi_post2(const zd* __restrict__ s, const zd* __restrict__ gy,
-                                                     const zd* __restrict__ g, long gs, zd* __restrict__ t, long N,
-                                                     int herm, const BiState* __restrict__ st,
+template <typename V>
+__global__ void __launch_bounds__(kBlock) k_bi_post2(const cplx<V>* __restrict__ s, const cplx<V>* __restrict__ gy,
+                                                     const zd* __restrict__ g, long gs, cplx<V>* __restrict__ t,
+                                                     long N, int herm, const BiState* __restrict__ st,
                                                      double2* __restrict__ part) {
   int rhs = blockIdx.y;
   if (!st[rhs].active) return;
   long o = rhs * N;
   zd acc[2] = {mkc<double>(0, 0), mkc<double>(0, 0)};
   ISG_VEC_LOOP(N) {
-    zd sv = s[o + i];
-    zd tv = post_val(sv, gy[o + i], g[rhs * gs + i], herm);
-    t[o + i] = tv;
+    zd sv = ldv(s, o + i);
+    zd tv = stv(t, o + i, post_val(sv, ldv(gy, o + i), g[rhs * gs + i], herm));
     acc[0].x += tv.x * sv.x + tv.y * sv.y;
     acc[0].y += tv.x * sv.y - tv.y * sv.x;
     acc[1].x += tv.x * tv.x + tv.y * tv.y;
@@ -161,9 +176,10 @@ __global__ void __launch_bounds__(kBlock) k_bi_post2(const zd* __restrict__ s, c
   block_store<2>(acc, part + (long)rhs * kSlots * gridDim.x, gridDim.x, blockIdx.x);
 }
 
-__global__ void __launch_bounds__(kBlock) k_bi_xrupd(zd* __restrict__ x, const zd* __restrict__ p,
-                                                     const zd* __restrict__ s, const zd* __restrict__ t,
-                                                     zd* __restrict__ r, const zd* __restrict__ rt, long N,
+template <typename V>
+__global__ void __launch_bounds__(kBlock) k_bi_xrupd(zd* __restrict__ x, const cplx<V>* __restrict__ p,
+                                                     const cplx<V>* __restrict__ s, const cplx<V>* __restrict__ t,
+                                                     cplx<V>* __restrict__ r, const cplx<V>* __restrict__ rt, long N,
                                                      const BiState* __restrict__ st, double2* __restrict__ part) {
   int rhs = blockIdx.y;
   const BiState& S = st[rhs];
@@ -172,13 +188,12 @@ __global__ void __launch_bounds__(kBlock) k_bi_xrupd(zd* __restrict__ x, const z
   zd al = S.alpha, om = S.omega, mom = mkc<double>(-S.omega.x, -S.omega.y);
   zd acc[2] = {mkc<double>(0, 0), mkc<double>(0, 0)};
   ISG_VEC_LOOP(N) {
-    zd sv = s[o + i];
-    zd xv = cfma(al, p[o + i], x[o + i]);
+    zd sv = ldv(s, o + i);
+    zd xv = cfma(al, ldv(p, o + i), x[o + i]);
     xv = cfma(om, sv, xv);
     x[o + i] = xv;
-    zd rv = cfma(mom, t[o + i], sv);
-    r[o + i] = rv;
-    zd a = rt[o + i];
+    zd rv = stv(r, o + i, cfma(mom, ldv(t, o + i), sv));
+    zd a = ldv(rt, o + i);
     acc[0].x += rv.x * rv.x + rv.y * rv.y;
     acc[1].x += a.x * rv.x + a.y * rv.y;
     acc[1].y += a.x * rv.y - a.y * rv.x;
@@ -276,17 +291,17 @@ __global__ void __launch_bounds__(kBlock) k_bi_fin(int phase, BiState* __restric
 
 // Blended start: ad = A d (post of the operator product), partials
 // <ad, r> (= sum conj(ad) r) and <ad, ad>.
-__global__ void __launch_bounds__(kBlock) k_bl_dots(const zd* __restrict__ d, const zd* __restrict__ gy,
-                                                    const zd* __restrict__ g, long gs, const zd* __restrict__ r,
-                                                    zd* __restrict__ ad, long N, int herm,
+template <typename V>
+__global__ void __launch_bounds__(kBlock) k_bl_dots(const zd* __restrict__ d, const cplx<V>* __restrict__ gy,
+                                                    const zd* __restrict__ g, long gs, const cplx<V>* __restrict__ r,
+                                                    cplx<V>* __restrict__ ad, long N, int herm,
                                                     double2* __restrict__ part) {
   int rhs = blockIdx.y;
   long o = rhs * N;
   zd acc[2] = {mkc<double>(0, 0), mkc<double>(0, 0)};
   ISG_VEC_LOOP(N) {
-    zd a = post_val(d[o + i], gy[o + i], g[rhs * gs + i], herm);
-    ad[o + i] = a;
-    zd rv = r[o + i];
+    zd a = stv(ad, o + i, post_val(d[o + i], ldv(gy, o + i), g[rhs * gs + i], herm));
+    zd rv = ldv(r, o + i);
     acc[0].x += a.x * rv.x + a.y * rv.y;
     acc[0].y += a.x * rv.y - a.y * rv.x;
     acc[1].x += a.x * a.x + a.y * a.y;
@@ -319,15 +334,16 @@ __global__ void __launch_bounds__(kBlock) k_bl_fin(const double2* __restrict__ p
 }
 
 // x += lambda d, r -= lambda ad
+template <typename V>
 __global__ void __launch_bounds__(kBlock) k_bl_apply(zd* __restrict__ x, const zd* __restrict__ d,
-                                                     zd* __restrict__ r, const zd* __restrict__ ad,
+                                                     cplx<V>* __restrict__ r, const cplx<V>* __restrict__ ad,
                                                      const zd* __restrict__ lam, long N) {
   int rhs = blockIdx.y;
   long o = rhs * N;
   zd l = lam[rhs], ml = mkc<double>(-l.x, -l.y);
   ISG_VEC_LOOP(N) {
     x[o + i] = cfma(l, d[o + i], x[o + i]);
-    r[o + i] = cfma(ml, ad[o + i], r[o + i]);
+    stv(r, o + i, cfma(ml, ldv(ad, o + i), ldv(r, o + i)));
   }
 }
 
@@ -339,11 +355,12 @@ __global__ void __launch_bounds__(kBlock) k_add(zd* __restrict__ x, const zd* __
 }
 
 // y = prep(x)
+template <typename V>
 __global__ void __launch_bounds__(kBlock) k_prep(const zd* __restrict__ x, const zd* __restrict__ g, long gs,
-                                                 zd* __restrict__ y, long N, int herm) {
+                                                 cplx<V>* __restrict__ y, long N, int herm) {
   int rhs = blockIdx.y;
   long o = rhs * N;
-  ISG_VEC_LOOP(N) y[o + i] = prep_val(x[o + i], g[rhs * gs + i], herm);
+  ISG_VEC_LOOP(N) stv(y, o + i, prep_val(x[o + i], g[rhs * gs + i], herm));
 }
 
 }  // namespace isg

@@ -5,7 +5,14 @@ import sys
 import numpy as np
 import pytest
 
-ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+# torch (device buffers for the *_dev entry points in a few GPU tests) must
+# load its HIP runtime before libisg.so starts HIP in this process.
+try:
+    import torch  # noqa: F401
+except ImportError:
+    torch = None
+
+ROOT =os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 PKG = os.path.join(ROOT, "inverse-solver-gpu_amd")
 for p in (ROOT, PKG):
     if p not in sys.path:

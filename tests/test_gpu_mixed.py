@@ -71,3 +71,22 @@ def test_mixed_batched_dbim_matches_oracle(isg):
         st = d.iterate()
         assert st["not_converged"] == 0
     assert relerr(d.get("o"), ref.get("o")) < 1e-4
+
+
+def test_fp32_vector_matvec(isg):
+    """isg_mlfma_f32_dev: FP32 operator on complex64 device vectors, 8
+    right-hand sides at 128x128, within the FP32 bar of the oracle."""
+    import torch
+    level = 5
+    o = oracle_for(level)
+    s = isg.Solver(level=level, max_rhs=8, precision=1)
+    rng = np.random.default_rng(4)
+    X = rng.standard_normal((8, o.N)) + 1j * rng.standard_normal((8, o.N))
+    x = torch.from_numpy(X.astype(np.complex64)).cuda()
+    r = torch.empty_like(x)
+    torch.cuda.synchronize()
+    s.mlfma_f32_dev(x.data_ptr(), r.data_ptr(), 8)
+    R = r.cpu().numpy()
+    for k in range(8):
+        e = relerr(R[k], o.mlfma(X[k].astype(np.complex64).astype(np.complex128)))
+        assert 1e-12 < e < 1e-5, e

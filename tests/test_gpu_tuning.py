@@ -13,7 +13,8 @@ from conftest import oracle_for, rand_c, relerr
 
 pytestmark = pytest.mark.gpu
 
-M2L_VARIANTS_F64 = [0, 1, 2, 3]
+M2L_TS = [4, 8, 8, 16, 8, 16, 8, 8, 8]  # tile width of each M2L variant
+M2L_VARIANTS_F64 = [0, 1, 2, 3, 6, 8]
 
 
 @pytest.fixture(scope="module")
@@ -36,12 +37,30 @@ def test_forced_m2l_variants_match_oracle(isg, level):
     for v in M2L_VARIANTS_F64:
         print("m2l variant", v, "level", level, flush=True)
         for i in range(2, level):
-            if (1 << i) >= [4, 8, 8, 16][v]:
+            if (1 << i) >= M2L_TS[v]:
                 s.tuning_set(0, 0, "m2l", i, v)
         check(s, o, 4)
 
 
-@pytest.mark.parametrize("kind", [0, 1, 2])
+def test_forced_m2l_variants_fp32(isg):
+    """Every M2L variant of the FP32 operator (4, 5 and 7 exist only there)
+    within the mixed-precision bar of the FP64 oracle."""
+    level = 6
+    o = oracle_for(level)
+    s = isg.Solver(level=level, max_rhs=4, precision=1)
+    X = np.stack([rand_c(np.random.default_rng(50 + r), o.N) for r in range(4)])
+    ref = [o.mlfma(X[r]) for r in range(4)]
+    for v in range(9):
+        print("fp32 m2l variant", v, flush=True)
+        for i in range(2, level):
+            s.tuning_set(1, 0, "m2l", i, v if (1 << i) >= M2L_TS[v] else -1)
+        Y = s.mlfma_many(X)
+        for r in range(4):
+            e = relerr(Y[r], ref[r])
+            assert 1e-12 < e < 1e-5, (v, e)
+
+
+@pytest.mark.parametrize("kind", [0, 1, 2, 3, 4])
 def test_forced_interp_kinds_match_oracle(isg, kind):
     level = 6
     o = oracle_for(level)
