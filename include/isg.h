@@ -119,6 +119,9 @@ int isg_mlfma_many(isg_context* ctx, const double* x, double* r, int nrhs);
 
 /* ---- device-pointer, batched surface (vectors [nrhs][numunk], receivers [nrhs][numrx]) ---- */
 int isg_mlfma_dev(isg_context* ctx, const void* x, void* r, int nrhs);
+/* waits for the work queued on the context's stream (the *_dev calls return
+ * before their kernels finish) */
+int isg_synchronize(isg_context* ctx);
 /* the same on interleaved FP32 complex vectors with the FP32 operator (precision 1
  * contexts): the matrix-vector product the mixed-precision solver runs */
 int isg_mlfma_f32_dev(isg_context* ctx, const void* x, void* r, int nrhs);

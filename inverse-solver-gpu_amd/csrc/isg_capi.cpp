@@ -252,6 +252,11 @@ int isg_mlfma_dev(isg_context* c, const void* x, void* r, int nrhs) {
   return check(c) ? ISG_OK : ISG_E_DEVICE;
 }
 
+int isg_synchronize(isg_context* c) {
+  if (!check(c)) return ISG_E_STATE;
+  return sync(c);
+}
+
 int isg_mlfma_f32_dev(isg_context* c, const void* x, void* r, int nrhs) {
   if (!check(c)) return ISG_E_STATE;
   if (!x || !r || nrhs < 1 || nrhs > c->cfg.max_rhs) return fail(c, ISG_E_ARG, "bad argument");

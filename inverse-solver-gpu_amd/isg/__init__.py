@@ -96,6 +96,7 @@ _SIGS = {
     "isg_mlfma_dev": (ctypes.c_int, [_V, _V, _V, ctypes.c_int]),
     "isg_aggregate_dev": (ctypes.c_int, [_V, _V, _V, ctypes.c_int]),
     "isg_mlfma_f32_dev": (ctypes.c_int, [_V, _V, _V, ctypes.c_int]),
+    "isg_synchronize": (ctypes.c_int, [_V]),
     "isg_aggregateh_dev": (ctypes.c_int, [_V, _V, _V, ctypes.c_int]),
     "isg_bicgs_dev": (ctypes.c_int, [_V, _V, _V, ctypes.c_long, _V, ctypes.c_int, ctypes.c_int,
                                      ctypes.c_int, _P(SolveInfo)]),
@@ -337,6 +338,10 @@ class Solver:
     # ---- device-pointer surface (pointers are integers, e.g. torch data_ptr())
     def mlfma_dev(self, x_ptr, r_ptr, nrhs=1):
         return self._err(self.lib.isg_mlfma_dev(self.h, _V(x_ptr), _V(r_ptr), nrhs), "isg_mlfma_dev")
+
+    def synchronize(self):
+        """Wait for the work queued on the context's stream by the *_dev calls."""
+        self._err(self.lib.isg_synchronize(self.h), "isg_synchronize")
 
     def mlfma_f32_dev(self, x_ptr, r_ptr, nrhs=1):
         """FP32 operator on interleaved complex64 device vectors (precision 1 contexts)."""

@@ -86,6 +86,7 @@ def test_fp32_vector_matvec(isg):
     r = torch.empty_like(x)
     torch.cuda.synchronize()
     s.mlfma_f32_dev(x.data_ptr(), r.data_ptr(), 8)
+    s.synchronize()
     R = r.cpu().numpy()
     for k in range(8):
         e = relerr(R[k], o.mlfma(X[k].astype(np.complex64).astype(np.complex128)))
