@@ -122,6 +122,13 @@ struct Mfma<double> {
     ci = mac(d.x, a.y, ci);
     ci = mac(d.y, a.x, ci);
   }
+  // three-product form (Gauss): p1 += dr ar, p2 += di ai, p3 += (dr + di)(ar + ai)
+  // with asum = ar + ai; then d a = (p1 - p2) + i (p3 - p1 - p2)
+  static __device__ __forceinline__ void cmac3(cplx<double> d, cplx<double> a, double asum, v4& p1, v4& p2, v4& p3) {
+    p1 = mac(d.x, a.x, p1);
+    p2 = mac(d.y, a.y, p2);
+    p3 = mac(d.x + d.y, asum, p3);
+  }
 };
 template <>
 struct Mfma<float> {
@@ -136,6 +143,11 @@ struct Mfma<float> {
     cr = mac(d.y, nai, cr);
     ci = mac(d.x, a.y, ci);
     ci = mac(d.y, a.x, ci);
+  }
+  static __device__ __forceinline__ void cmac3(cplx<float> d, cplx<float> a, float asum, v4& p1, v4& p2, v4& p3) {
+    p1 = mac(d.x, a.x, p1);
+    p2 = mac(d.y, a.y, p2);
+    p3 = mac(d.x + d.y, asum, p3);
   }
 };
 
@@ -172,11 +184,13 @@ __global__ void __launch_bounds__(kBlock) k_near_mfma(const cplx<TI>* __restrict
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int li = lane & 15, lk = lane >> 4;
   const long b0 = (long)xcd_block() * NB;
-  v4 cr[NN], ci[NN];
+  // complex products in the three-product form (Mfma::cmac3)
+  v4 p1[NN], p2[NN], p3[NN];
 #pragma unroll
   for (int j = 0; j < NN; j++) {
-    cr[j] = v4{0, 0, 0, 0};
-    ci[j] = v4{0, 0, 0, 0};
+    p1[j] = v4{0, 0, 0, 0};
+    p2[j] = v4{0, 0, 0, 0};
+    p3[j] = v4{0, 0, 0, 0};
   }
   const cplx<TI>* yr = y + rhs * ys;
   if (with_near) {
@@ -203,11 +217,11 @@ __global__ void __launch_bounds__(kBlock) k_near_mfma(const cplx<TI>* __restrict
       for (int kk = 0; kk < 16; kk++) {
         const int k = 4 * kk + lk;
         cplx<T> a = A[k * 64];
-        T nai = -a.y;
+        T asum = a.x + a.y;
 #pragma unroll
         for (int j = 0; j < NN; j++) {
           cplx<T> bv = tile[(hc[j] + off) * HROW + k];
-          Mfma<T>::cmac(bv, a, nai, cr[j], ci[j]);
+          Mfma<T>::cmac3(bv, a, asum, p1[j], p2[j], p3[j]);
         }
       }
     }
@@ -224,11 +238,11 @@ __global__ void __launch_bounds__(kBlock) k_near_mfma(const cplx<TI>* __restrict
   for (int kk = 0; kk < S / 4; kk++) {
     const int k = 4 * kk + lk;
     cplx<T> a = A[k * 64];
-    T nai = -a.y;
+    T asum = a.x + a.y;
 #pragma unroll
     for (int j = 0; j < NN; j++) {
       cplx<T> bv = tile[(16 * j + li) * row + k];
-      Mfma<T>::cmac(bv, a, nai, cr[j], ci[j]);
+      Mfma<T>::cmac3(bv, a, asum, p1[j], p2[j], p3[j]);
     }
   }
   // D[box][m]: lane li holds pixel m = 16w + li, so each store writes 16
@@ -239,7 +253,8 @@ __global__ void __launch_bounds__(kBlock) k_near_mfma(const cplx<TI>* __restrict
 #pragma unroll
     for (int q = 0; q < 4; q++) {
       long box = b0 + 16 * j + Mfma<T>::row(lk, q);
-      out[box * 64] = mkc<TI>((TI)cr[j][q], (TI)ci[j][q]);
+      const T re = p1[j][q] - p2[j][q], im = p3[j][q] - p1[j][q] - p2[j][q];
+      out[box * 64] = mkc<TI>((TI)re, (TI)im);
     }
   }
 }
@@ -605,22 +620,23 @@ __global__ void __launch_bounds__(kBlock) k_p2m_mfma(const cplx<TI>* __restrict_
   __syncthreads();
   const int nrb = (S + 15) / 16;
   for (int rb = w; rb < nrb; rb += 4) {
-    v4 cr[NN], ci[NN];
+    v4 p1[NN], p2[NN], p3[NN];
 #pragma unroll
     for (int j = 0; j < NN; j++) {
-      cr[j] = v4{0, 0, 0, 0};
-      ci[j] = v4{0, 0, 0, 0};
+      p1[j] = v4{0, 0, 0, 0};
+      p2[j] = v4{0, 0, 0, 0};
+      p3[j] = v4{0, 0, 0, 0};
     }
     const int krow = 16 * rb + li;
 #pragma unroll 4
     for (int kk = 0; kk < 16; kk++) {
       const int n = 4 * kk + lk;
       cplx<T> a = (krow < S) ? coeffT[n * S + krow] : mkc<T>(0, 0);
-      T nai = -a.y;
+      T asum = a.x + a.y;
 #pragma unroll
       for (int j = 0; j < NN; j++) {
         cplx<T> bv = tile[(16 * j + li) * 65 + n];
-        Mfma<T>::cmac(bv, a, nai, cr[j], ci[j]);
+        Mfma<T>::cmac3(bv, a, asum, p1[j], p2[j], p3[j]);
       }
     }
     cplx<T>* out = agm + rhs * as;
@@ -630,7 +646,7 @@ __global__ void __launch_bounds__(kBlock) k_p2m_mfma(const cplx<TI>* __restrict_
 #pragma unroll
         for (int q = 0; q < 4; q++) {
           long box = b0 + 16 * j + Mfma<T>::row(lk, q);
-          out[box * S + krow] = mkc<T>(cr[j][q], ci[j][q]);
+          out[box * S + krow] = mkc<T>(p1[j][q] - p2[j][q], p3[j][q] - p1[j][q] - p2[j][q]);
         }
       }
     }
