@@ -75,6 +75,12 @@ typedef struct isg_config {
                           needs a defect-correction round; 0 < mixed_margin <= 1 (default 0.5) */
   int graph;       /* 1 (default): each BiCGSTAB iteration is captured once per solve as a HIP graph
                       and replayed; 0: launched kernel by kernel.  Results are identical. */
+  double radius;   /* 0 (default): plane-wave transmitters and far-field receivers, as the reference
+                      (setup_born.cpp:62-65, matvec_gpu.cu:351-361).  > 0: line-source transmitters
+                      and point receivers equispaced on a circle of this radius (wavelengths):
+                      incident (i/4) H0(k0 |r - rho_t|), isg_aggregate = the scattered field
+                      sum_n (i/4) H0(k0 |rho_m - r_n|) res^2 x_n (dense, on the device),
+                      isg_aggregateh its exact adjoint */
 } isg_config;
 
 /* Per-solve result. */

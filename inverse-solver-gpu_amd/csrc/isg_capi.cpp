@@ -86,6 +86,7 @@ void isg_config_default(isg_config* c) {
   c->precision = 0;
   c->mixed_margin = 0.5;
   c->graph = 1;
+  c->radius = 0.0;
 }
 
 int isg_create(const isg_config* cfg, isg_context** out) {
@@ -94,6 +95,7 @@ int isg_create(const isg_config* cfg, isg_context** out) {
   if (cfg->max_rhs < 1) return ISG_E_ARG;
   if (cfg->precision != 0 && cfg->precision != 1) return ISG_E_ARG;
   if (!(cfg->mixed_margin > 0 && cfg->mixed_margin <= 1)) return ISG_E_ARG;
+  if (!(cfg->radius >= 0)) return ISG_E_ARG;
   isg_context* c = new isg_context();
   c->cfg = *cfg;
   isg::EngineConfig ec;
@@ -104,6 +106,7 @@ int isg_create(const isg_config* cfg, isg_context** out) {
   ec.plan.beta = cfg->beta;
   ec.plan.ninter = cfg->ninter;
   ec.plan.numrx = cfg->numrx;
+  ec.plan.radius = cfg->radius;
   ec.max_rhs = cfg->max_rhs;
   ec.precision = cfg->precision;
   ec.mixed_margin = cfg->mixed_margin;

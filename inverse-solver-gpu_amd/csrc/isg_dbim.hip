@@ -19,16 +19,25 @@ namespace isg {
 
 namespace {
 
-// inc[t][n] = exp(i k0 u_t . r_n): plane wave along transmitter t (setup_born.cpp:62-65)
-__global__ void __launch_bounds__(kBlock) k_incident(zd* inc, const zd* pos, const zd* tx, long N, double k0) {
+// inc[t][n] = exp(i k0 u_t . r_n): plane wave along transmitter t
+// (setup_born.cpp:62-65); with radius > 0 the field of a line source at
+// radius * u_t instead: (i/4) H0^(1)(k0 |r_n - radius u_t|).
+__global__ void __launch_bounds__(kBlock) k_incident(zd* inc, const zd* pos, const zd* tx, long N, double k0,
+                                                     double radius) {
   int t = blockIdx.y;
   zd u = tx[t];
   DB_LOOP(N) {
     zd p = pos[i];
-    double ph = k0 * (u.x * p.x + u.y * p.y);
-    double s, c;
-    sincos(ph, &s, &c);
-    inc[t * N + i] = mkc<double>(c, s);
+    if (radius > 0) {
+      const double dx = p.x - radius * u.x, dy = p.y - radius * u.y;
+      const double kd = k0 * sqrt(dx * dx + dy * dy);
+      inc[t * N + i] = mkc<double>(-0.25 * y0(kd), 0.25 * j0(kd));
+    } else {
+      double ph = k0 * (u.x * p.x + u.y * p.y);
+      double s, c;
+      sincos(ph, &s, &c);
+      inc[t * N + i] = mkc<double>(c, s);
+    }
   }
 }
 
@@ -285,7 +294,7 @@ void Dbim::setup(const zd* otrue, DbimStats& st) {
   hipStream_t s = eng_.stream();
   const long N = N_;
   dim3 blk(kBlock);
-  k_incident<<<dim3(nblk_, T_), blk, 0, s>>>(inc_, pos_, tx_, N, k0_);
+  k_incident<<<dim3(nblk_, T_), blk, 0, s>>>(inc_, pos_, tx_, N, k0_, eng_.plan().cfg.radius);
   DB_HIP(hipMemcpyAsync(o_, otrue, sizeof(zd) * N, hipMemcpyHostToDevice, s));
   k_weight<<<dim3(nblk_), blk, 0, s>>>(o_, g_, N, k0_);
   // synthetic measurements: mes_t = aggregate(g * A^-1 inc_t)  (setup_born.cpp:139-149)
@@ -306,7 +315,7 @@ void Dbim::setup(const zd* otrue, DbimStats& st) {
 void Dbim::setup_measured(const zd* mes_host, const zd* otrue, DbimStats& st) {
   if (!ok()) return;
   auto t_start = std::chrono::steady_clock::now();
-  k_incident<<<dim3(nblk_, T_), dim3(kBlock), 0, eng_.stream()>>>(inc_, pos_, tx_, N_, k0_);
+  k_incident<<<dim3(nblk_, T_), dim3(kBlock), 0, eng_.stream()>>>(inc_, pos_, tx_, N_, k0_, eng_.plan().cfg.radius);
   start(mes_host, otrue, st);
   st.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
 }

@@ -9,6 +9,8 @@
 #include <fstream>
 #include <sstream>
 
+#include <rocblas/rocblas.h>
+
 #include "isg_kernels.hpp"
 #include "isg_solver_kernels.hpp"
 
@@ -24,6 +26,38 @@ namespace isg {
 
 namespace {
 inline unsigned nblocks(long work) { return (unsigned)((work + kBlock - 1) / kBlock); }
+
+// Finite-radius receivers (line-source geometry): K[m][n] = (i/4) H0^(1)(k0
+// |rho_m - r_n|) res^2 with rho_m = radius (cos, sin)(2 pi m / R), the
+// midpoint-rule Green's function weight of pixel n seen from receiver m.
+__global__ void __launch_bounds__(kBlock) k_rx_matrix(const zd* __restrict__ pos, long N, int R, double radius,
+                                                      double k0, double res2, zd* __restrict__ K) {
+  const int m = blockIdx.y;
+  double sa, ca;
+  sincos(2.0 * M_PI * m / R, &sa, &ca);
+  const double rx = radius * ca, ry = radius * sa;
+  for (long n = (long)blockIdx.x * kBlock + threadIdx.x; n < N; n += (long)gridDim.x * kBlock) {
+    const double dx = rx - pos[n].x, dy = ry - pos[n].y;
+    const double kd = k0 * sqrt(dx * dx + dy * dy);
+    // (i/4)(J0 + i Y0) res^2
+    K[(long)m * N + n] = mkc<double>(-0.25 * y0(kd) * res2, 0.25 * j0(kd) * res2);
+  }
+}
+
+// out[i] = sum_b part[b][i] (fixed order) -- split-K reduction of the receiver product
+__global__ void __launch_bounds__(kBlock) k_sum_parts(const zd* __restrict__ part, long len, int nb, zd* __restrict__ out) {
+  for (long i = (long)blockIdx.x * kBlock + threadIdx.x; i < len; i += (long)gridDim.x * kBlock) {
+    zd s = mkc<double>(0, 0);
+    for (int b = 0; b < nb; b++) s = cadd(s, part[(long)b * len + i]);
+    out[i] = s;
+  }
+}
+
+// out[i] = conj(in[i]); in and out may be the same array
+__global__ void __launch_bounds__(kBlock) k_conj_copy(const zd* in, zd* out, long len) {
+  for (long i = (long)blockIdx.x * kBlock + threadIdx.x; i < len; i += (long)gridDim.x * kBlock) out[i] = cconj(in[i]);
+}
+
 enum { V_R = 0, V_RT, V_P, V_V, V_S, V_T, V_Y, V_OP, V_RES, V_COR, V_COUNT };
 // smallest power of two >= v (v >= 1)
 int pow2_ceil(int v) {
@@ -209,10 +243,75 @@ Engine::Engine(const EngineConfig& cfg) : cfg_(cfg) {
     ISG_HIP(hipMemset(state_, 0, (size_t)R * sizeof(BiState)));
     ISG_HIP(hipMemset(active_, 0, (size_t)R * sizeof(int)));
   }
+  if (ok() && cfg_.plan.radius > 0) setup_receivers();
+}
+
+// Finite-radius receivers: the numrx x N operator on the device (FP64) and a
+// rocBLAS handle on the engine stream.
+void Engine::setup_receivers() {
+  const int Rx = cfg_.plan.numrx, R = cfg_.max_rhs;
+  const long N = P_.N;
+  rxm_ = (zd*)dalloc((size_t)Rx * N * sizeof(zd));
+  zd* pos = (zd*)dalloc((size_t)N * sizeof(zd));
+  rx_kc_ = (int)std::min<long>(N, 4096);
+  rxpart_ = (zd*)dalloc((size_t)(N / rx_kc_) * Rx * R * sizeof(zd));
+  rxc_ = (zd*)dalloc((size_t)Rx * R * sizeof(zd));
+  if (!ok()) return;
+  ISG_HIP(hipMemcpy(pos, P_.pos.data(), (size_t)N * sizeof(zd), hipMemcpyHostToDevice));
+  k_rx_matrix<<<dim3(nblocks(N), Rx), dim3(kBlock), 0, stream_>>>(pos, N, Rx, cfg_.plan.radius, P_.k0,
+                                                                   cfg_.plan.res * cfg_.plan.res, rxm_);
+  ISG_HIP(hipGetLastError());
+  rocblas_handle h = nullptr;
+  if (rocblas_create_handle(&h) != rocblas_status_success || rocblas_set_stream(h, stream_) != rocblas_status_success) {
+    set_error("rocBLAS handle creation failed");
+    return;
+  }
+  blas_ = h;
+  ISG_HIP(hipStreamSynchronize(stream_));
+}
+
+// r[t][m] = sum_n K[m][n] x[t][n]: split-K strided-batched ZGEMM (one batch
+// per chunk of rx_kc_ unknowns) and a fixed-order sum of the chunks.
+void Engine::rx_apply(const zd* x, zd* r, int nrhs) {
+  const int Rx = cfg_.plan.numrx;
+  const long N = P_.N;
+  const int nb = (int)(N / rx_kc_);
+  const rocblas_double_complex one = {1.0, 0.0}, zero = {0.0, 0.0};
+  rocblas_status st = rocblas_zgemm_strided_batched(
+      (rocblas_handle)blas_, rocblas_operation_transpose, rocblas_operation_none, Rx, nrhs, rx_kc_, &one,
+      (const rocblas_double_complex*)rxm_, (rocblas_int)N, rx_kc_, (const rocblas_double_complex*)x, (rocblas_int)N,
+      rx_kc_, &zero, (rocblas_double_complex*)rxpart_, Rx, (rocblas_stride)Rx * nrhs, nb);
+  if (st != rocblas_status_success) {
+    set_error("rocblas_zgemm_strided_batched failed");
+    return;
+  }
+  const long len = (long)Rx * nrhs;
+  k_sum_parts<<<nblocks(len), kBlock, 0, stream_>>>(rxpart_, len, nb, r);
+  ISG_HIP(hipGetLastError());
+}
+
+// x[t][n] = sum_m conj(K[m][n]) r[t][m] = conj(sum_m K[m][n] conj(r[t][m])): one ZGEMM
+// with K read as the N x numrx column-major matrix, conjugations before and after.
+void Engine::rx_apply_h(const zd* r, zd* x, int nrhs) {
+  const int Rx = cfg_.plan.numrx;
+  const long N = P_.N;
+  const rocblas_double_complex one = {1.0, 0.0}, zero = {0.0, 0.0};
+  k_conj_copy<<<nblocks((long)Rx * nrhs), kBlock, 0, stream_>>>(r, rxc_, (long)Rx * nrhs);
+  rocblas_status st = rocblas_zgemm((rocblas_handle)blas_, rocblas_operation_none, rocblas_operation_none,
+                                    (rocblas_int)N, nrhs, Rx, &one, (const rocblas_double_complex*)rxm_,
+                                    (rocblas_int)N, (const rocblas_double_complex*)rxc_, Rx, &zero,
+                                    (rocblas_double_complex*)x, (rocblas_int)N);
+  if (st != rocblas_status_success) {
+    set_error("rocblas_zgemm failed");
+    return;
+  }
+  k_conj_copy<<<nblocks(N * nrhs), kBlock, 0, stream_>>>(x, x, N * nrhs);
+  ISG_HIP(hipGetLastError());
 }
 
 Engine::~Engine() {
   if (stream_) (void)hipStreamSynchronize(stream_);
+  if (blas_) (void)rocblas_destroy_handle((rocblas_handle)blas_);
   for (auto& k : kt_) {
     for (auto e : k.ev0) (void)hipEventDestroy(e);
     for (auto e : k.ev1) (void)hipEventDestroy(e);
@@ -531,6 +630,10 @@ void Engine::mlfma(const zd* x, zd* r, int nrhs, const int* act, int prec) {
 // on the device as well (FP64).
 void Engine::aggregate(const zd* x, zd* r, int nrhs) {
   if (!ok()) return;
+  if (cfg_.plan.radius > 0) {
+    rx_apply(x, r, nrhs);
+    return;
+  }
   const int L = P_.L;
   launch_p2m<double, double>(x, nrhs, nullptr);
   for (int i = L - 2; i >= 0; i--) launch_m2m<double>(i, nrhs, nullptr);
@@ -544,6 +647,10 @@ void Engine::aggregate(const zd* x, zd* r, int nrhs) {
 // every level on the device (FP64).
 void Engine::aggregateh(const zd* r, zd* x, int nrhs) {
   if (!ok()) return;
+  if (cfg_.plan.radius > 0) {
+    rx_apply_h(r, x, nrhs);
+    return;
+  }
   const int L = P_.L;
   int R = cfg_.plan.numrx;
   double sc = (double)R * cfg_.plan.res * cfg_.plan.res;

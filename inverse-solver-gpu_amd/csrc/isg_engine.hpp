@@ -184,6 +184,9 @@ class Engine {
   std::vector<InterpShape>& m2m_shapes(bool fp32) { return fp32 ? m2m32_ : m2m64_; }
 
   void apply_mlfma(const zd* y, zd* r, int nrhs, const int* act, int prec);
+  void setup_receivers();
+  void rx_apply(const zd* x, zd* r, int nrhs);
+  void rx_apply_h(const zd* r, zd* x, int nrhs);
   // solver work vectors of storage type V (FP32 for the mixed solver)
   template <typename V>
   cplx<V>* vb(int k);
@@ -221,6 +224,13 @@ class Engine {
   std::vector<size_t> lvl_size_;  // 4^i * S_i
   std::vector<zd*> vec_;          // solver vectors, each [max_rhs][N]
   std::vector<cplx<float>*> vec32_;  // FP32 copies of the work vectors (mixed solver)
+  // finite-radius receivers: dense operator K[m][n] = (i/4) H0(k0 |rho_m - r_n|) res^2,
+  // applied with rocBLAS (split-K partial products rxpart_, conjugated inputs rxc_)
+  zd* rxm_ = nullptr;
+  zd* rxpart_ = nullptr;
+  zd* rxc_ = nullptr;
+  int rx_kc_ = 0;
+  void* blas_ = nullptr;  // rocblas_handle
   double2* part_ = nullptr;       // reduction partials [max_rhs][3][nblk]
   double2* part_h_ = nullptr;     // pinned host mirror
   BiState* state_ = nullptr;      // [max_rhs]

@@ -33,6 +33,10 @@ struct PlanConfig {
   int beta = 5;         // excess-bandwidth parameter (setup_mlfma.cpp:3)
   int ninter = 30;      // Lagrange taps (setup_mlfma.cpp:4)
   int numrx = 16;       // far-field receivers (mom.cpp:10)
+  // 0: plane-wave transmitters and far-field receivers (the reference);
+  // > 0: line-source transmitters and point receivers on a circle of this
+  // radius (wavelengths), equispaced from angle 0
+  double radius = 0.0;
 };
 
 struct Plan {

@@ -41,6 +41,7 @@ class Config(ctypes.Structure):
         ("precision", ctypes.c_int),
         ("mixed_margin", ctypes.c_double),
         ("graph", ctypes.c_int),
+        ("radius", ctypes.c_double),
     ]
 
 
