@@ -3,9 +3,11 @@
 
 Workload (the headline configuration): 1024x1024 pixels of lambda/10
 (N = 1,048,576 unknowns, quadtree level 8, 8x8-pixel leaf boxes), 64
-plane-wave transmitters, 256 far-field receivers, contrast = 20 seeded random
-ellipses (seed 3, eps_r - 1 in U(0.05, 0.8) + 0.02i), BiCGSTAB relative
-residual 1e-6 (iteration limit 300), Tikhonov regular = sqrt(1e-3).
+line-source transmitters and 256 receivers on a circle of radius 60
+wavelengths (--radius 0: the reference's plane waves and far-field
+receivers), contrast = 20 seeded random ellipses (seed 3, eps_r - 1 in
+U(0.05, 0.8) + 0.02i), BiCGSTAB relative residual 1e-6 (iteration limit
+300), Tikhonov regular = sqrt(1e-3); 1 warm-up and 5 timed DBIM iterations.
 
 One step = one DBIM iteration (mom.cpp:277-400): per transmitter a Frechet
 derivative solve, a total-field solve and an adjoint solve, plus the line
@@ -137,7 +139,7 @@ def cpu_baseline(level, numrx, matvecs_per_iter, nsample):
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="dbim-1024", choices=sorted(WORKLOADS))
     ap.add_argument("--paths", default="fast,plain")
@@ -151,6 +153,8 @@ def main():
     ap.add_argument("--kernel-timing", default="on", choices=["on", "off"],
                     help="live HIP-event kernel timing in the timed region (off: BiCGSTAB iterations "
                          "replay as HIP graphs, no roofline figures)")
+    ap.add_argument("--radius", type=float, default=60.0,
+                    help="transmitter/receiver circle radius in wavelengths (0: plane waves, far-field receivers)")
     ap.add_argument("--cpu-sample", type=int, default=2, help="oracle matvecs timed for cpu_baseline (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -164,7 +168,7 @@ def main():
     side = 8 << (level - 1)
     tuning = {"committed": True, "none": False}.get(args.tuning, args.tuning)
     common = dict(level=level, numrx=nrx, numtx=ntx, toli=1e-6, iti=300, regular=float(np.sqrt(1e-3)),
-                  tuning=tuning)
+                  tuning=tuning, radius=args.radius)
     paths = [p for p in args.paths.split(",") if p]
     fast_prec = 1 if args.fast_precision == "mixed" else 0
 
@@ -233,8 +237,10 @@ def main():
         "dtype": "f32 operator + f64 solver" if mixed else "f64",
         "data": "synthetic (seeded ellipse phantom, measurements synthesized by the FP64 forward solver)",
         "config": {
-            "workload": "DBIM %dx%d px lambda/10, level %d, %d Tx, %d Rx, %d ellipses seed %d, rel-res 1e-6"
-                        % (side, side, level, ntx, nrx, wl["nell"], wl["seed"]),
+            "workload": "DBIM %dx%d px lambda/10, level %d, %d Tx, %d Rx %s, %d ellipses seed %d, rel-res 1e-6"
+                        % (side, side, level, ntx, nrx,
+                           ("on a circle of radius %g lambda (line sources)" % args.radius) if args.radius > 0
+                           else "(plane waves, far-field receivers)", wl["nell"], wl["seed"]),
             "path": ("batched %d RHS + deflation, %s, %s total-field start"
                      % (ntx, "FP32 operator / FP64 solver + FP64 defect correction" if mixed else "FP64",
                         args.fast_start)
