@@ -212,7 +212,8 @@ int isg_raster_bin_read(const char* path, int numunk, double* raster);
  * stage 0 near-field tile width (16 or 32 boxes), 1 P2M tile width, 2 M2L tile
  * variant of `level` (0: 4x4 boxes x 8 samples, 1: 8x8 x 8, 2: 8x8 x 16,
  * 3: 16x16 x 8, 4: 8x8 x 32, 5: 16x16 x 16; 6, 7, 8: 8x8 x 16, 32, 8 with the
- * operators held in registers; 4, 5 and 7 FP32 only), 3 / 4 M2M / L2L
+ * operators held in registers; 9: 8x8 x 16, one thread per parent with its 36
+ * sources in registers; 4, 5 and 7 FP32 only), 3 / 4 M2M / L2L
  * kernel of `level` (0 band MFMA, 1 chunked LDS, 2 direct, 3 / 4 band MFMA with
  * 16 / 32 columns per workgroup); value -1 = built-in
  * rule.  precision 0 = FP64 operator, 1 = FP32 operator of the mixed path;

@@ -495,10 +495,11 @@ void Engine::launch_l2l(int i, bool accumulate, int nrhs, const int* act, int fo
 }
 
 // M2L variants: TS x TS target boxes and KC samples per workgroup; 0-5
-// k_m2l_tile, 6-8 k_m2l_reg (operators in registers, 8x8 tiles)
-constexpr int kM2LVariants = 9;
+// k_m2l_tile, 6-8 k_m2l_reg (operators in registers), 9 k_m2l_par (one
+// thread per parent and sample, sources in registers)
+constexpr int kM2LVariants = 10;
 constexpr int kM2LShape[kM2LVariants][2] = {{4, 8}, {8, 8}, {8, 16}, {16, 8}, {8, 32}, {16, 16},
-                                            {8, 16}, {8, 32}, {8, 8}};
+                                            {8, 16}, {8, 32}, {8, 8}, {8, 16}};
 
 template <typename T>
 bool Engine::m2l_ok(int v, int i) const {
@@ -531,6 +532,7 @@ void Engine::launch_m2l(int i, int nrhs, const int* act, int force) {
     case 3: k_m2l_tile<T, 16, 8><<<grid, blk, 0, stream_>>>(m, l, st, d.trans[i], i, S, act); break;
     case 6: k_m2l_reg<T, 16><<<grid, blk, 0, stream_>>>(m, l, st, d.trans[i], i, S, act); break;
     case 8: k_m2l_reg<T, 8><<<grid, blk, 0, stream_>>>(m, l, st, d.trans[i], i, S, act); break;
+    case 9: k_m2l_par<T><<<grid, blk, 0, stream_>>>(m, l, st, d.trans[i], i, S, act); break;
     case 7:
       if constexpr (sizeof(T) == 4) k_m2l_reg<T, 32><<<grid, blk, 0, stream_>>>(m, l, st, d.trans[i], i, S, act);
       break;

@@ -600,6 +600,72 @@ __global__ void __launch_bounds__(kBlock) k_m2l_reg(const cplx<T>* __restrict__ 
   }
 }
 
+// M2L per parent: a workgroup owns 4x4 parents (8x8 target boxes) and 16
+// samples; a thread owns one parent and one sample.  The far lists of the
+// four children all lie in the parent's 6x6 neighbourhood of child-level
+// boxes: the thread loads those 36 sources into registers once, then walks
+// the 49 translation offsets, reading each operator once and applying it to
+// every child for which that offset is a far interaction (27 per child).
+// LDS reads per thread: 36 + 49 for 108 complex multiply-adds.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_m2l_par(const cplx<T>* __restrict__ agm, cplx<T>* __restrict__ agl,
+                                                    long st, const cplx<T>* __restrict__ trans, int lev, int S,
+                                                    const int* __restrict__ act) {
+  constexpr int TS = 8, SB = TS + 4, KC = 16;
+  __shared__ cplx<T> src[SB * SB][KC];
+  __shared__ cplx<T> tr[49][KC];
+  const int rhs = blockIdx.y;
+  if (rhs_off(act, rhs)) return;
+  const int nb = 1 << lev;
+  const int nchunk = (S + KC - 1) / KC;
+  const int lb = xcd_block();
+  const int tile = lb / nchunk, chunk = lb - tile * nchunk;
+  const int k0 = chunk * KC;
+  const unsigned m0 = (unsigned)tile * TS * TS;
+  const int X0 = (int)compact2(m0), Y0 = (int)compact2(m0 >> 1);
+  const cplx<T>* a = agm + rhs * st;
+  for (int e = threadIdx.x; e < SB * SB * KC; e += kBlock) {
+    int sb = e / KC, kk = e - sb * KC;
+    int sx = X0 - 2 + sb % SB, sy = Y0 - 2 + sb / SB;
+    cplx<T> v = mkc<T>(0, 0);
+    if (sx >= 0 && sy >= 0 && sx < nb && sy < nb && k0 + kk < S) v = a[(long)morton2(sx, sy) * S + k0 + kk];
+    src[sb][kk] = v;
+  }
+  for (int e = threadIdx.x; e < 49 * KC; e += kBlock) {
+    int q = e / KC, kk = e - q * KC;
+    tr[q][kk] = (k0 + kk < S) ? trans[(long)q * S + k0 + kk] : mkc<T>(0, 0);
+  }
+  __syncthreads();
+  const int kk = threadIdx.x % KC, pp = threadIdx.x / KC;
+  const int lx0 = 2 * (pp & 3), ly0 = 2 * (pp >> 2);  // first child of the parent, tile coordinates
+  cplx<T> sv[6][6];
+#pragma unroll
+  for (int dy = 0; dy < 6; dy++)
+#pragma unroll
+    for (int dx = 0; dx < 6; dx++) sv[dy][dx] = src[(ly0 + dy) * SB + lx0 + dx][kk];
+  cplx<T> acc[4];
+#pragma unroll
+  for (int c = 0; c < 4; c++) acc[c] = mkc<T>(0, 0);
+#pragma unroll
+  for (int ey = -3; ey <= 3; ey++)
+#pragma unroll
+    for (int ex = -3; ex <= 3; ex++) {
+      const cplx<T> t = tr[7 * (ey + 3) + ex + 3][kk];
+      if (ex >= -1 && ex <= 1 && ey >= -1 && ey <= 1) continue;  // near offsets
+#pragma unroll
+      for (int c = 0; c < 4; c++) {
+        const int sx = (c & 1) + ex, sy = (c >> 1) + ey;  // source, relative to the first child
+        if (sx >= -2 && sx <= 3 && sy >= -2 && sy <= 3) acc[c] = cfma(t, sv[sy + 2][sx + 2], acc[c]);
+      }
+    }
+  if (k0 + kk < S) {
+    cplx<T>* out = agl + rhs * st;
+#pragma unroll
+    for (int c = 0; c < 4; c++)
+      out[(long)(m0 + morton2((unsigned)(lx0 + (c & 1)), (unsigned)(ly0 + (c >> 1)))) * S + k0 + kk] = acc[c];
+  }
+}
+
 // P2M on the matrix cores: agm[box][k] = sum_n coeffT[n][k] y[box*64+n] for a
 // tile of NB boxes; rows k in blocks of 16 spread over the 4 waves.
 template <typename T, typename TI, int NB>

@@ -13,8 +13,8 @@ from conftest import oracle_for, rand_c, relerr
 
 pytestmark = pytest.mark.gpu
 
-M2L_TS = [4, 8, 8, 16, 8, 16, 8, 8, 8]  # tile width of each M2L variant
-M2L_VARIANTS_F64 = [0, 1, 2, 3, 6, 8]
+M2L_TS = [4, 8, 8, 16, 8, 16, 8, 8, 8, 8]  # tile width of each M2L variant
+M2L_VARIANTS_F64 = [0, 1, 2, 3, 6, 8, 9]
 
 
 @pytest.fixture(scope="module")
@@ -50,7 +50,7 @@ def test_forced_m2l_variants_fp32(isg):
     s = isg.Solver(level=level, max_rhs=4, precision=1)
     X = np.stack([rand_c(np.random.default_rng(50 + r), o.N) for r in range(4)])
     ref = [o.mlfma(X[r]) for r in range(4)]
-    for v in range(9):
+    for v in range(10):
         print("fp32 m2l variant", v, flush=True)
         for i in range(2, level):
             s.tuning_set(1, 0, "m2l", i, v if (1 << i) >= M2L_TS[v] else -1)
