@@ -258,9 +258,11 @@ void Dbim::gradient(DbimStats& st) {
     k_adj_src<<<grid, blk, 0, s>>>(g_, wy_, wc_, N);
     DB_HIP(hipMemsetAsync(wx_, 0, sizeof(zd) * N * nb, s));
     solve(wx_, wc_, true, nb, true, st);
-    // c = conj(G conj(x))  (mlfmah, solve.cpp:8-24); the conjugate of Gc is taken in k_grad_acc
+    // c = conj(G conj(x))  (mlfmah, solve.cpp:8-24); the conjugate of Gc is taken in k_grad_acc.
+    // In mixed precision this product uses the FP32 operator (relative error ~1e-6, the
+    // solves' own tolerance); only the solves' residual checks need the FP64 operator.
     k_conj<<<grid, blk, 0, s>>>(wx_, wc_, N);
-    eng_.mlfma(wc_, wx_, nb);
+    eng_.mlfma(wc_, wx_, nb, nullptr, opt_.precision);
     k_grad_acc<<<dim3(nblk_), blk, 0, s>>>(buff_, tot_ + (size_t)t0 * N, wy_, wx_, o_, N, nb,
                                            opt_.regular * opt_.regular);
   }
@@ -364,7 +366,7 @@ void Dbim::iterate(DbimStats& st) {
     int nb = std::min(B_, T_ - t0);
     dim3 grid(nblk_, nb);
     k_deriv_src<<<grid, blk, 0, s>>>(tot_ + (size_t)t0 * N, ot_, wy_, N, k0_);
-    eng_.mlfma(wy_, wc_, nb);
+    eng_.mlfma(wy_, wc_, nb, nullptr, opt_.precision);  // FP32 operator in mixed precision, as above
     zd* dx = dtot_ ? dtot_ + (size_t)t0 * N : wx_;  // kept for the first-order field start
     DB_HIP(hipMemsetAsync(dx, 0, sizeof(zd) * N * nb, s));
     solve(dx, wc_, false, nb, true, st);
