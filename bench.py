@@ -83,9 +83,13 @@ def measured_traffic():
     return None
 
 
-def m2l_bytes_per_rhs(level, numsamp, cbytes):
-    """Read every multipole once and write every local once, levels 2..leaf."""
-    return sum(2 * cbytes * 4 ** i * numsamp[i] for i in range(2, level))
+def m2l_bytes_per_rhs(level, numsamp, cbytes, fused_leaf=False):
+    """Read every multipole once and write every local once, levels 2..leaf;
+    with the fused leaf kernel also read the parent locals it interpolates."""
+    b = sum(2 * cbytes * 4 ** i * numsamp[i] for i in range(2, level))
+    if fused_leaf:
+        b += cbytes * 4 ** (level - 2) * numsamp[level - 2]
+    return b
 
 
 def run_path(solver, dbim, warmup, steps, label, timing=True):
@@ -195,6 +199,7 @@ def main():
         if args.fast_start == "learned":
             d_fast.set_warmnet(isg.WarmNet(max_batch=8, max_pixels=side * side, path=WARMNET_FILE))
         res["fast"] = run_path(fast, d_fast, args.warmup, args.steps, "fast", args.kernel_timing == "on")
+        fast_leaf_fused = fast.tuning_get(fast_prec, 1, "leaf") == 1
         del d_fast
         fast.close()
     if "plain" in paths:
@@ -205,6 +210,7 @@ def main():
         plain.close()
 
     main_path = "fast" if "fast" in res else "plain"
+    fast_leaf_fused = "fast" in res and fast_leaf_fused
     s_iter, stats, prof = res[main_path]
     mixed = main_path == "fast" and fast_prec == 1
     matvecs = sum(s["matvecs"] for s in stats)
@@ -219,7 +225,9 @@ def main():
         n_near, prec, cb = matvecs, "f64", 16
     vec_ms, vec_n = prof["bicgs_update"]
     achieved = n_near * flops_mv / (near_ms * 1e-3) / 1e12 if near_ms > 0 else 0.0
-    m2l_gbs = n_near * m2l_bytes_per_rhs(level, numsamp, cb) / (m2l_ms * 1e-3) / 1e9 if m2l_ms > 0 else 0.0
+    fused = main_path == "fast" and fast_leaf_fused
+    m2l_gbs = (n_near * m2l_bytes_per_rhs(level, numsamp, cb, fused) / (m2l_ms * 1e-3) / 1e9
+               if m2l_ms > 0 else 0.0)
     iters = sum(s["solve_iters"] for s in stats)
     traffic = measured_traffic() if (mixed and level == 8 and ntx == 64) else None
 
@@ -264,7 +272,8 @@ def main():
             "launches": near_n,
         },
         "m2l": {"achieved_GBs": round(m2l_gbs, 1), "peak_GBs": PEAK_HBM_GBS, "frac": round(m2l_gbs / PEAK_HBM_GBS, 4),
-                "ms_per_matvec_launchset": round(m2l_ms / max(m2l_n, 1), 4)},
+                "ms_per_matvec_launchset": round(m2l_ms / max(m2l_n, 1), 4),
+                "leaf_l2l_fused": fused},
         # x/r update reads x, p, s, t, r~ and writes x, r: x is FP64 (16 B), the
         # work vectors FP64 (plain) or FP32 (mixed): 7 x 16 B or 2 x 16 + 5 x 8 B per unknown
         "bicgs_update": {"GBs": round((72 if mixed else 112) * side * side * iters / max(vec_ms * 1e-3, 1e-12) / 1e9, 1),

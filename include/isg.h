@@ -215,8 +215,10 @@ int isg_raster_bin_read(const char* path, int numunk, double* raster);
  * operators held in registers; 9: 8x8 x 16, one thread per parent with its 36
  * sources in registers; 4, 5 and 7 FP32 only), 3 / 4 M2M / L2L
  * kernel of `level` (0 band MFMA, 1 chunked LDS, 2 direct, 3 / 4 band MFMA with
- * 16 / 32 columns per workgroup); value -1 = built-in
- * rule.  precision 0 = FP64 operator, 1 = FP32 operator of the mixed path;
+ * 16 / 32 columns per workgroup), 5 leaf fusion (`level` ignored; 1: the
+ * leaf-level M2L and the L2L into the leaf level run as one kernel that
+ * writes each leaf local expansion once, 0: separate); value -1 = built-in
+ * rule (separate).  precision 0 = FP64 operator, 1 = FP32 operator of the mixed path;
  * batch_class 0 for fewer than 16 right-hand sides, 1 otherwise.
  * isg_autotune times every admissible variant (reps launches each) and keeps
  * the fastest for batch_class(nrhs) and every operator precision the context has.

@@ -547,6 +547,33 @@ void Engine::launch_m2l(int i, int nrhs, const int* act, int force) {
   }
 }
 
+// k_m2l_leaf: leaf M2L plus the band L2L from level Lf-1, whose 16-row blocks
+// coincide with the kernel's 16-sample chunks.  Needs a leaf level of at
+// least 8x8 boxes with a parent level below it that has M2L (>= 2), the band
+// form of that L2L, and the LDS of both stages within the limit.
+template <typename T>
+bool Engine::leaf_fuse_ok() const {
+  const int Lf = P_.L - 1;
+  if (Lf < 3) return false;
+  const InterpShape& sh = (sizeof(T) == 4 ? l2l32_ : l2l64_)[Lf - 1];
+  if (sh.ncb <= 0 || sh.kb <= 0 || sh.kb > P_.nsamp[Lf - 1]) return false;
+  const size_t lds = ((size_t)(kParSB * kParSB + 49) * kParKC + 20 * (size_t)sh.kb) * sizeof(cplx<T>);
+  return lds <= kLdsLimit;
+}
+
+template <typename T>
+void Engine::launch_m2l_leaf(int nrhs, const int* act) {
+  const int Lf = P_.L - 1, S = P_.nsamp[Lf], Sp = P_.nsamp[Lf - 1], nb = 1 << Lf;
+  const InterpShape& sh = (sizeof(T) == 4 ? l2l32_ : l2l64_)[Lf - 1];
+  DevOp<T>& d = ops<T>();
+  const int nchunk = (S + kParKC - 1) / kParKC;
+  dim3 grid(nb * nb / (kParTS * kParTS) * nchunk, nrhs), blk(kBlock);
+  const size_t lds = 20 * (size_t)sh.kb * sizeof(cplx<T>);
+  k_m2l_leaf<T><<<grid, blk, lds, stream_>>>(d.agm + lvl_off_[Lf], d.agl + lvl_off_[Lf], (long)lvl_size_[Lf],
+                                             d.trans[Lf], Lf, S, d.agl + lvl_off_[Lf - 1], (long)lvl_size_[Lf - 1],
+                                             Sp, d.band_l2l[Lf - 1], d.bs0_l2l[Lf - 1], sh.kb, d.shiftL[Lf - 1], act);
+}
+
 // Near field + L2P on the matrix cores (k_near_mfma), or L2P alone when
 // with_near is false (receiver back-projection).
 template <typename T, typename TI>
@@ -580,10 +607,19 @@ void Engine::apply(const cplx<TI>* y, cplx<TI>* r, int nrhs, const int* act) {
   launch_p2m<T, TI>(y, nrhs, act);
   for (int i = L - 2; i >= 2; i--) launch_m2m<T>(i, nrhs, act);
   const int kt = sizeof(T) == 4 ? KT_M2L32 : KT_M2L;
+  // with the fused leaf kernel the last L2L (level Lf-1 -> Lf) runs inside the
+  // leaf M2L, after the L2L passes above it
+  const bool fuse = tune(sizeof(T) == 4 ? PREC_MIXED : PREC_FP64, nrhs).leaf == 1 && leaf_fuse_ok<T>();
+  const int mtop = fuse ? Lf - 1 : Lf, ltop = fuse ? L - 3 : L - 2;
   prof_begin(kt);
-  for (int i = 2; i <= Lf; i++) launch_m2l<T>(i, nrhs, act);
+  for (int i = 2; i <= mtop; i++) launch_m2l<T>(i, nrhs, act);
   prof_end(kt);
-  for (int i = 2; i <= L - 2; i++) launch_l2l<T>(i, true, nrhs, act);
+  for (int i = 2; i <= ltop; i++) launch_l2l<T>(i, true, nrhs, act);
+  if (fuse) {
+    prof_begin(kt, false);  // timed with the M2L launch set of this product
+    launch_m2l_leaf<T>(nrhs, act);
+    prof_end(kt);
+  }
   launch_near<T, TI>(y, r, nrhs, act, true);
 }
 
@@ -678,7 +714,7 @@ void Engine::profile(bool on) {
   }
 }
 
-void Engine::prof_begin(int id) {
+void Engine::prof_begin(int id, bool new_set) {
   if (!prof_) return;
   KTimer& k = kt_[id];
   if (k.used == (int)k.ev0.size()) {
@@ -691,8 +727,10 @@ void Engine::prof_begin(int id) {
       ISG_HIP(hipEventCreate(&b));
       k.ev0.push_back(a);
       k.ev1.push_back(b);
+      k.starts.push_back(0);
     }
   }
+  k.starts[k.used] = new_set ? 1 : 0;
   ISG_HIP(hipEventRecord(k.ev0[k.used], stream_));
 }
 
@@ -711,7 +749,7 @@ void Engine::harvest() {
       float ms = 0;
       if (hipEventSynchronize(k.ev1[i]) == hipSuccess && hipEventElapsedTime(&ms, k.ev0[i], k.ev1[i]) == hipSuccess) {
         k.ms += ms;
-        k.launches++;
+        k.launches += k.starts[i];
       }
     }
     k.used = 0;
@@ -993,6 +1031,21 @@ void Engine::autotune_prec(int nrhs, int reps) {
     tu.l2l[i] = pick(
         5, [&](int k) { return kind_ok(sl[i], k); }, [&](int k) { launch_l2l<T>(i, true, nrhs, nullptr, k); });
   }
+  // fused leaf kernel against the best separate leaf M2L + last L2L
+  tu.leaf = -1;
+  if (leaf_fuse_ok<T>()) {
+    const int vm = tu.m2l[Lf], kl = tu.l2l[L - 2];
+    tu.leaf = pick(
+        2, [](int) { return true; },
+        [&](int f) {
+          if (f == 1) {
+            launch_m2l_leaf<T>(nrhs, nullptr);
+          } else {
+            launch_m2l<T>(Lf, nrhs, nullptr, vm);
+            launch_l2l<T>(L - 2, true, nrhs, nullptr, kl);
+          }
+        });
+  }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   ISG_HIP(hipGetLastError());
@@ -1008,7 +1061,7 @@ void Engine::autotune(int nrhs, int reps) {
   if (has32_) autotune_prec<float>(nrhs, reps);
 }
 
-static const char* kStageName[Engine::ST_COUNT] = {"near", "p2m", "m2l", "m2m", "l2l"};
+static const char* kStageName[Engine::ST_COUNT] = {"near", "p2m", "m2l", "m2m", "l2l", "leaf"};
 
 bool Engine::tuning_set(int prec, int cls, int stage, int level, int value) {
   if ((prec != PREC_FP64 && prec != PREC_MIXED) || cls < 0 || cls > 1 || stage < 0 || stage >= ST_COUNT) return false;
@@ -1016,6 +1069,11 @@ bool Engine::tuning_set(int prec, int cls, int stage, int level, int value) {
   if (stage == ST_NEAR || stage == ST_P2M) {
     if (value != -1 && value != 16 && value != 32) return false;
     (stage == ST_NEAR ? t.near_nb : t.p2m_nb) = value;
+    return true;
+  }
+  if (stage == ST_LEAF) {
+    if (value < -1 || value > 1) return false;
+    t.leaf = value;
     return true;
   }
   if (level < 0 || level >= P_.L || value < -1) return false;
@@ -1034,6 +1092,7 @@ int Engine::tuning_get(int prec, int cls, int stage, int level) const {
   const Tuning& t = tune_[prec == PREC_MIXED ? 1 : 0][cls];
   if (stage == ST_NEAR) return t.near_nb;
   if (stage == ST_P2M) return t.p2m_nb;
+  if (stage == ST_LEAF) return t.leaf;
   if (level < 0 || level >= P_.L) return -1;
   return stage == ST_M2L ? t.m2l[level] : (stage == ST_M2M ? t.m2m[level] : t.l2l[level]);
 }
@@ -1046,7 +1105,7 @@ bool Engine::tuning_save(const std::string& path) {
   for (int p = 0; p < 2; p++)
     for (int c = 0; c < 2; c++)
       for (int s = 0; s < ST_COUNT; s++)
-        for (int i = 0; i < ((s == ST_NEAR || s == ST_P2M) ? 1 : P_.L); i++) {
+        for (int i = 0; i < ((s == ST_NEAR || s == ST_P2M || s == ST_LEAF) ? 1 : P_.L); i++) {
           int v = tuning_get(p, c, s, i);
           if (v >= 0) f << P_.L << ' ' << p << ' ' << c << ' ' << kStageName[s] << ' ' << i << ' ' << v << '\n';
         }

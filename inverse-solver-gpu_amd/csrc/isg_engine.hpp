@@ -119,9 +119,10 @@ class Engine {
   // boxes), M2L tile variant per level (kM2LVariants), M2M/L2L kernel kind per
   // level (0 band MFMA, 1 chunked LDS, 2 direct, 3 / 4 band MFMA with 16 / 32
   // columns per workgroup).
-  enum { ST_NEAR = 0, ST_P2M = 1, ST_M2L = 2, ST_M2M = 3, ST_L2L = 4, ST_COUNT = 5 };
+  enum { ST_NEAR = 0, ST_P2M = 1, ST_M2L = 2, ST_M2M = 3, ST_L2L = 4, ST_LEAF = 5, ST_COUNT = 6 };
   struct Tuning {
     int near_nb = -1, p2m_nb = -1;
+    int leaf = -1;  // 1: leaf M2L and the last L2L as one kernel (k_m2l_leaf), 0 / -1: separate
     std::vector<int> m2l, m2m, l2l;
   };
   static int batch_class(int nrhs) { return nrhs >= 16 ? 1 : 0; }
@@ -137,6 +138,7 @@ class Engine {
  private:
   struct KTimer {
     std::vector<hipEvent_t> ev0, ev1;
+    std::vector<char> starts;  // interval i opens a launch set
     int used = 0;
     double ms = 0;
     long long launches = 0;
@@ -172,6 +174,10 @@ class Engine {
   void launch_l2l(int i, bool accumulate, int nrhs, const int* act, int force = -1);
   template <typename T>
   void launch_m2l(int i, int nrhs, const int* act, int force = -1);
+  template <typename T>
+  void launch_m2l_leaf(int nrhs, const int* act);
+  template <typename T>
+  bool leaf_fuse_ok() const;
   template <typename T, typename TI>
   void launch_near(const cplx<TI>* y, cplx<TI>* r, int nrhs, const int* act, bool with_near, int force = -1);
   template <typename T>
@@ -202,7 +208,8 @@ class Engine {
   void blend(zd* x, const zd* d, const zd* g, long gs, const zd* b, bool herm, int nrhs, int prec);
   void true_residual(const zd* x, const zd* g, long gs, const zd* b, bool herm, int nrhs, zd* res,
                      std::vector<double>& rn, std::vector<double>& bn);
-  void prof_begin(int id);
+  // new_set false: the interval adds to the previous launch set's time
+  void prof_begin(int id, bool new_set = true);
   void prof_end(int id);
   void harvest();
   template <typename V>

@@ -32,6 +32,34 @@ __device__ __forceinline__ int xcd_block() {
   return (n & 7) ? x : (x & 7) * (n >> 3) + (x >> 3);
 }
 
+// Strided staging of n elements (element e = tid, tid + kBlock, ...): the
+// loads of U consecutive strides, ld(e), are issued before any of them is
+// stored with st(e, v), so each thread has U memory requests in flight
+// instead of waiting for every load in turn before the next one issues.
+// ld should only load (arithmetic on the loaded values belongs in st, or the
+// compiler waits for each load before issuing the next).
+template <typename T>
+struct CPair {
+  cplx<T> a, b;
+};
+
+template <int U, typename V, typename Ld, typename St>
+__device__ __forceinline__ void stage(int n, Ld ld, St st) {
+  for (int e0 = threadIdx.x; e0 < n; e0 += U * kBlock) {
+    V v[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int e = e0 + u * kBlock;
+      if (e < n) v[u] = ld(e);
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int e = e0 + u * kBlock;
+      if (e < n) st(e, v[u]);
+    }
+  }
+}
+
 // Direct-from-cache interpolation kernels, one thread per target sample; used
 // for root levels whose LDS staging would exceed 64 KB.
 // parent[p][k] = sum_c shift[c][k] * sum_t w[t][k] child[4p+c][(i0[k]+t) mod Sc]
@@ -195,13 +223,16 @@ __global__ void __launch_bounds__(kBlock) k_near_mfma(const cplx<TI>* __restrict
   const cplx<TI>* yr = y + rhs * ys;
   if (with_near) {
     const int X0 = (int)compact2((unsigned)b0), Y0 = (int)compact2((unsigned)b0 >> 1);
-    for (int idx = tid; idx < HX * HY * 64; idx += kBlock) {
-      int h = idx >> 6, n = idx & 63;
-      int sx = X0 - 1 + h % HX, sy = Y0 - 1 + h / HX;
-      cplx<T> v = mkc<T>(0, 0);
-      if (sx >= 0 && sy >= 0 && sx < nbs && sy < nbs) v = ccast<T>(yr[(long)morton2(sx, sy) * 64 + n]);
-      tile[h * HROW + n] = v;
-    }
+    stage<sizeof(T) == 4 ? 8 : 4, cplx<TI>>(
+        HX * HY * 64,
+        [&](int idx) {
+          int h = idx >> 6, n = idx & 63;
+          int sx = X0 - 1 + h % HX, sy = Y0 - 1 + h / HX;
+          cplx<TI> v = mkc<TI>(0, 0);
+          if (sx >= 0 && sy >= 0 && sx < nbs && sy < nbs) v = yr[(long)morton2(sx, sy) * 64 + n];
+          return v;
+        },
+        [&](int idx, cplx<TI> v) { tile[(idx >> 6) * HROW + (idx & 63)] = ccast<T>(v); });
     // halo row of column 16 j + li for the centre slot
     int hc[NN];
 #pragma unroll
@@ -229,10 +260,12 @@ __global__ void __launch_bounds__(kBlock) k_near_mfma(const cplx<TI>* __restrict
   // leaf locals -> pixels (L2P) as the last K block
   __syncthreads();
   const cplx<T>* lr = loc + rhs * ls + b0 * S;
-  for (int idx = tid; idx < NB * S; idx += kBlock) {
-    int j = idx / S, k = idx - j * S;
-    tile[j * row + k] = lr[idx];
-  }
+  stage<sizeof(T) == 4 ? 8 : 4, cplx<T>>(
+      NB * S, [&](int idx) { return lr[idx]; },
+      [&](int idx, cplx<T> v) {
+        int j = idx / S;
+        tile[j * row + idx - j * S] = v;
+      });
   __syncthreads();
   const cplx<T>* A = basisT + 16 * w + li;
   for (int kk = 0; kk < S / 4; kk++) {
@@ -287,12 +320,15 @@ __global__ void __launch_bounds__(kBlock) k_m2m_chunk(const cplx<T>* __restrict_
   if (W < 0) W += Sc;
   W += ntap;
   const cplx<T>* cr = child + rhs * cs + 4 * p0 * Sc;
-  for (int e = threadIdx.x; e < 4 * G * W; e += kBlock) {
-    int v = e / W, t = e - v * W;  // v = 4 g + c (consecutive children)
-    int j = base + t;
-    while (j >= Sc) j -= Sc;
-    lds[e] = cr[(long)v * Sc + j];
-  }
+  stage<sizeof(T) == 4 ? 8 : 4, cplx<T>>(
+      4 * G * W,
+      [&](int e) {
+        int v = e / W, t = e - v * W;  // v = 4 g + c (consecutive children)
+        int j = base + t;
+        while (j >= Sc) j -= Sc;
+        return cr[(long)v * Sc + j];
+      },
+      [&](int e, cplx<T> v) { lds[e] = v; });
   __syncthreads();
   int o = threadIdx.x;
   if (o >= G * kn) return;
@@ -331,12 +367,15 @@ __global__ void __launch_bounds__(kBlock) k_l2l_chunk(const cplx<T>* __restrict_
   if (W < 0) W += Sp;
   W += ntap;
   const cplx<T>* pv = par + rhs * ps + (long)p * Sp;
-  for (int e = threadIdx.x; e < 4 * W; e += kBlock) {
-    int c = e / W, t = e - c * W;
-    int j = base + t;
-    while (j >= Sp) j -= Sp;
-    lds[e] = cmul(shift[c * Sp + j], pv[j]);
-  }
+  stage<4, CPair<T>>(
+      4 * W,
+      [&](int e) {
+        int c = e / W, t = e - c * W;
+        int j = base + t;
+        while (j >= Sp) j -= Sp;
+        return CPair<T>{shift[c * Sp + j], pv[j]};
+      },
+      [&](int e, CPair<T> v) { lds[e] = cmul(v.a, v.b); });
   __syncthreads();
   for (int o = threadIdx.x; o < 4 * kn; o += kBlock) {
     int c = o / kn, k = k0 + (o - c * kn);
@@ -384,18 +423,22 @@ __global__ void __launch_bounds__(kBlock) k_interp_band(const cplx<T>* __restric
   const int li = lane & 15, lk = lane >> 4;
   const int row = K + 1;
   const long p0 = (long)xcd_block() * (NC / 4);
+  auto put = [&](int e, cplx<T> v) {
+    int col = e / K;
+    tile[col * row + e - col * K] = v;
+  };
   if (MODE == 0) {
     const cplx<T>* s = src + rhs * ss + 4 * p0 * K;  // NC consecutive children of length K
-    for (int e = tid; e < NC * K; e += kBlock) {
-      int col = e / K, j = e - col * K;
-      tile[col * row + j] = s[e];
-    }
+    stage<8, cplx<T>>(NC * K, [&](int e) { return s[e]; }, put);
   } else {
     const cplx<T>* s = src + rhs * ss + p0 * K;  // NC/4 parents of length K
-    for (int e = tid; e < NC * K; e += kBlock) {
-      int col = e / K, j = e - col * K;
-      tile[col * row + j] = cmul(shift[(col & 3) * K + j], s[(long)(col >> 2) * K + j]);
-    }
+    stage<8, CPair<T>>(
+        NC * K,
+        [&](int e) {
+          int col = e / K, j = e - col * K;
+          return CPair<T>{shift[(col & 3) * K + j], s[(long)(col >> 2) * K + j]};
+        },
+        [&](int e, CPair<T> v) { put(e, cmul(v.a, v.b)); });
   }
   __syncthreads();
   const int nrb = (Mrows + 15) / 16;
@@ -461,6 +504,33 @@ __global__ void __launch_bounds__(kBlock) k_interp_band(const cplx<T>* __restric
   }
 }
 
+// M2L staging for a TS x TS tile of target boxes at corner (X0, Y0) and
+// samples k0 .. k0+KC-1: the multipoles of the (TS+4)^2 boxes around it
+// (zero outside the level or past S) and the 49 translation operators.
+template <typename T, int TS, int KC>
+__device__ __forceinline__ void m2l_stage(cplx<T> (*src)[KC], cplx<T> (*tr)[KC], const cplx<T>* a,
+                                          const cplx<T>* trans, int X0, int Y0, int k0, int nb, int S) {
+  constexpr int SB = TS + 4, NS = SB * SB * KC;
+  constexpr int US = NS / kBlock < 9 ? (NS + kBlock - 1) / kBlock : 9;
+  stage<US, cplx<T>>(
+      NS,
+      [&](int e) {
+        int sb = e / KC, kk = e - sb * KC;
+        int sx = X0 - 2 + sb % SB, sy = Y0 - 2 + sb / SB;
+        cplx<T> v = mkc<T>(0, 0);
+        if (sx >= 0 && sy >= 0 && sx < nb && sy < nb && k0 + kk < S) v = a[(long)morton2(sx, sy) * S + k0 + kk];
+        return v;
+      },
+      [&](int e, cplx<T> v) { src[e / KC][e % KC] = v; });
+  stage<(49 * KC + kBlock - 1) / kBlock, cplx<T>>(
+      49 * KC,
+      [&](int e) {
+        int q = e / KC, kk = e - q * KC;
+        return (k0 + kk < S) ? trans[(long)q * S + k0 + kk] : mkc<T>(0, 0);
+      },
+      [&](int e, cplx<T> v) { tr[e / KC][e % KC] = v; });
+}
+
 // Tiled M2L.  A workgroup owns a TS x TS block of target boxes (Morton
 // consecutive) and KC consecutive samples.  The interaction lists of the
 // block lie in the (TS+4)^2 block of boxes around it; those multipoles and
@@ -483,18 +553,7 @@ __global__ void __launch_bounds__(kBlock) k_m2l_tile(const cplx<T>* __restrict__
   const int k0 = chunk * KC;
   const unsigned m0 = (unsigned)tile * TS * TS;
   const int X0 = (int)compact2(m0), Y0 = (int)compact2(m0 >> 1);
-  const cplx<T>* a = agm + rhs * st;
-  for (int e = threadIdx.x; e < SB * SB * KC; e += kBlock) {
-    int sb = e / KC, kk = e - sb * KC;
-    int sx = X0 - 2 + sb % SB, sy = Y0 - 2 + sb / SB;
-    cplx<T> v = mkc<T>(0, 0);
-    if (sx >= 0 && sy >= 0 && sx < nb && sy < nb && k0 + kk < S) v = a[(long)morton2(sx, sy) * S + k0 + kk];
-    src[sb][kk] = v;
-  }
-  for (int e = threadIdx.x; e < 49 * KC; e += kBlock) {
-    int q = e / KC, kk = e - q * KC;
-    tr[q][kk] = (k0 + kk < S) ? trans[(long)q * S + k0 + kk] : mkc<T>(0, 0);
-  }
+  m2l_stage<T, TS, KC>(src, tr, agm + rhs * st, trans, X0, Y0, k0, nb, S);
   __syncthreads();
   for (int o = threadIdx.x; o < TS * TS * KC; o += kBlock) {
     int tl = o / KC, kk = o - tl * KC;
@@ -576,18 +635,7 @@ __global__ void __launch_bounds__(kBlock) k_m2l_reg(const cplx<T>* __restrict__ 
   const int k0 = chunk * KC;
   const unsigned m0 = (unsigned)tile * TS * TS;
   const int X0 = (int)compact2(m0), Y0 = (int)compact2(m0 >> 1);
-  const cplx<T>* a = agm + rhs * st;
-  for (int e = threadIdx.x; e < SB * SB * KC; e += kBlock) {
-    int sb = e / KC, kk = e - sb * KC;
-    int sx = X0 - 2 + sb % SB, sy = Y0 - 2 + sb / SB;
-    cplx<T> v = mkc<T>(0, 0);
-    if (sx >= 0 && sy >= 0 && sx < nb && sy < nb && k0 + kk < S) v = a[(long)morton2(sx, sy) * S + k0 + kk];
-    src[sb][kk] = v;
-  }
-  for (int e = threadIdx.x; e < 49 * KC; e += kBlock) {
-    int q = e / KC, kk = e - q * KC;
-    tr[q][kk] = (k0 + kk < S) ? trans[(long)q * S + k0 + kk] : mkc<T>(0, 0);
-  }
+  m2l_stage<T, TS, KC>(src, tr, agm + rhs * st, trans, X0, Y0, k0, nb, S);
   __syncthreads();
   const int kk = threadIdx.x % KC, grp = threadIdx.x / KC;
   const int cls = grp & 3, part = grp >> 2;
@@ -607,43 +655,38 @@ __global__ void __launch_bounds__(kBlock) k_m2l_reg(const cplx<T>* __restrict__ 
 // the 49 translation offsets, reading each operator once and applying it to
 // every child for which that offset is a far interaction (27 per child).
 // LDS reads per thread: 36 + 49 for 108 complex multiply-adds.
+constexpr int kParTS = 8, kParSB = kParTS + 4, kParKC = 16;
+
+// Workgroup coordinates shared by k_m2l_par and k_m2l_leaf: 8x8 boxes
+// starting at Morton index m0 (corner X0, Y0), samples k0 .. k0+15.
+struct ParBlock {
+  int chunk, k0, X0, Y0;
+  unsigned m0;
+  __device__ __forceinline__ ParBlock(int S) {
+    const int nchunk = (S + kParKC - 1) / kParKC;
+    const int lb = xcd_block();
+    const int tile = lb / nchunk;
+    chunk = lb - tile * nchunk;
+    k0 = chunk * kParKC;
+    m0 = (unsigned)tile * kParTS * kParTS;
+    X0 = (int)compact2(m0);
+    Y0 = (int)compact2(m0 >> 1);
+  }
+};
+
+// the parent's 6x6 source neighbourhood (first child at tile position lx0, ly0)
 template <typename T>
-__global__ void __launch_bounds__(kBlock) k_m2l_par(const cplx<T>* __restrict__ agm, cplx<T>* __restrict__ agl,
-                                                    long st, const cplx<T>* __restrict__ trans, int lev, int S,
-                                                    const int* __restrict__ act) {
-  constexpr int TS = 8, SB = TS + 4, KC = 16;
-  __shared__ cplx<T> src[SB * SB][KC];
-  __shared__ cplx<T> tr[49][KC];
-  const int rhs = blockIdx.y;
-  if (rhs_off(act, rhs)) return;
-  const int nb = 1 << lev;
-  const int nchunk = (S + KC - 1) / KC;
-  const int lb = xcd_block();
-  const int tile = lb / nchunk, chunk = lb - tile * nchunk;
-  const int k0 = chunk * KC;
-  const unsigned m0 = (unsigned)tile * TS * TS;
-  const int X0 = (int)compact2(m0), Y0 = (int)compact2(m0 >> 1);
-  const cplx<T>* a = agm + rhs * st;
-  for (int e = threadIdx.x; e < SB * SB * KC; e += kBlock) {
-    int sb = e / KC, kk = e - sb * KC;
-    int sx = X0 - 2 + sb % SB, sy = Y0 - 2 + sb / SB;
-    cplx<T> v = mkc<T>(0, 0);
-    if (sx >= 0 && sy >= 0 && sx < nb && sy < nb && k0 + kk < S) v = a[(long)morton2(sx, sy) * S + k0 + kk];
-    src[sb][kk] = v;
-  }
-  for (int e = threadIdx.x; e < 49 * KC; e += kBlock) {
-    int q = e / KC, kk = e - q * KC;
-    tr[q][kk] = (k0 + kk < S) ? trans[(long)q * S + k0 + kk] : mkc<T>(0, 0);
-  }
-  __syncthreads();
-  const int kk = threadIdx.x % KC, pp = threadIdx.x / KC;
-  const int lx0 = 2 * (pp & 3), ly0 = 2 * (pp >> 2);  // first child of the parent, tile coordinates
-  cplx<T> sv[6][6];
+__device__ __forceinline__ void par_load(const cplx<T> (*src)[kParKC], int lx0, int ly0, int kk, cplx<T> (&sv)[6][6]) {
 #pragma unroll
   for (int dy = 0; dy < 6; dy++)
 #pragma unroll
-    for (int dx = 0; dx < 6; dx++) sv[dy][dx] = src[(ly0 + dy) * SB + lx0 + dx][kk];
-  cplx<T> acc[4];
+    for (int dx = 0; dx < 6; dx++) sv[dy][dx] = src[(ly0 + dy) * kParSB + lx0 + dx][kk];
+}
+
+// acc[c] = sum over the 27 far offsets of child c of T_offset * source
+template <typename T>
+__device__ __forceinline__ void par_translate(const cplx<T> (*tr)[kParKC], int kk, const cplx<T> (&sv)[6][6],
+                                              cplx<T> (&acc)[4]) {
 #pragma unroll
   for (int c = 0; c < 4; c++) acc[c] = mkc<T>(0, 0);
 #pragma unroll
@@ -658,11 +701,106 @@ __global__ void __launch_bounds__(kBlock) k_m2l_par(const cplx<T>* __restrict__ 
         if (sx >= -2 && sx <= 3 && sy >= -2 && sy <= 3) acc[c] = cfma(t, sv[sy + 2][sx + 2], acc[c]);
       }
     }
-  if (k0 + kk < S) {
-    cplx<T>* out = agl + rhs * st;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_m2l_par(const cplx<T>* __restrict__ agm, cplx<T>* __restrict__ agl,
+                                                    long st, const cplx<T>* __restrict__ trans, int lev, int S,
+                                                    const int* __restrict__ act) {
+  __shared__ cplx<T> src[kParSB * kParSB][kParKC];
+  __shared__ cplx<T> tr[49][kParKC];
+  const int rhs = blockIdx.y;
+  if (rhs_off(act, rhs)) return;
+  const ParBlock B(S);
+  m2l_stage<T, kParTS, kParKC>(src, tr, agm + rhs * st, trans, B.X0, B.Y0, B.k0, 1 << lev, S);
+  __syncthreads();
+  const int kk = threadIdx.x % kParKC, pp = threadIdx.x / kParKC;
+  const int lx0 = 2 * (pp & 3), ly0 = 2 * (pp >> 2);  // first child of the parent, tile coordinates
+  cplx<T> sv[6][6], acc[4];
+  par_load<T>(src, lx0, ly0, kk, sv);
+  par_translate<T>(tr, kk, sv, acc);
+  if (B.k0 + kk < S) {
+    cplx<T>* out = agl + rhs * st + (long)(B.m0 + 4 * morton2((unsigned)(pp & 3), (unsigned)(pp >> 2))) * S;
 #pragma unroll
-    for (int c = 0; c < 4; c++)
-      out[(long)(m0 + morton2((unsigned)(lx0 + (c & 1)), (unsigned)(ly0 + (c >> 1)))) * S + k0 + kk] = acc[c];
+    for (int c = 0; c < 4; c++) out[(long)c * S + B.k0 + kk] = acc[c];
+  }
+}
+
+// Leaf M2L fused with the last L2L (parent level lev-1 -> leaf level lev):
+// the k_m2l_par workgroup (4x4 parents, 16 samples = one 16-row block of the
+// L2L band matrix) also interpolates its parents' local expansions to the
+// children's samples on the matrix cores (as k_interp_band in L2L mode,
+// columns = the 64 children, window = the row block's KB parent samples
+// from s0v[chunk], mod Sp) and writes  local = translations + interpolation
+// once.  The separate L2L pass would read the parents again and read and
+// rewrite every leaf local after the M2L wrote it.  Dynamic LDS: the parent
+// window, 16 x KB, and the child shifts on it, 4 x KB.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_m2l_leaf(const cplx<T>* __restrict__ agm, cplx<T>* __restrict__ agl,
+                                                     long st, const cplx<T>* __restrict__ trans, int lev, int S,
+                                                     const cplx<T>* __restrict__ par, long ps, int Sp,
+                                                     const T* __restrict__ AB, const int* __restrict__ s0v, int KB,
+                                                     const cplx<T>* __restrict__ shift, const int* __restrict__ act) {
+  typedef typename Mfma<T>::v4 v4;
+  __shared__ cplx<T> src[kParSB * kParSB][kParKC];  // after the MFMA phase: the 64 x 16 interpolated locals
+  __shared__ cplx<T> tr[49][kParKC];
+  extern __shared__ char smem_raw[];
+  cplx<T>* pw = reinterpret_cast<cplx<T>*>(smem_raw);  // [16 parents][KB]
+  cplx<T>* sw = pw + 16 * KB;                          // [4 children][KB]
+  const int rhs = blockIdx.y;
+  if (rhs_off(act, rhs)) return;
+  const ParBlock B(S);
+  m2l_stage<T, kParTS, kParKC>(src, tr, agm + rhs * st, trans, B.X0, B.Y0, B.k0, 1 << lev, S);
+  const int s0 = s0v[B.chunk];
+  const cplx<T>* pr = par + rhs * ps + (long)(B.m0 >> 2) * Sp;  // the tile's 16 parents, Morton order
+  stage<4, cplx<T>>(
+      16 * KB,
+      [&](int e) {
+        int p = e / KB, j = s0 + e - p * KB;
+        if (j >= Sp) j -= Sp;
+        return pr[(long)p * Sp + j];
+      },
+      [&](int e, cplx<T> v) { pw[e] = v; });
+  stage<2, cplx<T>>(
+      4 * KB,
+      [&](int e) {
+        int c = e / KB, j = s0 + e - c * KB;
+        if (j >= Sp) j -= Sp;
+        return shift[c * Sp + j];
+      },
+      [&](int e, cplx<T> v) { sw[e] = v; });
+  __syncthreads();
+  const int tid = threadIdx.x, kk = tid % kParKC, pp = tid / kParKC;
+  cplx<T> sv[6][6], acc[4];
+  par_load<T>(src, 2 * (pp & 3), 2 * (pp >> 2), kk, sv);
+  // interpolation: wave w owns columns 16w .. 16w+15 (children 16w + i, i.e.
+  // parent 4w + i/4, child position i%4); lane (li, lk) feeds column li at
+  // window sample 4q + lk and receives rows Mfma::row(lk, q) at sample li
+  const int lane = tid & 63, w = tid >> 6, li = lane & 15, lk = lane >> 4;
+  const T* ab = AB + (long)B.chunk * KB * 16;
+  const cplx<T>* pcol = pw + (4 * w + (li >> 2)) * KB;
+  const cplx<T>* scol = sw + (li & 3) * KB;
+  v4 cr = v4{0, 0, 0, 0}, ci = v4{0, 0, 0, 0};
+  const int nkq = KB >> 2;
+#pragma unroll 4
+  for (int q = 0; q < nkq; q++) {
+    const int jj = 4 * q + lk;
+    const cplx<T> bv = cmul(scol[jj], pcol[jj]);
+    const T av = ab[jj * 16 + li];
+    cr = Mfma<T>::mac(bv.x, av, cr);
+    ci = Mfma<T>::mac(bv.y, av, ci);
+  }
+  __syncthreads();  // every thread holds its sources: src is free
+  cplx<T>* ll = &src[0][0];  // [64 children][16 samples]
+#pragma unroll
+  for (int q = 0; q < 4; q++) ll[(16 * w + Mfma<T>::row(lk, q)) * kParKC + li] = mkc<T>(cr[q], ci[q]);
+  par_translate<T>(tr, kk, sv, acc);
+  __syncthreads();
+  const unsigned pm = morton2((unsigned)(pp & 3), (unsigned)(pp >> 2));
+  if (B.k0 + kk < S) {
+    cplx<T>* out = agl + rhs * st + (long)(B.m0 + 4 * pm) * S;
+#pragma unroll
+    for (int c = 0; c < 4; c++) out[(long)c * S + B.k0 + kk] = cadd(acc[c], ll[(4 * pm + c) * kParKC + kk]);
   }
 }
 
@@ -694,15 +832,22 @@ __global__ void __launch_bounds__(kBlock) k_p2m_mfma(const cplx<TI>* __restrict_
       p3[j] = v4{0, 0, 0, 0};
     }
     const int krow = 16 * rb + li;
-#pragma unroll 4
-    for (int kk = 0; kk < 16; kk++) {
-      const int n = 4 * kk + lk;
-      cplx<T> a = (krow < S) ? coeffT[n * S + krow] : mkc<T>(0, 0);
-      T asum = a.x + a.y;
+    // the coefficients of PRE k-steps are loaded together ahead of their MFMAs
+    constexpr int PRE = sizeof(T) == 4 ? 8 : 4;
+#pragma unroll 1
+    for (int k0 = 0; k0 < 16; k0 += PRE) {
+      cplx<T> a[PRE];
 #pragma unroll
-      for (int j = 0; j < NN; j++) {
-        cplx<T> bv = tile[(16 * j + li) * 65 + n];
-        Mfma<T>::cmac3(bv, a, asum, p1[j], p2[j], p3[j]);
+      for (int u = 0; u < PRE; u++) a[u] = (krow < S) ? coeffT[(4 * (k0 + u) + lk) * S + krow] : mkc<T>(0, 0);
+#pragma unroll
+      for (int u = 0; u < PRE; u++) {
+        const int n = 4 * (k0 + u) + lk;
+        const T asum = a[u].x + a[u].y;
+#pragma unroll
+        for (int j = 0; j < NN; j++) {
+          cplx<T> bv = tile[(16 * j + li) * 65 + n];
+          Mfma<T>::cmac3(bv, a[u], asum, p1[j], p2[j], p3[j]);
+        }
       }
     }
     cplx<T>* out = agm + rhs * as;

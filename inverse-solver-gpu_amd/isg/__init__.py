@@ -306,8 +306,8 @@ class Solver:
                   "isg_profile_read")
         return ms.value, n.value
 
-    # ---- autotuner (stage: 0 near, 1 p2m, 2 m2l, 3 m2m, 4 l2l)
-    TUNE_STAGES = {"near": 0, "p2m": 1, "m2l": 2, "m2m": 3, "l2l": 4}
+    # ---- autotuner (stage: 0 near, 1 p2m, 2 m2l, 3 m2m, 4 l2l, 5 leaf = fused leaf M2L + last L2L, 0 / 1)
+    TUNE_STAGES = {"near": 0, "p2m": 1, "m2l": 2, "m2m": 3, "l2l": 4, "leaf": 5}
     TUNING_FILE = os.path.normpath(os.path.join(_HERE, "..", "tuning", "mi355x.txt"))
 
     def autotune(self, nrhs=1, reps=3):

@@ -75,6 +75,31 @@ def test_forced_interp_kinds_match_oracle(isg, kind):
         check(s, o, 4)
 
 
+@pytest.mark.parametrize("level", [5, 6])
+def test_fused_leaf_matches_oracle(isg, level):
+    """Leaf M2L fused with the last L2L (tuning stage "leaf" = 1): the FP64
+    operator to 1e-12 of the oracle with every upper-level interpolation
+    kind, the FP32 operator within the mixed-precision bar."""
+    o = oracle_for(level)
+    s64 = isg.Solver(level=level, max_rhs=4)
+    s32 = isg.Solver(level=level, max_rhs=4, precision=1)
+    s64.tuning_set(0, 0, "leaf", 0, 1)
+    s32.tuning_set(1, 0, "leaf", 0, 1)
+    assert s64.tuning_get(0, 0, "leaf") == 1 and s32.tuning_get(1, 0, "leaf") == 1
+    X = np.stack([rand_c(np.random.default_rng(70 + r), o.N) for r in range(4)])
+    ref = [o.mlfma(X[r]) for r in range(4)]
+    for kind in (-1, 0, 1):
+        print("fused leaf, level", level, "upper l2l kind", kind, flush=True)
+        for i in range(0, level - 2):
+            s64.tuning_set(0, 0, "l2l", i, kind)
+            s32.tuning_set(1, 0, "l2l", i, kind)
+        Y = s64.mlfma_many(X)
+        Y32 = s32.mlfma_many(X)
+        for r in range(4):
+            assert relerr(Y[r], ref[r]) < 1e-12
+            assert 1e-12 < relerr(Y32[r], ref[r]) < 1e-5
+
+
 def test_autotune_save_load(isg, tmp_path):
     level = 6
     o = oracle_for(level)
@@ -93,7 +118,7 @@ def test_autotune_save_load(isg, tmp_path):
     t = isg.Solver(level=level, max_rhs=16)
     t.tuning_load(p)
     for prec in (0, 1):
-        for st in ("near", "p2m"):
+        for st in ("near", "p2m", "leaf"):
             assert t.tuning_get(prec, 1, st) == s.tuning_get(prec, 1, st)
         for st in ("m2l", "m2m", "l2l"):
             for i in range(level):
