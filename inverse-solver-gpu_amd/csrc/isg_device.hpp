@@ -93,4 +93,21 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+// A BiCGSTAB step done in the near-field kernel's epilogue instead of a
+// separate pass over the operator output (vectors of storage type V):
+//   mode 1: out = v = post(p, G y) and the partials of <r~, v>
+//   mode 2: out = t = post(s, G y) and the partials of <t, s>, <t, t>
+// post() is the contrast product of the solver (forward x - Gy, adjoint
+// x - conj(g Gy)); partials go to part[rhs][slot][workgroup].
+template <typename V>
+struct NearPost {
+  int mode = 0;
+  const cplx<V>* x = nullptr;   // p (mode 1) or s (mode 2)
+  const cplx<V>* rt = nullptr;  // r~ (mode 1)
+  const zd* g = nullptr;
+  long gs = 0;
+  int herm = 0;
+  double2* part = nullptr;
+};
+
 }  // namespace isg

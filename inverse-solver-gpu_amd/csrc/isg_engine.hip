@@ -8,6 +8,7 @@
 #include <cstring>
 #include <fstream>
 #include <sstream>
+#include <type_traits>
 
 #include <rocblas/rocblas.h>
 
@@ -230,7 +231,9 @@ Engine::Engine(const EngineConfig& cfg) : cfg_(cfg) {
   if (has32_)
     for (int k = 0; k < V_COUNT; k++) vec32_[k] = (cplx<float>*)dalloc((size_t)R * P_.N * sizeof(cplx<float>));
   nblk_ = (int)std::min<long>(1024, std::max<long>(1, (P_.N + kBlock * 4 - 1) / (kBlock * 4)));
-  part_ = (double2*)dalloc((size_t)R * kSlots * nblk_ * sizeof(double2));
+  // partial sums of the vector kernels (nblk_ workgroups) and of the
+  // near-field post steps (one per 16 leaf boxes at most)
+  part_ = (double2*)dalloc((size_t)R * kSlots * std::max<long>(nblk_, P_.B / 16) * sizeof(double2));
   state_ = (BiState*)dalloc((size_t)R * sizeof(BiState));
   active_ = (int*)dalloc((size_t)R * sizeof(int));
   tolv_ = (double*)dalloc((size_t)R * sizeof(double));
@@ -577,7 +580,8 @@ void Engine::launch_m2l_leaf(int nrhs, const int* act) {
 // Near field + L2P on the matrix cores (k_near_mfma), or L2P alone when
 // with_near is false (receiver back-projection).
 template <typename T, typename TI>
-void Engine::launch_near(const cplx<TI>* y, cplx<TI>* r, int nrhs, const int* act, bool with_near, int force) {
+int Engine::launch_near(const cplx<TI>* y, cplx<TI>* r, int nrhs, const int* act, bool with_near, int force,
+                        const NearPost<TI>* post) {
   const int Lf = P_.L - 1, S = P_.nsamp[Lf], B = P_.B;
   const long N = P_.N, ls = (long)lvl_size_[Lf];
   const int tuned = tune(sizeof(T) == 4 ? PREC_MIXED : PREC_FP64, nrhs).near_nb;
@@ -588,21 +592,44 @@ void Engine::launch_near(const cplx<TI>* y, cplx<TI>* r, int nrhs, const int* ac
   size_t lds = (size_t)near_lds_elems(nb, S) * sizeof(cplx<T>);
   dim3 grid(B / nb, nrhs), blk(kBlock);
   const int kt = sizeof(T) == 4 ? KT_NEAR32 : KT_NEAR;
+  const NearPost<TI> pp = post ? *post : NearPost<TI>();
+  const int wn = with_near ? 1 : 0, lv = 1 << Lf;
+  // one instantiation per post mode: the plain product keeps the register
+  // budget (and occupancy) of a kernel without the post epilogue
+  auto go = [&](auto nbc, auto modec) {
+    k_near_mfma<T, TI, decltype(nbc)::value, decltype(modec)::value>
+        <<<grid, blk, lds, stream_>>>(y, N, loc, ls, r, N, d.near, d.basisT, S, lv, wn, act, pp);
+  };
+  using I16 = std::integral_constant<int, 16>;
+  using I32 = std::integral_constant<int, 32>;
+  using M0 = std::integral_constant<int, 0>;
+  using M1 = std::integral_constant<int, 1>;
+  using M2 = std::integral_constant<int, 2>;
   if (with_near) prof_begin(kt);
-  if (nb == 32)
-    k_near_mfma<T, TI, 32><<<grid, blk, lds, stream_>>>(y, N, loc, ls, r, N, d.near, d.basisT, S, 1 << Lf,
-                                                        with_near ? 1 : 0, act);
-  else
-    k_near_mfma<T, TI, 16><<<grid, blk, lds, stream_>>>(y, N, loc, ls, r, N, d.near, d.basisT, S, 1 << Lf,
-                                                        with_near ? 1 : 0, act);
+  if (nb == 32) {
+    if (pp.mode == 1)
+      go(I32(), M1());
+    else if (pp.mode == 2)
+      go(I32(), M2());
+    else
+      go(I32(), M0());
+  } else {
+    if (pp.mode == 1)
+      go(I16(), M1());
+    else if (pp.mode == 2)
+      go(I16(), M2());
+    else
+      go(I16(), M0());
+  }
   if (with_near) prof_end(kt);
+  return B / nb;
 }
 
 // r = G y (y already contrast-weighted).  Stage order of matvec_gpu.cu:103-264:
 // P2M, M2M up to level 2, M2L on levels 2..leaf, L2L down, then near field
 // and L2P together.
 template <typename T, typename TI>
-void Engine::apply(const cplx<TI>* y, cplx<TI>* r, int nrhs, const int* act) {
+void Engine::apply(const cplx<TI>* y, cplx<TI>* r, int nrhs, const int* act, const NearPost<TI>* post) {
   const int L = P_.L, Lf = L - 1;
   launch_p2m<T, TI>(y, nrhs, act);
   for (int i = L - 2; i >= 2; i--) launch_m2m<T>(i, nrhs, act);
@@ -620,7 +647,7 @@ void Engine::apply(const cplx<TI>* y, cplx<TI>* r, int nrhs, const int* act) {
     launch_m2l_leaf<T>(nrhs, act);
     prof_end(kt);
   }
-  launch_near<T, TI>(y, r, nrhs, act, true);
+  near_blocks_ = launch_near<T, TI>(y, r, nrhs, act, true, -1, post);
 }
 
 void Engine::apply_mlfma(const zd* y, zd* r, int nrhs, const int* act, int prec) { apply_vec<double>(y, r, nrhs, act, prec); }
@@ -635,15 +662,15 @@ cplx<V>* Engine::vb(int k) {
 
 // FP32 vectors always go with the FP32 operator; FP64 vectors with either
 template <typename V>
-void Engine::apply_vec(const cplx<V>* y, cplx<V>* r, int nrhs, const int* act, int prec) {
+void Engine::apply_vec(const cplx<V>* y, cplx<V>* r, int nrhs, const int* act, int prec, const NearPost<V>* post) {
   if (!ok()) return;
   if constexpr (sizeof(V) == 4) {
-    apply<float, float>(y, r, nrhs, act);
+    apply<float, float>(y, r, nrhs, act, post);
   } else {
     if (prec == PREC_MIXED && has32_)
-      apply<float, double>(y, r, nrhs, act);
+      apply<float, double>(y, r, nrhs, act, post);
     else
-      apply<double, double>(y, r, nrhs, act);
+      apply<double, double>(y, r, nrhs, act, post);
   }
   ISG_HIP(hipGetLastError());
 }
@@ -826,18 +853,29 @@ template <typename V>
 void Engine::bicgs_body(zd* x, const zd* g, long gs, bool herm, int nrhs, int prec) {
   const long N = P_.N;
   cplx<V>*r = vb<V>(V_R), *rt = vb<V>(V_RT), *p = vb<V>(V_P), *v = vb<V>(V_V), *s = vb<V>(V_S), *t = vb<V>(V_T),
-     *y = vb<V>(V_Y), *op = vb<V>(V_OP);
+     *y = vb<V>(V_Y);
   const int h = herm ? 1 : 0;
   const int maxit = maxit_;
   dim3 grid(nblk_, nrhs), blk(kBlock);
+  // v = post(p, G y) and t = post(s, G y) with their dot products are the
+  // near-field kernel's epilogue (NearPost); its partials are per near workgroup
+  NearPost<V> post;
+  post.g = g;
+  post.gs = gs;
+  post.herm = h;
+  post.part = part_;
   k_bi_pupd<V><<<grid, blk, 0, stream_>>>(r, p, v, g, gs, y, N, h, state_);
-  apply_vec<V>(y, op, nrhs, active_, prec);
-  k_bi_post1<V><<<grid, blk, 0, stream_>>>(p, op, g, gs, v, rt, N, h, state_, part_);
-  k_bi_fin<<<dim3(nrhs), blk, 0, stream_>>>(PH_ALPHA, state_, active_, part_, nblk_, tolv_, maxit);
+  post.mode = 1;
+  post.x = p;
+  post.rt = rt;
+  apply_vec<V>(y, v, nrhs, active_, prec, &post);
+  k_bi_fin<<<dim3(nrhs), blk, 0, stream_>>>(PH_ALPHA, state_, active_, part_, near_blocks_, tolv_, maxit);
   k_bi_supd<V><<<grid, blk, 0, stream_>>>(r, v, g, gs, s, y, N, h, state_);
-  apply_vec<V>(y, op, nrhs, active_, prec);
-  k_bi_post2<V><<<grid, blk, 0, stream_>>>(s, op, g, gs, t, N, h, state_, part_);
-  k_bi_fin<<<dim3(nrhs), blk, 0, stream_>>>(PH_OMEGA, state_, active_, part_, nblk_, tolv_, maxit);
+  post.mode = 2;
+  post.x = s;
+  post.rt = nullptr;
+  apply_vec<V>(y, t, nrhs, active_, prec, &post);
+  k_bi_fin<<<dim3(nrhs), blk, 0, stream_>>>(PH_OMEGA, state_, active_, part_, near_blocks_, tolv_, maxit);
   prof_begin(KT_VEC);
   k_bi_xrupd<V><<<grid, blk, 0, stream_>>>(x, p, s, t, r, rt, N, state_, part_);
   prof_end(KT_VEC);

@@ -48,6 +48,7 @@ struct BiState {
   int iter, status, active, first;
 };
 
+
 // Device copies of the operator tables and the multipole/local stores in
 // precision T.
 template <typename T>
@@ -164,7 +165,7 @@ class Engine {
   void plan_interp(DevOp<T>& d, std::vector<InterpShape>& m2m, std::vector<InterpShape>& l2l);
   // operator with tables / multipoles of type T on vectors of type TI
   template <typename T, typename TI>
-  void apply(const cplx<TI>* y, cplx<TI>* r, int nrhs, const int* act);
+  void apply(const cplx<TI>* y, cplx<TI>* r, int nrhs, const int* act, const NearPost<TI>* post = nullptr);
   // `force` >= 0 overrides the tuned / built-in choice (autotuner)
   template <typename T, typename TI>
   void launch_p2m(const cplx<TI>* y, int nrhs, const int* act, int force = -1);
@@ -179,7 +180,9 @@ class Engine {
   template <typename T>
   bool leaf_fuse_ok() const;
   template <typename T, typename TI>
-  void launch_near(const cplx<TI>* y, cplx<TI>* r, int nrhs, const int* act, bool with_near, int force = -1);
+  // returns the number of workgroups per right-hand side (partial sums of a post step)
+  int launch_near(const cplx<TI>* y, cplx<TI>* r, int nrhs, const int* act, bool with_near, int force = -1,
+                  const NearPost<TI>* post = nullptr);
   template <typename T>
   void autotune_prec(int nrhs, int reps);
   template <typename T>
@@ -197,7 +200,8 @@ class Engine {
   template <typename V>
   cplx<V>* vb(int k);
   template <typename V>
-  void apply_vec(const cplx<V>* y, cplx<V>* r, int nrhs, const int* act, int prec);
+  void apply_vec(const cplx<V>* y, cplx<V>* r, int nrhs, const int* act, int prec,
+                 const NearPost<V>* post = nullptr);
   // r0_in_res: the initial residual is already in vb<V>(V_RES) (blended start)
   void solve(zd* x, const zd* g, long gs, const zd* b, bool herm, int nrhs, bool x_zero, double tol, int maxit,
              std::vector<SolveStat>& out, int prec, bool r0_in_res);
@@ -251,6 +255,7 @@ class Engine {
   int maxit_ = 0;
   bool prof_ = false;
   KTimer kt_[KT_COUNT];
+  int near_blocks_ = 0;  // workgroups per right-hand side of the last near-field launch
   Tuning tune_[2][2];  // [FP64, FP32 operator][batch class]
   std::vector<void*> allocs_;
 };

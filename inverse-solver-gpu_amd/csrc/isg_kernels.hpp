@@ -179,6 +179,56 @@ struct Mfma<float> {
   }
 };
 
+// ---- BiCGSTAB helpers shared by the vector kernels and the near-field epilogue
+constexpr int kSlots = 3;
+
+__device__ __forceinline__ zd prep_val(zd x, zd g, bool herm) { return herm ? cconj(x) : cmul(x, g); }
+__device__ __forceinline__ zd post_val(zd x, zd gy, zd g, bool herm) {
+  return herm ? csub(x, cconj(cmul(g, gy))) : csub(x, gy);
+}
+
+// Block-level sum of up to 3 complex partials; thread 0 writes part[slot].
+template <int NS>
+__device__ __forceinline__ void block_store(zd (&v)[NS], double2* part, int nblk, int blk) {
+  __shared__ double sh[NS * 2][kBlock / 64];
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int s = 0; s < NS; s++) {
+    double a = wave_sum(v[s].x), b = wave_sum(v[s].y);
+    if (lane == 0) {
+      sh[2 * s][w] = a;
+      sh[2 * s + 1][w] = b;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+      double a = 0, b = 0;
+      for (int i = 0; i < kBlock / 64; i++) {
+        a += sh[2 * s][i];
+        b += sh[2 * s + 1][i];
+      }
+      part[s * nblk + blk] = make_double2(a, b);
+    }
+  }
+}
+
+#define ISG_VEC_LOOP(N) for (long i = (long)blockIdx.x * kBlock + threadIdx.x; i < (N); i += (long)gridDim.x * kBlock)
+
+// Vector kernels are templated on the storage type V of the work vectors
+// (double, or float for the mixed-precision solver); arithmetic and the dot
+// product partials are FP64 either way, the solution x and the right-hand
+// side b are always FP64.
+template <typename V>
+__device__ __forceinline__ zd ldv(const cplx<V>* a, long i) { return ccast<double>(a[i]); }
+template <typename V>
+__device__ __forceinline__ zd stv(cplx<V>* a, long i, zd v) {
+  cplx<V> w = ccast<V>(v);
+  a[i] = w;
+  return ccast<double>(w);  // the value as stored
+}
+
 // LDS row length (complex elements) of the near/L2P tile: 64 pixels or S
 // locals, plus one element of padding so 16 lanes reading 16 different rows
 // hit 16 different banks.
@@ -190,13 +240,14 @@ __host__ __device__ inline int near_lds_elems(int NB, int S) {
   return halo > l2p ? halo : l2p;
 }
 
-template <typename T, typename TI, int NB>
+template <typename T, typename TI, int NB, int POST>
 __global__ void __launch_bounds__(kBlock) k_near_mfma(const cplx<TI>* __restrict__ y, long ys,
                                                       const cplx<T>* __restrict__ loc, long ls,
                                                       cplx<TI>* __restrict__ r, long rs,
                                                       const cplx<T>* __restrict__ near,
                                                       const cplx<T>* __restrict__ basisT, int S, int nbs,
-                                                      int with_near, const int* __restrict__ act) {
+                                                      int with_near, const int* __restrict__ act,
+                                                      NearPost<TI> post) {
   typedef typename Mfma<T>::v4 v4;
   constexpr int NN = NB / 16;
   // Morton-consecutive NB boxes form a TX x TY block; with its one-box halo
@@ -204,6 +255,7 @@ __global__ void __launch_bounds__(kBlock) k_near_mfma(const cplx<TI>* __restrict
   // once (each input pixel enters LDS once per tile instead of once per slot).
   constexpr int TX = (NB == 16) ? 4 : 8, TY = NB / TX, HX = TX + 2, HY = TY + 2;
   constexpr int HROW = 65;  // 64 pixels + 1 padding element
+  constexpr int PRE = sizeof(T) == 4 ? 8 : 4;  // L2P table values loaded ahead per MFMA group
   extern __shared__ char smem_raw[];
   cplx<T>* tile = reinterpret_cast<cplx<T>*>(smem_raw);
   const int row = S + 1;    // L2P rows: S locals + 1 padding element
@@ -268,27 +320,89 @@ __global__ void __launch_bounds__(kBlock) k_near_mfma(const cplx<TI>* __restrict
       });
   __syncthreads();
   const cplx<T>* A = basisT + 16 * w + li;
-  for (int kk = 0; kk < S / 4; kk++) {
-    const int k = 4 * kk + lk;
-    cplx<T> a = A[k * 64];
-    T asum = a.x + a.y;
+  const int nk = S / 4;  // S is a multiple of 4
+#pragma unroll 1
+  for (int k0 = 0; k0 < nk; k0 += PRE) {
+    cplx<T> a[PRE];
 #pragma unroll
-    for (int j = 0; j < NN; j++) {
-      cplx<T> bv = tile[(16 * j + li) * row + k];
-      Mfma<T>::cmac3(bv, a, asum, p1[j], p2[j], p3[j]);
+    for (int u = 0; u < PRE; u++)
+      if (k0 + u < nk) a[u] = A[(4 * (k0 + u) + lk) * 64];
+#pragma unroll
+    for (int u = 0; u < PRE; u++) {
+      if (k0 + u < nk) {
+        const int k = 4 * (k0 + u) + lk;
+        const T asum = a[u].x + a[u].y;
+#pragma unroll
+        for (int j = 0; j < NN; j++) {
+          cplx<T> bv = tile[(16 * j + li) * row + k];
+          Mfma<T>::cmac3(bv, a[u], asum, p1[j], p2[j], p3[j]);
+        }
+      }
     }
   }
   // D[box][m]: lane li holds pixel m = 16w + li, so each store writes 16
   // consecutive pixels of 4 boxes.
-  cplx<TI>* out = r + rhs * rs + 16 * w + li;
+  if constexpr (POST == 0) {
+    cplx<TI>* out = r + rhs * rs + 16 * w + li;
+#pragma unroll
+    for (int j = 0; j < NN; j++) {
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        long box = b0 + 16 * j + Mfma<T>::row(lk, q);
+        const T re = p1[j][q] - p2[j][q], im = p3[j][q] - p1[j][q] - p2[j][q];
+        out[box * 64] = mkc<TI>((TI)re, (TI)im);
+      }
+    }
+    return;
+  } else {
+  // BiCGSTAB post step on the operator output.  The output goes through LDS
+  // (the L2P tile is free once every wave has passed the barrier) so that
+  // each thread then takes consecutive elements of the workgroup's
+  // contiguous output range [b0*64, (b0+NB)*64): coalesced accesses and few
+  // live registers.  Values are rounded to the vector type TI first, as a
+  // separate pass reading the stored product back would see them.
+  __syncthreads();
 #pragma unroll
   for (int j = 0; j < NN; j++) {
 #pragma unroll
     for (int q = 0; q < 4; q++) {
-      long box = b0 + 16 * j + Mfma<T>::row(lk, q);
       const T re = p1[j][q] - p2[j][q], im = p3[j][q] - p1[j][q] - p2[j][q];
-      out[box * 64] = mkc<TI>((TI)re, (TI)im);
+      tile[(16 * j + Mfma<T>::row(lk, q)) * 64 + 16 * w + li] = mkc<T>(re, im);
     }
+  }
+  __syncthreads();
+  const long o = rhs * rs + b0 * 64;
+  const zd* gr = post.g + rhs * post.gs + b0 * 64;
+  zd acc[2] = {mkc<double>(0, 0), mkc<double>(0, 0)};
+  struct In {
+    cplx<TI> x, rt;
+    zd g;
+  };
+  stage<2, In>(
+      NB * 64,
+      [&](int e) {
+        In v;
+        v.x = post.x[o + e];
+        if (POST == 1) v.rt = post.rt[o + e];
+        v.g = gr[e];
+        return v;
+      },
+      [&](int e, In v) {
+        const cplx<T> gt = tile[e];
+        const zd gy = mkc<double>((double)(TI)gt.x, (double)(TI)gt.y);
+        const zd xv = ccast<double>(v.x);
+        const zd res = stv(r, o + e, post_val(xv, gy, v.g, post.herm != 0));
+        if (POST == 1) {
+          const zd a = ccast<double>(v.rt);
+          acc[0].x += a.x * res.x + a.y * res.y;
+          acc[0].y += a.x * res.y - a.y * res.x;
+        } else {
+          acc[0].x += res.x * xv.x + res.y * xv.y;
+          acc[0].y += res.x * xv.y - res.y * xv.x;
+          acc[1].x += res.x * res.x + res.y * res.y;
+        }
+      });
+    block_store<2>(acc, post.part + (long)rhs * kSlots * gridDim.x, gridDim.x, blockIdx.x);
   }
 }
 

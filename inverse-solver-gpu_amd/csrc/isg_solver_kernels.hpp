@@ -6,9 +6,9 @@
 // fused pass that also produces the dot products the next scalar needs:
 //   k_bi_init   r = b - A x, r~ = r, partial <b,b>, <r,r>
 //   k_bi_pupd   p = r + beta (p - omega v), y = prep(p)
-//   k_bi_post1  v = post(p, G y), partial <r~,v>
+//   (near)      v = post(p, G y), partial <r~,v>    -- epilogue of k_near_mfma
 //   k_bi_supd   s = r - alpha v, y = prep(s)
-//   k_bi_post2  t = post(s, G y), partial <t,s>, <t,t>
+//   (near)      t = post(s, G y), partial <t,s>, <t,t>    -- likewise
 //   k_bi_xrupd  x += alpha p; x += omega s; r = s - omega t; partial <r,r>, <r~,r>
 // Partial sums are FP64, one per block, reduced in a fixed order by
 // k_bi_fin, which also advances the per-right-hand-side scalars on the
@@ -19,59 +19,11 @@
 
 #include "isg_device.hpp"
 #include "isg_engine.hpp"
+#include "isg_kernels.hpp"
 
 namespace isg {
 
 enum { PH_INIT = 0, PH_ALPHA = 1, PH_OMEGA = 2, PH_RESID = 3 };
-constexpr int kSlots = 3;
-
-__device__ __forceinline__ zd prep_val(zd x, zd g, bool herm) { return herm ? cconj(x) : cmul(x, g); }
-__device__ __forceinline__ zd post_val(zd x, zd gy, zd g, bool herm) {
-  return herm ? csub(x, cconj(cmul(g, gy))) : csub(x, gy);
-}
-
-// Block-level sum of up to 3 complex partials; thread 0 writes part[slot].
-template <int NS>
-__device__ __forceinline__ void block_store(zd (&v)[NS], double2* part, int nblk, int blk) {
-  __shared__ double sh[NS * 2][kBlock / 64];
-  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-#pragma unroll
-  for (int s = 0; s < NS; s++) {
-    double a = wave_sum(v[s].x), b = wave_sum(v[s].y);
-    if (lane == 0) {
-      sh[2 * s][w] = a;
-      sh[2 * s + 1][w] = b;
-    }
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int s = 0; s < NS; s++) {
-      double a = 0, b = 0;
-      for (int i = 0; i < kBlock / 64; i++) {
-        a += sh[2 * s][i];
-        b += sh[2 * s + 1][i];
-      }
-      part[s * nblk + blk] = make_double2(a, b);
-    }
-  }
-}
-
-#define ISG_VEC_LOOP(N) for (long i = (long)blockIdx.x * kBlock + threadIdx.x; i < (N); i += (long)gridDim.x * kBlock)
-
-// Vector kernels are templated on the storage type V of the work vectors
-// (double, or float for the mixed-precision solver); arithmetic and the dot
-// product partials are FP64 either way, the solution x and the right-hand
-// side b are always FP64.
-template <typename V>
-__device__ __forceinline__ zd ldv(const cplx<V>* a, long i) { return ccast<double>(a[i]); }
-template <typename V>
-__device__ __forceinline__ zd stv(cplx<V>* a, long i, zd v) {
-  cplx<V> w = ccast<V>(v);
-  a[i] = w;
-  return ccast<double>(w);  // the value as stored
-}
-
 // r0 (may be null) supplies a residual computed beforehand (blended start).
 template <typename V>
 __global__ void __launch_bounds__(kBlock) k_bi_init(const zd* __restrict__ x, const cplx<V>* __restrict__ gy,
@@ -124,24 +76,6 @@ __global__ void __launch_bounds__(kBlock) k_bi_pupd(const cplx<V>* __restrict__ 
 }
 
 template <typename V>
-__global__ void __launch_bounds__(kBlock) k_bi_post1(const cplx<V>* __restrict__ p, const cplx<V>* __restrict__ gy,
-                                                     const zd* __restrict__ g, long gs, cplx<V>* __restrict__ v,
-                                                     const cplx<V>* __restrict__ rt, long N, int herm,
-                                                     const BiState* __restrict__ st, double2* __restrict__ part) {
-  int rhs = blockIdx.y;
-  if (!st[rhs].active) return;
-  long o = rhs * N;
-  zd acc[1] = {mkc<double>(0, 0)};
-  ISG_VEC_LOOP(N) {
-    zd vv = stv(v, o + i, post_val(ldv(p, o + i), ldv(gy, o + i), g[rhs * gs + i], herm));
-    zd a = ldv(rt, o + i);
-    acc[0].x += a.x * vv.x + a.y * vv.y;
-    acc[0].y += a.x * vv.y - a.y * vv.x;
-  }
-  block_store<1>(acc, part + (long)rhs * kSlots * gridDim.x, gridDim.x, blockIdx.x);
-}
-
-template <typename V>
 __global__ void __launch_bounds__(kBlock) k_bi_supd(const cplx<V>* __restrict__ r, const cplx<V>* __restrict__ v,
                                                     const zd* __restrict__ g, long gs, cplx<V>* __restrict__ s,
                                                     cplx<V>* __restrict__ y, long N, int herm,
@@ -155,25 +89,6 @@ __global__ void __launch_bounds__(kBlock) k_bi_supd(const cplx<V>* __restrict__ 
     zd sv = stv(s, o + i, cfma(mal, ldv(v, o + i), ldv(r, o + i)));
     stv(y, o + i, prep_val(sv, g[rhs * gs + i], herm));
   }
-}
-
-template <typename V>
-__global__ void __launch_bounds__(kBlock) k_bi_post2(const cplx<V>* __restrict__ s, const cplx<V>* __restrict__ gy,
-                                                     const zd* __restrict__ g, long gs, cplx<V>* __restrict__ t,
-                                                     long N, int herm, const BiState* __restrict__ st,
-                                                     double2* __restrict__ part) {
-  int rhs = blockIdx.y;
-  if (!st[rhs].active) return;
-  long o = rhs * N;
-  zd acc[2] = {mkc<double>(0, 0), mkc<double>(0, 0)};
-  ISG_VEC_LOOP(N) {
-    zd sv = ldv(s, o + i);
-    zd tv = stv(t, o + i, post_val(sv, ldv(gy, o + i), g[rhs * gs + i], herm));
-    acc[0].x += tv.x * sv.x + tv.y * sv.y;
-    acc[0].y += tv.x * sv.y - tv.y * sv.x;
-    acc[1].x += tv.x * tv.x + tv.y * tv.y;
-  }
-  block_store<2>(acc, part + (long)rhs * kSlots * gridDim.x, gridDim.x, blockIdx.x);
 }
 
 template <typename V>
