@@ -409,12 +409,15 @@ void Engine::launch_p2m(const cplx<TI>* y, int nrhs, const int* act, int force) 
   const int Lf = P_.L - 1, S = P_.nsamp[Lf], B = P_.B;
   const int tuned = tune(sizeof(T) == 4 ? PREC_MIXED : PREC_FP64, nrhs).p2m_nb;
   int nb = force > 0 ? force : (tuned > 0 ? tuned : (B >= 32 ? 32 : 16));
+  if (nb == 64 && sizeof(T) != 4) nb = 32;  // 64-box tiles: FP32 only (LDS)
   if (B % nb != 0) nb = 16;
   dim3 grid(B / nb, nrhs), blk(kBlock);
   DevOp<T>& d = ops<T>();
   cplx<T>* out = d.agm + lvl_off_[Lf];
   long as = (long)lvl_size_[Lf];
-  if (nb == 32)
+  if (nb == 64) {
+    if constexpr (sizeof(T) == 4) k_p2m_mfma<T, TI, 64><<<grid, blk, 0, stream_>>>(y, P_.N, out, as, d.coeffT, S, act);
+  } else if (nb == 32)
     k_p2m_mfma<T, TI, 32><<<grid, blk, 0, stream_>>>(y, P_.N, out, as, d.coeffT, S, act);
   else
     k_p2m_mfma<T, TI, 16><<<grid, blk, 0, stream_>>>(y, P_.N, out, as, d.coeffT, S, act);
@@ -586,6 +589,7 @@ int Engine::launch_near(const cplx<TI>* y, cplx<TI>* r, int nrhs, const int* act
   const long N = P_.N, ls = (long)lvl_size_[Lf];
   const int tuned = tune(sizeof(T) == 4 ? PREC_MIXED : PREC_FP64, nrhs).near_nb;
   int nb = force > 0 ? force : (tuned > 0 ? tuned : (B >= 32 ? 32 : 16));
+  if (nb == 64 && sizeof(T) != 4) nb = 32;  // 64-box tiles: FP32 only (LDS)
   if (B % nb != 0) nb = 16;
   DevOp<T>& d = ops<T>();
   const cplx<T>* loc = d.agl + lvl_off_[Lf];
@@ -602,11 +606,21 @@ int Engine::launch_near(const cplx<TI>* y, cplx<TI>* r, int nrhs, const int* act
   };
   using I16 = std::integral_constant<int, 16>;
   using I32 = std::integral_constant<int, 32>;
+  using I64 = std::integral_constant<int, 64>;
   using M0 = std::integral_constant<int, 0>;
   using M1 = std::integral_constant<int, 1>;
   using M2 = std::integral_constant<int, 2>;
   if (with_near) prof_begin(kt);
-  if (nb == 32) {
+  if (nb == 64) {
+    if constexpr (sizeof(T) == 4) {
+      if (pp.mode == 1)
+        go(I64(), M1());
+      else if (pp.mode == 2)
+        go(I64(), M2());
+      else
+        go(I64(), M0());
+    }
+  } else if (nb == 32) {
     if (pp.mode == 1)
       go(I32(), M1());
     else if (pp.mode == 2)
@@ -1052,11 +1066,11 @@ void Engine::autotune_prec(int nrhs, int reps) {
     }
     return best;
   };
-  const int widths[2] = {16, 32};
-  auto width_ok = [&](int c) { return P_.B % widths[c] == 0; };
-  int c = pick(2, width_ok, [&](int c) { launch_near<T, double>(y, op, nrhs, nullptr, true, widths[c]); });
+  const int widths[3] = {16, 32, 64};
+  auto width_ok = [&](int c) { return P_.B % widths[c] == 0 && (widths[c] < 64 || sizeof(T) == 4); };
+  int c = pick(3, width_ok, [&](int c) { launch_near<T, double>(y, op, nrhs, nullptr, true, widths[c]); });
   tu.near_nb = c < 0 ? -1 : widths[c];
-  c = pick(2, width_ok, [&](int c) { launch_p2m<T, double>(y, nrhs, nullptr, widths[c]); });
+  c = pick(3, width_ok, [&](int c) { launch_p2m<T, double>(y, nrhs, nullptr, widths[c]); });
   tu.p2m_nb = c < 0 ? -1 : widths[c];
   for (int i = 2; i <= Lf; i++)
     tu.m2l[i] = pick(
@@ -1105,7 +1119,7 @@ bool Engine::tuning_set(int prec, int cls, int stage, int level, int value) {
   if ((prec != PREC_FP64 && prec != PREC_MIXED) || cls < 0 || cls > 1 || stage < 0 || stage >= ST_COUNT) return false;
   Tuning& t = tune_[prec == PREC_MIXED ? 1 : 0][cls];
   if (stage == ST_NEAR || stage == ST_P2M) {
-    if (value != -1 && value != 16 && value != 32) return false;
+    if (value != -1 && value != 16 && value != 32 && value != 64) return false;
     (stage == ST_NEAR ? t.near_nb : t.p2m_nb) = value;
     return true;
   }

@@ -275,13 +275,20 @@ __global__ void __launch_bounds__(kBlock) k_near_mfma(const cplx<TI>* __restrict
   const cplx<TI>* yr = y + rhs * ys;
   if (with_near) {
     const int X0 = (int)compact2((unsigned)b0), Y0 = (int)compact2((unsigned)b0 >> 1);
+    // pixel offset of each halo box (Morton index * 64), -1 outside the grid:
+    // one Morton conversion per box instead of one per pixel
+    __shared__ int hoff[HX * HY];
+    for (int h = tid; h < HX * HY; h += kBlock) {
+      const int sx = X0 - 1 + h % HX, sy = Y0 - 1 + h / HX;
+      hoff[h] = (sx >= 0 && sy >= 0 && sx < nbs && sy < nbs) ? (int)morton2(sx, sy) * 64 : -1;
+    }
+    __syncthreads();
     stage<sizeof(T) == 4 ? 8 : 4, cplx<TI>>(
         HX * HY * 64,
         [&](int idx) {
-          int h = idx >> 6, n = idx & 63;
-          int sx = X0 - 1 + h % HX, sy = Y0 - 1 + h / HX;
+          const int off = hoff[idx >> 6];
           cplx<TI> v = mkc<TI>(0, 0);
-          if (sx >= 0 && sy >= 0 && sx < nbs && sy < nbs) v = yr[(long)morton2(sx, sy) * 64 + n];
+          if (off >= 0) v = yr[(long)off + (idx & 63)];
           return v;
         },
         [&](int idx, cplx<TI> v) { tile[(idx >> 6) * HROW + (idx & 63)] = ccast<T>(v); });
@@ -312,12 +319,34 @@ __global__ void __launch_bounds__(kBlock) k_near_mfma(const cplx<TI>* __restrict
   // leaf locals -> pixels (L2P) as the last K block
   __syncthreads();
   const cplx<T>* lr = loc + rhs * ls + b0 * S;
-  stage<sizeof(T) == 4 ? 8 : 4, cplx<T>>(
-      NB * S, [&](int idx) { return lr[idx]; },
-      [&](int idx, cplx<T> v) {
-        int j = idx / S;
-        tile[j * row + idx - j * S] = v;
-      });
+  {
+    // element idx = j * S + k goes to tile[j * (S + 1) + k] = tile[idx + j];
+    // j is stepped rather than divided out of idx
+    constexpr int U = sizeof(T) == 4 ? 8 : 4;
+    const int n = NB * S, dj = kBlock / S, dk = kBlock - dj * S;
+    int jb = tid / S, kq = tid - jb * S;
+    for (int e0 = tid; e0 < n; e0 += U * kBlock) {
+      cplx<T> v[U];
+      int ju[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int e = e0 + u * kBlock;
+        ju[u] = jb;
+        if (e < n) v[u] = lr[e];
+        kq += dk;
+        jb += dj;
+        if (kq >= S) {
+          kq -= S;
+          jb++;
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int e = e0 + u * kBlock;
+        if (e < n) tile[e + ju[u]] = v[u];
+      }
+    }
+  }
   __syncthreads();
   const cplx<T>* A = basisT + 16 * w + li;
   const int nk = S / 4;  // S is a multiple of 4
@@ -537,22 +566,43 @@ __global__ void __launch_bounds__(kBlock) k_interp_band(const cplx<T>* __restric
   const int li = lane & 15, lk = lane >> 4;
   const int row = K + 1;
   const long p0 = (long)xcd_block() * (NC / 4);
-  auto put = [&](int e, cplx<T> v) {
-    int col = e / K;
-    tile[col * row + e - col * K] = v;
-  };
-  if (MODE == 0) {
-    const cplx<T>* s = src + rhs * ss + 4 * p0 * K;  // NC consecutive children of length K
-    stage<8, cplx<T>>(NC * K, [&](int e) { return s[e]; }, put);
-  } else {
-    const cplx<T>* s = src + rhs * ss + p0 * K;  // NC/4 parents of length K
-    stage<8, CPair<T>>(
-        NC * K,
-        [&](int e) {
-          int col = e / K, j = e - col * K;
-          return CPair<T>{shift[(col & 3) * K + j], s[(long)(col >> 2) * K + j]};
-        },
-        [&](int e, CPair<T> v) { put(e, cmul(v.a, v.b)); });
+  // Element e = col * K + j of the NC x K block goes to tile[col * (K + 1) + j]
+  // = tile[e + col].  Each thread starts at e = tid and advances by kBlock;
+  // (col, j) are stepped rather than divided out of e, and the loads of U
+  // strides are issued before the first store.
+  {
+    constexpr int U = 8;
+    const int n = NC * K, dcol = kBlock / K, dj = kBlock - dcol * K;
+    int col = tid / K, j = tid - col * K;
+    const cplx<T>* s = src + rhs * ss + (MODE == 0 ? 4 * p0 : p0) * K;
+    for (int e0 = tid; e0 < n; e0 += U * kBlock) {
+      cplx<T> a[U], b[U];
+      int cu[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int e = e0 + u * kBlock;
+        cu[u] = col;
+        if (e < n) {
+          if (MODE == 0) {
+            a[u] = s[e];  // NC consecutive children of length K
+          } else {
+            a[u] = shift[(col & 3) * K + j];  // NC/4 parents of length K
+            b[u] = s[(long)(col >> 2) * K + j];
+          }
+        }
+        j += dj;
+        col += dcol;
+        if (j >= K) {
+          j -= K;
+          col++;
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int e = e0 + u * kBlock;
+        if (e < n) tile[e + cu[u]] = (MODE == 0) ? a[u] : cmul(a[u], b[u]);
+      }
+    }
   }
   __syncthreads();
   const int nrb = (Mrows + 15) / 16;
@@ -566,17 +616,26 @@ __global__ void __launch_bounds__(kBlock) k_interp_band(const cplx<T>* __restric
     for (int q = 0; q < KQ; q++) a[q] = (q < nkq) ? ab[(4 * q + lk) * 16 + li] : (T)0;
     const int s0 = s0v[rb];
     const cplx<T>* tcol = tile + (16 * cb + li) * row;
-    v4 cr0 = v4{0, 0, 0, 0}, ci0 = v4{0, 0, 0, 0};
+    // even and odd k-steps accumulate separately: four independent MFMA
+    // chains per wave instead of two
+    v4 cr0 = v4{0, 0, 0, 0}, ci0 = v4{0, 0, 0, 0}, cr1 = v4{0, 0, 0, 0}, ci1 = v4{0, 0, 0, 0};
 #pragma unroll
     for (int q = 0; q < KQ; q++) {
       if (q < nkq) {
         int j = s0 + 4 * q + lk;
         if (j >= K) j -= K;
         const cplx<T> bv = tcol[j];
-        cr0 = Mfma<T>::mac(bv.x, a[q], cr0);
-        ci0 = Mfma<T>::mac(bv.y, a[q], ci0);
+        if (q & 1) {
+          cr1 = Mfma<T>::mac(bv.x, a[q], cr1);
+          ci1 = Mfma<T>::mac(bv.y, a[q], ci1);
+        } else {
+          cr0 = Mfma<T>::mac(bv.x, a[q], cr0);
+          ci0 = Mfma<T>::mac(bv.y, a[q], ci0);
+        }
       }
     }
+    cr0 += cr1;
+    ci0 += ci1;
     // D[col][k]: lane li holds target sample k = 16 rb + li; col = 16 cb + row(lk, q)
     const int k = 16 * rb + li;
     if (MODE == 0) {
@@ -626,23 +685,29 @@ __device__ __forceinline__ void m2l_stage(cplx<T> (*src)[KC], cplx<T> (*tr)[KC],
                                           const cplx<T>* trans, int X0, int Y0, int k0, int nb, int S) {
   constexpr int SB = TS + 4, NS = SB * SB * KC;
   constexpr int US = NS / kBlock < 9 ? (NS + kBlock - 1) / kBlock : 9;
+  // offset of each halo box's multipole (Morton index * S), -1 outside the
+  // level: one Morton conversion per box instead of one per element
+  __shared__ int boff[SB * SB];
+  for (int b = threadIdx.x; b < SB * SB; b += kBlock) {
+    const int sx = X0 - 2 + b % SB, sy = Y0 - 2 + b / SB;
+    boff[b] = (sx >= 0 && sy >= 0 && sx < nb && sy < nb) ? (int)morton2(sx, sy) * S : -1;
+  }
+  __syncthreads();
+  // kBlock is a multiple of KC: each thread keeps one sample kk
+  const int kk = threadIdx.x % KC;
+  const bool kin = k0 + kk < S;
   stage<US, cplx<T>>(
       NS,
       [&](int e) {
-        int sb = e / KC, kk = e - sb * KC;
-        int sx = X0 - 2 + sb % SB, sy = Y0 - 2 + sb / SB;
+        const int off = boff[e / KC];
         cplx<T> v = mkc<T>(0, 0);
-        if (sx >= 0 && sy >= 0 && sx < nb && sy < nb && k0 + kk < S) v = a[(long)morton2(sx, sy) * S + k0 + kk];
+        if (off >= 0 && kin) v = a[(long)off + k0 + kk];
         return v;
       },
-      [&](int e, cplx<T> v) { src[e / KC][e % KC] = v; });
+      [&](int e, cplx<T> v) { src[e / KC][kk] = v; });
   stage<(49 * KC + kBlock - 1) / kBlock, cplx<T>>(
-      49 * KC,
-      [&](int e) {
-        int q = e / KC, kk = e - q * KC;
-        return (k0 + kk < S) ? trans[(long)q * S + k0 + kk] : mkc<T>(0, 0);
-      },
-      [&](int e, cplx<T> v) { tr[e / KC][e % KC] = v; });
+      49 * KC, [&](int e) { return kin ? trans[(long)(e / KC) * S + k0 + kk] : mkc<T>(0, 0); },
+      [&](int e, cplx<T> v) { tr[e / KC][kk] = v; });
 }
 
 // Tiled M2L.  A workgroup owns a TS x TS block of target boxes (Morton
@@ -894,16 +959,24 @@ __global__ void __launch_bounds__(kBlock) k_m2l_leaf(const cplx<T>* __restrict__
   const T* ab = AB + (long)B.chunk * KB * 16;
   const cplx<T>* pcol = pw + (4 * w + (li >> 2)) * KB;
   const cplx<T>* scol = sw + (li & 3) * KB;
-  v4 cr = v4{0, 0, 0, 0}, ci = v4{0, 0, 0, 0};
+  v4 cr = v4{0, 0, 0, 0}, ci = v4{0, 0, 0, 0}, cr1 = v4{0, 0, 0, 0}, ci1 = v4{0, 0, 0, 0};
   const int nkq = KB >> 2;
-#pragma unroll 4
-  for (int q = 0; q < nkq; q++) {
+#pragma unroll 2
+  for (int q = 0; q < nkq; q += 2) {  // even and odd k-steps in separate chains
     const int jj = 4 * q + lk;
     const cplx<T> bv = cmul(scol[jj], pcol[jj]);
     const T av = ab[jj * 16 + li];
     cr = Mfma<T>::mac(bv.x, av, cr);
     ci = Mfma<T>::mac(bv.y, av, ci);
+    if (q + 1 < nkq) {
+      const cplx<T> bw = cmul(scol[jj + 4], pcol[jj + 4]);
+      const T aw = ab[(jj + 4) * 16 + li];
+      cr1 = Mfma<T>::mac(bw.x, aw, cr1);
+      ci1 = Mfma<T>::mac(bw.y, aw, ci1);
+    }
   }
+  cr += cr1;
+  ci += ci1;
   __syncthreads();  // every thread holds its sources: src is free
   cplx<T>* ll = &src[0][0];  // [64 children][16 samples]
 #pragma unroll

@@ -100,6 +100,31 @@ def test_fused_leaf_matches_oracle(isg, level):
             assert 1e-12 < relerr(Y32[r], ref[r]) < 1e-5
 
 
+@pytest.mark.parametrize("herm", [False, True])
+def test_fp32_tile_width_64(isg, herm):
+    """64-box near-field and P2M tiles (FP32 operator only): the batched
+    product within the mixed-precision bar of the oracle, and a mixed
+    BiCGSTAB solve, whose near-field launches carry the fused post steps,
+    to the FP64 residual of the tolerance."""
+    level, tol = 5, 1e-6
+    o = oracle_for(level)
+    s = isg.Solver(level=level, max_rhs=4, toli=tol, iti=300, precision=1)
+    for st in ("near", "p2m"):
+        s.tuning_set(1, 0, st, 0, 64)
+        assert s.tuning_get(1, 0, st) == 64
+    X = np.stack([rand_c(np.random.default_rng(90 + r), o.N) for r in range(4)])
+    Y = s.mlfma_many(X)
+    for r in range(4):
+        assert 1e-12 < relerr(Y[r], o.mlfma(X[r])) < 1e-5
+    pos = o.pos()
+    g = 0.4 * np.exp(-np.abs(pos) ** 2 / (2 * 2.5 ** 2)) * (1 + 0.05j) * o.k0 ** 2
+    b = np.exp(1j * o.k0 * (0.6 * pos.real + 0.8 * pos.imag))
+    x, info = s.bicgs(g, b, herm=herm)
+    assert info["status"] == 0, info
+    ax = x - np.conj(g * o.mlfma(np.conj(x))) if herm else x - o.mlfma(g * x)
+    assert np.linalg.norm(b - ax) / np.linalg.norm(b) < tol * 1.05
+
+
 def test_autotune_save_load(isg, tmp_path):
     level = 6
     o = oracle_for(level)
