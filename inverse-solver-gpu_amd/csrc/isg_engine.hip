@@ -563,7 +563,8 @@ bool Engine::leaf_fuse_ok() const {
   if (Lf < 3) return false;
   const InterpShape& sh = (sizeof(T) == 4 ? l2l32_ : l2l64_)[Lf - 1];
   if (sh.ncb <= 0 || sh.kb <= 0 || sh.kb > P_.nsamp[Lf - 1]) return false;
-  const size_t lds = ((size_t)(kParSB * kParSB + 49) * kParKC + 20 * (size_t)sh.kb) * sizeof(cplx<T>);
+  const size_t dyn = std::max<size_t>(16 * (size_t)sh.kb, 64 * kParKC) + 4 * (size_t)sh.kb;
+  const size_t lds = ((size_t)(kParSB * kParSB + 49) * kParKC + dyn) * sizeof(cplx<T>);
   return lds <= kLdsLimit;
 }
 
@@ -574,7 +575,7 @@ void Engine::launch_m2l_leaf(int nrhs, const int* act) {
   DevOp<T>& d = ops<T>();
   const int nchunk = (S + kParKC - 1) / kParKC;
   dim3 grid(nb * nb / (kParTS * kParTS) * nchunk, nrhs), blk(kBlock);
-  const size_t lds = 20 * (size_t)sh.kb * sizeof(cplx<T>);
+  const size_t lds = (std::max<size_t>(16 * (size_t)sh.kb, 64 * kParKC) + 4 * (size_t)sh.kb) * sizeof(cplx<T>);
   k_m2l_leaf<T><<<grid, blk, lds, stream_>>>(d.agm + lvl_off_[Lf], d.agl + lvl_off_[Lf], (long)lvl_size_[Lf],
                                              d.trans[Lf], Lf, S, d.agl + lvl_off_[Lf - 1], (long)lvl_size_[Lf - 1],
                                              Sp, d.band_l2l[Lf - 1], d.bs0_l2l[Lf - 1], sh.kb, d.shiftL[Lf - 1], act);

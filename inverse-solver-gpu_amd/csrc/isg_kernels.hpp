@@ -913,7 +913,8 @@ __global__ void __launch_bounds__(kBlock) k_m2l_par(const cplx<T>* __restrict__ 
 // from s0v[chunk], mod Sp) and writes  local = translations + interpolation
 // once.  The separate L2L pass would read the parents again and read and
 // rewrite every leaf local after the M2L wrote it.  Dynamic LDS: the parent
-// window, 16 x KB, and the child shifts on it, 4 x KB.
+// window, max(16 KB, 1024) elements (later the 64 x 16 interpolated
+// locals), and the child shifts on it, 4 x KB.
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_m2l_leaf(const cplx<T>* __restrict__ agm, cplx<T>* __restrict__ agl,
                                                      long st, const cplx<T>* __restrict__ trans, int lev, int S,
@@ -921,11 +922,11 @@ __global__ void __launch_bounds__(kBlock) k_m2l_leaf(const cplx<T>* __restrict__
                                                      const T* __restrict__ AB, const int* __restrict__ s0v, int KB,
                                                      const cplx<T>* __restrict__ shift, const int* __restrict__ act) {
   typedef typename Mfma<T>::v4 v4;
-  __shared__ cplx<T> src[kParSB * kParSB][kParKC];  // after the MFMA phase: the 64 x 16 interpolated locals
+  __shared__ cplx<T> src[kParSB * kParSB][kParKC];
   __shared__ cplx<T> tr[49][kParKC];
   extern __shared__ char smem_raw[];
   cplx<T>* pw = reinterpret_cast<cplx<T>*>(smem_raw);  // [16 parents][KB]
-  cplx<T>* sw = pw + 16 * KB;                          // [4 children][KB]
+  cplx<T>* sw = pw + max(16 * KB, 64 * kParKC);         // [4 children][KB]
   const int rhs = blockIdx.y;
   if (rhs_off(act, rhs)) return;
   const ParBlock B(S);
@@ -950,9 +951,9 @@ __global__ void __launch_bounds__(kBlock) k_m2l_leaf(const cplx<T>* __restrict__
       [&](int e, cplx<T> v) { sw[e] = v; });
   __syncthreads();
   const int tid = threadIdx.x, kk = tid % kParKC, pp = tid / kParKC;
-  cplx<T> sv[6][6], acc[4];
-  par_load<T>(src, 2 * (pp & 3), 2 * (pp >> 2), kk, sv);
-  // interpolation: wave w owns columns 16w .. 16w+15 (children 16w + i, i.e.
+  // interpolation first (the 36 sources of the translation phase are loaded
+  // only after it, so the two phases do not hold registers at the same time):
+  // wave w owns columns 16w .. 16w+15 (children 16w + i, i.e.
   // parent 4w + i/4, child position i%4); lane (li, lk) feeds column li at
   // window sample 4q + lk and receives rows Mfma::row(lk, q) at sample li
   const int lane = tid & 63, w = tid >> 6, li = lane & 15, lk = lane >> 4;
@@ -977,12 +978,14 @@ __global__ void __launch_bounds__(kBlock) k_m2l_leaf(const cplx<T>* __restrict__
   }
   cr += cr1;
   ci += ci1;
-  __syncthreads();  // every thread holds its sources: src is free
-  cplx<T>* ll = &src[0][0];  // [64 children][16 samples]
+  __syncthreads();  // the parent window is no longer read: it takes the result
+  cplx<T>* ll = pw;  // [64 children][16 samples]
 #pragma unroll
   for (int q = 0; q < 4; q++) ll[(16 * w + Mfma<T>::row(lk, q)) * kParKC + li] = mkc<T>(cr[q], ci[q]);
-  par_translate<T>(tr, kk, sv, acc);
   __syncthreads();
+  cplx<T> sv[6][6], acc[4];
+  par_load<T>(src, 2 * (pp & 3), 2 * (pp >> 2), kk, sv);
+  par_translate<T>(tr, kk, sv, acc);
   const unsigned pm = morton2((unsigned)(pp & 3), (unsigned)(pp >> 2));
   if (B.k0 + kk < S) {
     cplx<T>* out = agl + rhs * st + (long)(B.m0 + 4 * pm) * S;
