@@ -303,7 +303,7 @@ __global__ void __launch_bounds__(kBlock) k_near_mfma(const cplx<TI>* __restrict
     for (int s = 0; s < 9; s++) {
       const int off = (s / 3 - 1) * HX + (s % 3 - 1);
       const cplx<T>* A = near + (long)s * 4096 + 16 * w + li;
-#pragma unroll 4
+#pragma unroll 8
       for (int kk = 0; kk < 16; kk++) {
         const int k = 4 * kk + lk;
         cplx<T> a = A[k * 64];
@@ -369,27 +369,11 @@ __global__ void __launch_bounds__(kBlock) k_near_mfma(const cplx<TI>* __restrict
       }
     }
   }
-  // D[box][m]: lane li holds pixel m = 16w + li, so each store writes 16
-  // consecutive pixels of 4 boxes.
-  if constexpr (POST == 0) {
-    cplx<TI>* out = r + rhs * rs + 16 * w + li;
-#pragma unroll
-    for (int j = 0; j < NN; j++) {
-#pragma unroll
-      for (int q = 0; q < 4; q++) {
-        long box = b0 + 16 * j + Mfma<T>::row(lk, q);
-        const T re = p1[j][q] - p2[j][q], im = p3[j][q] - p1[j][q] - p2[j][q];
-        out[box * 64] = mkc<TI>((TI)re, (TI)im);
-      }
-    }
-    return;
-  } else {
-  // BiCGSTAB post step on the operator output.  The output goes through LDS
-  // (the L2P tile is free once every wave has passed the barrier) so that
-  // each thread then takes consecutive elements of the workgroup's
-  // contiguous output range [b0*64, (b0+NB)*64): coalesced accesses and few
-  // live registers.  Values are rounded to the vector type TI first, as a
-  // separate pass reading the stored product back would see them.
+  // D[box][m]: lane li holds pixel m = 16w + li.  The product goes through
+  // LDS (the L2P tile is free once every wave has passed the barrier) so
+  // that each thread then takes consecutive elements of the workgroup's
+  // contiguous output range [b0*64, (b0+NB)*64): coalesced stores, and for a
+  // post step coalesced loads and few live registers.
   __syncthreads();
 #pragma unroll
   for (int j = 0; j < NN; j++) {
@@ -400,6 +384,13 @@ __global__ void __launch_bounds__(kBlock) k_near_mfma(const cplx<TI>* __restrict
     }
   }
   __syncthreads();
+  if constexpr (POST == 0) {
+    cplx<TI>* out = r + rhs * rs + b0 * 64;
+#pragma unroll 4
+    for (int e = tid; e < NB * 64; e += kBlock) out[e] = ccast<TI>(tile[e]);
+  } else {
+  // BiCGSTAB post step on the operator output, rounded to the vector type TI
+  // first, as a separate pass reading the stored product back would see it.
   const long o = rhs * rs + b0 * 64;
   const zd* gr = post.g + rhs * post.gs + b0 * 64;
   zd acc[2] = {mkc<double>(0, 0), mkc<double>(0, 0)};
