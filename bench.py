@@ -71,8 +71,10 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_fp32_operator_1024x64
 
 def measured_traffic():
     """HBM bytes (FETCH_SIZE + WRITE_SIZE, copy-calibrated) of one 64-RHS
-    launch of the FP32 near-field kernel at level 8, from the committed
-    rocprofv3 counter summary; None when that summary is absent."""
+    launch of the FP32 near-field kernel at level 8, averaged over the
+    launches of a timed fast-path iteration (most carry the fused BiCGSTAB
+    post step), from the committed rocprofv3 counter summary; None when that
+    summary is absent."""
     try:
         with open(TRAFFIC_FILE) as f:
             for line in f:
@@ -263,7 +265,8 @@ def main():
             "unit": "TFLOP/s",
             "frac": round(achieved / PEAK_TFLOPS[prec], 4),
             "traffic": traffic,
-            "traffic_launch": "64 RHS, level 8 (rocprofv3 FETCH_SIZE x2 + WRITE_SIZE x1, copy-calibrated)"
+            "traffic_launch": "64 RHS, level 8, mean over the near-field launches of a fast-path DBIM iteration "
+                              "(rocprofv3 FETCH_SIZE x2 + WRITE_SIZE x1, copy-calibrated)"
                               if traffic else None,
             "flops_per_rhs_launch": flops_mv,
             "note": "complex products in the three-product form: the matrix cores execute 6 of the 8 "

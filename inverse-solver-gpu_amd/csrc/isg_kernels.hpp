@@ -182,9 +182,11 @@ struct Mfma<float> {
 // ---- BiCGSTAB helpers shared by the vector kernels and the near-field epilogue
 constexpr int kSlots = 3;
 
-__device__ __forceinline__ zd prep_val(zd x, zd g, bool herm) { return herm ? cconj(x) : cmul(x, g); }
-__device__ __forceinline__ zd post_val(zd x, zd gy, zd g, bool herm) {
-  return herm ? csub(x, cconj(cmul(g, gy))) : csub(x, gy);
+// The contrast weight g[i] enters prep only in the forward product and post
+// only in the adjoint one; it is loaded only there (herm is uniform per launch).
+__device__ __forceinline__ zd prep_val(zd x, const zd* g, long i, bool herm) { return herm ? cconj(x) : cmul(x, g[i]); }
+__device__ __forceinline__ zd post_val(zd x, zd gy, const zd* g, long i, bool herm) {
+  return herm ? csub(x, cconj(cmul(g[i], gy))) : csub(x, gy);
 }
 
 // Block-level sum of up to 3 complex partials; thread 0 writes part[slot].
@@ -398,20 +400,21 @@ __global__ void __launch_bounds__(kBlock) k_near_mfma(const cplx<TI>* __restrict
     cplx<TI> x, rt;
     zd g;
   };
+  const bool herm = post.herm != 0;
   stage<2, In>(
       NB * 64,
       [&](int e) {
         In v;
         v.x = post.x[o + e];
         if (POST == 1) v.rt = post.rt[o + e];
-        v.g = gr[e];
+        if (herm) v.g = gr[e];  // the forward post step does not use g
         return v;
       },
       [&](int e, In v) {
         const cplx<T> gt = tile[e];
         const zd gy = mkc<double>((double)(TI)gt.x, (double)(TI)gt.y);
         const zd xv = ccast<double>(v.x);
-        const zd res = stv(r, o + e, post_val(xv, gy, v.g, post.herm != 0));
+        const zd res = stv(r, o + e, herm ? csub(xv, cconj(cmul(v.g, gy))) : csub(xv, gy));
         if (POST == 1) {
           const zd a = ccast<double>(v.rt);
           acc[0].x += a.x * res.x + a.y * res.y;

@@ -40,7 +40,7 @@ __global__ void __launch_bounds__(kBlock) k_bi_init(const zd* __restrict__ x, co
     if (r0) {
       rv = ldv(r0, o + i);
     } else {
-      zd ax = xzero ? mkc<double>(0, 0) : post_val(x[o + i], ldv(gy, o + i), g[rhs * gs + i], herm);
+      zd ax = xzero ? mkc<double>(0, 0) : post_val(x[o + i], ldv(gy, o + i), g, rhs * gs + i, herm);
       rv = csub(bv, ax);
     }
     rv = stv(r, o + i, rv);
@@ -71,7 +71,7 @@ __global__ void __launch_bounds__(kBlock) k_bi_pupd(const cplx<V>* __restrict__ 
       pv = cfma(beta, tmp, ldv(r, o + i));
     }
     pv = stv(p, o + i, pv);
-    stv(y, o + i, prep_val(pv, g[rhs * gs + i], herm));
+    stv(y, o + i, prep_val(pv, g, rhs * gs + i, herm));
   }
 }
 
@@ -87,7 +87,7 @@ __global__ void __launch_bounds__(kBlock) k_bi_supd(const cplx<V>* __restrict__ 
   zd mal = mkc<double>(-S.alpha.x, -S.alpha.y);
   ISG_VEC_LOOP(N) {
     zd sv = stv(s, o + i, cfma(mal, ldv(v, o + i), ldv(r, o + i)));
-    stv(y, o + i, prep_val(sv, g[rhs * gs + i], herm));
+    stv(y, o + i, prep_val(sv, g, rhs * gs + i, herm));
   }
 }
 
@@ -215,7 +215,7 @@ __global__ void __launch_bounds__(kBlock) k_bl_dots(const zd* __restrict__ d, co
   long o = rhs * N;
   zd acc[2] = {mkc<double>(0, 0), mkc<double>(0, 0)};
   ISG_VEC_LOOP(N) {
-    zd a = stv(ad, o + i, post_val(d[o + i], ldv(gy, o + i), g[rhs * gs + i], herm));
+    zd a = stv(ad, o + i, post_val(d[o + i], ldv(gy, o + i), g, rhs * gs + i, herm));
     zd rv = ldv(r, o + i);
     acc[0].x += a.x * rv.x + a.y * rv.y;
     acc[0].y += a.x * rv.y - a.y * rv.x;
@@ -275,7 +275,7 @@ __global__ void __launch_bounds__(kBlock) k_prep(const zd* __restrict__ x, const
                                                  cplx<V>* __restrict__ y, long N, int herm) {
   int rhs = blockIdx.y;
   long o = rhs * N;
-  ISG_VEC_LOOP(N) stv(y, o + i, prep_val(x[o + i], g[rhs * gs + i], herm));
+  ISG_VEC_LOOP(N) stv(y, o + i, prep_val(x[o + i], g, rhs * gs + i, herm));
 }
 
 }  // namespace isg
