@@ -36,6 +36,31 @@ def test_fp32_matvec_within_1e5(isg, level, nrhs):
         assert e > 1e-12  # really computed in FP32
 
 
+@pytest.mark.parametrize("level", [4, 5])
+def test_matvec_vs_direct_mom(isg, level):
+    """Both operators against the dense direct MoM product (the matrix the
+    MLFMA approximates, sampled by the oracle).  Measured on MI355X: FP64
+    MLFMA 3e-7 .. 5.4e-7, FP32 operator 1.4e-6 .. 2.5e-6, i.e. the FP32
+    rounding is about 4x the MLFMA's own approximation error (with three- or
+    four-product complex MACs alike).  Bars: FP64 < 1e-6, FP32 < 5e-6; every
+    mixed solve is certified against the FP64 operator.  Level 5 compares a
+    1024-row window."""
+    print("matvec vs direct MoM, level", level, flush=True)
+    o = oracle_for(level)
+    s64 = isg.Solver(level=level, numrx=16, max_rhs=2)
+    s32 = isg.Solver(level=level, numrx=16, max_rhs=2, precision=1)
+    X = rand_c(np.random.default_rng(700 + level), (2, o.N))
+    R64, R32 = s64.mlfma_many(X), s32.mlfma_many(X)
+    r0, nr = (0, o.N) if level == 4 else (2048, 1024)
+    for k in range(2):
+        d = o.direct(X[k], r0, nr)
+        e64 = relerr(R64[k][r0:r0 + nr], d)
+        e32 = relerr(R32[k][r0:r0 + nr], d)
+        print("  rhs", k, "fp64 error", e64, "fp32 error", e32, flush=True)
+        assert 1e-12 < e64 < 1e-6, e64
+        assert e32 < 5e-6, e32
+
+
 @pytest.mark.parametrize("herm", [False, True])
 def test_mixed_bicgs_reaches_fp64_residual(isg, herm):
     level, tol = 5, 1e-6
