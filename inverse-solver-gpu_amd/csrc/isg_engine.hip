@@ -60,12 +60,6 @@ __global__ void __launch_bounds__(kBlock) k_conj_copy(const zd* in, zd* out, lon
 }
 
 enum { V_R = 0, V_RT, V_P, V_V, V_S, V_T, V_Y, V_OP, V_RES, V_COR, V_COUNT };
-// smallest power of two >= v (v >= 1)
-int pow2_ceil(int v) {
-  int p = 1;
-  while (p < v) p <<= 1;
-  return p;
-}
 // largest power of two <= v (v >= 1)
 int pow2_floor(int v) {
   int p = 1;

@@ -956,7 +956,6 @@ __global__ void __launch_bounds__(kBlock) k_m2l_leaf(const cplx<T>* __restrict__
   const cplx<T>* scol = sw + (li & 3) * KB;
   v4 cr = v4{0, 0, 0, 0}, ci = v4{0, 0, 0, 0}, cr1 = v4{0, 0, 0, 0}, ci1 = v4{0, 0, 0, 0};
   const int nkq = KB >> 2;
-#pragma unroll 2
   for (int q = 0; q < nkq; q += 2) {  // even and odd k-steps in separate chains
     const int jj = 4 * q + lk;
     const cplx<T> bv = cmul(scol[jj], pcol[jj]);
