@@ -876,8 +876,10 @@ __device__ __forceinline__ void par_translate(const cplx<T> (*tr)[kParKC], int k
     }
 }
 
+// (at least 2 waves per SIMD: the FP64 instantiation otherwise takes 256+ registers
+// and runs one wave per SIMD; measured 2.00 -> 1.52 ms per FP64 launch set)
 template <typename T>
-__global__ void __launch_bounds__(kBlock) k_m2l_par(const cplx<T>* __restrict__ agm, cplx<T>* __restrict__ agl,
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) k_m2l_par(const cplx<T>* __restrict__ agm, cplx<T>* __restrict__ agl,
                                                     long st, const cplx<T>* __restrict__ trans, int lev, int S,
                                                     const int* __restrict__ act) {
   __shared__ cplx<T> src[kParSB * kParSB][kParKC];
