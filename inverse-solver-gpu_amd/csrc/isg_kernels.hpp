@@ -610,33 +610,39 @@ __global__ void __launch_bounds__(kBlock) k_interp_band(const cplx<T>* __restric
     for (int q = 0; q < KQ; q++) a[q] = (q < nkq) ? ab[(4 * q + lk) * 16 + li] : (T)0;
     const int s0 = s0v[rb];
     const cplx<T>* tcol = tile + (16 * cb + li) * row;
-    // even and odd k-steps accumulate separately: four independent MFMA
-    // chains per wave instead of two
-    v4 cr0 = v4{0, 0, 0, 0}, ci0 = v4{0, 0, 0, 0}, cr1 = v4{0, 0, 0, 0}, ci1 = v4{0, 0, 0, 0};
+    // the epilogue's operands (M2M: sibling shifts; L2L: the child locals it
+    // adds to) are loaded here, so their latency overlaps the MFMA chain
+    const int k = 16 * rb + li;
+    cplx<T> pre[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      pre[q] = mkc<T>(0, 0);
+      if (k < Mrows) {
+        if (MODE == 0) {
+          if (!Mfma<T>::kRowsAcrossLanes || q == 0)
+            pre[q] = shift[(Mfma<T>::kRowsAcrossLanes ? lk : q) * Mrows + k];
+        } else if (accumulate) {
+          pre[q] = dst[rhs * ds + (4 * p0 + 16 * cb + Mfma<T>::row(lk, q)) * (long)Mrows + k];
+        }
+      }
+    }
+    v4 cr0 = v4{0, 0, 0, 0}, ci0 = v4{0, 0, 0, 0};
 #pragma unroll
     for (int q = 0; q < KQ; q++) {
       if (q < nkq) {
         int j = s0 + 4 * q + lk;
         if (j >= K) j -= K;
         const cplx<T> bv = tcol[j];
-        if (q & 1) {
-          cr1 = Mfma<T>::mac(bv.x, a[q], cr1);
-          ci1 = Mfma<T>::mac(bv.y, a[q], ci1);
-        } else {
-          cr0 = Mfma<T>::mac(bv.x, a[q], cr0);
-          ci0 = Mfma<T>::mac(bv.y, a[q], ci0);
-        }
+        cr0 = Mfma<T>::mac(bv.x, a[q], cr0);
+        ci0 = Mfma<T>::mac(bv.y, a[q], ci0);
       }
     }
-    cr0 += cr1;
-    ci0 += ci1;
     // D[col][k]: lane li holds target sample k = 16 rb + li; col = 16 cb + row(lk, q)
-    const int k = 16 * rb + li;
     if (MODE == 0) {
       // parent = sum over the 4 sibling columns of shiftM[c][k] * D[col][k]
       if (Mfma<T>::kRowsAcrossLanes) {
         // col = 16cb + lk + 4q: sibling c = lk, parent 4cb + q
-        const cplx<T> s1 = (k < Mrows) ? shift[lk * Mrows + k] : mkc<T>(0, 0);
+        const cplx<T> s1 = pre[0];
 #pragma unroll
         for (int q = 0; q < 4; q++) {
           T vr = cr0[q], vi = ci0[q];
@@ -652,7 +658,7 @@ __global__ void __launch_bounds__(kBlock) k_interp_band(const cplx<T>* __restric
         T xr = 0, xi = 0;
 #pragma unroll
         for (int q = 0; q < 4; q++) {
-          const cplx<T> sq = (k < Mrows) ? shift[q * Mrows + k] : mkc<T>(0, 0);
+          const cplx<T> sq = pre[q];
           T vr = cr0[q], vi = ci0[q];
           xr += vr * sq.x - vi * sq.y;
           xi += vr * sq.y + vi * sq.x;
@@ -665,7 +671,7 @@ __global__ void __launch_bounds__(kBlock) k_interp_band(const cplx<T>* __restric
         const int col = 16 * cb + Mfma<T>::row(lk, q);
         cplx<T>* o = dst + rhs * ds + (4 * p0 + col) * (long)Mrows + k;
         cplx<T> v = mkc<T>(cr0[q], ci0[q]);
-        *o = accumulate ? cadd(*o, v) : v;
+        *o = accumulate ? cadd(pre[q], v) : v;
       }
     }
   }
