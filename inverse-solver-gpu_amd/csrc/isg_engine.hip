@@ -893,6 +893,7 @@ void Engine::bicgs_body(zd* x, const zd* g, long gs, bool herm, int nrhs, int pr
 }
 
 // res = b - A x with the FP64 operator; rn, bn: per-system norms of res and b.
+// res may be null (norms only); A x stays in vec_[V_OP] until the next product.
 void Engine::true_residual(const zd* x, const zd* g, long gs, const zd* b, bool herm, int nrhs, zd* res,
                            std::vector<double>& rn, std::vector<double>& bn) {
   const long N = P_.N;
@@ -902,7 +903,8 @@ void Engine::true_residual(const zd* x, const zd* g, long gs, const zd* b, bool 
   k_prep<double><<<grid, blk, 0, stream_>>>(x, g, gs, y, N, h);
   apply_mlfma(y, op, nrhs, nullptr, PREC_FP64);
   matvecs += nrhs;
-  k_bi_init<double><<<grid, blk, 0, stream_>>>(x, op, g, gs, b, nullptr, res, vec_[V_RT], N, h, 0, part_);
+  // res == nullptr: only the norms (the vector is formed later if a correction needs it)
+  k_bi_init<double><<<grid, blk, 0, stream_>>>(x, op, g, gs, b, nullptr, res, nullptr, N, h, 0, part_);
   ISG_HIP(hipMemcpyAsync(part_h_, part_, (size_t)nrhs * kSlots * nblk_ * sizeof(double2), hipMemcpyDeviceToHost,
                          stream_));
   ISG_HIP(hipStreamSynchronize(stream_));
@@ -992,7 +994,7 @@ void Engine::solve(zd* x, const zd* g, long gs, const zd* b, bool herm, int nrhs
     zd *res = vec_[V_RES], *cor = vec_[V_COR];
     std::vector<double> rn, bn;
     for (int round = 0; round <= kRefineRounds; round++) {
-      true_residual(x, g, gs, b, herm, nrhs, res, rn, bn);
+      true_residual(x, g, gs, b, herm, nrhs, nullptr, rn, bn);
       if (!ok()) return;
       bool any = false;
       for (int i = 0; i < nrhs; i++) {
@@ -1012,6 +1014,9 @@ void Engine::solve(zd* x, const zd* g, long gs, const zd* b, bool herm, int nrhs
       }
       if (!any || round == kRefineRounds) break;
       refinements++;
+      // the residual vector itself, from the FP64 product the check left in V_OP
+      k_bi_init<double><<<dim3(nblk_, nrhs), dim3(kBlock), 0, stream_>>>(x, vec_[V_OP], g, gs, b, nullptr, res,
+                                                                        nullptr, P_.N, herm ? 1 : 0, 0, part_);
       ISG_HIP(hipMemsetAsync(cor, 0, (size_t)nrhs * P_.N * sizeof(zd), stream_));
       run_bicgs<float>(cor, g, gs, res, herm, nrhs, true, maxit, PREC_MIXED, nullptr);
       if (!ok()) return;

@@ -43,8 +43,8 @@ __global__ void __launch_bounds__(kBlock) k_bi_init(const zd* __restrict__ x, co
       zd ax = xzero ? mkc<double>(0, 0) : post_val(x[o + i], ldv(gy, o + i), g, rhs * gs + i, herm);
       rv = csub(bv, ax);
     }
-    rv = stv(r, o + i, rv);
-    stv(rt, o + i, rv);
+    if (r) rv = stv(r, o + i, rv);  // r, rt null: the norms only (residual check)
+    if (rt) stv(rt, o + i, rv);
     acc[0].x += bv.x * bv.x + bv.y * bv.y;
     acc[1].x += rv.x * rv.x + rv.y * rv.y;
   }

@@ -62,6 +62,25 @@ def test_matvec_vs_direct_mom(isg, level):
 
 
 @pytest.mark.parametrize("herm", [False, True])
+def test_mixed_defect_correction(isg, herm):
+    """A tolerance below the FP32 operator's accuracy (1e-8 on a strong
+    scatterer): the FP32 solve's recursive residual meets its target but the
+    FP64 check does not, so at least one defect-correction round runs, and the
+    final FP64 residual (recomputed with the oracle) meets the tolerance."""
+    level, tol = 5, 1e-8
+    o = oracle_for(level)
+    s = isg.Solver(level=level, numrx=16, toli=tol, iti=300, precision=1)
+    pos = o.pos()
+    g = 0.4 * np.exp(-np.abs(pos) ** 2 / (2 * 2.5 ** 2)) * (1 + 0.05j) * o.k0 ** 2
+    b = np.exp(1j * o.k0 * (0.6 * pos.real + 0.8 * pos.imag))
+    x, info = s.bicgs(g, b, herm=herm)
+    assert info["status"] == 0, info
+    assert s.counters()["refinements"] >= 1
+    ax = x - np.conj(g * o.mlfma(np.conj(x))) if herm else x - o.mlfma(g * x)
+    assert np.linalg.norm(b - ax) / np.linalg.norm(b) < tol * 1.05
+
+
+@pytest.mark.parametrize("herm", [False, True])
 def test_mixed_bicgs_reaches_fp64_residual(isg, herm):
     level, tol = 5, 1e-6
     o = oracle_for(level)
