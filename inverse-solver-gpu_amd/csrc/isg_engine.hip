@@ -556,7 +556,8 @@ bool Engine::leaf_fuse_ok() const {
   const int Lf = P_.L - 1;
   if (Lf < 3) return false;
   const InterpShape& sh = (sizeof(T) == 4 ? l2l32_ : l2l64_)[Lf - 1];
-  if (sh.ncb <= 0 || sh.kb <= 0 || sh.kb > P_.nsamp[Lf - 1]) return false;
+  // kb <= 96: the kernel's register window
+  if (sh.ncb <= 0 || sh.kb <= 0 || sh.kb > P_.nsamp[Lf - 1] || sh.kb > 96) return false;
   const size_t dyn = std::max<size_t>(16 * (size_t)sh.kb, 64 * kParKC) + 4 * (size_t)sh.kb;
   const size_t lds = ((size_t)(kParSB * kParSB + 49) * kParKC + dyn) * sizeof(cplx<T>);
   return lds <= kLdsLimit;

@@ -935,22 +935,39 @@ __global__ void __launch_bounds__(kBlock) k_m2l_leaf(const cplx<T>* __restrict__
   m2l_stage<T, kParTS, kParKC>(src, tr, agm + rhs * st, trans, B.X0, B.Y0, B.k0, 1 << lev, S);
   const int s0 = s0v[B.chunk];
   const cplx<T>* pr = par + rhs * ps + (long)(B.m0 >> 2) * Sp;  // the tile's 16 parents, Morton order
-  stage<4, cplx<T>>(
-      16 * KB,
-      [&](int e) {
-        int p = e / KB, j = s0 + e - p * KB;
+  // window element e = p * KB + jj (parent p, or child position p for the
+  // shifts), parent sample (s0 + jj) mod Sp; (p, jj) stepped, not divided
+  {
+    const int n = 20 * KB, dp = kBlock / KB, dq = kBlock - dp * KB;
+    int p = threadIdx.x / KB, jj = threadIdx.x - p * KB;
+    for (int e0 = threadIdx.x; e0 < n; e0 += 4 * kBlock) {
+      cplx<T> v[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int e = e0 + u * kBlock;
+        int j = s0 + jj;
         if (j >= Sp) j -= Sp;
-        return pr[(long)p * Sp + j];
-      },
-      [&](int e, cplx<T> v) { pw[e] = v; });
-  stage<2, cplx<T>>(
-      4 * KB,
-      [&](int e) {
-        int c = e / KB, j = s0 + e - c * KB;
-        if (j >= Sp) j -= Sp;
-        return shift[c * Sp + j];
-      },
-      [&](int e, cplx<T> v) { sw[e] = v; });
+        if (e < 16 * KB)
+          v[u] = pr[(long)p * Sp + j];
+        else if (e < n)
+          v[u] = shift[(p - 16) * Sp + j];
+        jj += dq;
+        p += dp;
+        if (jj >= KB) {
+          jj -= KB;
+          p++;
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int e = e0 + u * kBlock;
+        if (e < 16 * KB)
+          pw[e] = v[u];
+        else if (e < n)
+          sw[e - 16 * KB] = v[u];
+      }
+    }
+  }
   __syncthreads();
   const int tid = threadIdx.x, kk = tid % kParKC, pp = tid / kParKC;
   // interpolation first (the 36 sources of the translation phase are loaded
@@ -962,23 +979,22 @@ __global__ void __launch_bounds__(kBlock) k_m2l_leaf(const cplx<T>* __restrict__
   const T* ab = AB + (long)B.chunk * KB * 16;
   const cplx<T>* pcol = pw + (4 * w + (li >> 2)) * KB;
   const cplx<T>* scol = sw + (li & 3) * KB;
-  v4 cr = v4{0, 0, 0, 0}, ci = v4{0, 0, 0, 0}, cr1 = v4{0, 0, 0, 0}, ci1 = v4{0, 0, 0, 0};
+  // the band rows of the window, loaded together ahead of the MFMA chain
+  constexpr int KQ = 24;  // windows of up to 96 samples (the L2L band limit)
   const int nkq = KB >> 2;
-  for (int q = 0; q < nkq; q += 2) {  // even and odd k-steps in separate chains
-    const int jj = 4 * q + lk;
-    const cplx<T> bv = cmul(scol[jj], pcol[jj]);
-    const T av = ab[jj * 16 + li];
-    cr = Mfma<T>::mac(bv.x, av, cr);
-    ci = Mfma<T>::mac(bv.y, av, ci);
-    if (q + 1 < nkq) {
-      const cplx<T> bw = cmul(scol[jj + 4], pcol[jj + 4]);
-      const T aw = ab[(jj + 4) * 16 + li];
-      cr1 = Mfma<T>::mac(bw.x, aw, cr1);
-      ci1 = Mfma<T>::mac(bw.y, aw, ci1);
+  T a[KQ];
+#pragma unroll
+  for (int q = 0; q < KQ; q++) a[q] = (q < nkq) ? ab[(4 * q + lk) * 16 + li] : (T)0;
+  v4 cr = v4{0, 0, 0, 0}, ci = v4{0, 0, 0, 0};
+#pragma unroll
+  for (int q = 0; q < KQ; q++) {
+    if (q < nkq) {
+      const int jj = 4 * q + lk;
+      const cplx<T> bv = cmul(scol[jj], pcol[jj]);
+      cr = Mfma<T>::mac(bv.x, a[q], cr);
+      ci = Mfma<T>::mac(bv.y, a[q], ci);
     }
   }
-  cr += cr1;
-  ci += ci1;
   __syncthreads();  // the parent window is no longer read: it takes the result
   cplx<T>* ll = pw;  // [64 children][16 samples]
 #pragma unroll

@@ -52,6 +52,24 @@ def test_linearity_symmetry_fp32(isg, s64):
         assert 1e-12 < e < 1e-5, e
 
 
+def test_fused_leaf_level8(isg, s64):
+    """The fused leaf M2L + L2L kernel at this size (FP32 operator, 64
+    right-hand sides): within 1e-5 of the FP64 product, like the separate
+    kernels, and within FP32 rounding of them."""
+    print("level 8 fused leaf kernel", flush=True)
+    X = rand_c(np.random.default_rng(83), (2, s64.N))
+    ref = s64.mlfma_many(X)
+    s = isg.Solver(level=LEVEL, numrx=16, max_rhs=64, precision=1)
+    Y = {}
+    for leaf in (0, 1):
+        s.tuning_set(1, 1, "leaf", 0, leaf)
+        Y[leaf] = s.mlfma_many(np.concatenate([X] * 8))[:2]  # 16 RHS: batch class 1
+    s.close()
+    for k in range(2):
+        assert relerr(Y[1][k], ref[k]) < 1e-5
+        assert relerr(Y[1][k], Y[0][k]) < 2e-6
+
+
 def test_mixed_solve_64_rhs(isg, s64):
     print("level 8 mixed solve, 64 rhs", flush=True)
     tol = 1e-6
