@@ -48,9 +48,12 @@ START_KINDS = {"zero": 0, "previous": 1, "first-order": 2, "learned": 3}
 WARMNET_FILE = os.path.join(ROOT, "inverse-solver-gpu_amd", "weights", "warmnet_512.bin")
 
 WORKLOADS = {
-    # name: level, transmitters, receivers, ellipses, seed
-    "dbim-1024": dict(level=8, numtx=64, numrx=256, nell=20, seed=3),
-    "dbim-256": dict(level=6, numtx=16, numrx=64, nell=8, seed=3),
+    # name: level, transmitters, receivers, ellipses, seed, default circle radius (wavelengths)
+    "dbim-1024": dict(level=8, numtx=64, numrx=256, nell=20, seed=3, radius=60.0),
+    "dbim-256": dict(level=6, numtx=16, numrx=64, nell=8, seed=3, radius=60.0),
+    # 2048^2 (204.8 wavelengths a side): the circle passes outside the corners; transmitters in
+    # two batches of 64 (the 128-RHS work vectors would take most of the 288 GB)
+    "dbim-2048": dict(level=9, numtx=128, numrx=512, nell=20, seed=5, radius=160.0, batch=64),
 }
 
 
@@ -159,8 +162,9 @@ def main():
     ap.add_argument("--kernel-timing", default="on", choices=["on", "off"],
                     help="live HIP-event kernel timing in the timed region (off: BiCGSTAB iterations "
                          "replay as HIP graphs, no roofline figures)")
-    ap.add_argument("--radius", type=float, default=60.0,
-                    help="transmitter/receiver circle radius in wavelengths (0: plane waves, far-field receivers)")
+    ap.add_argument("--radius", type=float, default=None,
+                    help="transmitter/receiver circle radius in wavelengths (0: plane waves, far-field "
+                         "receivers; default: the workload's, 60 at 1024^2)")
     ap.add_argument("--cpu-sample", type=int, default=2, help="oracle matvecs timed for cpu_baseline (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -170,6 +174,8 @@ def main():
 
     import isg
     wl = WORKLOADS[args.workload]
+    if args.radius is None:
+        args.radius = wl["radius"]
     level, ntx, nrx = wl["level"], wl["numtx"], wl["numrx"]
     side = 8 << (level - 1)
     tuning = {"committed": True, "none": False}.get(args.tuning, args.tuning)
@@ -180,7 +186,8 @@ def main():
 
     # synthetic measurements of the true contrast, made once by the batched FP64 path
     t0 = time.perf_counter()
-    gen = isg.Solver(max_rhs=ntx, **common)
+    batch = wl.get("batch", ntx)
+    gen = isg.Solver(max_rhs=batch, **common)
     pos, r2t = gen.geometry()
     otrue = isg.raster_to_tree(isg.phantom_ellipses(side, wl["nell"], wl["seed"]), r2t)
     d_gen = isg.Dbim(gen, otrue, field_start=1)
@@ -196,11 +203,12 @@ def main():
 
     res = {}
     if "fast" in paths:
-        fast = isg.Solver(max_rhs=ntx, precision=fast_prec, mixed_margin=args.mixed_margin, **common)
+        fast = isg.Solver(max_rhs=batch, precision=fast_prec, mixed_margin=args.mixed_margin, **common)
         d_fast = isg.Dbim(fast, otrue, field_start=START_KINDS[args.fast_start], measurements=mes)
         if args.fast_start == "learned":
             d_fast.set_warmnet(isg.WarmNet(max_batch=8, max_pixels=side * side, path=WARMNET_FILE))
         res["fast"] = run_path(fast, d_fast, args.warmup, args.steps, "fast", args.kernel_timing == "on")
+        contrast = {"fast": d_fast.get("o")}
         fast_leaf_fused = fast.tuning_get(fast_prec, 1, "leaf") == 1
         del d_fast
         fast.close()
@@ -208,6 +216,7 @@ def main():
         plain = isg.Solver(max_rhs=1, **common)
         d_plain = isg.Dbim(plain, otrue, field_start=0, measurements=mes)
         res["plain"] = run_path(plain, d_plain, args.warmup, args.steps, "plain", args.kernel_timing == "on")
+        contrast_plain = d_plain.get("o")
         del d_plain
         plain.close()
 
@@ -252,7 +261,7 @@ def main():
                            ("on a circle of radius %g lambda (line sources)" % args.radius) if args.radius > 0
                            else "(plane waves, far-field receivers)", wl["nell"], wl["seed"]),
             "path": ("batched %d RHS + deflation, %s, %s total-field start"
-                     % (ntx, "FP32 operator / FP64 solver + FP64 defect correction" if mixed else "FP64",
+                     % (batch, "FP32 operator / FP64 solver + FP64 defect correction" if mixed else "FP64",
                         args.fast_start)
                      if main_path == "fast" else "1 RHS, FP64, zero start"),
             "kernel_tuning": args.tuning,
@@ -293,6 +302,9 @@ def main():
         out["plain_s_per_iter"] = round(res["plain"][0], 4)
         out["speedup_vs_plain"] = round(res["plain"][0] / s_iter, 3)
         out["plain_misfit"] = res["plain"][1][-1]["misfit"]
+        # reconstruction after the same iterations: fast path against the FP64 plain path
+        out["contrast_rel_diff_vs_plain"] = float(np.linalg.norm(contrast["fast"] - contrast_plain)
+                                                  / np.linalg.norm(contrast_plain))
     if args.cpu_sample > 0 and "plain" in res:
         plain_mv = int(round(sum(s["matvecs"] for s in res["plain"][1]) / len(res["plain"][1])))
         try:
