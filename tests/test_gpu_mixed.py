@@ -36,22 +36,22 @@ def test_fp32_matvec_within_1e5(isg, level, nrhs):
         assert e > 1e-12  # really computed in FP32
 
 
-@pytest.mark.parametrize("level", [4, 5])
-def test_matvec_vs_direct_mom(isg, level):
+@pytest.mark.parametrize("level,r0,nr", [(4, 0, 4096), (5, 2048, 1024), (6, 4096, 4096)])
+def test_matvec_vs_direct_mom(isg, level, r0, nr):
     """Both operators against the dense direct MoM product (the matrix the
     MLFMA approximates, sampled by the oracle).  Measured on MI355X: FP64
     MLFMA 3e-7 .. 5.4e-7, FP32 operator 1.4e-6 .. 2.5e-6, i.e. the FP32
     rounding is about 4x the MLFMA's own approximation error (with three- or
     four-product complex MACs alike).  Bars: FP64 < 1e-6, FP32 < 5e-6; every
     mixed solve is certified against the FP64 operator.  Level 5 compares a
-    1024-row window."""
+    1024-row window; level 6 (256^2) a 64x64-pixel sub-window (8x8 leaf boxes,
+    Morton-contiguous rows 4096..8191)."""
     print("matvec vs direct MoM, level", level, flush=True)
     o = oracle_for(level)
     s64 = isg.Solver(level=level, numrx=16, max_rhs=2)
     s32 = isg.Solver(level=level, numrx=16, max_rhs=2, precision=1)
     X = rand_c(np.random.default_rng(700 + level), (2, o.N))
     R64, R32 = s64.mlfma_many(X), s32.mlfma_many(X)
-    r0, nr = (0, o.N) if level == 4 else (2048, 1024)
     for k in range(2):
         d = o.direct(X[k], r0, nr)
         e64 = relerr(R64[k][r0:r0 + nr], d)
