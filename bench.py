@@ -243,7 +243,7 @@ def main():
     traffic = measured_traffic() if (mixed and level == 8 and ntx == 64) else None
 
     out = {
-        "metric": METRIC,
+        "metric": METRIC if args.workload == "dbim-1024" else METRIC.replace("1024² px ×64 Tx", "%d² px ×%d Tx" % (side, ntx)),
         "value": round(s_iter, 4),
         "unit": "s",
         "n_gpus": 1,
