@@ -81,6 +81,11 @@ typedef struct isg_config {
                       incident (i/4) H0(k0 |r - rho_t|), isg_aggregate = the scattered field
                       sum_n (i/4) H0(k0 |rho_m - r_n|) res^2 x_n (dense, on the device),
                       isg_aggregateh its exact adjoint */
+  int products;    /* operator of the DBIM's derivative source G (k0^2 ot tot) (mom.cpp:284-290):
+                      0 (default) FP64, the plain path's matrix-vector error; 1 the FP32 operator
+                      (precision 1 only).  The gradient's G^H c (mom.cpp:247-271) is FP64 either
+                      way: with precision 1 it is the product that the adjoint solve's final FP64
+                      residual check has already computed */
 } isg_config;
 
 /* Per-solve result. */

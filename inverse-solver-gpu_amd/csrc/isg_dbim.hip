@@ -259,12 +259,16 @@ void Dbim::gradient(DbimStats& st) {
     DB_HIP(hipMemsetAsync(wx_, 0, sizeof(zd) * N * nb, s));
     solve(wx_, wc_, true, nb, true, st);
     // c = conj(G conj(x))  (mlfmah, solve.cpp:8-24); the conjugate of Gc is taken in k_grad_acc.
-    // In mixed precision this product uses the FP32 operator (relative error 1.4e-6 - 2.4e-6
-    // against the FP64 one, the order of the solves' tolerance); only the solves' residual
-    // checks need the FP64 operator.
-    k_conj<<<grid, blk, 0, s>>>(wx_, wc_, N);
-    eng_.mlfma(wc_, wx_, nb, nullptr, opt_.precision);
-    k_grad_acc<<<dim3(nblk_), blk, 0, s>>>(buff_, tot_ + (size_t)t0 * N, wy_, wx_, o_, N, nb,
+    // A mixed-precision solve ends with an FP64 residual check that computed G conj(x) of the
+    // final x: that is this product, with the FP64 operator, at no cost.  After an FP64 solve
+    // it is computed here.
+    const zd* gcx = eng_.checked_product(wx_, nb, true);
+    if (!gcx) {
+      k_conj<<<grid, blk, 0, s>>>(wx_, wc_, N);
+      eng_.mlfma(wc_, wx_, nb, nullptr, opt_.products);
+      gcx = wx_;
+    }
+    k_grad_acc<<<dim3(nblk_), blk, 0, s>>>(buff_, tot_ + (size_t)t0 * N, wy_, gcx, o_, N, nb,
                                            opt_.regular * opt_.regular);
   }
   DB_HIP(hipMemcpyAsync(od_, buff_, sizeof(zd) * N, hipMemcpyDeviceToDevice, s));
@@ -367,7 +371,10 @@ void Dbim::iterate(DbimStats& st) {
     int nb = std::min(B_, T_ - t0);
     dim3 grid(nblk_, nb);
     k_deriv_src<<<grid, blk, 0, s>>>(tot_ + (size_t)t0 * N, ot_, wy_, N, k0_);
-    eng_.mlfma(wy_, wc_, nb, nullptr, opt_.precision);  // FP32 operator in mixed precision, as above
+    // opt_.products picks the operator: the FP32 one has a relative error of 1.3e-6 - 2.4e-6
+    // against the FP64 one (the order of the solves' tolerance), most of it from the FP32 sums
+    // of P2M and M2L, amplified by the leaf-level translations
+    eng_.mlfma(wy_, wc_, nb, nullptr, opt_.products);
     zd* dx = dtot_ ? dtot_ + (size_t)t0 * N : wx_;  // kept for the first-order field start
     DB_HIP(hipMemsetAsync(dx, 0, sizeof(zd) * N * nb, s));
     solve(dx, wc_, false, nb, true, st);

@@ -27,6 +27,7 @@ struct DbimOptions {
   // from tot_prev + lambda d with the residual-minimising lambda
   int field_start = 0;
   int precision = 0;      // solver operator: PREC_FP64 or PREC_MIXED
+  int products = 0;       // operator of the explicit products G^H c and G y: PREC_FP64 or PREC_MIXED
 };
 
 struct DbimStats {

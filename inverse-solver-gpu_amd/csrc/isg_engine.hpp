@@ -99,6 +99,12 @@ class Engine {
   void bicgs_blend(zd* x, const zd* d, const zd* g, long g_stride, const zd* b, bool herm, int nrhs, double tol,
                    int maxit, std::vector<SolveStat>& out, int prec, std::vector<zd>* lam);
 
+  // The FP64 product G prep(x) (prep(x) = g x forward, conj(x) adjoint) that
+  // the last true-residual check of a mixed-precision solve computed for its
+  // final iterate x, nrhs rows; null unless that check is the latest work on
+  // the operator output and was made for this x, nrhs and direction.
+  const zd* checked_product(const zd* x, int nrhs, bool herm) const;
+
   // multipoles (which=0) or locals (1) of `level` for one rhs, FP64 store
   void stage(int which, int level, int rhs, std::vector<zd>& out);
 
@@ -225,6 +231,10 @@ class Engine {
   std::string err_;
   hipStream_t stream_ = nullptr;
   bool has32_ = false;
+  // what the operator output buffer holds after a mixed solve's last check
+  const zd* checked_x_ = nullptr;
+  int checked_n_ = 0;
+  bool checked_h_ = false;
   DevOp<double> op64_;
   DevOp<float> op32_;
   std::vector<InterpShape> m2m64_, l2l64_, m2m32_, l2l32_;  // per parent level

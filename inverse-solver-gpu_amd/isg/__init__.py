@@ -42,6 +42,7 @@ class Config(ctypes.Structure):
         ("mixed_margin", ctypes.c_double),
         ("graph", ctypes.c_int),
         ("radius", ctypes.c_double),
+        ("products", ctypes.c_int),
     ]
 
 

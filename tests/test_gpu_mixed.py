@@ -102,18 +102,34 @@ def test_mixed_bicgs_reaches_fp64_residual(isg, herm):
     assert relerr(x, x_ref) < 10 * tol * 10
 
 
-def test_mixed_batched_dbim_matches_oracle(isg):
+@pytest.mark.parametrize("products", [1, 0])
+def test_mixed_batched_dbim_matches_oracle(isg, products):
+    """products=1: the derivative source G y on the FP32 operator; 0: on the
+    FP64 one.  The gradient's G^H c is never computed separately: it is the
+    FP64 product of the adjoint solve's final residual check, so the FP64
+    products of an iteration are exactly the checks, plus numtx when
+    products == 0."""
     from oracle.oracle import Dbim as OrcDbim
+    print("mixed dbim parity, products", products, flush=True)
     level, ntx, nrx, tol, reg = 4, 16, 16, 1e-6, float(np.sqrt(1e-3))
     o = oracle_for(level, numrx=nrx)
     otrue = isg.raster_to_tree(isg.phantom_blobs(o.side, 3, 1), o.unkmap())
     ref = OrcDbim(o, ntx, otrue, regular=reg, tol=tol, maxit=300)
-    s = isg.Solver(level=level, numrx=nrx, numtx=ntx, toli=tol, iti=300, regular=reg, max_rhs=ntx, precision=1)
+    s = isg.Solver(level=level, numrx=nrx, numtx=ntx, toli=tol, iti=300, regular=reg, max_rhs=ntx, precision=1,
+                   products=products)
     d = isg.Dbim(s, otrue, field_start=1)
     for _ in range(3):
         ref.iterate()
+        c0 = s.counters()
         st = d.iterate()
+        c1 = s.counters()
         assert st["not_converged"] == 0
+        # FP64 products: the true-residual check of every solved system, once
+        # more per defect-correction round (each covers the batch of ntx), plus
+        # the derivative source when products == 0
+        fp64 = (c1["matvecs"] - c1["matvecs_fp32"]) - (c0["matvecs"] - c0["matvecs_fp32"])
+        checks = (c1["solves"] - c0["solves"]) + ntx * (c1["refinements"] - c0["refinements"])
+        assert fp64 == checks + (ntx if products == 0 else 0), (fp64, checks, c0, c1)
     assert relerr(d.get("o"), ref.get("o")) < 1e-4
 
 
