@@ -158,6 +158,10 @@ def main():
     ap.add_argument("--fast-products", default="fp64", choices=["fp64", "fp32"],
                     help="operator of the DBIM's derivative-source product on the fast path (fp32 needs "
                          "--fast-precision mixed); the gradient's product is FP64 either way")
+    ap.add_argument("--fast-solve-start", default="zero", choices=["zero", "previous"],
+                    help="adjoint and derivative-field solves of the fast path: zero (reference) or the "
+                         "previous DBIM iteration's solution scaled by the residual-minimising lambda "
+                         "(pays only where the misfit stalls)")
     ap.add_argument("--mixed-margin", type=float, default=0.5,
                     help="FP32 solve target as a fraction of the tolerance (mixed precision)")
     ap.add_argument("--tuning", default="committed",
@@ -207,7 +211,8 @@ def main():
     res = {}
     if "fast" in paths:
         fast = isg.Solver(max_rhs=batch, precision=fast_prec, mixed_margin=args.mixed_margin,
-                          products=1 if args.fast_products == "fp32" and fast_prec == 1 else 0, **common)
+                          products=1 if args.fast_products == "fp32" and fast_prec == 1 else 0,
+                          solve_start=1 if args.fast_solve_start == "previous" else 0, **common)
         d_fast = isg.Dbim(fast, otrue, field_start=START_KINDS[args.fast_start], measurements=mes)
         if args.fast_start == "learned":
             d_fast.set_warmnet(isg.WarmNet(max_batch=8, max_pixels=side * side, path=WARMNET_FILE))
@@ -264,9 +269,11 @@ def main():
                         % (side, side, level, ntx, nrx,
                            ("on a circle of radius %g lambda (line sources)" % args.radius) if args.radius > 0
                            else "(plane waves, far-field receivers)", wl["nell"], wl["seed"]),
-            "path": ("batched %d RHS + deflation, %s, %s total-field start, %s derivative-source product"
+            "path": ("batched %d RHS + deflation, %s, %s total-field start, %s adjoint/derivative starts, "
+                     "%s derivative-source product"
                      % (batch, "FP32 operator / FP64 solver + FP64 defect correction" if mixed else "FP64",
-                        args.fast_start, "FP32" if args.fast_products == "fp32" and mixed else "FP64")
+                        args.fast_start, args.fast_solve_start,
+                        "FP32" if args.fast_products == "fp32" and mixed else "FP64")
                      if main_path == "fast" else "1 RHS, FP64, zero start"),
             "kernel_tuning": args.tuning,
         },

@@ -86,6 +86,11 @@ typedef struct isg_config {
                       (precision 1 only).  The gradient's G^H c (mom.cpp:247-271) is FP64 either
                       way: with precision 1 it is the product that the adjoint solve's final FP64
                       residual check has already computed */
+  int solve_start; /* DBIM adjoint and derivative-field solves: 0 (default) zero start, as the
+                      reference (mom.cpp:286, :347); 1 the previous DBIM iteration's solution of
+                      the same transmitter scaled by the residual-minimising lambda (one operator
+                      application; the start is never worse than zero); the stores are dropped by
+                      isg_dbim_setup* and isg_dbim_set */
 } isg_config;
 
 /* Per-solve result. */

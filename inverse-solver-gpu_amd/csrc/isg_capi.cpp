@@ -88,6 +88,7 @@ void isg_config_default(isg_config* c) {
   c->graph = 1;
   c->radius = 0.0;
   c->products = 0;
+  c->solve_start = 0;
 }
 
 int isg_create(const isg_config* cfg, isg_context** out) {
@@ -98,6 +99,7 @@ int isg_create(const isg_config* cfg, isg_context** out) {
   if (!(cfg->mixed_margin > 0 && cfg->mixed_margin <= 1)) return ISG_E_ARG;
   if (!(cfg->radius >= 0)) return ISG_E_ARG;
   if (cfg->products != 0 && !(cfg->products == 1 && cfg->precision == 1)) return ISG_E_ARG;
+  if (cfg->solve_start != 0 && cfg->solve_start != 1) return ISG_E_ARG;
   isg_context* c = new isg_context();
   c->cfg = *cfg;
   isg::EngineConfig ec;
@@ -357,6 +359,7 @@ int isg_dbim_setup(isg_context* c, const double* o_true, int field_start, isg_db
   opt.field_start = field_start;
   opt.precision = c->cfg.precision;
   opt.products = c->cfg.products;
+  opt.solve_start = c->cfg.solve_start;
   c->dbim.reset();
   c->dbim.reset(new isg::Dbim(*c->eng, opt));
   if (!c->dbim->ok()) return fail(c, ISG_E_DEVICE, c->dbim->error());
@@ -379,6 +382,7 @@ int isg_dbim_setup_measured(isg_context* c, const double* mes, const double* o_t
   opt.field_start = field_start;
   opt.precision = c->cfg.precision;
   opt.products = c->cfg.products;
+  opt.solve_start = c->cfg.solve_start;
   c->dbim.reset();
   c->dbim.reset(new isg::Dbim(*c->eng, opt));
   if (!c->dbim->ok()) return fail(c, ISG_E_DEVICE, c->dbim->error());

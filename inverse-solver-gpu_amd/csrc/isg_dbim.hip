@@ -215,7 +215,8 @@ Dbim::Dbim(Engine& eng, const DbimOptions& opt) : eng_(eng), opt_(opt) {
   wy_ = (zd*)dalloc(vb * B_);
   wc_ = (zd*)dalloc(vb * B_);
   wx_ = (zd*)dalloc(vb * B_);
-  if (opt_.field_start == 2) dtot_ = (zd*)dalloc(vb * T_);
+  if (opt_.field_start == 2 || opt_.solve_start == 1) dtot_ = (zd*)dalloc(vb * T_);
+  if (opt_.solve_start == 1) adj_ = (zd*)dalloc(vb * T_);
   nblk_ = (int)std::min<long>(1024, std::max<long>(1, (N_ + kBlock * 4 - 1) / (kBlock * 4)));
   part_ = (double*)dalloc(sizeof(double) * kSums * nblk_);
   DB_HIP(hipHostMalloc((void**)&part_h_, sizeof(double) * kSums * nblk_, hipHostMallocDefault));
@@ -246,6 +247,21 @@ void Dbim::solve(zd* x, const zd* b, bool herm, int nb, bool x_zero, DbimStats& 
   }
 }
 
+void Dbim::solve_from(zd* x, zd* prev, const zd* b, bool herm, int nb, DbimStats& st) {
+  hipStream_t s = eng_.stream();
+  const size_t bytes = sizeof(zd) * (size_t)N_ * nb;
+  DB_HIP(hipMemsetAsync(x, 0, bytes, s));
+  std::vector<SolveStat> ss;
+  eng_.bicgs_blend(x, prev, g_, 0, b, herm, nb, opt_.tol, opt_.maxit, ss, opt_.precision, nullptr, true);
+  for (auto& r : ss) {
+    st.solves++;
+    st.solve_iters += r.iters;
+    if (r.status != 0) st.not_converged++;
+  }
+  // the copy leaves the engine's operator output (a mixed solve's checked product) untouched
+  DB_HIP(hipMemcpyAsync(prev, x, bytes, hipMemcpyDeviceToDevice, s));
+}
+
 // mom.cpp:247-271 / :338-362: od = sum_t conj(tot_t) (A_t^H mesd_t) + reg^2 o
 void Dbim::gradient(DbimStats& st) {
   hipStream_t s = eng_.stream();
@@ -256,8 +272,13 @@ void Dbim::gradient(DbimStats& st) {
     dim3 grid(nblk_, nb), blk(kBlock);
     eng_.aggregateh(mesd_ + (size_t)t0 * R_, wy_, nb);
     k_adj_src<<<grid, blk, 0, s>>>(g_, wy_, wc_, N);
-    DB_HIP(hipMemsetAsync(wx_, 0, sizeof(zd) * N * nb, s));
-    solve(wx_, wc_, true, nb, true, st);
+    if (adj_ && adj_ok_) {
+      solve_from(wx_, adj_ + (size_t)t0 * N, wc_, true, nb, st);
+    } else {
+      DB_HIP(hipMemsetAsync(wx_, 0, sizeof(zd) * N * nb, s));
+      solve(wx_, wc_, true, nb, true, st);
+      if (adj_) DB_HIP(hipMemcpyAsync(adj_ + (size_t)t0 * N, wx_, sizeof(zd) * N * nb, hipMemcpyDeviceToDevice, s));
+    }
     // c = conj(G conj(x))  (mlfmah, solve.cpp:8-24); the conjugate of Gc is taken in k_grad_acc.
     // A mixed-precision solve ends with an FP64 residual check that computed G conj(x) of the
     // final x: that is this product, with the FP64 operator, at no cost.  After an FP64 solve
@@ -271,6 +292,7 @@ void Dbim::gradient(DbimStats& st) {
     k_grad_acc<<<dim3(nblk_), blk, 0, s>>>(buff_, tot_ + (size_t)t0 * N, wy_, gcx, o_, N, nb,
                                            opt_.regular * opt_.regular);
   }
+  adj_ok_ = adj_ != nullptr;
   DB_HIP(hipMemcpyAsync(od_, buff_, sizeof(zd) * N, hipMemcpyDeviceToDevice, s));
 }
 
@@ -334,6 +356,7 @@ void Dbim::start(const zd* mes_host, const zd* otrue, DbimStats& st) {
   const long N = N_;
   std::vector<zd> mes(mes_host, mes_host + (size_t)T_ * R_);
   DB_HIP(hipMemcpyAsync(mes_, mes.data(), sizeof(zd) * mes.size(), hipMemcpyHostToDevice, s));
+  adj_ok_ = dtot_ok_ = false;
   objnorm_ = 0;
   if (otrue) {
     DB_HIP(hipMemcpyAsync(omes_, otrue, sizeof(zd) * N, hipMemcpyHostToDevice, s));
@@ -376,11 +399,16 @@ void Dbim::iterate(DbimStats& st) {
     // of P2M and M2L, amplified by the leaf-level translations
     eng_.mlfma(wy_, wc_, nb, nullptr, opt_.products);
     zd* dx = dtot_ ? dtot_ + (size_t)t0 * N : wx_;  // kept for the first-order field start
-    DB_HIP(hipMemsetAsync(dx, 0, sizeof(zd) * N * nb, s));
-    solve(dx, wc_, false, nb, true, st);
+    if (opt_.solve_start == 1 && dtot_ok_) {
+      solve_from(wx_, dx, wc_, false, nb, st);
+    } else {
+      DB_HIP(hipMemsetAsync(dx, 0, sizeof(zd) * N * nb, s));
+      solve(dx, wc_, false, nb, true, st);
+    }
     k_add_gx<<<grid, blk, 0, s>>>(wy_, g_, dx, N);
     eng_.aggregate(wy_, rhs_ + (size_t)t0 * R_, nb);
   }
+  dtot_ok_ = opt_.solve_start == 1;
   std::vector<zd> rhs((size_t)T_ * R_), mesd((size_t)T_ * R_);
   DB_HIP(hipMemcpyAsync(rhs.data(), rhs_, sizeof(zd) * rhs.size(), hipMemcpyDeviceToHost, s));
   DB_HIP(hipMemcpyAsync(mesd.data(), mesd_, sizeof(zd) * mesd.size(), hipMemcpyDeviceToHost, s));
@@ -405,7 +433,8 @@ void Dbim::iterate(DbimStats& st) {
     bool zero = opt_.field_start == 0;
     if (zero) DB_HIP(hipMemsetAsync(tot, 0, sizeof(zd) * N * nb, s));
     // o moved by -alpha ot, so to first order tot moves by -alpha dtot
-    if (dtot_) k_axpy_rows<<<dim3(nblk_, nb), blk, 0, s>>>(tot, dtot_ + (size_t)t0 * N, mkc<double>(-alpha, 0.0), N);
+    if (opt_.field_start == 2)
+      k_axpy_rows<<<dim3(nblk_, nb), blk, 0, s>>>(tot, dtot_ + (size_t)t0 * N, mkc<double>(-alpha, 0.0), N);
     if (opt_.field_start == 3) {
       learned_solve(tot, inc_ + (size_t)t0 * N, nb, st);
     } else {
@@ -536,6 +565,8 @@ bool Dbim::set(int which, const std::vector<zd>& in) {
     default: return false;
   }
   if (in.size() != n) return false;
+  // restored state: the stored previous solutions belong to another history
+  adj_ok_ = dtot_ok_ = false;
   DB_HIP(hipStreamSynchronize(eng_.stream()));
   DB_HIP(hipMemcpy(dst, in.data(), sizeof(zd) * n, hipMemcpyHostToDevice));
   if (which == 0) {

@@ -28,6 +28,10 @@ struct DbimOptions {
   int field_start = 0;
   int precision = 0;      // solver operator: PREC_FP64 or PREC_MIXED
   int products = 0;       // operator of the explicit products G^H c and G y: PREC_FP64 or PREC_MIXED
+  // adjoint and derivative-field solves: 0 zero start (mom.cpp:286, :347),
+  // 1 the previous DBIM iteration's solution of the same transmitter, scaled
+  // by the residual-minimising lambda (never a worse start than zero)
+  int solve_start = 0;
 };
 
 struct DbimStats {
@@ -70,6 +74,8 @@ class Dbim {
   void gradient(DbimStats& st);
   void metrics(DbimStats& st);
   void solve(zd* x, const zd* b, bool herm, int nb, bool x_zero, DbimStats& st);
+  // solve into x (zeroed here) started from lambda * prev; then prev = x
+  void solve_from(zd* x, zd* prev, const zd* b, bool herm, int nb, DbimStats& st);
   void learned_solve(zd* tot, const zd* inc, int nb, DbimStats& st);
   void* dalloc(size_t bytes);
 
@@ -84,7 +90,9 @@ class Dbim {
   zd *inc_ = nullptr, *tot_ = nullptr;
   zd *mes_ = nullptr, *mesd_ = nullptr, *rhs_ = nullptr;
   zd *wy_ = nullptr, *wc_ = nullptr, *wx_ = nullptr;
-  zd* dtot_ = nullptr;  // [numtx][N] derivative fields, field_start 2 only
+  zd* dtot_ = nullptr;  // [numtx][N] derivative fields (field_start 2 or solve_start 1)
+  zd* adj_ = nullptr;   // [numtx][N] adjoint solutions (solve_start 1)
+  bool dtot_ok_ = false, adj_ok_ = false;  // the stores hold a previous iteration's solutions
   WarmNet* net_ = nullptr;
   int net_chunk_ = 0;
   float *net_in_ = nullptr, *net_out_ = nullptr;  // [chunk][side][side][6], [..][2]

@@ -937,7 +937,7 @@ void Engine::bicgs(zd* x, const zd* g, long gs, const zd* b, bool herm, int nrhs
 // r_x = b - A x and A d with the solver's operator, lambda = <Ad, r_x> / <Ad, Ad>,
 // x += lambda d, r0 = r_x - lambda Ad; BiCGSTAB then starts from (x, r0).
 void Engine::bicgs_blend(zd* x, const zd* d, const zd* g, long gs, const zd* b, bool herm, int nrhs, double tol,
-                         int maxit, std::vector<SolveStat>& out, int prec, std::vector<zd>* lam) {
+                         int maxit, std::vector<SolveStat>& out, int prec, std::vector<zd>* lam, bool x_zero) {
   out.assign(nrhs, SolveStat{-1, 0, 0.0});
   if (!ok()) return;
   if (nrhs > cfg_.max_rhs) {
@@ -946,11 +946,12 @@ void Engine::bicgs_blend(zd* x, const zd* d, const zd* g, long gs, const zd* b, 
   }
   const bool mixed = prec == PREC_MIXED && has32_;
   if (mixed)
-    blend<float>(x, d, g, gs, b, herm, nrhs, PREC_MIXED);
+    blend<float>(x, d, g, gs, b, herm, nrhs, PREC_MIXED, x_zero);
   else
-    blend<double>(x, d, g, gs, b, herm, nrhs, PREC_FP64);
-  matvecs += 2 * nrhs;
-  if (mixed) matvecs32 += 2 * nrhs;
+    blend<double>(x, d, g, gs, b, herm, nrhs, PREC_FP64, x_zero);
+  const int napp = x_zero ? 1 : 2;
+  matvecs += napp * nrhs;
+  if (mixed) matvecs32 += napp * nrhs;
   if (lam) {
     lam->resize(nrhs);
     ISG_HIP(hipMemcpyAsync(lam->data(), lam_, nrhs * sizeof(zd), hipMemcpyDeviceToHost, stream_));
@@ -960,15 +961,18 @@ void Engine::bicgs_blend(zd* x, const zd* d, const zd* g, long gs, const zd* b, 
 }
 
 template <typename V>
-void Engine::blend(zd* x, const zd* d, const zd* g, long gs, const zd* b, bool herm, int nrhs, int prec) {
+void Engine::blend(zd* x, const zd* d, const zd* g, long gs, const zd* b, bool herm, int nrhs, int prec,
+                   bool x_zero) {
   const long N = P_.N;
   checked_x_ = nullptr;
   const int h = herm ? 1 : 0;
   dim3 grid(nblk_, nrhs), blk(kBlock);
   cplx<V>*y = vb<V>(V_Y), *op = vb<V>(V_OP), *res = vb<V>(V_RES), *ad = vb<V>(V_COR);
-  k_prep<V><<<grid, blk, 0, stream_>>>(x, g, gs, y, N, h);
-  apply_vec<V>(y, op, nrhs, nullptr, prec);
-  k_bi_init<V><<<grid, blk, 0, stream_>>>(x, op, g, gs, b, nullptr, res, vb<V>(V_RT), N, h, 0, part_);
+  if (!x_zero) {
+    k_prep<V><<<grid, blk, 0, stream_>>>(x, g, gs, y, N, h);
+    apply_vec<V>(y, op, nrhs, nullptr, prec);
+  }
+  k_bi_init<V><<<grid, blk, 0, stream_>>>(x, op, g, gs, b, nullptr, res, vb<V>(V_RT), N, h, x_zero ? 1 : 0, part_);
   k_prep<V><<<grid, blk, 0, stream_>>>(d, g, gs, y, N, h);
   apply_vec<V>(y, op, nrhs, nullptr, prec);
   k_bl_dots<V><<<grid, blk, 0, stream_>>>(d, op, g, gs, res, ad, N, h, part_);

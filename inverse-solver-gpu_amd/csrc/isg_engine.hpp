@@ -95,9 +95,10 @@ class Engine {
              int maxit, std::vector<SolveStat>& out, int prec = PREC_FP64);
   // The same solve started from x + lambda d, lambda chosen per system to
   // minimise |b - A (x + lambda d)| (so the start is never worse than x); two
-  // operator applications replace the usual one.  lam (host) receives lambda.
+  // operator applications replace the usual one (one when x_zero: x holds
+  // zeros and r_x = b).  lam (host, may be null) receives lambda.
   void bicgs_blend(zd* x, const zd* d, const zd* g, long g_stride, const zd* b, bool herm, int nrhs, double tol,
-                   int maxit, std::vector<SolveStat>& out, int prec, std::vector<zd>* lam);
+                   int maxit, std::vector<SolveStat>& out, int prec, std::vector<zd>* lam, bool x_zero = false);
 
   // The FP64 product G prep(x) (prep(x) = g x forward, conj(x) adjoint) that
   // the last true-residual check of a mixed-precision solve computed for its
@@ -215,7 +216,7 @@ class Engine {
   void run_bicgs(zd* x, const zd* g, long gs, const zd* b, bool herm, int nrhs, bool x_zero, int maxit, int prec,
                  const cplx<V>* r0);
   template <typename V>
-  void blend(zd* x, const zd* d, const zd* g, long gs, const zd* b, bool herm, int nrhs, int prec);
+  void blend(zd* x, const zd* d, const zd* g, long gs, const zd* b, bool herm, int nrhs, int prec, bool x_zero);
   void true_residual(const zd* x, const zd* g, long gs, const zd* b, bool herm, int nrhs, zd* res,
                      std::vector<double>& rn, std::vector<double>& bn);
   // new_set false: the interval adds to the previous launch set's time

@@ -43,6 +43,7 @@ class Config(ctypes.Structure):
         ("graph", ctypes.c_int),
         ("radius", ctypes.c_double),
         ("products", ctypes.c_int),
+        ("solve_start", ctypes.c_int),
     ]
 
 

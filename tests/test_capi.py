@@ -34,7 +34,7 @@ def test_defaults_are_reference_constants():
     assert (c.res, c.box, c.level, c.beta, c.ninter) == (0.1, 8, 7, 5, 30)
     assert (c.toli, c.iti, c.regular) == (1e-4, 300, 0.0)
     assert c.max_rhs == 1 and c.precision == 0 and c.mixed_margin == 0.5
-    assert c.graph == 1 and c.radius == 0.0 and c.products == 0
+    assert c.graph == 1 and c.radius == 0.0 and c.products == 0 and c.solve_start == 0
 
 
 @pytest.mark.skipif(gpu_present(), reason="checks the no-device error path")
@@ -55,6 +55,8 @@ def test_bad_config_rejected():
         isg.Solver(level=4, precision=0, products=1)
     with pytest.raises(isg.IsgError):
         isg.Solver(level=4, precision=1, products=-1)
+    with pytest.raises(isg.IsgError):
+        isg.Solver(level=4, solve_start=2)
 
 
 class SplitMix64:
