@@ -31,12 +31,17 @@ def case():
     return isg, otrue, ref, np.array(hist_ref)
 
 
-@pytest.mark.parametrize("batch", [1, 16])
-def test_dbim_matches_oracle(case, batch):
+@pytest.mark.parametrize("batch,precision,field_start,solve_start",
+                         [(1, 0, 0, 0), (16, 0, 0, 0), (6, 0, 0, 0), (6, 1, 2, 1)])
+def test_dbim_matches_oracle(case, batch, precision, field_start, solve_start):
+    """batch 6 leaves a ragged last batch (6 + 6 + 4 transmitters); the last
+    case also runs every option of the fast path on it (mixed precision,
+    first-order field start, previous-solution adjoint/derivative starts)."""
     isg, otrue, ref, hist_ref = case
-    print("dbim parity batch", batch, flush=True)
-    s = isg.Solver(level=LEVEL, numrx=NRX, numtx=NTX, toli=TOL, iti=300, regular=REG, max_rhs=batch)
-    d = isg.Dbim(s, otrue)
+    print("dbim parity batch", batch, "precision", precision, "starts", field_start, solve_start, flush=True)
+    s = isg.Solver(level=LEVEL, numrx=NRX, numtx=NTX, toli=TOL, iti=300, regular=REG, max_rhs=batch,
+                   precision=precision, solve_start=solve_start)
+    d = isg.Dbim(s, otrue, field_start=field_start)
     st0 = d.setup_stats
     hist = [[st0["misfit"], st0["object_error"], st0["object_cost"], st0["gradient_norm2"], st0["object_norm2"]]]
     for _ in range(3):
