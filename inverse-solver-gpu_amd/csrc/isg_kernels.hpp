@@ -1012,7 +1012,10 @@ __global__ void __launch_bounds__(kBlock) k_m2l_leaf(const cplx<T>* __restrict__
 }
 
 // P2M on the matrix cores: agm[box][k] = sum_n coeffT[n][k] y[box*64+n] for a
-// tile of NB boxes; rows k in blocks of 16 spread over the 4 waves.
+// tile of NB boxes.  The work is cut into units of 16 rows k x 16 boxes, dealt
+// round-robin to the 4 waves: with S = 68 (5 row blocks) and 64-box tiles
+// every wave gets 5 units (whole row blocks per wave would leave three waves
+// idle half the time).
 template <typename T, typename TI, int NB>
 __global__ void __launch_bounds__(kBlock) k_p2m_mfma(const cplx<TI>* __restrict__ y, long ys,
                                                      cplx<T>* __restrict__ agm, long as,
@@ -1027,45 +1030,36 @@ __global__ void __launch_bounds__(kBlock) k_p2m_mfma(const cplx<TI>* __restrict_
   const int li = lane & 15, lk = lane >> 4;
   const long b0 = (long)xcd_block() * NB;
   const cplx<TI>* yr = y + rhs * ys + b0 * 64;
-  for (int idx = tid; idx < NB * 64; idx += kBlock) tile[(idx >> 6) * 65 + (idx & 63)] = ccast<T>(yr[idx]);
+  // loads issued in groups ahead of their LDS stores
+  stage<sizeof(T) == 4 ? 8 : 4, cplx<TI>>(
+      NB * 64, [&](int idx) { return yr[idx]; },
+      [&](int idx, cplx<TI> v) { tile[(idx >> 6) * 65 + (idx & 63)] = ccast<T>(v); });
   __syncthreads();
-  const int nrb = (S + 15) / 16;
-  for (int rb = w; rb < nrb; rb += 4) {
-    v4 p1[NN], p2[NN], p3[NN];
-#pragma unroll
-    for (int j = 0; j < NN; j++) {
-      p1[j] = v4{0, 0, 0, 0};
-      p2[j] = v4{0, 0, 0, 0};
-      p3[j] = v4{0, 0, 0, 0};
-    }
+  const int nrb = (S + 15) / 16, nunit = nrb * NN;
+  cplx<T>* out = agm + rhs * as;
+  for (int u = w; u < nunit; u += 4) {
+    const int rb = u / NN, j = u - rb * NN;
+    v4 p1 = v4{0, 0, 0, 0}, p2 = v4{0, 0, 0, 0}, p3 = v4{0, 0, 0, 0};
     const int krow = 16 * rb + li;
+    const cplx<T>* trow = tile + (16 * j + li) * 65;
     // the coefficients of PRE k-steps are loaded together ahead of their MFMAs
     constexpr int PRE = sizeof(T) == 4 ? 8 : 4;
 #pragma unroll 1
     for (int k0 = 0; k0 < 16; k0 += PRE) {
       cplx<T> a[PRE];
 #pragma unroll
-      for (int u = 0; u < PRE; u++) a[u] = (krow < S) ? coeffT[(4 * (k0 + u) + lk) * S + krow] : mkc<T>(0, 0);
+      for (int v = 0; v < PRE; v++) a[v] = (krow < S) ? coeffT[(4 * (k0 + v) + lk) * S + krow] : mkc<T>(0, 0);
 #pragma unroll
-      for (int u = 0; u < PRE; u++) {
-        const int n = 4 * (k0 + u) + lk;
-        const T asum = a[u].x + a[u].y;
-#pragma unroll
-        for (int j = 0; j < NN; j++) {
-          cplx<T> bv = tile[(16 * j + li) * 65 + n];
-          Mfma<T>::cmac3(bv, a[u], asum, p1[j], p2[j], p3[j]);
-        }
+      for (int v = 0; v < PRE; v++) {
+        const T asum = a[v].x + a[v].y;
+        Mfma<T>::cmac3(trow[4 * (k0 + v) + lk], a[v], asum, p1, p2, p3);
       }
     }
-    cplx<T>* out = agm + rhs * as;
     if (krow < S) {
 #pragma unroll
-      for (int j = 0; j < NN; j++) {
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-          long box = b0 + 16 * j + Mfma<T>::row(lk, q);
-          out[box * S + krow] = mkc<T>(p1[j][q] - p2[j][q], p3[j][q] - p1[j][q] - p2[j][q]);
-        }
+      for (int q = 0; q < 4; q++) {
+        const long box = b0 + 16 * j + Mfma<T>::row(lk, q);
+        out[box * S + krow] = mkc<T>(p1[q] - p2[q], p3[q] - p1[q] - p2[q]);
       }
     }
   }
