@@ -31,16 +31,18 @@ def case():
     return isg, otrue, ref, np.array(hist_ref)
 
 
-@pytest.mark.parametrize("batch,precision,field_start,solve_start",
-                         [(1, 0, 0, 0), (16, 0, 0, 0), (6, 0, 0, 0), (6, 1, 2, 1)])
-def test_dbim_matches_oracle(case, batch, precision, field_start, solve_start):
+@pytest.mark.parametrize("batch,precision,field_start,solve_start,margin",
+                         [(1, 0, 0, 0, 0.5), (16, 0, 0, 0, 0.5), (6, 0, 0, 0, 0.5), (6, 1, 2, 1, 0.9)])
+def test_dbim_matches_oracle(case, batch, precision, field_start, solve_start, margin):
     """batch 6 leaves a ragged last batch (6 + 6 + 4 transmitters); the last
-    case also runs every option of the fast path on it (mixed precision,
-    first-order field start, previous-solution adjoint/derivative starts)."""
+    case also runs every option of the fast path on it (mixed precision with
+    the benchmark's solve margin, first-order field start, previous-solution
+    adjoint/derivative starts)."""
     isg, otrue, ref, hist_ref = case
-    print("dbim parity batch", batch, "precision", precision, "starts", field_start, solve_start, flush=True)
+    print("dbim parity batch", batch, "precision", precision, "starts", field_start, solve_start, "margin", margin,
+          flush=True)
     s = isg.Solver(level=LEVEL, numrx=NRX, numtx=NTX, toli=TOL, iti=300, regular=REG, max_rhs=batch,
-                   precision=precision, solve_start=solve_start)
+                   precision=precision, solve_start=solve_start, mixed_margin=margin)
     d = isg.Dbim(s, otrue, field_start=field_start)
     st0 = d.setup_stats
     hist = [[st0["misfit"], st0["object_error"], st0["object_cost"], st0["gradient_norm2"], st0["object_norm2"]]]
