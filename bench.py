@@ -272,7 +272,8 @@ def main():
                            else "(plane waves, far-field receivers)", wl["nell"], wl["seed"]),
             "path": ("batched %d RHS + deflation, %s, %s total-field start, %s adjoint/derivative starts, "
                      "%s derivative-source product"
-                     % (batch, "FP32 operator / FP64 solver + FP64 defect correction" if mixed else "FP64",
+                     % (batch, ("FP32 operator / FP64 solver (FP32 solves to %g x tol) + FP64 defect correction"
+                                % args.mixed_margin) if mixed else "FP64",
                         args.fast_start, args.fast_solve_start,
                         "FP32" if args.fast_products == "fp32" and mixed else "FP64")
                      if main_path == "fast" else "1 RHS, FP64, zero start"),

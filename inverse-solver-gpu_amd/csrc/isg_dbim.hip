@@ -282,11 +282,11 @@ void Dbim::gradient(DbimStats& st) {
     // c = conj(G conj(x))  (mlfmah, solve.cpp:8-24); the conjugate of Gc is taken in k_grad_acc.
     // A mixed-precision solve ends with an FP64 residual check that computed G conj(x) of the
     // final x: that is this product, with the FP64 operator, at no cost.  After an FP64 solve
-    // it is computed here.
+    // it is computed here, also with the FP64 operator.
     const zd* gcx = eng_.checked_product(wx_, nb, true);
     if (!gcx) {
       k_conj<<<grid, blk, 0, s>>>(wx_, wc_, N);
-      eng_.mlfma(wc_, wx_, nb, nullptr, opt_.products);
+      eng_.mlfma(wc_, wx_, nb, nullptr, PREC_FP64);
       gcx = wx_;
     }
     k_grad_acc<<<dim3(nblk_), blk, 0, s>>>(buff_, tot_ + (size_t)t0 * N, wy_, gcx, o_, N, nb,

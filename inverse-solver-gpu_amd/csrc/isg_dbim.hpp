@@ -27,7 +27,7 @@ struct DbimOptions {
   // from tot_prev + lambda d with the residual-minimising lambda
   int field_start = 0;
   int precision = 0;      // solver operator: PREC_FP64 or PREC_MIXED
-  int products = 0;       // operator of the explicit products G^H c and G y: PREC_FP64 or PREC_MIXED
+  int products = 0;       // operator of the derivative source G (k0^2 ot tot): PREC_FP64 or PREC_MIXED
   // adjoint and derivative-field solves: 0 zero start (mom.cpp:286, :347),
   // 1 the previous DBIM iteration's solution of the same transmitter, scaled
   // by the residual-minimising lambda (never a worse start than zero)
