@@ -151,7 +151,9 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="dbim-1024", choices=sorted(WORKLOADS))
-    ap.add_argument("--paths", default="fast,plain")
+    ap.add_argument("--paths", default="fast,plain,learned",
+                    help="fast: the headline path; plain: the reference's FP64 single-RHS path; learned: the "
+                         "fast path with the learned total-field start instead of the first-order one")
     ap.add_argument("--fast-precision", default="mixed", choices=["mixed", "fp64"])
     ap.add_argument("--fast-start", default="first-order", choices=sorted(START_KINDS),
                     help="total-field start of the fast path")
@@ -190,6 +192,9 @@ def main():
     common = dict(level=level, numrx=nrx, numtx=ntx, toli=1e-6, iti=300, regular=float(np.sqrt(1e-3)),
                   tuning=tuning, radius=args.radius)
     paths = [p for p in args.paths.split(",") if p]
+    if not set(paths) <= {"fast", "plain", "learned"} or not ({"fast", "plain"} & set(paths)):
+        print("--paths: a comma list of fast, plain, learned with fast or plain in it", file=sys.stderr)
+        sys.exit(2)
     fast_prec = 1 if args.fast_precision == "mixed" else 0
 
     # synthetic measurements of the true contrast, made once by the batched FP64 path
@@ -222,6 +227,15 @@ def main():
         fast_leaf_fused = fast.tuning_get(fast_prec, 1, "leaf") == 1
         del d_fast
         fast.close()
+    if "learned" in paths:
+        lrn = isg.Solver(max_rhs=batch, precision=fast_prec, mixed_margin=args.mixed_margin,
+                         products=1 if args.fast_products == "fp32" and fast_prec == 1 else 0,
+                         solve_start=1 if args.fast_solve_start == "previous" else 0, **common)
+        d_lrn = isg.Dbim(lrn, otrue, field_start=START_KINDS["learned"], measurements=mes)
+        d_lrn.set_warmnet(isg.WarmNet(max_batch=8, max_pixels=side * side, path=WARMNET_FILE))
+        res["learned"] = run_path(lrn, d_lrn, args.warmup, args.steps, "learned", args.kernel_timing == "on")
+        del d_lrn
+        lrn.close()
     if "plain" in paths:
         plain = isg.Solver(max_rhs=1, **common)
         d_plain = isg.Dbim(plain, otrue, field_start=0, measurements=mes)
@@ -318,6 +332,17 @@ def main():
         # reconstruction after the same iterations: fast path against the FP64 plain path
         out["contrast_rel_diff_vs_plain"] = float(np.linalg.norm(contrast["fast"] - contrast_plain)
                                                   / np.linalg.norm(contrast_plain))
+    if "learned" in res:
+        lst = res["learned"][1]
+        out["learned_start"] = {
+            "s_per_iter": round(res["learned"][0], 4),
+            "speedup_vs_plain": round(res["plain"][0] / res["learned"][0], 3) if "plain" in res else None,
+            "field_bicgstab_iterations": sum(s["field_iters"] for s in lst),
+            "network_s_per_iter": round(sum(s["net_seconds"] for s in lst) / len(lst), 4),
+            "mean_abs_lambda": round(sum(s["lambda_abs"] for s in lst) / len(lst), 4),
+            "note": "the fast path with the learned total-field start (network + residual-minimising blend) "
+                    "in place of the first-order start; field_bicgstab_iterations compares with the line's",
+        }
     if args.cpu_sample > 0 and "plain" in res:
         plain_mv = int(round(sum(s["matvecs"] for s in res["plain"][1]) / len(res["plain"][1])))
         try:
