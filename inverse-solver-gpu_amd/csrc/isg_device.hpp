@@ -98,13 +98,16 @@ __device__ __forceinline__ double wave_sum(double v) {
 // separate pass over the operator output (vectors of storage type V):
 //   mode 1: out = v = post(p, G y) and the partials of <r~, v>
 //   mode 2: out = t = post(s, G y) and the partials of <t, s>, <t, t>
+//   mode 3 (FP64 only): out = G y itself and the partials of |b|^2 and
+//           |b - post(x, G y)|^2 -- the true-residual check of an iterate x
 // post() is the contrast product of the solver (forward x - Gy, adjoint
 // x - conj(g Gy)); partials go to part[rhs][slot][workgroup].
 template <typename V>
 struct NearPost {
   int mode = 0;
-  const cplx<V>* x = nullptr;   // p (mode 1) or s (mode 2)
+  const cplx<V>* x = nullptr;   // p (mode 1), s (mode 2) or the iterate (mode 3)
   const cplx<V>* rt = nullptr;  // r~ (mode 1)
+  const cplx<V>* b = nullptr;   // right-hand side (mode 3)
   const zd* g = nullptr;
   long gs = 0;
   int herm = 0;

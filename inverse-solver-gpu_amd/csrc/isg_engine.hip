@@ -593,6 +593,10 @@ int Engine::launch_near(const cplx<TI>* y, cplx<TI>* r, int nrhs, const int* act
   dim3 grid(B / nb, nrhs), blk(kBlock);
   const int kt = sizeof(T) == 4 ? KT_NEAR32 : KT_NEAR;
   const NearPost<TI> pp = post ? *post : NearPost<TI>();
+  if (pp.mode == 3 && !(sizeof(T) == 8 && sizeof(TI) == 8 && nb <= 32)) {
+    set_error("near field: the residual-check epilogue is FP64 only");
+    return B / nb;
+  }
   const int wn = with_near ? 1 : 0, lv = 1 << Lf;
   // one instantiation per post mode: the plain product keeps the register
   // budget (and occupancy) of a kernel without the post epilogue
@@ -606,6 +610,7 @@ int Engine::launch_near(const cplx<TI>* y, cplx<TI>* r, int nrhs, const int* act
   using M0 = std::integral_constant<int, 0>;
   using M1 = std::integral_constant<int, 1>;
   using M2 = std::integral_constant<int, 2>;
+  using M3 = std::integral_constant<int, 3>;
   if (with_near) prof_begin(kt);
   if (nb == 64) {
     if constexpr (sizeof(T) == 4) {
@@ -621,14 +626,18 @@ int Engine::launch_near(const cplx<TI>* y, cplx<TI>* r, int nrhs, const int* act
       go(I32(), M1());
     else if (pp.mode == 2)
       go(I32(), M2());
-    else
+    else if (pp.mode == 3) {
+      if constexpr (sizeof(T) == 8 && sizeof(TI) == 8) go(I32(), M3());
+    } else
       go(I32(), M0());
   } else {
     if (pp.mode == 1)
       go(I16(), M1());
     else if (pp.mode == 2)
       go(I16(), M2());
-    else
+    else if (pp.mode == 3) {
+      if constexpr (sizeof(T) == 8 && sizeof(TI) == 8) go(I16(), M3());
+    } else
       go(I16(), M0());
   }
   if (with_near) prof_end(kt);
@@ -894,9 +903,9 @@ void Engine::bicgs_body(zd* x, const zd* g, long gs, bool herm, int nrhs, int pr
   ISG_HIP(hipMemcpyAsync(active_h_, active_, nrhs * sizeof(int), hipMemcpyDeviceToHost, stream_));
 }
 
-// res = b - A x with the FP64 operator; rn, bn: per-system norms of res and b.
-// res may be null (norms only); A x stays in vec_[V_OP] until the next product.
-void Engine::true_residual(const zd* x, const zd* g, long gs, const zd* b, bool herm, int nrhs, zd* res,
+// rn, bn: per-system norms of b - A x (FP64 operator) and of b; G prep(x)
+// stays in vec_[V_OP] until the next product.
+void Engine::true_residual(const zd* x, const zd* g, long gs, const zd* b, bool herm, int nrhs,
                            std::vector<double>& rn, std::vector<double>& bn) {
   const long N = P_.N;
   const int h = herm ? 1 : 0;
@@ -904,21 +913,31 @@ void Engine::true_residual(const zd* x, const zd* g, long gs, const zd* b, bool 
   zd *y = vec_[V_Y], *op = vec_[V_OP];
   checked_x_ = nullptr;
   k_prep<double><<<grid, blk, 0, stream_>>>(x, g, gs, y, N, h);
-  apply_mlfma(y, op, nrhs, nullptr, PREC_FP64);
+  // op = G y with the FP64 operator; the near-field kernel's epilogue also
+  // forms the partial sums of |b|^2 and |b - A x|^2 (the residual vector
+  // itself is formed later, from op, if a correction needs it)
+  NearPost<double> pp;
+  pp.mode = 3;
+  pp.x = x;
+  pp.b = b;
+  pp.g = g;
+  pp.gs = gs;
+  pp.herm = h;
+  pp.part = part_;
+  apply_vec<double>(y, op, nrhs, nullptr, PREC_FP64, &pp);
   matvecs += nrhs;
-  // res == nullptr: only the norms (the vector is formed later if a correction needs it)
-  k_bi_init<double><<<grid, blk, 0, stream_>>>(x, op, g, gs, b, nullptr, res, nullptr, N, h, 0, part_);
-  ISG_HIP(hipMemcpyAsync(part_h_, part_, (size_t)nrhs * kSlots * nblk_ * sizeof(double2), hipMemcpyDeviceToHost,
+  const int nb = near_blocks_;
+  ISG_HIP(hipMemcpyAsync(part_h_, part_, (size_t)nrhs * kSlots * nb * sizeof(double2), hipMemcpyDeviceToHost,
                          stream_));
   ISG_HIP(hipStreamSynchronize(stream_));
   rn.assign(nrhs, 0.0);
   bn.assign(nrhs, 0.0);
   for (int k = 0; k < nrhs; k++) {
-    const double2* pr = part_h_ + (size_t)k * kSlots * nblk_;
+    const double2* pr = part_h_ + (size_t)k * kSlots * nb;
     double sb = 0, sr = 0;
-    for (int j = 0; j < nblk_; j++) {
+    for (int j = 0; j < nb; j++) {
       sb += pr[j].x;
-      sr += pr[nblk_ + j].x;
+      sr += pr[nb + j].x;
     }
     bn[k] = std::sqrt(sb);
     rn[k] = std::sqrt(sr);
@@ -1006,7 +1025,7 @@ void Engine::solve(zd* x, const zd* g, long gs, const zd* b, bool herm, int nrhs
     zd *res = vec_[V_RES], *cor = vec_[V_COR];
     std::vector<double> rn, bn;
     for (int round = 0; round <= kRefineRounds; round++) {
-      true_residual(x, g, gs, b, herm, nrhs, nullptr, rn, bn);
+      true_residual(x, g, gs, b, herm, nrhs, rn, bn);
       if (!ok()) return;
       bool any = false;
       for (int i = 0; i < nrhs; i++) {

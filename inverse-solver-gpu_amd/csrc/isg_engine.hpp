@@ -217,8 +217,8 @@ class Engine {
                  const cplx<V>* r0);
   template <typename V>
   void blend(zd* x, const zd* d, const zd* g, long gs, const zd* b, bool herm, int nrhs, int prec, bool x_zero);
-  void true_residual(const zd* x, const zd* g, long gs, const zd* b, bool herm, int nrhs, zd* res,
-                     std::vector<double>& rn, std::vector<double>& bn);
+  void true_residual(const zd* x, const zd* g, long gs, const zd* b, bool herm, int nrhs, std::vector<double>& rn,
+                     std::vector<double>& bn);
   // new_set false: the interval adds to the previous launch set's time
   void prof_begin(int id, bool new_set = true);
   void prof_end(int id);

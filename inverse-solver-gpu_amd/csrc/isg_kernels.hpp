@@ -391,13 +391,14 @@ __global__ void __launch_bounds__(kBlock) k_near_mfma(const cplx<TI>* __restrict
 #pragma unroll 4
     for (int e = tid; e < NB * 64; e += kBlock) out[e] = ccast<TI>(tile[e]);
   } else {
-  // BiCGSTAB post step on the operator output, rounded to the vector type TI
-  // first, as a separate pass reading the stored product back would see it.
+  // BiCGSTAB post step (modes 1, 2) or residual check (mode 3, FP64) on the
+  // operator output, rounded to the vector type TI first, as a separate pass
+  // reading the stored product back would see it.
   const long o = rhs * rs + b0 * 64;
   const zd* gr = post.g + rhs * post.gs + b0 * 64;
   zd acc[2] = {mkc<double>(0, 0), mkc<double>(0, 0)};
   struct In {
-    cplx<TI> x, rt;
+    cplx<TI> x, aux;  // aux: r~ (mode 1) or b (mode 3)
     zd g;
   };
   const bool herm = post.herm != 0;
@@ -406,7 +407,8 @@ __global__ void __launch_bounds__(kBlock) k_near_mfma(const cplx<TI>* __restrict
       [&](int e) {
         In v;
         v.x = post.x[o + e];
-        if (POST == 1) v.rt = post.rt[o + e];
+        if (POST == 1) v.aux = post.rt[o + e];
+        if (POST == 3) v.aux = post.b[o + e];
         if (herm) v.g = gr[e];  // the forward post step does not use g
         return v;
       },
@@ -414,15 +416,23 @@ __global__ void __launch_bounds__(kBlock) k_near_mfma(const cplx<TI>* __restrict
         const cplx<T> gt = tile[e];
         const zd gy = mkc<double>((double)(TI)gt.x, (double)(TI)gt.y);
         const zd xv = ccast<double>(v.x);
-        const zd res = stv(r, o + e, herm ? csub(xv, cconj(cmul(v.g, gy))) : csub(xv, gy));
-        if (POST == 1) {
-          const zd a = ccast<double>(v.rt);
-          acc[0].x += a.x * res.x + a.y * res.y;
-          acc[0].y += a.x * res.y - a.y * res.x;
+        const zd px = herm ? csub(xv, cconj(cmul(v.g, gy))) : csub(xv, gy);
+        if constexpr (POST == 3) {
+          r[o + e] = ccast<TI>(gt);  // the product itself, as the plain kernel stores it
+          const zd bv = ccast<double>(v.aux), rv = csub(bv, px);
+          acc[0].x += bv.x * bv.x + bv.y * bv.y;
+          acc[1].x += rv.x * rv.x + rv.y * rv.y;
         } else {
-          acc[0].x += res.x * xv.x + res.y * xv.y;
-          acc[0].y += res.x * xv.y - res.y * xv.x;
-          acc[1].x += res.x * res.x + res.y * res.y;
+          const zd res = stv(r, o + e, px);
+          if (POST == 1) {
+            const zd a = ccast<double>(v.aux);
+            acc[0].x += a.x * res.x + a.y * res.y;
+            acc[0].y += a.x * res.y - a.y * res.x;
+          } else {
+            acc[0].x += res.x * xv.x + res.y * xv.y;
+            acc[0].y += res.x * xv.y - res.y * xv.x;
+            acc[1].x += res.x * res.x + res.y * res.y;
+          }
         }
       });
     block_store<2>(acc, post.part + (long)rhs * kSlots * gridDim.x, gridDim.x, blockIdx.x);
