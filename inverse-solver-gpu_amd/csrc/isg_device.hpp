@@ -99,7 +99,9 @@ __device__ __forceinline__ double wave_sum(double v) {
 //   mode 1: out = v = post(p, G y) and the partials of <r~, v>
 //   mode 2: out = t = post(s, G y) and the partials of <t, s>, <t, t>
 //   mode 3 (FP64 only): out = G y itself and the partials of |b|^2 and
-//           |b - post(x, G y)|^2 -- the true-residual check of an iterate x
+//           |b - post(x, G y)|^2 -- the true-residual check of an iterate x;
+//           the operator's input is then x itself, and P2M and the near field
+//           form y = prep(x) while loading it
 // post() is the contrast product of the solver (forward x - Gy, adjoint
 // x - conj(g Gy)); partials go to part[rhs][slot][workgroup].
 template <typename V>

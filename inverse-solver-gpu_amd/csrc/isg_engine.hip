@@ -399,7 +399,7 @@ int Engine::interp_kind(const InterpShape& sh, int np, int nrhs) const {
 // --------------------------------------------------------------- launches
 // leaf radiation patterns on the matrix cores
 template <typename T, typename TI>
-void Engine::launch_p2m(const cplx<TI>* y, int nrhs, const int* act, int force) {
+void Engine::launch_p2m(const cplx<TI>* y, int nrhs, const int* act, int force, const NearPost<TI>* prep) {
   const int Lf = P_.L - 1, S = P_.nsamp[Lf], B = P_.B;
   const int tuned = tune(sizeof(T) == 4 ? PREC_MIXED : PREC_FP64, nrhs).p2m_nb;
   int nb = force > 0 ? force : (tuned > 0 ? tuned : (B >= 32 ? 32 : 16));
@@ -409,6 +409,19 @@ void Engine::launch_p2m(const cplx<TI>* y, int nrhs, const int* act, int force) 
   DevOp<T>& d = ops<T>();
   cplx<T>* out = d.agm + lvl_off_[Lf];
   long as = (long)lvl_size_[Lf];
+  if (prep) {  // residual check: y is the iterate, contrast-weighted while loading (FP64 only)
+    if constexpr (sizeof(T) == 8 && sizeof(TI) == 8) {
+      if (nb == 32)
+        k_p2m_mfma<T, TI, 32, true><<<grid, blk, 0, stream_>>>(y, P_.N, out, as, d.coeffT, S, act, prep->g, prep->gs,
+                                                               prep->herm);
+      else
+        k_p2m_mfma<T, TI, 16, true><<<grid, blk, 0, stream_>>>(y, P_.N, out, as, d.coeffT, S, act, prep->g, prep->gs,
+                                                               prep->herm);
+    } else {
+      set_error("P2M: contrast weighting on load is FP64 only");
+    }
+    return;
+  }
   if (nb == 64) {
     if constexpr (sizeof(T) == 4) k_p2m_mfma<T, TI, 64><<<grid, blk, 0, stream_>>>(y, P_.N, out, as, d.coeffT, S, act);
   } else if (nb == 32)
@@ -650,7 +663,8 @@ int Engine::launch_near(const cplx<TI>* y, cplx<TI>* r, int nrhs, const int* act
 template <typename T, typename TI>
 void Engine::apply(const cplx<TI>* y, cplx<TI>* r, int nrhs, const int* act, const NearPost<TI>* post) {
   const int L = P_.L, Lf = L - 1;
-  launch_p2m<T, TI>(y, nrhs, act);
+  // post mode 3 (residual check): y is the iterate, weighted while P2M and the near field load it
+  launch_p2m<T, TI>(y, nrhs, act, -1, (post && post->mode == 3) ? post : nullptr);
   for (int i = L - 2; i >= 2; i--) launch_m2m<T>(i, nrhs, act);
   const int kt = sizeof(T) == 4 ? KT_M2L32 : KT_M2L;
   // with the fused leaf kernel the last L2L (level Lf-1 -> Lf) runs inside the
@@ -907,15 +921,13 @@ void Engine::bicgs_body(zd* x, const zd* g, long gs, bool herm, int nrhs, int pr
 // stays in vec_[V_OP] until the next product.
 void Engine::true_residual(const zd* x, const zd* g, long gs, const zd* b, bool herm, int nrhs,
                            std::vector<double>& rn, std::vector<double>& bn) {
-  const long N = P_.N;
   const int h = herm ? 1 : 0;
-  dim3 grid(nblk_, nrhs), blk(kBlock);
-  zd *y = vec_[V_Y], *op = vec_[V_OP];
+  zd* op = vec_[V_OP];
   checked_x_ = nullptr;
-  k_prep<double><<<grid, blk, 0, stream_>>>(x, g, gs, y, N, h);
-  // op = G y with the FP64 operator; the near-field kernel's epilogue also
-  // forms the partial sums of |b|^2 and |b - A x|^2 (the residual vector
-  // itself is formed later, from op, if a correction needs it)
+  // op = G prep(x) with the FP64 operator, prep(x) formed while P2M and the
+  // near field load x; the near-field kernel's epilogue also forms the partial
+  // sums of |b|^2 and |b - A x|^2 (the residual vector itself is formed later,
+  // from op, if a correction needs it)
   NearPost<double> pp;
   pp.mode = 3;
   pp.x = x;
@@ -924,7 +936,7 @@ void Engine::true_residual(const zd* x, const zd* g, long gs, const zd* b, bool 
   pp.gs = gs;
   pp.herm = h;
   pp.part = part_;
-  apply_vec<double>(y, op, nrhs, nullptr, PREC_FP64, &pp);
+  apply_vec<double>(x, op, nrhs, nullptr, PREC_FP64, &pp);
   matvecs += nrhs;
   const int nb = near_blocks_;
   ISG_HIP(hipMemcpyAsync(part_h_, part_, (size_t)nrhs * kSlots * nb * sizeof(double2), hipMemcpyDeviceToHost,

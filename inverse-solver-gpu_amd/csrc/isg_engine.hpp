@@ -175,7 +175,8 @@ class Engine {
   void apply(const cplx<TI>* y, cplx<TI>* r, int nrhs, const int* act, const NearPost<TI>* post = nullptr);
   // `force` >= 0 overrides the tuned / built-in choice (autotuner)
   template <typename T, typename TI>
-  void launch_p2m(const cplx<TI>* y, int nrhs, const int* act, int force = -1);
+  // prep (mode 3): y is the iterate x and P2M loads prep(x) (FP64 only)
+  void launch_p2m(const cplx<TI>* y, int nrhs, const int* act, int force = -1, const NearPost<TI>* prep = nullptr);
   template <typename T>
   void launch_m2m(int i, int nrhs, const int* act, int force = -1);
   template <typename T>

@@ -290,7 +290,11 @@ __global__ void __launch_bounds__(kBlock) k_near_mfma(const cplx<TI>* __restrict
         [&](int idx) {
           const int off = hoff[idx >> 6];
           cplx<TI> v = mkc<TI>(0, 0);
-          if (off >= 0) v = yr[(long)off + (idx & 63)];
+          if (off >= 0) {
+            v = yr[(long)off + (idx & 63)];
+            // residual check: the input is the iterate x, contrast-weighted here
+            if constexpr (POST == 3) v = prep_val(v, post.g, rhs * post.gs + off + (idx & 63), post.herm != 0);
+          }
           return v;
         },
         [&](int idx, cplx<TI> v) { tile[(idx >> 6) * HROW + (idx & 63)] = ccast<T>(v); });
@@ -1025,12 +1029,14 @@ __global__ void __launch_bounds__(kBlock) k_m2l_leaf(const cplx<T>* __restrict__
 // tile of NB boxes.  The work is cut into units of 16 rows k x 16 boxes, dealt
 // round-robin to the 4 waves: with S = 68 (5 row blocks) and 64-box tiles
 // every wave gets 5 units (whole row blocks per wave would leave three waves
-// idle half the time).
-template <typename T, typename TI, int NB>
+// idle half the time).  PREP (FP64 only): y holds x and the contrast product
+// prep(x) (g x forward, conj(x) adjoint) is formed while loading it.
+template <typename T, typename TI, int NB, bool PREP = false>
 __global__ void __launch_bounds__(kBlock) k_p2m_mfma(const cplx<TI>* __restrict__ y, long ys,
                                                      cplx<T>* __restrict__ agm, long as,
                                                      const cplx<T>* __restrict__ coeffT, int S,
-                                                     const int* __restrict__ act) {
+                                                     const int* __restrict__ act, const zd* __restrict__ g = nullptr,
+                                                     long gs = 0, int herm = 0) {
   typedef typename Mfma<T>::v4 v4;
   constexpr int NN = NB / 16;
   __shared__ cplx<T> tile[NB * 65];
@@ -1042,7 +1048,12 @@ __global__ void __launch_bounds__(kBlock) k_p2m_mfma(const cplx<TI>* __restrict_
   const cplx<TI>* yr = y + rhs * ys + b0 * 64;
   // loads issued in groups ahead of their LDS stores
   stage<sizeof(T) == 4 ? 8 : 4, cplx<TI>>(
-      NB * 64, [&](int idx) { return yr[idx]; },
+      NB * 64,
+      [&](int idx) {
+        cplx<TI> v = yr[idx];
+        if constexpr (PREP) v = prep_val(v, g, rhs * gs + b0 * 64 + idx, herm != 0);
+        return v;
+      },
       [&](int idx, cplx<TI> v) { tile[(idx >> 6) * 65 + (idx & 63)] = ccast<T>(v); });
   __syncthreads();
   const int nrb = (S + 15) / 16, nunit = nrb * NN;
