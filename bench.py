@@ -179,8 +179,8 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=2, help="oracle matvecs timed for cpu_baseline (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (transmitter sharding "
-              "over RCCL is described in DESIGN.md but not built); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: there is no multi-GPU path (sharding the transmitters "
+              "over RCCL is designed but not built); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     import isg
