@@ -15,15 +15,19 @@ search, gradient and Polak-Ribiere update.  Inputs are resident in HBM before
 the timed region; the timed region brackets K iterations with device
 synchronisation on both sides (each iteration ends with a stream sync).
 
-Two paths run from the same synthetic measurements:
+Three paths run from the same synthetic measurements:
   plain   FP64 operator, one right-hand side at a time, every solve from zero
-          -- the reference's loop on this library's kernels;
+          -- the reference's loop on this library's kernels, each BiCGSTAB
+          iteration replayed as a HIP graph;
   fast    every transmitter of a phase in one multi-RHS BiCGSTAB with
           deflation, FP32 operator with FP64 solver scalars/vectors and an FP64
           true-residual defect correction (every solve still reaches 1e-6 in
-          FP64), total-field solves started from the previous DBIM total field
-          moved to first order along the step (--fast-start).
-`value` is the fast path's s/iteration; the plain figure is reported beside it.
+          FP64), FP64 gradient and derivative products, total-field solves
+          started from the previous DBIM total field moved to first order along
+          the step (--fast-start);
+  learned the fast path with the learned total-field start instead.
+`value` is the fast path's s/iteration; the plain and learned figures are
+reported beside it.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1]
 """
@@ -182,6 +186,10 @@ def main():
         print("bench.py measures one GPU: there is no multi-GPU path (sharding the transmitters "
               "over RCCL is designed but not built); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    paths = [p for p in args.paths.split(",") if p]
+    if not set(paths) <= {"fast", "plain", "learned"} or not ({"fast", "plain"} & set(paths)):
+        print("--paths: a comma list of fast, plain, learned with fast or plain in it", file=sys.stderr)
+        sys.exit(2)
 
     import isg
     wl = WORKLOADS[args.workload]
@@ -192,10 +200,6 @@ def main():
     tuning = {"committed": True, "none": False}.get(args.tuning, args.tuning)
     common = dict(level=level, numrx=nrx, numtx=ntx, toli=1e-6, iti=300, regular=float(np.sqrt(1e-3)),
                   tuning=tuning, radius=args.radius)
-    paths = [p for p in args.paths.split(",") if p]
-    if not set(paths) <= {"fast", "plain", "learned"} or not ({"fast", "plain"} & set(paths)):
-        print("--paths: a comma list of fast, plain, learned with fast or plain in it", file=sys.stderr)
-        sys.exit(2)
     fast_prec = 1 if args.fast_precision == "mixed" else 0
 
     # synthetic measurements of the true contrast, made once by the batched FP64 path
