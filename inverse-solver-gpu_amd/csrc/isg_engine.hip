@@ -226,8 +226,10 @@ Engine::Engine(const EngineConfig& cfg) : cfg_(cfg) {
     for (int k = 0; k < V_COUNT; k++) vec32_[k] = (cplx<float>*)dalloc((size_t)R * P_.N * sizeof(cplx<float>));
   nblk_ = (int)std::min<long>(1024, std::max<long>(1, (P_.N + kBlock * 4 - 1) / (kBlock * 4)));
   // partial sums of the vector kernels (nblk_ workgroups) and of the
-  // near-field post steps (one per 16 leaf boxes at most)
-  part_ = (double2*)dalloc((size_t)R * kSlots * std::max<long>(nblk_, P_.B / 16) * sizeof(double2));
+  // near-field epilogues (one per 16 leaf boxes at most); the host copy takes
+  // either (the residual check reads the near-field ones)
+  part_blocks_ = std::max<long>(nblk_, P_.B / 16);
+  part_ = (double2*)dalloc((size_t)R * kSlots * part_blocks_ * sizeof(double2));
   state_ = (BiState*)dalloc((size_t)R * sizeof(BiState));
   active_ = (int*)dalloc((size_t)R * sizeof(int));
   tolv_ = (double*)dalloc((size_t)R * sizeof(double));
@@ -235,7 +237,7 @@ Engine::Engine(const EngineConfig& cfg) : cfg_(cfg) {
   ISG_HIP(hipHostMalloc((void**)&active_h_, (size_t)R * sizeof(int), hipHostMallocDefault));
   ISG_HIP(hipHostMalloc((void**)&state_h_, (size_t)R * sizeof(BiState), hipHostMallocDefault));
   ISG_HIP(hipHostMalloc((void**)&tol_h_, (size_t)R * sizeof(double), hipHostMallocDefault));
-  ISG_HIP(hipHostMalloc((void**)&part_h_, (size_t)R * kSlots * nblk_ * sizeof(double2), hipHostMallocDefault));
+  ISG_HIP(hipHostMalloc((void**)&part_h_, (size_t)R * kSlots * part_blocks_ * sizeof(double2), hipHostMallocDefault));
   if (ok()) {
     ISG_HIP(hipMemset(state_, 0, (size_t)R * sizeof(BiState)));
     ISG_HIP(hipMemset(active_, 0, (size_t)R * sizeof(int)));
@@ -939,6 +941,10 @@ void Engine::true_residual(const zd* x, const zd* g, long gs, const zd* b, bool 
   apply_vec<double>(x, op, nrhs, nullptr, PREC_FP64, &pp);
   matvecs += nrhs;
   const int nb = near_blocks_;
+  if (nb > part_blocks_) {  // cannot happen with the buffers sized above; never copy past part_h_
+    set_error("residual check: more near-field workgroups than partial-sum slots");
+    return;
+  }
   ISG_HIP(hipMemcpyAsync(part_h_, part_, (size_t)nrhs * kSlots * nb * sizeof(double2), hipMemcpyDeviceToHost,
                          stream_));
   ISG_HIP(hipStreamSynchronize(stream_));

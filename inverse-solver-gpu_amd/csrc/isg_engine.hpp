@@ -254,8 +254,9 @@ class Engine {
   zd* rxc_ = nullptr;
   int rx_kc_ = 0;
   void* blas_ = nullptr;  // rocblas_handle
-  double2* part_ = nullptr;       // reduction partials [max_rhs][3][nblk]
-  double2* part_h_ = nullptr;     // pinned host mirror
+  double2* part_ = nullptr;       // reduction partials [max_rhs][3][part_blocks_]
+  double2* part_h_ = nullptr;     // pinned host mirror, same size
+  long part_blocks_ = 0;          // max(vector-kernel, near-field) workgroups per rhs and slot
   BiState* state_ = nullptr;      // [max_rhs]
   int* active_ = nullptr;         // [max_rhs]
   double* tolv_ = nullptr;        // [max_rhs] per-system tolerance

@@ -98,3 +98,23 @@ def test_mixed_solve_64_rhs(isg, s64):
         assert info["status"] == 0
         assert relerr(x1, X[k]) < 1e-4, (k, relerr(x1, X[k]))
     s.close()
+
+
+def test_mixed_solve_level9(isg):
+    """2048x2048 pixels (level 9): the residual check's partial sums come from
+    the near-field kernel, one per 16 or 32 leaf boxes (4096 or 2048 per
+    right-hand side, more than the vector kernels' 1024), and must all reach
+    the host; the mixed solve then meets its FP64 tolerance, recomputed here
+    with the FP64 operator."""
+    print("level 9 mixed solve, 1 rhs", flush=True)
+    tol = 1e-6
+    s = isg.Solver(level=9, numrx=16, max_rhs=1, precision=1, toli=tol, iti=300)
+    pos, _ = s.geometry()
+    k0 = 2 * np.pi
+    g = 0.02 * np.exp(-np.abs(pos - pos.mean()) ** 2 / (2 * 40.0 ** 2)) * (1 + 0.05j) * k0 ** 2
+    b = np.exp(1j * k0 * pos.real)
+    x, info = s.bicgs(g, b)
+    assert info["status"] == 0, info
+    res = np.linalg.norm(b - (x - s.mlfma(g * x))) / np.linalg.norm(b)
+    assert res < tol * 1.05, res
+    s.close()
