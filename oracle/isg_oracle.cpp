@@ -22,7 +22,7 @@
 // How each function is pinned (tests/test_oracle_*.py):
 //   green_pixel        scipy hankel1 (off-diagonal), 2-D quadrature of the
 //                      reference's small-argument model (self term)
-//   hn                 scipy jv/yv fixtures (tests/golden/bessel.json)
+//   hn                 scipy jv/yv evaluated in the test (tests/test_oracle_green.py)
 //   tree / lists       brute-force geometry (distances, Morton order, raster map)
 //   near blocks        direct evaluation of green_pixel on the box pair
 //   trans              addition theorem: plane-wave sum vs H0(k|r-r'|)
@@ -35,7 +35,14 @@
 //   aggregateh         adjoint identity <aggregate x, y> vs <x, aggregateh y>
 //                      up to the reference's scale -4i numrx res^2 (closed form)
 //   bicgs              numpy.linalg.solve on the dense operator
-//   dbim               misfit decrease and gradient finite differences
+//   dbim               gradient (with the per-transmitter Tikhonov term) by
+//                      central finite differences of the regularised misfit;
+//                      step length alpha against a finite-difference Frechet
+//                      derivative (minimiser of the linearised misfit);
+//                      Polak-Ribiere beta by conjugacy of successive
+//                      directions in the Born limit (tests/test_oracle_dbim.py).
+//                      Unpinned: where the conjugate sits in beta's numerator
+//                      (the imaginary part a quadratic misfit cannot see)
 //   phantoms           closed forms (blob peak, ellipse membership)
 // No function is left without such a check.
 
@@ -676,6 +683,9 @@ struct Dbim {
   int iter = 0;
   long solves = 0, solve_iters = 0;
   double hist[5] = {0, 0, 0, 0, 0};
+  // planted mistakes for the tests that pin this loop (1 = the reference's
+  // algorithm): scale of alpha, of beta and of the regulariser's gradient term
+  double mut_alpha = 1, mut_beta = 1, mut_reg = 1;
 };
 
 void track(Dbim& d, const SolveInfo& si) {
@@ -697,7 +707,7 @@ void dbim_gradient(Oracle& o, Dbim& d) {
     mlfmah(o, x.data(), c.data());
     for (int n = 0; n < N; n++) y[n] = y[n] + c[n];
     for (int n = 0; n < N; n++) buff[n] = buff[n] + std::conj(d.tot[(size_t)i * N + n]) * y[n];
-    for (int n = 0; n < N; n++) buff[n] = buff[n] + d.regular * d.regular * d.o[n];
+    for (int n = 0; n < N; n++) buff[n] = buff[n] + d.mut_reg * d.regular * d.regular * d.o[n];
   }
   d.od = buff;
 }
@@ -790,7 +800,7 @@ double dbim_iterate(Oracle& o, Dbim& d) {
       t1 = t1 + std::conj(d.mesd[(size_t)i * R + m]) * d.rhs[(size_t)i * R + m];
       t2 = t2 + std::norm(d.rhs[(size_t)i * R + m]);
     }
-  double alpha = t1.real() / t2;
+  double alpha = d.mut_alpha * t1.real() / t2;
   for (int n = 0; n < N; n++) d.o[n] = d.o[n] - alpha * d.ot[n];
   // total-field update (mom.cpp:322-336)
   for (int i = 0; i < T; i++) {
@@ -812,7 +822,7 @@ double dbim_iterate(Oracle& o, Dbim& d) {
     n1 = n1 + d.od[n] * std::conj(d.od[n] - d.save[n]);
     n2 = n2 + std::norm(d.save[n]);
   }
-  cd beta = n1 / n2;
+  cd beta = d.mut_beta * n1 / n2;
   for (int n = 0; n < N; n++) d.ot[n] = d.od[n] + beta * d.ot[n];
   d.iter++;
   return alpha;
@@ -962,6 +972,13 @@ void* orc_dbim_create(void* h, int numtx, double regular, double tol, int maxit,
   return d;
 }
 void orc_dbim_destroy(void* d) { delete (Dbim*)d; }
+// test hook: plant a scale error in alpha, beta or the regulariser term
+void orc_dbim_mutate(void* dd, double alpha_scale, double beta_scale, double reg_scale) {
+  Dbim* d = (Dbim*)dd;
+  d->mut_alpha = alpha_scale;
+  d->mut_beta = beta_scale;
+  d->mut_reg = reg_scale;
+}
 double orc_dbim_iterate(void* h, void* dd) { return dbim_iterate(*(Oracle*)h, *(Dbim*)dd); }
 void orc_dbim_hist(void* dd, double* out) { memcpy(out, ((Dbim*)dd)->hist, sizeof(double) * 5); }
 void orc_dbim_get(void* h, void* dd, int which, double* out) {

@@ -78,6 +78,7 @@ def lib():
             "orc_dbim_create": (_V, [_V, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                      ctypes.c_int, _D]),
             "orc_dbim_destroy": (None, [_V]),
+            "orc_dbim_mutate": (None, [_V, ctypes.c_double, ctypes.c_double, ctypes.c_double]),
             "orc_dbim_iterate": (ctypes.c_double, [_V, _V]),
             "orc_dbim_hist": (None, [_V, _D]),
             "orc_dbim_get": (None, [_V, _V, ctypes.c_int, _D]),
@@ -303,6 +304,11 @@ class Dbim:
 
     def iterate(self):
         return self.o.L.orc_dbim_iterate(self.o.h, self.h)
+
+    def mutate(self, alpha=1.0, beta=1.0, reg=1.0):
+        """Plant a scale error in the step length, the Polak-Ribiere
+        coefficient or the regulariser's gradient term (tests of the pins)."""
+        self.o.L.orc_dbim_mutate(self.h, alpha, beta, reg)
 
     def hist(self):
         out = np.empty(5)
