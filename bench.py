@@ -4,30 +4,39 @@
 Workload (the headline configuration): 1024x1024 pixels of lambda/10
 (N = 1,048,576 unknowns, quadtree level 8, 8x8-pixel leaf boxes), 64
 line-source transmitters and 256 receivers on a circle of radius 60
-wavelengths (--radius 0: the reference's plane waves and far-field
-receivers), contrast = 20 seeded random ellipses (seed 3, eps_r - 1 in
-U(0.05, 0.8) + 0.02i), BiCGSTAB relative residual 1e-6 (iteration limit
-300), Tikhonov regular = sqrt(1e-3); 1 warm-up and 5 timed DBIM iterations.
+wavelengths, contrast = 20 seeded random ellipses (seed 3, eps_r - 1 in
+U(0.05, 0.8) + 0.02i), BiCGSTAB relative residual 1e-6 (iteration limit 300),
+Tikhonov regular = sqrt(1e-3), 5 DBIM iterations.
 
 One step = one DBIM iteration (mom.cpp:277-400): per transmitter a Frechet
 derivative solve, a total-field solve and an adjoint solve, plus the line
-search, gradient and Polak-Ribiere update.  Inputs are resident in HBM before
-the timed region; the timed region brackets K iterations with device
-synchronisation on both sides (each iteration ends with a stream sync).
+search, gradient and Polak-Ribiere update.  The steps cycle through the
+inversion's first 5 iterations from a cold start: after every 5th step the
+inversion returns to its initial state (zero object, isg_dbim_restart: device
+copies only, inside the timed region), so K steps with K a multiple of 5
+average whole 5-iteration inversions.  Inputs are resident in HBM before the
+timed region; every step ends with a stream synchronisation.
 
-Three paths run from the same synthetic measurements:
-  plain   FP64 operator, one right-hand side at a time, every solve from zero
-          -- the reference's loop on this library's kernels, each BiCGSTAB
-          iteration replayed as a HIP graph;
-  fast    every transmitter of a phase in one multi-RHS BiCGSTAB with
+The synthetic measurements are made once, before any timing, by the batched
+mixed-precision solver with an FP64 true-residual certification and an
+iteration limit raised for this synthesis only (--synth-iti), so that every
+forward solve of the true contrast reaches the 1e-6 tolerance (counts in the
+line's "synthesis").
+
+Paths, all from the same measurements:
+  learned (value)  every transmitter of a phase in one multi-RHS BiCGSTAB with
           deflation, FP32 operator with FP64 solver scalars/vectors and an FP64
           true-residual defect correction (every solve still reaches 1e-6 in
           FP64), FP64 gradient and derivative products, total-field solves
-          started from the previous DBIM total field moved to first order along
-          the step (--fast-start);
-  learned the fast path with the learned total-field start instead.
-`value` is the fast path's s/iteration; the plain and learned figures are
-reported beside it.
+          started from the warm-start network's prediction blended onto the
+          first-order start (isg_dbim field_start 3);
+  fast    the same with the first-order total-field start alone (no network);
+  plain   FP64 operator, one right-hand side at a time, every solve from zero:
+          the reference's loop on this library's kernels, each BiCGSTAB
+          iteration replayed as a HIP graph, the dense receiver operator
+          applied once per phase for all transmitters.
+The reference geometry (plane waves, far-field receivers; --radius 0) is timed
+as well, one 5-iteration cycle per path ("reference_geometry").
 
     python bench.py [--steps K] [--warmup W] [--gpus 1]
 """
@@ -46,8 +55,9 @@ import numpy as np  # noqa: E402
 METRIC = "s per DBIM iteration, 1024² px ×64 Tx, rel-res 1e-6: plain vs batched+learned"
 PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3}
 PEAK_HBM_GBS = 8000.0
+CYCLE = 5  # DBIM iterations of one inversion (the workload's 5)
 
-# total-field start of the fast path (DbimOptions::field_start)
+# total-field start (DbimOptions::field_start)
 START_KINDS = {"zero": 0, "previous": 1, "first-order": 2, "learned": 3}
 WARMNET_FILE = os.path.join(ROOT, "inverse-solver-gpu_amd", "weights", "warmnet_512.bin")
 
@@ -102,19 +112,34 @@ def m2l_bytes_per_rhs(level, numsamp, cbytes, fused_leaf=False):
 
 
 def run_path(solver, dbim, warmup, steps, label, timing=True):
-    for w in range(warmup):
+    """W untimed then K timed steps; step p is iteration p mod CYCLE + 1 of
+    the inversion, the state going back to the start before every
+    CYCLE-th step (inside the timed region when it falls there)."""
+    pos = [0]
+
+    def step():
+        if pos[0] > 0 and pos[0] % CYCLE == 0:
+            dbim.restart()
         st = dbim.iterate()
-        log("[%s] warmup iteration %d: %.3fs, %d solves, %d BiCGSTAB iterations, misfit %.4e"
-            % (label, w, st["seconds"], st["solves"], st["solve_iters"], st["misfit"]))
+        st["iteration"] = pos[0] % CYCLE + 1
+        pos[0] += 1
+        return st
+
+    for w in range(warmup):
+        st = step()
+        log("[%s] warmup step %d (iteration %d): %.3fs, %d solves, %d BiCGSTAB iterations, misfit %.4e"
+            % (label, w, st["iteration"], st["seconds"], st["solves"], st["solve_iters"], st["misfit"]))
     solver.profile(timing)
     c0 = solver.counters()
     t0 = time.perf_counter()
     stats = []
     for k in range(steps):
-        st = dbim.iterate()
+        st = step()
         stats.append(st)
-        log("[%s] timed iteration %d: %.3fs, %d solves, %d BiCGSTAB iterations, %d not converged, misfit %.4e"
-            % (label, k, st["seconds"], st["solves"], st["solve_iters"], st["not_converged"], st["misfit"]))
+        log("[%s] timed step %d (iteration %d): %.3fs, %d solves, %d BiCGSTAB iterations (%d total-field), "
+            "%d not converged, alpha %.3e, misfit %.4e"
+            % (label, k, st["iteration"], st["seconds"], st["solves"], st["solve_iters"], st["field_iters"],
+               st["not_converged"], st["alpha"], st["misfit"]))
     t1 = time.perf_counter()
     c1 = solver.counters()
     stats[-1]["refinements_timed"] = c1["refinements"] - c0["refinements"]
@@ -123,29 +148,139 @@ def run_path(solver, dbim, warmup, steps, label, timing=True):
     return (t1 - t0) / steps, stats, prof
 
 
-def cpu_baseline(level, numrx, matvecs_per_iter, nsample):
-    """Time the CPU oracle (FP64 C++/OpenMP restatement of the reference CPU
-    path) on `nsample` matvecs of the same size and scale by the matvec count
-    of one plain DBIM iteration measured on the GPU."""
-    from oracle.oracle import Oracle
-    o = Oracle(level, numrx=numrx)
-    rng = np.random.default_rng(0)
-    x = rng.standard_normal(o.N) + 1j * rng.standard_normal(o.N)
-    o.mlfma(x)  # first touch
+def misfit_by_iteration(stats):
+    """Misfit after each of the cycle's iterations (the last timed visit)."""
+    out = {}
+    for s in stats:
+        out[s["iteration"]] = s["misfit"]
+    return [out[k] for k in sorted(out)]
+
+
+def synthesize(isg, common, batch, otrue, iti):
+    """Measurements of the true contrast: batched mixed-precision forward
+    solves certified in FP64 to the tolerance, iteration limit `iti`."""
     t0 = time.perf_counter()
-    for _ in range(nsample):
-        o.mlfma(x)
-    t_mv = (time.perf_counter() - t0) / nsample
-    threads = o.L.orc_threads()
+    kw = dict(common)
+    kw["iti"] = iti
+    gen = isg.Solver(max_rhs=batch, precision=1, mixed_margin=0.5, **kw)
+    d = isg.Dbim(gen, otrue, field_start=1)
+    mes = d.get("mes")
+    s = d.setup_stats
+    info = {"solver": "batched, FP32 operator + FP64 true-residual defect correction to rel-res %g"
+                      % common["toli"],
+            "iteration_limit": iti, "solves": s["solves"], "bicgstab_iterations": s["solve_iters"],
+            "not_converged": s["not_converged"], "seconds": round(time.perf_counter() - t0, 2)}
+    log("measurements: %.2fs, %d solves, %d BiCGSTAB iterations, %d not converged (limit %d)"
+        % (info["seconds"], s["solves"], s["solve_iters"], s["not_converged"], iti))
+    numsamp = list(gen.numsamp)
+    del d
+    gen.close()
+    return mes, info, numsamp
+
+
+def run_paths(isg, args, wl, common, otrue, mes, paths, warmup, steps, timing):
+    """Time the requested paths from the same measurements; returns per-path
+    (s/step, stats, kernel timers), the contrasts and the leaf-fusion flag."""
+    batch = wl.get("batch", wl["numtx"])
+    fast_prec = 1 if args.fast_precision == "mixed" else 0
+    fast_kw = dict(max_rhs=batch, precision=fast_prec, mixed_margin=args.mixed_margin,
+                   products=1 if args.fast_products == "fp32" and fast_prec == 1 else 0,
+                   solve_start=1 if args.fast_solve_start == "previous" else 0)
+    res, contrast, errors = {}, {}, {}
+    fused = False
+    side = 8 << (wl["level"] - 1)
+    for p in paths:
+        if p == "plain":
+            s = isg.Solver(max_rhs=1, **common)
+            d = isg.Dbim(s, otrue, field_start=0, measurements=mes)
+            # timed at its best: without live kernel timers, so that each BiCGSTAB iteration
+            # replays as a HIP graph (its single-RHS kernels are short enough for launch
+            # overhead to show), unless it is the only path and supplies the roofline figures
+            ptime = timing and paths == ["plain"]
+        else:
+            s = isg.Solver(**fast_kw, **common)
+            start = "learned" if p == "learned" else args.fast_start
+            try:
+                d = isg.Dbim(s, otrue, field_start=START_KINDS[start], measurements=mes)
+                if start == "learned":
+                    d.set_warmnet(isg.WarmNet(max_batch=8, max_pixels=side * side, path=WARMNET_FILE))
+            except (isg.IsgError, OSError) as e:
+                errors[p] = str(e)
+                log("[%s] skipped: %s" % (p, e))
+                s.close()
+                continue
+            ptime = timing
+            fused = fused or s.tuning_get(fast_prec, 1, "leaf") == 1
+        res[p] = run_path(s, d, warmup, steps, p, ptime)
+        contrast[p] = d.get("o")
+        del d
+        s.close()
+    return res, contrast, fused, errors
+
+
+def cpu_baselines(isg, level, radius, o_rec, inc0, plain_iter_solves):
+    """The CPU oracle (FP64 C++/OpenMP restatement of the reference CPU path:
+    solve.cpp MLFMA, bicgs.cpp/bicgs_gpu.cu BiCGSTAB, mom.cpp DBIM) on the
+    host's cores: the 64^2 DBIM run in full, per-matvec and per-solve times at
+    256^2, and one 1024^2 solve of the benchmark's kind scaled to a DBIM
+    iteration."""
+    from oracle.oracle import Oracle, Dbim
+    out = {"nproc": os.cpu_count()}
+    # 64^2 DBIM: 16 Tx / 16 Rx on a 10-lambda circle, 3 blobs (seed 1), 3 iterations
+    o4 = Oracle(4, numrx=16, radius=10.0)
+    out["threads"] = o4.L.orc_threads()
+    ot4 = isg.raster_to_tree(isg.phantom_blobs(o4.side, 3, 1), o4.unkmap())
+    t0 = time.perf_counter()
+    d4 = Dbim(o4, 16, ot4, regular=float(np.sqrt(1e-3)), tol=1e-6, maxit=300)
+    t_setup = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for _ in range(3):
+        d4.iterate()
+    t_it = (time.perf_counter() - t0) / 3
+    out["dbim_64"] = {"s_per_iter": round(t_it, 4), "setup_s": round(t_setup, 4),
+                      "misfit_after_3": float(d4.hist()[0]), "kind": "measured, full run"}
+    # 256^2, 1 Tx: matvec and one forward solve of the 8-blob phantom (seed 2)
+    o6 = Oracle(6, numrx=16)
+    rng = np.random.default_rng(0)
+    x = rng.standard_normal(o6.N) + 1j * rng.standard_normal(o6.N)
+    o6.mlfma(x)
+    t0 = time.perf_counter()
+    for _ in range(5):
+        o6.mlfma(x)
+    t_mv6 = (time.perf_counter() - t0) / 5
+    ot6 = isg.raster_to_tree(isg.phantom_blobs(o6.side, 8, 2), o6.unkmap())
+    inc6 = np.exp(1j * o6.k0 * o6.pos().real)
+    t0 = time.perf_counter()
+    _, info6 = o6.bicgs(ot6 * o6.k0 ** 2, inc6, tol=1e-6, maxit=300)
+    t_solve6 = time.perf_counter() - t0
+    out["single_rhs_256"] = {"s_per_matvec": round(t_mv6, 5), "s_per_solve": round(t_solve6, 4),
+                             "solve_iterations": info6["iters"], "kind": "measured"}
+    # 1024^2: one forward solve with the benchmark's reconstructed contrast and transmitter 0's
+    # incident field, to 1e-6; a DBIM iteration has 3 solves per transmitter
+    del o4, o6, d4
+    o8 = Oracle(level, numrx=16, radius=radius)
+    o8.mlfma(np.zeros(o8.N, np.complex128))  # first touch
+    y = rng.standard_normal(o8.N) + 1j * rng.standard_normal(o8.N)
+    t0 = time.perf_counter()
+    o8.mlfma(y)
+    t_mv8 = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    _, info8 = o8.bicgs(o_rec * o8.k0 ** 2, inc0, tol=1e-6, maxit=300)
+    t_solve8 = time.perf_counter() - t0
+    out["solve_1024"] = {"s_per_matvec": round(t_mv8, 4), "s_per_solve": round(t_solve8, 3),
+                         "solve_iterations": info8["iters"], "kind": "measured"}
     return {
-        "value": round(t_mv * matvecs_per_iter, 2),
+        "value": round(t_solve8 * plain_iter_solves, 2),
         "unit": "s",
-        "cores": threads,
+        "cores": out["threads"],
         "kind": "port",
-        "sample": ("%d FP64 MLFMA matvecs at %dx%d on the CPU oracle (%.3f s each, %d OpenMP threads), "
-                   "scaled by the %d matvecs of one plain DBIM iteration counted on the GPU run; "
-                   "BiCGSTAB vector algebra and receiver operators not included (extrapolated)"
-                   % (nsample, 8 << (level - 1), 8 << (level - 1), t_mv, threads, matvecs_per_iter)),
+        "sample": ("one FP64 BiCGSTAB solve at %dx%d on the CPU oracle (%.2f s, %d iterations, %d OpenMP "
+                   "threads on %d host CPUs) with the reconstructed contrast and transmitter 0's incident field, "
+                   "x %d solves of one DBIM iteration (extrapolated: receivers and DBIM vector algebra "
+                   "not included); 64^2 DBIM and 256^2 matvec/solve timings measured in full below"
+                   % (o8.side, o8.side, t_solve8, info8["iters"], out["threads"], out["nproc"],
+                      plain_iter_solves)),
+        "detail": out,
     }
 
 
@@ -153,42 +288,45 @@ def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="dbim-1024", choices=sorted(WORKLOADS))
-    ap.add_argument("--paths", default="fast,plain,learned",
-                    help="fast: the headline path; plain: the reference's FP64 single-RHS path; learned: the "
-                         "fast path with the learned total-field start instead of the first-order one")
+    ap.add_argument("--paths", default="learned,fast,plain",
+                    help="learned: the headline path; fast: the same with the first-order total-field start "
+                         "alone; plain: the reference's FP64 single-RHS path")
     ap.add_argument("--fast-precision", default="mixed", choices=["mixed", "fp64"])
     ap.add_argument("--fast-start", default="first-order", choices=sorted(START_KINDS),
                     help="total-field start of the fast path")
     ap.add_argument("--fast-products", default="fp64", choices=["fp64", "fp32"],
-                    help="operator of the DBIM's derivative-source product on the fast path (fp32 needs "
+                    help="operator of the DBIM's derivative-source product on the batched paths (fp32 needs "
                          "--fast-precision mixed); the gradient's product is FP64 either way")
     ap.add_argument("--fast-solve-start", default="zero", choices=["zero", "previous"],
-                    help="adjoint and derivative-field solves of the fast path: zero (reference) or the "
-                         "previous DBIM iteration's solution scaled by the residual-minimising lambda "
-                         "(pays only where the misfit stalls)")
+                    help="adjoint and derivative-field solves of the batched paths: zero (reference) or the "
+                         "previous DBIM iteration's solution scaled by the residual-minimising lambda")
     ap.add_argument("--mixed-margin", type=float, default=0.9,
-                    help="FP32 solve target as a fraction of the tolerance (mixed precision); 0.9 needed no "
-                         "defect correction at 1024^2 (both geometries) and 2048^2, the library default is 0.5")
+                    help="FP32 solve target as a fraction of the tolerance (mixed precision); the library "
+                         "default is 0.5")
+    ap.add_argument("--synth-iti", type=int, default=3000,
+                    help="BiCGSTAB iteration limit of the measurement synthesis only (the timed solves keep 300)")
     ap.add_argument("--tuning", default="committed",
                     help="kernel-variant cache: 'committed' (inverse-solver-gpu_amd/tuning), 'none' or a path")
     ap.add_argument("--kernel-timing", default="on", choices=["on", "off"],
-                    help="live HIP-event kernel timing in the fast and learned paths' timed regions (off: "
-                         "BiCGSTAB iterations replay as HIP graphs, no roofline figures); the plain path "
-                         "replays graphs unless it is run without the fast path")
+                    help="live HIP-event kernel timing in the batched paths' timed regions (off: BiCGSTAB "
+                         "iterations replay as HIP graphs, no roofline figures)")
     ap.add_argument("--radius", type=float, default=None,
                     help="transmitter/receiver circle radius in wavelengths (0: plane waves, far-field "
                          "receivers; default: the workload's, 60 at 1024^2)")
-    ap.add_argument("--cpu-sample", type=int, default=2, help="oracle matvecs timed for cpu_baseline (0: skip)")
+    ap.add_argument("--ref-geometry", default="on", choices=["on", "off"],
+                    help="also time the learned and plain paths in the reference geometry (plane waves, "
+                         "far-field receivers), one 5-iteration cycle each")
+    ap.add_argument("--cpu-baseline", default="on", choices=["on", "off"])
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: there is no multi-GPU path (sharding the transmitters "
               "over RCCL is designed but not built); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     paths = [p for p in args.paths.split(",") if p]
-    if not set(paths) <= {"fast", "plain", "learned"} or not ({"fast", "plain"} & set(paths)):
-        print("--paths: a comma list of fast, plain, learned with fast or plain in it", file=sys.stderr)
+    if not paths or not set(paths) <= {"fast", "plain", "learned"}:
+        print("--paths: a comma list of learned, fast, plain", file=sys.stderr)
         sys.exit(2)
 
     import isg
@@ -197,76 +335,30 @@ def main():
         args.radius = wl["radius"]
     level, ntx, nrx = wl["level"], wl["numtx"], wl["numrx"]
     side = 8 << (level - 1)
+    batch = wl.get("batch", ntx)
     tuning = {"committed": True, "none": False}.get(args.tuning, args.tuning)
     common = dict(level=level, numrx=nrx, numtx=ntx, toli=1e-6, iti=300, regular=float(np.sqrt(1e-3)),
                   tuning=tuning, radius=args.radius)
     fast_prec = 1 if args.fast_precision == "mixed" else 0
+    timing = args.kernel_timing == "on"
 
-    # synthetic measurements of the true contrast, made once by the batched FP64 path
-    t0 = time.perf_counter()
-    batch = wl.get("batch", ntx)
-    gen = isg.Solver(max_rhs=batch, **common)
-    pos, r2t = gen.geometry()
+    probe = isg.Solver(max_rhs=1, level=level, numrx=nrx, numtx=ntx, tuning=False, radius=args.radius)
+    _, r2t = probe.geometry()
+    probe.close()
     otrue = isg.raster_to_tree(isg.phantom_ellipses(side, wl["nell"], wl["seed"]), r2t)
-    d_gen = isg.Dbim(gen, otrue, field_start=1)
-    mes = d_gen.get("mes")
-    log("measurements (FP64, batched): %.2fs, %d solves, %d BiCGSTAB iterations, %d not converged"
-        % (time.perf_counter() - t0, d_gen.setup_stats["solves"], d_gen.setup_stats["solve_iters"],
-           d_gen.setup_stats["not_converged"]))
-    S_leaf = gen.numsamp[-1]
-    numsamp = list(gen.numsamp)
-    del d_gen
-    gen.close()
+    mes, synth, numsamp = synthesize(isg, common, batch, otrue, args.synth_iti)
+    S_leaf = numsamp[-1]
     flops_mv, pairs = near_flops_per_rhs(level, S_leaf)
 
-    res = {}
-    if "fast" in paths:
-        fast = isg.Solver(max_rhs=batch, precision=fast_prec, mixed_margin=args.mixed_margin,
-                          products=1 if args.fast_products == "fp32" and fast_prec == 1 else 0,
-                          solve_start=1 if args.fast_solve_start == "previous" else 0, **common)
-        d_fast = isg.Dbim(fast, otrue, field_start=START_KINDS[args.fast_start], measurements=mes)
-        if args.fast_start == "learned":
-            d_fast.set_warmnet(isg.WarmNet(max_batch=8, max_pixels=side * side, path=WARMNET_FILE))
-        res["fast"] = run_path(fast, d_fast, args.warmup, args.steps, "fast", args.kernel_timing == "on")
-        contrast = {"fast": d_fast.get("o")}
-        fast_leaf_fused = fast.tuning_get(fast_prec, 1, "leaf") == 1
-        del d_fast
-        fast.close()
-    learned_error = None
-    if "learned" in paths:
-        # a side figure: a failure here (e.g. missing network weights) is reported in the
-        # line instead of costing the headline measurement
-        lrn = None
-        try:
-            lrn = isg.Solver(max_rhs=batch, precision=fast_prec, mixed_margin=args.mixed_margin,
-                             products=1 if args.fast_products == "fp32" and fast_prec == 1 else 0,
-                             solve_start=1 if args.fast_solve_start == "previous" else 0, **common)
-            d_lrn = isg.Dbim(lrn, otrue, field_start=START_KINDS["learned"], measurements=mes)
-            d_lrn.set_warmnet(isg.WarmNet(max_batch=8, max_pixels=side * side, path=WARMNET_FILE))
-            res["learned"] = run_path(lrn, d_lrn, args.warmup, args.steps, "learned", args.kernel_timing == "on")
-            del d_lrn
-        except (isg.IsgError, OSError) as e:
-            learned_error = str(e)
-            log("[learned] skipped: %s" % e)
-        if lrn is not None:
-            lrn.close()
-    if "plain" in paths:
-        plain = isg.Solver(max_rhs=1, **common)
-        d_plain = isg.Dbim(plain, otrue, field_start=0, measurements=mes)
-        # the plain path is timed at its best: without live kernel timing, so that each BiCGSTAB
-        # iteration replays as a HIP graph (its single-RHS kernels are short enough for launch
-        # overhead to show: measured 1.002 vs 1.022 s per DBIM iteration); it keeps the timers
-        # only when it is the line's main path and supplies the roofline figures
-        plain_timing = args.kernel_timing == "on" and "fast" not in paths
-        res["plain"] = run_path(plain, d_plain, args.warmup, args.steps, "plain", plain_timing)
-        contrast_plain = d_plain.get("o")
-        del d_plain
-        plain.close()
+    res, contrast, fast_leaf_fused, errors = run_paths(isg, args, wl, common, otrue, mes, paths,
+                                                       args.warmup, args.steps, timing)
+    if not res:
+        print(json.dumps({"metric": METRIC, "value": None, "error": errors}), flush=True)
+        sys.exit(1)
 
-    main_path = "fast" if "fast" in res else "plain"
-    fast_leaf_fused = "fast" in res and fast_leaf_fused
+    main_path = next(p for p in ("learned", "fast", "plain") if p in res)
     s_iter, stats, prof = res[main_path]
-    mixed = main_path == "fast" and fast_prec == 1
+    mixed = main_path != "plain" and fast_prec == 1
     matvecs = sum(s["matvecs"] for s in stats)
     mv32 = sum(s["matvecs_fp32"] for s in stats)
     if mixed:
@@ -279,11 +371,24 @@ def main():
         n_near, prec, cb = matvecs, "f64", 16
     vec_ms, vec_n = prof["bicgs_update"]
     achieved = n_near * flops_mv / (near_ms * 1e-3) / 1e12 if near_ms > 0 else 0.0
-    fused = main_path == "fast" and fast_leaf_fused
+    fused = main_path != "plain" and fast_leaf_fused
     m2l_gbs = (n_near * m2l_bytes_per_rhs(level, numsamp, cb, fused) / (m2l_ms * 1e-3) / 1e9
                if m2l_ms > 0 else 0.0)
     iters = sum(s["solve_iters"] for s in stats)
     traffic = measured_traffic() if (mixed and level == 8 and ntx == 64) else None
+    geom = (("on a circle of radius %g lambda (line sources)" % args.radius) if args.radius > 0
+            else "(plane waves, far-field receivers)")
+
+    def path_desc(p):
+        if p == "plain":
+            return "1 RHS, FP64, zero start"
+        return ("batched %d RHS + deflation, %s, %s total-field start, %s adjoint/derivative starts, "
+                "%s derivative-source product"
+                % (batch, ("FP32 operator / FP64 solver (FP32 solves to %g x tol) + FP64 defect correction"
+                           % args.mixed_margin) if fast_prec == 1 else "FP64",
+                   "learned (network + residual-minimising blend onto the first-order start)" if p == "learned"
+                   else args.fast_start, args.fast_solve_start,
+                   "FP32" if args.fast_products == "fp32" and fast_prec == 1 else "FP64"))
 
     out = {
         "metric": METRIC if args.workload == "dbim-1024" else METRIC.replace("1024² px ×64 Tx", "%d² px ×%d Tx" % (side, ntx)),
@@ -297,19 +402,12 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32 operator + f64 solver" if mixed else "f64",
-        "data": "synthetic (seeded ellipse phantom, measurements synthesized by the FP64 forward solver)",
+        "data": "synthetic (seeded ellipse phantom, measurements synthesized by the solver, every solve to 1e-6)",
         "config": {
-            "workload": "DBIM %dx%d px lambda/10, level %d, %d Tx, %d Rx %s, %d ellipses seed %d, rel-res 1e-6"
-                        % (side, side, level, ntx, nrx,
-                           ("on a circle of radius %g lambda (line sources)" % args.radius) if args.radius > 0
-                           else "(plane waves, far-field receivers)", wl["nell"], wl["seed"]),
-            "path": ("batched %d RHS + deflation, %s, %s total-field start, %s adjoint/derivative starts, "
-                     "%s derivative-source product"
-                     % (batch, ("FP32 operator / FP64 solver (FP32 solves to %g x tol) + FP64 defect correction"
-                                % args.mixed_margin) if mixed else "FP64",
-                        args.fast_start, args.fast_solve_start,
-                        "FP32" if args.fast_products == "fp32" and mixed else "FP64")
-                     if main_path == "fast" else "1 RHS, FP64, zero start"),
+            "workload": "DBIM %dx%d px lambda/10, level %d, %d Tx, %d Rx %s, %d ellipses seed %d, rel-res 1e-6, "
+                        "steps cycle through iterations 1-%d from a cold start"
+                        % (side, side, level, ntx, nrx, geom, wl["nell"], wl["seed"], CYCLE),
+            "path": main_path + ": " + path_desc(main_path),
             "kernel_tuning": args.tuning,
         },
         "roofline": {
@@ -342,34 +440,61 @@ def main():
         "matvecs": matvecs,
         "matvecs_fp32": mv32,
         "defect_corrections": stats[-1].get("refinements_timed"),
-        "misfit": stats[-1]["misfit"],
+        "misfit_by_iteration": misfit_by_iteration(stats),
+        "synthesis": synth,
     }
-    if "plain" in res and "fast" in res:
-        out["plain_s_per_iter"] = round(res["plain"][0], 4)
+    if main_path == "learned":
+        out["network_s_per_iter"] = round(sum(s["net_seconds"] for s in stats) / len(stats), 4)
+        out["mean_abs_lambda"] = round(sum(s["lambda_abs"] for s in stats) / len(stats), 4)
+    for p in ("fast", "plain"):
+        if p in res and p != main_path:
+            ps = res[p][1]
+            out[p] = {"s_per_iter": round(res[p][0], 4), "path": path_desc(p),
+                      "bicgstab_iterations": sum(s["solve_iters"] for s in ps),
+                      "field_bicgstab_iterations": sum(s["field_iters"] for s in ps),
+                      "misfit_by_iteration": misfit_by_iteration(ps)}
+            if main_path in contrast and p in contrast:
+                out[p]["contrast_rel_diff_vs_%s" % main_path] = float(
+                    np.linalg.norm(contrast[main_path] - contrast[p]) / np.linalg.norm(contrast[p]))
+    if "plain" in res and main_path != "plain":
         out["speedup_vs_plain"] = round(res["plain"][0] / s_iter, 3)
-        out["plain_misfit"] = res["plain"][1][-1]["misfit"]
-        # reconstruction after the same iterations: fast path against the FP64 plain path
-        out["contrast_rel_diff_vs_plain"] = float(np.linalg.norm(contrast["fast"] - contrast_plain)
-                                                  / np.linalg.norm(contrast_plain))
-    if learned_error is not None:
-        out["learned_start"] = {"error": learned_error}
-    if "learned" in res:
-        lst = res["learned"][1]
-        out["learned_start"] = {
-            "s_per_iter": round(res["learned"][0], 4),
-            "speedup_vs_plain": round(res["plain"][0] / res["learned"][0], 3) if "plain" in res else None,
-            "field_bicgstab_iterations": sum(s["field_iters"] for s in lst),
-            "network_s_per_iter": round(sum(s["net_seconds"] for s in lst) / len(lst), 4),
-            "mean_abs_lambda": round(sum(s["lambda_abs"] for s in lst) / len(lst), 4),
-            "note": "the fast path with the learned total-field start (network + residual-minimising blend) "
-                    "in place of the first-order start; field_bicgstab_iterations compares with the line's",
-        }
-    if args.cpu_sample > 0 and "plain" in res:
-        plain_mv = int(round(sum(s["matvecs"] for s in res["plain"][1]) / len(res["plain"][1])))
+        if "fast" in res:
+            out["fast"]["speedup_vs_plain"] = round(res["plain"][0] / res["fast"][0], 3)
+    if errors:
+        out["errors"] = errors
+
+    if args.ref_geometry == "on" and args.radius > 0:
+        # the reference's own geometry: plane-wave transmitters, far-field receivers
+        rg_common = dict(common)
+        rg_common["radius"] = 0.0
+        rg_mes, rg_synth, _ = synthesize(isg, rg_common, batch, otrue, args.synth_iti)
+        rg_paths = [p for p in ("learned", "plain") if p in paths]
+        rres, rcon, _, rerr = run_paths(isg, args, wl, rg_common, otrue, rg_mes, rg_paths, 1, CYCLE, False)
+        rg = {"note": "plane waves and far-field receivers (the reference's geometry), one warm-up step and "
+                      "one 5-iteration cycle per path, no live kernel timers", "synthesis": rg_synth}
+        for p in rres:
+            rg[p + "_s_per_iter"] = round(rres[p][0], 4)
+            rg[p + "_misfit_by_iteration"] = misfit_by_iteration(rres[p][1])
+        if "learned" in rres and "plain" in rres:
+            rg["speedup_vs_plain"] = round(rres["plain"][0] / rres["learned"][0], 3)
+            rg["contrast_rel_diff"] = float(np.linalg.norm(rcon["learned"] - rcon["plain"])
+                                            / np.linalg.norm(rcon["plain"]))
+        if rerr:
+            rg["errors"] = rerr
+        out["reference_geometry"] = rg
+
+    if args.cpu_baseline == "on":
         try:
-            out["cpu_baseline"] = cpu_baseline(level, nrx, plain_mv, args.cpu_sample)
+            o_rec = contrast[main_path]
+            # transmitter 0's incident field from a context of the same geometry
+            s0 = isg.Solver(max_rhs=1, **common)
+            d0 = isg.Dbim(s0, otrue, field_start=0, measurements=mes)
+            inc0 = d0.get("inc")[: side * side].copy()
+            del d0
+            s0.close()
+            out["cpu_baseline"] = cpu_baselines(isg, level, args.radius, o_rec, inc0, 3 * ntx)
         except Exception as e:  # the CPU oracle needs its .so; report instead of failing the bench
-            out["cpu_baseline"] = {"error": str(e)}
+            out["cpu_baseline"] = {"error": "%s: %s" % (type(e).__name__, e)}
     print(json.dumps(out), flush=True)
 
 

@@ -193,6 +193,13 @@ int isg_dbim_setup_measured(isg_context* ctx, const double* mes, const double* o
                             isg_dbim_stats* st);
 /* One DBIM iteration (mom.cpp:277-400). */
 int isg_dbim_iterate(isg_context* ctx, isg_dbim_stats* st);
+/* Back to the state isg_dbim_setup* left (zero object, total fields = incident
+ * fields, mesd = -mes, the first gradient and direction kept from setup rather
+ * than recomputed, no previous solutions), so that the next isg_dbim_iterate
+ * is iteration 1 again (mom.cpp:222-276 without its adjoint solves; no
+ * counterpart in the reference, whose loop runs once: repeated timed runs of
+ * the first iterations). */
+int isg_dbim_restart(isg_context* ctx);
 /* which: 0 o, 1 od (gradient), 2 ot (direction), 3 mes, 4 mesd, 5 inc, 6 tot,
  * 7 true contrast (get only).
  * Vectors are numunk (x numtx for inc/tot) complex; mes/mesd numtx x numrx.

@@ -115,6 +115,8 @@ int isg_create(const isg_config* cfg, isg_context** out) {
   ec.precision = cfg->precision;
   ec.mixed_margin = cfg->mixed_margin;
   ec.graph = cfg->graph != 0;
+  // the DBIM driver applies the receiver operator once per phase for every transmitter
+  ec.rx_rhs = std::max(cfg->max_rhs, cfg->numtx);
   c->eng.reset(new isg::Engine(ec));
   if (!check(c)) {
     *out = c;  // keep the context so the caller can read the error
@@ -309,9 +311,12 @@ int isg_bicgs_dev(isg_context* c, void* x, const void* o, long o_stride, const v
 int isg_stage(isg_context* c, int which, int level, int rhs, double* out) {
   if (!check(c)) return ISG_E_STATE;
   if (!out || level < 0 || level >= c->eng->plan().L) return fail(c, ISG_E_ARG, "bad argument");
+  if (which != 0 && which != 1) return fail(c, ISG_E_ARG, "which must be 0 (multipoles) or 1 (locals)");
+  if (rhs < 0 || rhs >= c->eng->max_rhs()) return fail(c, ISG_E_ARG, "rhs out of range [0, max_rhs)");
   std::vector<isg::zd> v;
   c->eng->stage(which, level, rhs, v);
   if (!check(c)) return ISG_E_DEVICE;
+  if (v.empty()) return fail(c, ISG_E_STATE, "stage store unavailable");
   std::memcpy(out, v.data(), v.size() * sizeof(isg::zd));
   return ISG_OK;
 }
@@ -445,6 +450,14 @@ int isg_dbim_iterate(isg_context* c, isg_dbim_stats* st) {
   c->dbim->iterate(s);
   if (!c->dbim->ok()) return fail(c, ISG_E_DEVICE, c->dbim->error());
   fill_stats(s, st);
+  return ISG_OK;
+}
+
+int isg_dbim_restart(isg_context* c) {
+  if (!check(c)) return ISG_E_STATE;
+  if (!c->dbim) return fail(c, ISG_E_STATE, "isg_dbim_setup has not been called");
+  c->dbim->restart();
+  if (!c->dbim->ok()) return fail(c, ISG_E_DEVICE, c->dbim->error());
   return ISG_OK;
 }
 

@@ -98,6 +98,11 @@ __global__ void __launch_bounds__(kBlock) k_grad_acc(zd* buff, const zd* tot, co
   }
 }
 
+// b = -a
+__global__ void __launch_bounds__(kBlock) k_neg(const zd* a, zd* b, long N) {
+  DB_LOOP(N) b[i] = mkc<double>(-a[i].x, -a[i].y);
+}
+
 // a = a + s * b  (complex s)
 __global__ void __launch_bounds__(kBlock) k_axpy(zd* a, const zd* b, zd s, long N) {
   DB_LOOP(N) a[i] = cadd(a[i], cmul(s, b[i]));
@@ -207,6 +212,7 @@ Dbim::Dbim(Engine& eng, const DbimOptions& opt) : eng_(eng), opt_(opt) {
   omes_ = (zd*)dalloc(vb);
   g_ = (zd*)dalloc(vb);
   buff_ = (zd*)dalloc(vb);
+  od0_ = (zd*)dalloc(vb);
   inc_ = (zd*)dalloc(vb * T_);
   tot_ = (zd*)dalloc(vb * T_);
   mes_ = (zd*)dalloc(sizeof(zd) * (size_t)T_ * R_);
@@ -215,6 +221,13 @@ Dbim::Dbim(Engine& eng, const DbimOptions& opt) : eng_(eng), opt_(opt) {
   wy_ = (zd*)dalloc(vb * B_);
   wc_ = (zd*)dalloc(vb * B_);
   wx_ = (zd*)dalloc(vb * B_);
+  if (P.cfg.radius > 0 && B_ < T_) {
+    if (eng_.rx_rhs() < T_) {
+      err_ = "receiver operator capacity below numtx";
+      return;
+    }
+    rxw_ = (zd*)dalloc(vb * T_);
+  }
   if (opt_.field_start == 2 || opt_.solve_start == 1) dtot_ = (zd*)dalloc(vb * T_);
   if (opt_.solve_start == 1) adj_ = (zd*)dalloc(vb * T_);
   nblk_ = (int)std::min<long>(1024, std::max<long>(1, (N_ + kBlock * 4 - 1) / (kBlock * 4)));
@@ -267,11 +280,16 @@ void Dbim::gradient(DbimStats& st) {
   hipStream_t s = eng_.stream();
   const long N = N_;
   DB_HIP(hipMemsetAsync(buff_, 0, sizeof(zd) * N, s));
+  // the receiver back-projection of every transmitter in one call when the
+  // transmitters run in several batches (the dense finite-radius operator is
+  // then read once per phase)
+  if (rxw_) eng_.aggregateh(mesd_, rxw_, T_);
   for (int t0 = 0; t0 < T_; t0 += B_) {
     int nb = std::min(B_, T_ - t0);
     dim3 grid(nblk_, nb), blk(kBlock);
-    eng_.aggregateh(mesd_ + (size_t)t0 * R_, wy_, nb);
-    k_adj_src<<<grid, blk, 0, s>>>(g_, wy_, wc_, N);
+    zd* wy = rxw_ ? rxw_ + (size_t)t0 * N : wy_;
+    if (!rxw_) eng_.aggregateh(mesd_ + (size_t)t0 * R_, wy, nb);
+    k_adj_src<<<grid, blk, 0, s>>>(g_, wy, wc_, N);
     if (adj_ && adj_ok_) {
       solve_from(wx_, adj_ + (size_t)t0 * N, wc_, true, nb, st);
     } else {
@@ -289,7 +307,7 @@ void Dbim::gradient(DbimStats& st) {
       eng_.mlfma(wc_, wx_, nb, nullptr, PREC_FP64);
       gcx = wx_;
     }
-    k_grad_acc<<<dim3(nblk_), blk, 0, s>>>(buff_, tot_ + (size_t)t0 * N, wy_, gcx, o_, N, nb,
+    k_grad_acc<<<dim3(nblk_), blk, 0, s>>>(buff_, tot_ + (size_t)t0 * N, wy, gcx, o_, N, nb,
                                            opt_.regular * opt_.regular);
   }
   adj_ok_ = adj_ != nullptr;
@@ -331,9 +349,11 @@ void Dbim::setup(const zd* otrue, DbimStats& st) {
     int nb = std::min(B_, T_ - t0);
     DB_HIP(hipMemsetAsync(wx_, 0, sizeof(zd) * N * nb, s));
     solve(wx_, inc_ + (size_t)t0 * N, false, nb, true, st);
-    k_rowmul<<<dim3(nblk_, nb), blk, 0, s>>>(wx_, g_, wy_, N);
-    eng_.aggregate(wy_, mes_ + (size_t)t0 * R_, nb);
+    zd* wy = rxw_ ? rxw_ + (size_t)t0 * N : wy_;
+    k_rowmul<<<dim3(nblk_, nb), blk, 0, s>>>(wx_, g_, wy, N);
+    if (!rxw_) eng_.aggregate(wy, mes_ + (size_t)t0 * R_, nb);
   }
+  if (rxw_) eng_.aggregate(rxw_, mes_, T_);
   std::vector<zd> mes((size_t)T_ * R_);
   DB_HIP(hipMemcpyAsync(mes.data(), mes_, sizeof(zd) * mes.size(), hipMemcpyDeviceToHost, s));
   DB_HIP(hipStreamSynchronize(s));
@@ -377,9 +397,27 @@ void Dbim::start(const zd* mes_host, const zd* otrue, DbimStats& st) {
   gradient(st);
   metrics(st);
   DB_HIP(hipMemcpyAsync(ot_, od_, sizeof(zd) * N, hipMemcpyDeviceToDevice, s));
+  DB_HIP(hipMemcpyAsync(od0_, od_, sizeof(zd) * N, hipMemcpyDeviceToDevice, s));
   DB_HIP(hipStreamSynchronize(s));
   iter_ = 0;
   st.matvecs = eng_.matvecs;
+}
+
+void Dbim::restart() {
+  if (!ok()) return;
+  hipStream_t s = eng_.stream();
+  const long N = N_;
+  const size_t vb = sizeof(zd) * (size_t)N;
+  k_neg<<<dim3((unsigned)std::min<long>(1024, ((long)T_ * R_ + kBlock - 1) / kBlock)), dim3(kBlock), 0, s>>>(mes_, mesd_, (long)T_ * R_);
+  DB_HIP(hipMemsetAsync(o_, 0, vb, s));
+  DB_HIP(hipMemsetAsync(g_, 0, vb, s));
+  DB_HIP(hipMemsetAsync(save_, 0, vb, s));
+  DB_HIP(hipMemcpyAsync(tot_, inc_, vb * T_, hipMemcpyDeviceToDevice, s));
+  DB_HIP(hipMemcpyAsync(od_, od0_, vb, hipMemcpyDeviceToDevice, s));
+  DB_HIP(hipMemcpyAsync(ot_, od0_, vb, hipMemcpyDeviceToDevice, s));
+  DB_HIP(hipStreamSynchronize(s));
+  adj_ok_ = dtot_ok_ = false;
+  iter_ = 0;
 }
 
 void Dbim::iterate(DbimStats& st) {
@@ -393,11 +431,12 @@ void Dbim::iterate(DbimStats& st) {
   for (int t0 = 0; t0 < T_; t0 += B_) {
     int nb = std::min(B_, T_ - t0);
     dim3 grid(nblk_, nb);
-    k_deriv_src<<<grid, blk, 0, s>>>(tot_ + (size_t)t0 * N, ot_, wy_, N, k0_);
+    zd* wy = rxw_ ? rxw_ + (size_t)t0 * N : wy_;
+    k_deriv_src<<<grid, blk, 0, s>>>(tot_ + (size_t)t0 * N, ot_, wy, N, k0_);
     // opt_.products picks the operator: the FP32 one has a relative error of 1.3e-6 - 2.4e-6
     // against the FP64 one (the order of the solves' tolerance), most of it from the FP32 sums
     // of P2M and M2L, amplified by the leaf-level translations
-    eng_.mlfma(wy_, wc_, nb, nullptr, opt_.products);
+    eng_.mlfma(wy, wc_, nb, nullptr, opt_.products);
     zd* dx = dtot_ ? dtot_ + (size_t)t0 * N : wx_;  // kept for the first-order field start
     if (opt_.solve_start == 1 && dtot_ok_) {
       solve_from(wx_, dx, wc_, false, nb, st);
@@ -405,9 +444,10 @@ void Dbim::iterate(DbimStats& st) {
       DB_HIP(hipMemsetAsync(dx, 0, sizeof(zd) * N * nb, s));
       solve(dx, wc_, false, nb, true, st);
     }
-    k_add_gx<<<grid, blk, 0, s>>>(wy_, g_, dx, N);
-    eng_.aggregate(wy_, rhs_ + (size_t)t0 * R_, nb);
+    k_add_gx<<<grid, blk, 0, s>>>(wy, g_, dx, N);
+    if (!rxw_) eng_.aggregate(wy, rhs_ + (size_t)t0 * R_, nb);
   }
+  if (rxw_) eng_.aggregate(rxw_, rhs_, T_);
   dtot_ok_ = opt_.solve_start == 1;
   std::vector<zd> rhs((size_t)T_ * R_), mesd((size_t)T_ * R_);
   DB_HIP(hipMemcpyAsync(rhs.data(), rhs_, sizeof(zd) * rhs.size(), hipMemcpyDeviceToHost, s));
@@ -440,9 +480,11 @@ void Dbim::iterate(DbimStats& st) {
     } else {
       solve(tot, inc_ + (size_t)t0 * N, false, nb, zero, st);
     }
-    k_rowmul<<<dim3(nblk_, nb), blk, 0, s>>>(tot, g_, wx_, N);
-    eng_.aggregate(wx_, rhs_ + (size_t)t0 * R_, nb);
+    zd* wx = rxw_ ? rxw_ + (size_t)t0 * N : wx_;
+    k_rowmul<<<dim3(nblk_, nb), blk, 0, s>>>(tot, g_, wx, N);
+    if (!rxw_) eng_.aggregate(wx, rhs_ + (size_t)t0 * R_, nb);
   }
+  if (rxw_) eng_.aggregate(rxw_, rhs_, T_);
   DB_HIP(hipStreamSynchronize(s));
   st.field_iters = st.solve_iters - it_field;
   st.field_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_field).count();

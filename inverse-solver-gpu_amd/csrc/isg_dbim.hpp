@@ -62,6 +62,10 @@ class Dbim {
   void setup_measured(const zd* mes_host, const zd* otrue_host, DbimStats& st);
   // One DBIM iteration (mom.cpp:277-400).
   void iterate(DbimStats& st);
+  // Back to the state right after setup: zero object, total fields equal to
+  // the incident fields, mesd = -mes, the first gradient and direction (kept
+  // from setup, not recomputed), no previous solutions.  Device copies only.
+  void restart();
   // which: 0 o, 1 od, 2 ot, 3 mes, 4 mesd, 5 inc, 6 tot, 7 true contrast; copies to host
   bool get(int which, std::vector<zd>& out);
   bool set(int which, const std::vector<zd>& in);
@@ -90,6 +94,11 @@ class Dbim {
   zd *inc_ = nullptr, *tot_ = nullptr;
   zd *mes_ = nullptr, *mesd_ = nullptr, *rhs_ = nullptr;
   zd *wy_ = nullptr, *wc_ = nullptr, *wx_ = nullptr;
+  // finite-radius receivers with several transmitter batches: the receiver
+  // operator's inputs (or aggregateh's outputs) of every transmitter [numtx][N],
+  // so that the dense operator is applied once per phase, not once per batch
+  zd* rxw_ = nullptr;
+  zd* od0_ = nullptr;   // first gradient (restart)
   zd* dtot_ = nullptr;  // [numtx][N] derivative fields (field_start 2 or solve_start 1)
   zd* adj_ = nullptr;   // [numtx][N] adjoint solutions (solve_start 1)
   bool dtot_ok_ = false, adj_ok_ = false;  // the stores hold a previous iteration's solutions

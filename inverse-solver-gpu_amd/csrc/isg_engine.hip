@@ -248,7 +248,7 @@ Engine::Engine(const EngineConfig& cfg) : cfg_(cfg) {
 // Finite-radius receivers: the numrx x N operator on the device (FP64) and a
 // rocBLAS handle on the engine stream.
 void Engine::setup_receivers() {
-  const int Rx = cfg_.plan.numrx, R = cfg_.max_rhs;
+  const int Rx = cfg_.plan.numrx, R = rx_rhs();
   const long N = P_.N;
   rxm_ = (zd*)dalloc((size_t)Rx * N * sizeof(zd));
   zd* pos = (zd*)dalloc((size_t)N * sizeof(zd));

@@ -14,6 +14,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -32,6 +33,10 @@ struct EngineConfig {
   bool graph = true;          // replay each BiCGSTAB iteration as one captured HIP graph
                               // (not while live kernel timing is on: HIP events recorded
                               // by graph nodes carry no timestamps)
+  int rx_rhs = 0;             // finite-radius receivers: right-hand sides one aggregate /
+                              // aggregateh call may take (at least max_rhs), so that a
+                              // caller solving one transmitter at a time can still apply
+                              // the dense receiver operator once for all of them
 };
 
 // Result of one BiCGSTAB solve (one right-hand side).
@@ -75,6 +80,8 @@ class Engine {
   const Plan& plan() const { return P_; }
   hipStream_t stream() const { return stream_; }
   int max_rhs() const { return cfg_.max_rhs; }
+  // right-hand sides one aggregate / aggregateh call accepts
+  int rx_rhs() const { return cfg_.plan.radius > 0 ? std::max(cfg_.max_rhs, cfg_.rx_rhs) : cfg_.max_rhs; }
   bool has_mixed() const { return has32_; }
 
   // r = G x for nrhs right-hand sides (device pointers, FP64, [nrhs][N]).

@@ -87,12 +87,18 @@ void aggregateh(complex<double>* x, complex<double>* r) {
   if (rc != ISG_OK) die("isg_aggregateh", rc);
 }
 
-// bicgs_gpu.cu:116-224: non-convergence prints, as the reference does.
+// bicgs_gpu.cu:116-224: prints what the reference prints -- non-convergence,
+// then the iteration count and the solve's matrix-vector products
+// (bicgs_gpu.cu:219-220).
 void bicgs(complex<double>* x, complex<double>* o, complex<double>* b, bool herm) {
   isg_solve_info info;
+  long long c0[5], c1[5];
+  isg_counters(g_ctx, c0);
   int rc = isg_bicgs(g_ctx, reinterpret_cast<double*>(x), reinterpret_cast<const double*>(o),
                      reinterpret_cast<const double*>(b), herm ? 1 : 0, &info);
   if (rc < 0) die("isg_bicgs", rc);
+  isg_counters(g_ctx, c1);
   if (info.status != ISG_SOLVE_CONVERGED)
     printf("NOT CONVERGED! status %d iter: %d relres %e\n", info.status, info.iters, info.relres);
+  printf("NUMBER OF %d ITERATIONS: %d (%lld)\n", herm ? 1 : 0, info.iters, c1[0] - c0[0]);
 }

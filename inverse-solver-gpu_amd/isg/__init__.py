@@ -110,6 +110,7 @@ _SIGS = {
     "isg_profile": (ctypes.c_int, [_V, ctypes.c_int]),
     "isg_profile_read": (ctypes.c_int, [_V, ctypes.c_int, _D, _P(ctypes.c_longlong)]),
     "isg_dbim_iterate": (ctypes.c_int, [_V, _P(DbimStats)]),
+    "isg_dbim_restart": (ctypes.c_int, [_V]),
     "isg_dbim_get": (ctypes.c_int, [_V, ctypes.c_int, _D]),
     "isg_dbim_set": (ctypes.c_int, [_V, ctypes.c_int, _D]),
     "isg_warmnet_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_long, ctypes.c_ulonglong, _P(_V)]),
@@ -394,6 +395,11 @@ class Dbim:
         st = DbimStats()
         self.s._err(self.s.lib.isg_dbim_iterate(self.s.h, ctypes.byref(st)), "isg_dbim_iterate")
         return _stats_dict(st)
+
+    def restart(self):
+        """Back to the state after setup: the next iterate() is iteration 1
+        again (the first gradient is kept, not recomputed)."""
+        self.s._err(self.s.lib.isg_dbim_restart(self.s.h), "isg_dbim_restart")
 
     def _size(self, name):
         T, R, N = self.s.cfg.numtx, self.s.cfg.numrx, self.s.N

@@ -20,7 +20,7 @@ def test_more_than_one_gpu_refused():
 
 
 def test_unknown_path_refused():
-    for paths in ("fast,turbo", "learned"):
+    for paths in ("fast,turbo", "first-order", ","):
         p = run("--paths", paths)
         assert p.returncode == 2, paths
         assert "--paths" in p.stderr

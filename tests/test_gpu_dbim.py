@@ -74,6 +74,28 @@ def test_dbim_checkpoint_resume(case):
     assert relerr(b.get("o"), a.get("o")) < 1e-12
 
 
+@pytest.mark.parametrize("field_start,solve_start", [(0, 0), (2, 1)])
+def test_dbim_restart_repeats_iterations(case, field_start, solve_start):
+    """isg_dbim_restart returns to the state after setup: iterations 1 and 2
+    after a restart reproduce the first run's iterations 1 and 2 bitwise
+    (also with the first-order start and the previous-solution starts, whose
+    stores the restart drops)."""
+    isg, otrue, ref, _ = case
+    kw = dict(level=LEVEL, numrx=NRX, numtx=NTX, toli=TOL, iti=300, regular=REG, max_rhs=16, precision=1,
+              solve_start=solve_start)
+    d = isg.Dbim(isg.Solver(**kw), otrue, field_start=field_start)
+    first = []
+    for _ in range(2):
+        st = d.iterate()
+        first.append((d.get("o"), st["misfit"], st["solve_iters"]))
+    d.iterate()
+    d.restart()
+    for k in range(2):
+        st = d.iterate()
+        assert np.array_equal(d.get("o"), first[k][0]), k
+        assert st["misfit"] == first[k][1] and st["solve_iters"] == first[k][2], k
+
+
 def test_dbim_checkpoint_resume_previous_solution_start(case):
     """With solve_start 1 a restored state drops the stored adjoint and
     derivative solutions (they belong to another history), so the resumed

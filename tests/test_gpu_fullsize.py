@@ -11,6 +11,10 @@ kernel is established at 64^2 .. 512^2 in the other GPU tests; here:
     converged, the FP64 relative residual (recomputed here with the FP64
     operator) below the tolerance, and systems 0 and 63 solved alone agree
     with the batched solutions to 1e-4 (the same residual criterion).
+At 2048x2048 (level 9, the full-scale configuration): a 64-right-hand-side
+mixed solve with its FP64 residuals recomputed, and one batched DBIM
+iteration (64 line-source transmitters, 512 receivers) of the mixed path
+against the batched FP64 path.
 """
 import numpy as np
 import pytest
@@ -118,3 +122,68 @@ def test_mixed_solve_level9(isg):
     res = np.linalg.norm(b - (x - s.mlfma(g * x))) / np.linalg.norm(b)
     assert res < tol * 1.05, res
     s.close()
+
+
+@pytest.mark.timeout(300)
+def test_mixed_solve_level9_64_rhs(isg):
+    """2048x2048 pixels (level 9) with 64 right-hand sides in one batch, the
+    full-scale configuration's batch: every system converges, and the FP64
+    relative residuals of systems 0, 31 and 63, recomputed here with a
+    separate FP64 context, meet the tolerance."""
+    print("level 9 mixed solve, 64 rhs", flush=True)
+    import torch
+    tol = 1e-6
+    s = isg.Solver(level=9, numrx=16, max_rhs=64, precision=1, toli=tol, iti=300)
+    pos, _ = s.geometry()
+    k0 = 2 * np.pi
+    g = 0.02 * np.exp(-np.abs(pos - pos.mean()) ** 2 / (2 * 40.0 ** 2)) * (1 + 0.05j) * k0 ** 2
+    pd = torch.from_numpy(pos).cuda()
+    ang = torch.arange(64, dtype=torch.float64, device="cuda")[:, None] * (2 * np.pi / 64)
+    B_d = torch.exp(1j * k0 * (torch.cos(ang) * pd.real[None, :] + torch.sin(ang) * pd.imag[None, :])).contiguous()
+    g_d = torch.from_numpy(np.ascontiguousarray(g)).cuda()
+    X_d = torch.zeros_like(B_d)
+    torch.cuda.synchronize()
+    _, infos = s.bicgs_dev(X_d.data_ptr(), g_d.data_ptr(), B_d.data_ptr(), 64, False, 0, True)
+    s.synchronize()
+    assert all(i["status"] == 0 for i in infos), infos
+    rows = {k: (X_d[k].cpu().numpy(), B_d[k].cpu().numpy()) for k in (0, 31, 63)}
+    del X_d, B_d
+    s.close()
+    f = isg.Solver(level=9, numrx=16, max_rhs=1)
+    for k, (x, b) in rows.items():
+        res = np.linalg.norm(b - (x - f.mlfma(g * x))) / np.linalg.norm(b)
+        assert res < tol * 1.05, (k, res)
+    f.close()
+
+
+@pytest.mark.timeout(600)
+def test_dbim_level9_batch64_vs_fp64(isg):
+    """One DBIM iteration at 2048x2048 pixels (level 9) with 64 line-source
+    transmitters and 512 receivers on a 160-wavelength circle, all in one
+    batch: the mixed-precision path with the first-order total-field start
+    (the full-scale configuration's solver) against the batched FP64 path
+    from the same measurements; reconstructed contrast within 1e-4."""
+    print("level 9 DBIM iteration, 64 tx batch, fp64 vs mixed", flush=True)
+    common = dict(level=9, numrx=512, numtx=64, toli=1e-6, iti=300, regular=float(np.sqrt(1e-3)), max_rhs=64,
+                  radius=160.0)
+    s = isg.Solver(precision=0, **common)
+    _, r2t = s.geometry()
+    otrue = isg.raster_to_tree(isg.phantom_ellipses(2048, 20, 5, vmin=0.002, vmax=0.01, imag=0.001), r2t)
+    d = isg.Dbim(s, otrue, field_start=0)
+    assert d.setup_stats["not_converged"] == 0
+    mes = d.get("mes")
+    st64 = d.iterate()
+    o64 = d.get("o")
+    del d
+    s.close()
+    s = isg.Solver(precision=1, mixed_margin=0.9, **common)
+    d = isg.Dbim(s, otrue, field_start=2, measurements=mes)
+    st32 = d.iterate()
+    o32 = d.get("o")
+    del d
+    s.close()
+    print("fp64: %d BiCGSTAB iterations, misfit %.6e; mixed: %d, misfit %.6e"
+          % (st64["solve_iters"], st64["misfit"], st32["solve_iters"], st32["misfit"]), flush=True)
+    assert st64["not_converged"] == 0 and st32["not_converged"] == 0
+    assert relerr(o32, o64) < 1e-4, relerr(o32, o64)
+    assert abs(st32["misfit"] - st64["misfit"]) < 1e-4 * st64["misfit"]

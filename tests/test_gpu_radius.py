@@ -32,7 +32,7 @@ def test_receivers_match_oracle(isg):
     assert relerr(s.aggregateh(r), o.aggregateh(r)) < 1e-12
 
 
-@pytest.mark.parametrize("batch,precision", [(1, 0), (16, 1)])
+@pytest.mark.parametrize("batch,precision", [(1, 0), (16, 1), (6, 1)])
 def test_dbim_matches_oracle(isg, batch, precision):
     from oracle.oracle import Dbim as OrcDbim
     print("finite-radius DBIM batch", batch, "precision", precision, flush=True)
