@@ -148,6 +148,21 @@ int isg_aggregateh_dev(isg_context* ctx, void* x, const void* r, int nrhs);
 int isg_bicgs_dev(isg_context* ctx, void* x, const void* o, long o_stride, const void* b, int herm, int nrhs,
                   int x_zero, isg_solve_info* info);
 
+/* isg_bicgs_dev without allocating: no counterpart in the reference.  For
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same arguments,
+ * results and errors as isg_bicgs_dev; the solver's reduction partials,
+ * per-system scalars, activity flags and tolerances live in a 16 MiB static
+ * device array of the library (one per GPU, kept until the process ends)
+ * instead of the context's buffers, and the iterations are launched kernel by
+ * kernel (no HIP graph is instantiated).  It calls no hipMalloc/hipFree.
+ * Needs 48 bytes per 1024 unknowns and system plus a few hundred bytes per
+ * system (2048^2 pixels x 64 systems: 12.6 MB); larger requests fail with
+ * ISG_E_ARG and a message.  Every call on one GPU shares the array: calls
+ * that overlap on one GPU (two host threads, two contexts) race. */
+int isg_bicgs_dev_static(isg_context* ctx, void* x, const void* o, long o_stride, const void* b, int herm, int nrhs,
+                         int x_zero, isg_solve_info* info);
+
 /* multipoles (which 0) or locals (which 1) of one level after the last operator call,
  * 4^level * numsamp[level] complex values (stage checkpoints of solve.cpp:218-245) */
 int isg_stage(isg_context* ctx, int which, int level, int rhs, double* out);

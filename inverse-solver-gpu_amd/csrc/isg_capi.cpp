@@ -34,6 +34,8 @@ struct isg_context {
   isg::zd* dr = nullptr;
   isg::zd* dg = nullptr;
   isg::zd* drx = nullptr;
+  // per-system results of isg_bicgs_dev_static (capacity max_rhs, reserved at creation)
+  std::vector<isg::SolveStat> sst;
 };
 
 namespace {
@@ -118,6 +120,7 @@ int isg_create(const isg_config* cfg, isg_context** out) {
   // the DBIM driver applies the receiver operator once per phase for every transmitter
   ec.rx_rhs = std::max(cfg->max_rhs, cfg->numtx);
   c->eng.reset(new isg::Engine(ec));
+  c->sst.reserve(cfg->max_rhs);
   if (!check(c)) {
     *out = c;  // keep the context so the caller can read the error
     return ISG_E_DEVICE;
@@ -304,6 +307,31 @@ int isg_bicgs_dev(isg_context* c, void* x, const void* o, long o_stride, const v
       info[i].relres = st[i].relres;
     }
     if (st[i].status > worst) worst = st[i].status;
+  }
+  return worst;
+}
+
+int isg_bicgs_dev_static(isg_context* c, void* x, const void* o, long o_stride, const void* b, int herm, int nrhs,
+                         int x_zero, isg_solve_info* info) {
+  if (!check(c)) return ISG_E_STATE;
+  if (!x || !o || !b || nrhs < 1 || nrhs > c->cfg.max_rhs) return fail(c, ISG_E_ARG, "bad argument");
+  if (o_stride != 0 && o_stride != c->eng->plan().N) return fail(c, ISG_E_ARG, "o_stride must be 0 or numunk");
+  if (c->eng->static_scratch_bytes(nrhs) > isg::kStaticScratchBytes)
+    return fail(c, ISG_E_ARG, "isg_bicgs_dev_static: this problem needs " +
+                                  std::to_string(c->eng->static_scratch_bytes(nrhs)) +
+                                  " bytes of solver scratch, more than the static " +
+                                  std::to_string(isg::kStaticScratchBytes) + "; use isg_bicgs_dev");
+  c->eng->bicgs_static((isg::zd*)x, (const isg::zd*)o, o_stride, (const isg::zd*)b, herm != 0, nrhs, x_zero != 0,
+                       c->cfg.toli, c->cfg.iti, c->sst, c->cfg.precision);
+  if (!check(c)) return ISG_E_DEVICE;
+  int worst = 0;
+  for (int i = 0; i < nrhs; i++) {
+    if (info) {
+      info[i].status = c->sst[i].status;
+      info[i].iters = c->sst[i].iters;
+      info[i].relres = c->sst[i].relres;
+    }
+    if (c->sst[i].status > worst) worst = c->sst[i].status;
   }
   return worst;
 }

@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <mutex>
 #include <sstream>
 #include <type_traits>
 
@@ -25,8 +26,24 @@ namespace isg {
     }                                                                                      \
   } while (0)
 
+// Device scratch of the allocation-free solve (Engine::bicgs_static,
+// isg_bicgs_dev_static): the solver's reduction partials, per-system scalars,
+// activity flags, tolerances and blend coefficients.  One copy per GPU, held
+// until the process ends; host code reaches it only through the address
+// hipGetSymbolAddress returns for the current device (g_scratch_ptr).
+static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+
 namespace {
 inline unsigned nblocks(long work) { return (unsigned)((work + kBlock - 1) / kBlock); }
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// bytes[0..n) = 0 (a fill kernel, not hipMemsetAsync, so that a caller may capture it)
+__global__ void __launch_bounds__(kBlock) k_clear_words(unsigned* __restrict__ p, long n) {
+  for (long i = (long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long)gridDim.x * kBlock) p[i] = 0u;
+}
 
 // Finite-radius receivers (line-source geometry): K[m][n] = (i/4) H0^(1)(k0
 // |rho_m - r_n|) res^2 with rho_m = radius (cos, sin)(2 pi m / R), the
@@ -185,6 +202,23 @@ Engine::Engine(const EngineConfig& cfg) : cfg_(cfg) {
     return;
   }
   ISG_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+  {
+    // this device's copy of the static scratch: its address, once per device
+    int dev = 0;
+    ISG_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (dev >= 0 && dev < kMaxDevices) {
+      if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e == hipSuccess && p)
+          g_scratch_ptr[dev] = p;
+        else
+          set_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+      }
+      scratch_ = g_scratch_ptr[dev];
+    }
+  }
   int L = P_.L, R = cfg_.max_rhs;
   // multipole and local stores, all levels, [level][rhs][4^i S_i]
   lvl_off_.assign(L, 0);
@@ -238,6 +272,8 @@ Engine::Engine(const EngineConfig& cfg) : cfg_(cfg) {
   ISG_HIP(hipHostMalloc((void**)&state_h_, (size_t)R * sizeof(BiState), hipHostMallocDefault));
   ISG_HIP(hipHostMalloc((void**)&tol_h_, (size_t)R * sizeof(double), hipHostMallocDefault));
   ISG_HIP(hipHostMalloc((void**)&part_h_, (size_t)R * kSlots * part_blocks_ * sizeof(double2), hipHostMallocDefault));
+  rn_.reserve(R);
+  bn_.reserve(R);
   if (ok()) {
     ISG_HIP(hipMemset(state_, 0, (size_t)R * sizeof(BiState)));
     ISG_HIP(hipMemset(active_, 0, (size_t)R * sizeof(int)));
@@ -971,6 +1007,68 @@ void Engine::bicgs(zd* x, const zd* g, long gs, const zd* b, bool herm, int nrhs
   solve(x, g, gs, b, herm, nrhs, x_zero, tol, maxit, out, prec, false);
 }
 
+size_t Engine::static_scratch_bytes(int nrhs) const {
+  size_t off = 0;
+  auto take = [&](size_t bytes) { off += (bytes + 255) & ~(size_t)255; };
+  take((size_t)nrhs * kSlots * part_blocks_ * sizeof(double2));
+  take((size_t)nrhs * sizeof(BiState));
+  take((size_t)nrhs * sizeof(int));
+  take((size_t)nrhs * sizeof(double));
+  take((size_t)nrhs * sizeof(zd));
+  return off;
+}
+
+// The solve of bicgs() with its scratch (partials, scalars, flags,
+// tolerances, blend coefficients) in this device's static array instead of
+// the engine's allocations, launched kernel by kernel (no graph is
+// instantiated) and without the live timers: no device memory is allocated.
+// The caller has checked static_scratch_bytes(nrhs) <= kStaticScratchBytes.
+void Engine::bicgs_static(zd* x, const zd* g, long gs, const zd* b, bool herm, int nrhs, bool x_zero, double tol,
+                          int maxit, std::vector<SolveStat>& out, int prec) {
+  if (!ok()) return;
+  if (!scratch_ || static_scratch_bytes(nrhs) > kStaticScratchBytes) {
+    set_error("bicgs_static: static scratch unavailable or too small");
+    return;
+  }
+  char* base = (char*)scratch_;
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    char* p = base + off;
+    off += (bytes + 255) & ~(size_t)255;
+    return p;
+  };
+  double2* part = (double2*)take((size_t)nrhs * kSlots * part_blocks_ * sizeof(double2));
+  BiState* state = (BiState*)take((size_t)nrhs * sizeof(BiState));
+  int* active = (int*)take((size_t)nrhs * sizeof(int));
+  double* tolv = (double*)take((size_t)nrhs * sizeof(double));
+  zd* lam = (zd*)take((size_t)nrhs * sizeof(zd));
+  // the array keeps the previous call's contents: clear the scalars and flags
+  const long words = (long)(((char*)tolv - (char*)state) / sizeof(unsigned));
+  k_clear_words<<<nblocks(words), kBlock, 0, stream_>>>((unsigned*)state, words);
+  ISG_HIP(hipGetLastError());
+  double2* s_part = part_;
+  BiState* s_state = state_;
+  int* s_active = active_;
+  double* s_tolv = tolv_;
+  zd* s_lam = lam_;
+  const bool s_graph = cfg_.graph, s_prof = prof_;
+  part_ = part;
+  state_ = state;
+  active_ = active;
+  tolv_ = tolv;
+  lam_ = lam;
+  cfg_.graph = false;
+  prof_ = false;
+  solve(x, g, gs, b, herm, nrhs, x_zero, tol, maxit, out, prec, false);
+  part_ = s_part;
+  state_ = s_state;
+  active_ = s_active;
+  tolv_ = s_tolv;
+  lam_ = s_lam;
+  cfg_.graph = s_graph;
+  prof_ = s_prof;
+}
+
 // r_x = b - A x and A d with the solver's operator, lambda = <Ad, r_x> / <Ad, Ad>,
 // x += lambda d, r0 = r_x - lambda Ad; BiCGSTAB then starts from (x, r0).
 void Engine::bicgs_blend(zd* x, const zd* d, const zd* g, long gs, const zd* b, bool herm, int nrhs, double tol,
@@ -1041,7 +1139,7 @@ void Engine::solve(zd* x, const zd* g, long gs, const zd* b, bool herm, int nrhs
     // FP64 true residual; defect correction with the FP32 operator until the
     // FP64 relative residual meets tol (or the rounds are used up)
     zd *res = vec_[V_RES], *cor = vec_[V_COR];
-    std::vector<double> rn, bn;
+    std::vector<double>&rn = rn_, &bn = bn_;
     for (int round = 0; round <= kRefineRounds; round++) {
       true_residual(x, g, gs, b, herm, nrhs, rn, bn);
       if (!ok()) return;

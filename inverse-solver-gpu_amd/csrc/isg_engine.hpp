@@ -25,6 +25,9 @@ namespace isg {
 
 enum { PREC_FP64 = 0, PREC_MIXED = 1 };
 
+// Size of the static device scratch of the allocation-free solve (one per GPU).
+constexpr size_t kStaticScratchBytes = (size_t)16 << 20;
+
 struct EngineConfig {
   PlanConfig plan;
   int max_rhs = 1;        // right-hand sides solved together
@@ -106,6 +109,16 @@ class Engine {
   // zeros and r_x = b).  lam (host, may be null) receives lambda.
   void bicgs_blend(zd* x, const zd* d, const zd* g, long g_stride, const zd* b, bool herm, int nrhs, double tol,
                    int maxit, std::vector<SolveStat>& out, int prec, std::vector<zd>* lam, bool x_zero = false);
+
+  // bicgs() without device allocations: the solver's partial sums, scalars,
+  // flags, tolerances and blend coefficients live in this GPU's static scratch
+  // array (kStaticScratchBytes, shared by every engine on the GPU, so calls
+  // that overlap on one GPU race); iterations are launched kernel by kernel.
+  // Needs static_scratch_bytes(nrhs) <= kStaticScratchBytes and out with
+  // capacity for nrhs entries if the call must not touch the host heap either.
+  void bicgs_static(zd* x, const zd* g, long g_stride, const zd* b, bool herm, int nrhs, bool x_zero, double tol,
+                    int maxit, std::vector<SolveStat>& out, int prec = PREC_FP64);
+  size_t static_scratch_bytes(int nrhs) const;
 
   // The FP64 product G prep(x) (prep(x) = g x forward, conj(x) adjoint) that
   // the last true-residual check of a mixed-precision solve computed for its
@@ -278,6 +291,8 @@ class Engine {
   int near_blocks_ = 0;  // workgroups per right-hand side of the last near-field launch
   Tuning tune_[2][2];  // [FP64, FP32 operator][batch class]
   std::vector<void*> allocs_;
+  void* scratch_ = nullptr;          // this device's static scratch (bicgs_static)
+  std::vector<double> rn_, bn_;      // true-residual norms (capacity max_rhs)
 };
 
 }  // namespace isg

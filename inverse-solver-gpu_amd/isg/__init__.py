@@ -103,6 +103,8 @@ _SIGS = {
     "isg_aggregateh_dev": (ctypes.c_int, [_V, _V, _V, ctypes.c_int]),
     "isg_bicgs_dev": (ctypes.c_int, [_V, _V, _V, ctypes.c_long, _V, ctypes.c_int, ctypes.c_int,
                                      ctypes.c_int, _P(SolveInfo)]),
+    "isg_bicgs_dev_static": (ctypes.c_int, [_V, _V, _V, ctypes.c_long, _V, ctypes.c_int, ctypes.c_int,
+                                            ctypes.c_int, _P(SolveInfo)]),
     "isg_stage": (ctypes.c_int, [_V, ctypes.c_int, ctypes.c_int, ctypes.c_int, _D]),
     "isg_counters": (ctypes.c_int, [_V, _P(ctypes.c_longlong)]),
     "isg_dbim_setup": (ctypes.c_int, [_V, _D, ctypes.c_int, _P(DbimStats)]),
@@ -356,6 +358,14 @@ class Solver:
 
     def aggregateh_dev(self, x_ptr, r_ptr, nrhs=1):
         return self._err(self.lib.isg_aggregateh_dev(self.h, _V(x_ptr), _V(r_ptr), nrhs), "isg_aggregateh_dev")
+
+    def bicgs_dev_static(self, x_ptr, o_ptr, b_ptr, nrhs=1, herm=False, o_stride=0, x_zero=False):
+        """bicgs_dev with the solver's scratch in the library's static device
+        array (no device allocation; calls overlapping on one GPU race)."""
+        infos = (SolveInfo * nrhs)()
+        rc = self._err(self.lib.isg_bicgs_dev_static(self.h, _V(x_ptr), _V(o_ptr), o_stride, _V(b_ptr),
+                                                     int(herm), nrhs, int(x_zero), infos), "isg_bicgs_dev_static")
+        return rc, [{"status": i.status, "iters": i.iters, "relres": i.relres} for i in infos]
 
     def bicgs_dev(self, x_ptr, o_ptr, b_ptr, nrhs=1, herm=False, o_stride=0, x_zero=False):
         infos = (SolveInfo * nrhs)()
