@@ -59,7 +59,7 @@ CYCLE = 5  # DBIM iterations of one inversion (the workload's 5)
 
 # total-field start (DbimOptions::field_start)
 START_KINDS = {"zero": 0, "previous": 1, "first-order": 2, "learned": 3}
-WARMNET_FILE = os.path.join(ROOT, "inverse-solver-gpu_amd", "weights", "warmnet_512.bin")
+WARMNET_FILE = os.path.join(ROOT, "inverse-solver-gpu_amd", "weights", "warmnet.bin")
 
 WORKLOADS = {
     # name: level, transmitters, receivers, ellipses, seed, default circle radius (wavelengths)

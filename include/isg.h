@@ -197,8 +197,9 @@ typedef struct isg_dbim_stats {
  * as in the reference (mom.cpp:324), 1 = from the previous total field, 2 = from
  * the previous total field moved to first order along the step: tot - alpha dtot,
  * dtot being the derivative field of the step-length solve (mom.cpp:284-290),
- * 3 = learned: tot_prev + lambda d with d the warm-start network's correction and
- * lambda minimising the initial residual per transmitter (needs
+ * 3 = learned: the first-order start + lambda d with d the warm-start
+ * network's correction of it and lambda minimising the initial residual per
+ * transmitter, so never a worse start than the first-order one (needs
  * isg_dbim_set_warmnet before isg_dbim_iterate). */
 int isg_dbim_setup(isg_context* ctx, const double* o_true, int field_start, isg_dbim_stats* st);
 /* Same start from given measurements mes (numtx x numrx complex, the reference's
@@ -216,7 +217,9 @@ int isg_dbim_iterate(isg_context* ctx, isg_dbim_stats* st);
  * the first iterations). */
 int isg_dbim_restart(isg_context* ctx);
 /* which: 0 o, 1 od (gradient), 2 ot (direction), 3 mes, 4 mesd, 5 inc, 6 tot,
- * 7 true contrast (get only).
+ * 7 true contrast (get only), 8 dtot: the Frechet-derivative fields of the
+ * last step-length phase, numtx x numunk (get only; field_start 2 or 3 or
+ * solve_start 1).
  * Vectors are numunk (x numtx for inc/tot) complex; mes/mesd numtx x numrx.
  * isg_dbim_set accepts 0, 1, 2, 4 and 6 (checkpoint / resume). */
 int isg_dbim_get(isg_context* ctx, int which, double* out);
@@ -276,12 +279,16 @@ int isg_profile_read(isg_context* ctx, int kernel, double* total_ms, long long* 
 /* ---- learned warm start of the total-field solves (no counterpart in the
  * reference, whose total-field solves restart from zero, mom.cpp:324) ----
  * A 4-level convolutional encoder-decoder (32 base channels, BF16 weights,
- * FP32 accumulation) mapping per transmitter (Re/Im contrast, Re/Im incident
- * field, Re/Im previous total field) to a 2-channel correction of the previous
- * total field.  Images are raster order, NHWC FP32, H and W multiples of 64.
+ * FP32 accumulation) mapping per transmitter 8 channels (Re/Im contrast,
+ * Re/Im incident field, Re/Im first-order start tot_prev - alpha dtot, Re/Im
+ * its step -alpha dtot; the field channels divided by the transmitter's
+ * incident-field rms) to a 2-channel correction of the first-order start.
+ * The image enters space-to-depth by 2 (the network runs at half the pixel
+ * resolution with 32 input channels) and leaves depth-to-space.  Images are
+ * raster order, NHWC FP32, H and W multiples of 16.
  * Calls without a context run on the null stream and return after the work. */
 typedef struct isg_warmnet isg_warmnet;
-/* Workspace for max_batch images of max_pixels pixels (multiple of 4096);
+/* Workspace for max_batch images of max_pixels pixels (multiple of 16 x 16);
  * He-normal weights from seed. */
 int isg_warmnet_create(int max_batch, long max_pixels, unsigned long long seed, isg_warmnet** out);
 void isg_warmnet_destroy(isg_warmnet* net);
@@ -291,10 +298,10 @@ long isg_warmnet_num_params(void);
 int isg_warmnet_get_params(isg_warmnet* net, float* out);
 int isg_warmnet_set_params(isg_warmnet* net, const float* in);  /* also resets Adam */
 int isg_warmnet_get_grads(isg_warmnet* net, float* out);
-/* 7 scales: input channel c is multiplied by s[c] (c < 6), the output by s[6]. */
+/* 9 scales: input channel c is multiplied by s[c] (c < 8), the output by s[8]. */
 int isg_warmnet_set_scales(isg_warmnet* net, const float* s);
 int isg_warmnet_get_scales(isg_warmnet* net, float* s);
-/* in [batch][H][W][6], out [batch][H][W][2], device pointers */
+/* in [batch][H][W][8], out [batch][H][W][2], device pointers */
 int isg_warmnet_forward_dev(isg_warmnet* net, const void* in, void* out, int batch, int H, int W);
 /* forward + backward of mean((out - target)^2) over the 2 output channels;
  * gradients kept for isg_warmnet_adam / isg_warmnet_get_grads */
@@ -306,7 +313,7 @@ int isg_warmnet_loss_grad(isg_warmnet* net, const float* in, const float* target
                           double* loss);
 /* Adam step (beta1 0.9, beta2 0.999, eps 1e-8) with learning rate lr */
 int isg_warmnet_adam(isg_warmnet* net, float lr);
-/* weight file: "ISGWNET1", uint32 count, 7 float scales, float parameters */
+/* weight file: "ISGWNET2", uint32 count, 9 float scales, float parameters */
 int isg_warmnet_save(isg_warmnet* net, const char* path);
 int isg_warmnet_load(isg_warmnet* net, const char* path);
 /* Network used by field_start 3 of this context's DBIM (not owned). */

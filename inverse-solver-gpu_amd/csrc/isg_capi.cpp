@@ -592,7 +592,7 @@ static bool net_stage(isg_warmnet* n, size_t pix) {
   if (n->dout) (void)hipFree(n->dout);
   n->din = n->dout = nullptr;
   n->cap = 0;
-  if (hipMalloc((void**)&n->din, pix * 6 * sizeof(float)) != hipSuccess ||
+  if (hipMalloc((void**)&n->din, pix * isg::WarmNet::kInCh * sizeof(float)) != hipSuccess ||
       hipMalloc((void**)&n->dout, pix * 2 * sizeof(float)) != hipSuccess)
     return false;
   n->cap = pix;
@@ -604,7 +604,7 @@ int isg_warmnet_forward(isg_warmnet* n, const float* in, float* out, int batch, 
   if (!in || !out || batch < 1 || H < 1 || W < 1) return net_fail(n, ISG_E_ARG, "bad argument");
   const size_t pix = (size_t)batch * H * W;
   if (!net_stage(n, pix)) return net_fail(n, ISG_E_DEVICE, "hipMalloc failed");
-  if (hipMemcpy(n->din, in, pix * 6 * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+  if (hipMemcpy(n->din, in, pix * isg::WarmNet::kInCh * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
     return net_fail(n, ISG_E_DEVICE, "copy to device failed");
   int rc = isg_warmnet_forward_dev(n, n->din, n->dout, batch, H, W);
   if (rc != ISG_OK) return rc;
@@ -619,7 +619,7 @@ int isg_warmnet_loss_grad(isg_warmnet* n, const float* in, const float* target, 
   if (!in || !target || !loss || batch < 1 || H < 1 || W < 1) return net_fail(n, ISG_E_ARG, "bad argument");
   const size_t pix = (size_t)batch * H * W;
   if (!net_stage(n, pix)) return net_fail(n, ISG_E_DEVICE, "hipMalloc failed");
-  if (hipMemcpy(n->din, in, pix * 6 * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
+  if (hipMemcpy(n->din, in, pix * isg::WarmNet::kInCh * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(n->dout, target, pix * 2 * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
     return net_fail(n, ISG_E_DEVICE, "copy to device failed");
   return isg_warmnet_loss_grad_dev(n, n->din, n->dout, batch, H, W, loss);

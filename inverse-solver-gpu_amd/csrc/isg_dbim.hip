@@ -114,30 +114,38 @@ __global__ void __launch_bounds__(kBlock) k_axpy_rows(zd* a, const zd* b, zd s, 
   DB_LOOP(N) a[t * N + i] = cadd(a[t * N + i], cmul(s, b[t * N + i]));
 }
 
-// Network input of transmitters t0..t0+nc-1 in raster order:
-// in[t][q] = (o, inc_t, tot_t)[r2t[q]] as 6 floats.
-__global__ void __launch_bounds__(kBlock) k_net_in(const zd* o, const zd* inc, const zd* tot, const int* r2t,
-                                                   float* in, long N) {
+// Network input of transmitters t0..t0+nc-1 in raster order, the field
+// channels divided by the transmitter's incident-field rms (inv[t]):
+// in[t][q] = (o, inc_t, tot_t, -alpha dtot_t)[r2t[q]] as 8 floats, tot_t
+// being the first-order start tot_prev - alpha dtot.
+__global__ void __launch_bounds__(kBlock) k_net_in(const zd* o, const zd* inc, const zd* tot, const zd* dtot,
+                                                   double alpha, const double* inv, const int* r2t, float* in,
+                                                   long N) {
   int t = blockIdx.y;
+  const double s = inv[t];
   DB_LOOP(N) {
     int n = r2t[i];
-    zd a = o[n], b = inc[t * N + n], c = tot[t * N + n];
-    float* p = in + (t * N + i) * 6;
+    zd a = o[n], b = inc[t * N + n], c = tot[t * N + n], d = dtot[t * N + n];
+    float* p = in + (t * N + i) * 8;
     p[0] = (float)a.x;
     p[1] = (float)a.y;
-    p[2] = (float)b.x;
-    p[3] = (float)b.y;
-    p[4] = (float)c.x;
-    p[5] = (float)c.y;
+    p[2] = (float)(b.x * s);
+    p[3] = (float)(b.y * s);
+    p[4] = (float)(c.x * s);
+    p[5] = (float)(c.y * s);
+    p[6] = (float)(-alpha * d.x * s);
+    p[7] = (float)(-alpha * d.y * s);
   }
 }
 
-// d[t][r2t[q]] = out[t][q] (complex from 2 floats)
-__global__ void __launch_bounds__(kBlock) k_net_dir(const float* out, const int* r2t, zd* d, long N) {
+// d[t][r2t[q]] = out[t][q] * rms[t] (complex from 2 floats)
+__global__ void __launch_bounds__(kBlock) k_net_dir(const float* out, const double* rms, const int* r2t, zd* d,
+                                                    long N) {
   int t = blockIdx.y;
+  const double s = rms[t];
   DB_LOOP(N) {
     const float* p = out + (t * N + i) * 2;
-    d[t * N + r2t[i]] = mkc<double>(p[0], p[1]);
+    d[t * N + r2t[i]] = mkc<double>(p[0] * s, p[1] * s);
   }
 }
 
@@ -228,7 +236,7 @@ Dbim::Dbim(Engine& eng, const DbimOptions& opt) : eng_(eng), opt_(opt) {
     }
     rxw_ = (zd*)dalloc(vb * T_);
   }
-  if (opt_.field_start == 2 || opt_.solve_start == 1) dtot_ = (zd*)dalloc(vb * T_);
+  if (opt_.field_start >= 2 || opt_.solve_start == 1) dtot_ = (zd*)dalloc(vb * T_);
   if (opt_.solve_start == 1) adj_ = (zd*)dalloc(vb * T_);
   nblk_ = (int)std::min<long>(1024, std::max<long>(1, (N_ + kBlock * 4 - 1) / (kBlock * 4)));
   part_ = (double*)dalloc(sizeof(double) * kSums * nblk_);
@@ -473,10 +481,10 @@ void Dbim::iterate(DbimStats& st) {
     bool zero = opt_.field_start == 0;
     if (zero) DB_HIP(hipMemsetAsync(tot, 0, sizeof(zd) * N * nb, s));
     // o moved by -alpha ot, so to first order tot moves by -alpha dtot
-    if (opt_.field_start == 2)
+    if (opt_.field_start >= 2)
       k_axpy_rows<<<dim3(nblk_, nb), blk, 0, s>>>(tot, dtot_ + (size_t)t0 * N, mkc<double>(-alpha, 0.0), N);
     if (opt_.field_start == 3) {
-      learned_solve(tot, inc_ + (size_t)t0 * N, nb, st);
+      learned_solve(tot, t0, nb, alpha, st);
     } else {
       solve(tot, inc_ + (size_t)t0 * N, false, nb, zero, st);
     }
@@ -520,54 +528,82 @@ void Dbim::set_warmnet(WarmNet* net) {
   if (!net || !ok()) return;
   const Plan& P = eng_.plan();
   const long side = (long)P.side;
-  if (side % 64 != 0 || side * side > net->max_pixels()) {
-    err_ = "warm-start network workspace does not fit this domain (side multiple of 64 needed)";
+  if (side % WarmNet::kTile != 0 || side * side > net->max_pixels()) {
+    err_ = "warm-start network workspace does not fit this domain (side multiple of 16 needed)";
     return;
   }
   if (!r2t_) {
     r2t_ = (int*)dalloc(sizeof(int) * N_);
     wdir_ = (zd*)dalloc(sizeof(zd) * (size_t)N_ * B_);
+    inc_rms_ = (double*)dalloc(sizeof(double) * 2 * T_);
     if (!ok()) return;
     DB_HIP(hipMemcpy(r2t_, P.raster2tree.data(), sizeof(int) * N_, hipMemcpyHostToDevice));
   }
   if (net->max_batch() > net_chunk_) {
     net_chunk_ = std::min(net->max_batch(), B_);
-    net_in_ = (float*)dalloc(sizeof(float) * 6 * (size_t)N_ * net_chunk_);
+    net_in_ = (float*)dalloc(sizeof(float) * WarmNet::kInCh * (size_t)N_ * net_chunk_);
     net_out_ = (float*)dalloc(sizeof(float) * 2 * (size_t)N_ * net_chunk_);
   }
+  rms_ok_ = false;
 }
 
-// Learned start (field_start 3): per chunk of transmitters, pack
-// (o, inc_t, tot_t) into raster images, run the network, scatter its
-// correction back to tree order, then one blended solve for the batch.
-void Dbim::learned_solve(zd* tot, const zd* inc, int nb, DbimStats& st) {
+// rms of every transmitter's incident field (the network's field scale):
+// inc_rms_[t] and its inverse inc_rms_[T + t]
+void Dbim::incident_rms() {
+  hipStream_t s = eng_.stream();
+  std::vector<double> h(2 * T_);
+  for (int t = 0; t < T_; t++) {
+    k_sums<<<dim3(nblk_), dim3(kBlock), 0, s>>>(inc_ + (size_t)t * N_, inc_ + (size_t)t * N_, inc_ + (size_t)t * N_,
+                                                inc_ + (size_t)t * N_, N_, part_);
+    DB_HIP(hipMemcpyAsync(part_h_, part_, sizeof(double) * kSums * nblk_, hipMemcpyDeviceToHost, s));
+    DB_HIP(hipStreamSynchronize(s));
+    double e = 0;
+    for (int j = 0; j < nblk_; j++) e += part_h_[4 * nblk_ + j];  // |o|^2 slot: sum |inc_t|^2
+    const double r = std::sqrt(e / N_);
+    h[t] = r > 0 ? r : 1.0;
+    h[T_ + t] = 1.0 / h[t];
+  }
+  DB_HIP(hipMemcpy(inc_rms_, h.data(), sizeof(double) * 2 * T_, hipMemcpyHostToDevice));
+  rms_ok_ = ok();
+}
+
+// Learned start (field_start 3): tot already holds the first-order start
+// tot_prev - alpha dtot; per chunk of transmitters, pack (o, inc_t, tot_t,
+// -alpha dtot_t) into raster images, run the network, scatter its correction
+// back to tree order, then one blended solve for the batch.
+void Dbim::learned_solve(zd* tot, int t0, int nb, double alpha, DbimStats& st) {
   if (!net_ || !net_in_) {
     if (err_.empty()) err_ = "field_start 3 needs a warm-start network (isg_dbim_set_warmnet)";
     return;
   }
+  if (!rms_ok_) incident_rms();
   hipStream_t s = eng_.stream();
   const long N = N_;
   const int side = eng_.plan().side;
-  auto t0 = std::chrono::steady_clock::now();
+  const zd* inc = inc_ + (size_t)t0 * N;
+  const zd* dtot = dtot_ + (size_t)t0 * N;
+  auto tn = std::chrono::steady_clock::now();
   for (int c0 = 0; c0 < nb; c0 += net_chunk_) {
     int nc = std::min(net_chunk_, nb - c0);
-    k_net_in<<<dim3(nblk_, nc), dim3(kBlock), 0, s>>>(o_, inc + (size_t)c0 * N, tot + (size_t)c0 * N, r2t_,
+    k_net_in<<<dim3(nblk_, nc), dim3(kBlock), 0, s>>>(o_, inc + (size_t)c0 * N, tot + (size_t)c0 * N,
+                                                       dtot + (size_t)c0 * N, alpha, inc_rms_ + T_ + t0 + c0, r2t_,
                                                        net_in_, N);
     net_->forward(net_in_, net_out_, nc, side, side, s);
     if (!net_->ok()) {
       if (err_.empty()) err_ = net_->error();
       return;
     }
-    k_net_dir<<<dim3(nblk_, nc), dim3(kBlock), 0, s>>>(net_out_, r2t_, wdir_ + (size_t)c0 * N, N);
+    k_net_dir<<<dim3(nblk_, nc), dim3(kBlock), 0, s>>>(net_out_, inc_rms_ + t0 + c0, r2t_, wdir_ + (size_t)c0 * N,
+                                                        N);
   }
   DB_HIP(hipStreamSynchronize(s));
-  st.net_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  st.net_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - tn).count();
   std::vector<SolveStat> ss;
   std::vector<zd> lam;
   eng_.bicgs_blend(tot, wdir_, g_, 0, inc, false, nb, opt_.tol, opt_.maxit, ss, opt_.precision, &lam);
   double la = 0;
   for (auto& l : lam) la += std::sqrt(l.x * l.x + l.y * l.y);
-  st.lambda_abs = lam.empty() ? 0 : la / lam.size();
+  st.lambda_abs += lam.empty() ? 0 : la / T_;
   for (auto& q : ss) {
     st.solves++;
     st.solve_iters += q.iters;
@@ -587,8 +623,10 @@ bool Dbim::get(int which, std::vector<zd>& out) {
     case 5: src = inc_; n = (size_t)T_ * N_; break;
     case 6: src = tot_; n = (size_t)T_ * N_; break;
     case 7: src = omes_; n = N_; break;
+    case 8: src = dtot_; n = (size_t)T_ * N_; break;  // null unless field_start >= 2 or solve_start 1
     default: return false;
   }
+  if (!src) return false;
   out.resize(n);
   DB_HIP(hipStreamSynchronize(eng_.stream()));
   DB_HIP(hipMemcpy(out.data(), src, sizeof(zd) * n, hipMemcpyDeviceToHost));

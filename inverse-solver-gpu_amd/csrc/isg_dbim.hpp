@@ -23,8 +23,8 @@ struct DbimOptions {
   // 2 previous total field + first-order update -alpha * dtot, where dtot is the
   // Frechet derivative field the step-length phase already solved for
   // ((I - G g) dtot = G(k0^2 ot tot), mom.cpp:284-290), 3 learned: the
-  // warm-start network's correction d of the previous total field, started
-  // from tot_prev + lambda d with the residual-minimising lambda
+  // warm-start network's correction d of that first-order start, started
+  // from tot_prev - alpha dtot + lambda d with the residual-minimising lambda
   int field_start = 0;
   int precision = 0;      // solver operator: PREC_FP64 or PREC_MIXED
   int products = 0;       // operator of the derivative source G (k0^2 ot tot): PREC_FP64 or PREC_MIXED
@@ -80,7 +80,9 @@ class Dbim {
   void solve(zd* x, const zd* b, bool herm, int nb, bool x_zero, DbimStats& st);
   // solve into x (zeroed here) started from lambda * prev; then prev = x
   void solve_from(zd* x, zd* prev, const zd* b, bool herm, int nb, DbimStats& st);
-  void learned_solve(zd* tot, const zd* inc, int nb, DbimStats& st);
+  // tot: the first-order starts of transmitters t0..t0+nb-1 (solved in place)
+  void learned_solve(zd* tot, int t0, int nb, double alpha, DbimStats& st);
+  void incident_rms();
   void* dalloc(size_t bytes);
 
   Engine& eng_;
@@ -107,6 +109,8 @@ class Dbim {
   float *net_in_ = nullptr, *net_out_ = nullptr;  // [chunk][side][side][6], [..][2]
   zd* wdir_ = nullptr;                             // [max_rhs][N] learned corrections
   int* r2t_ = nullptr;                             // raster -> tree index
+  double* inc_rms_ = nullptr;                      // [2][numtx]: rms of inc_t, then 1 / rms
+  bool rms_ok_ = false;
   double* part_ = nullptr;
   double* part_h_ = nullptr;
   int nblk_ = 0;

@@ -11,10 +11,12 @@ namespace {
 
 constexpr int kNB = 256;  // threads per block
 constexpr int kCK = 32;   // input channels staged per pass
-constexpr int kCKP = kCK + 4;  // padded LDS row (keeps 8-byte alignment, spreads banks)
+constexpr int kCKP = kCK + 8;  // padded LDS row: 80 bytes keep the 16-byte loads aligned
 constexpr int kCOT = 64;  // output channels per workgroup
 
 typedef short v4s __attribute__((ext_vector_type(4)));
+typedef short v8s __attribute__((ext_vector_type(8)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 
 // FP32 -> BF16, round to nearest even (inputs are finite).
@@ -37,7 +39,11 @@ __host__ __device__ __forceinline__ float bf2f(uint16_t b) {
 // One workgroup: an 8x8 pixel tile of one image and up to 64 output channels;
 // wave w owns tile pixels 16w..16w+15 (the MFMA's 16 rows), every 16-channel
 // output block is one accumulator.  Input halo and weights are staged through
-// LDS in chunks of 32 (or 16) input channels as BF16.
+// LDS in chunks of 32 (or 16) input channels as BF16; a chunk of 32 is one
+// v_mfma_f32_16x16x32_bf16 per tap and output block (lane l supplies row or
+// column l % 16 and the 8 channels 8 (l / 16) .. +7 of both operands; the
+// same channel order on both sides, so the sum over the chunk is exact in
+// any lane-to-channel assignment), a chunk of 16 one v_mfma_f32_16x16x16_bf16.
 template <int KS>
 __global__ void __launch_bounds__(kNB) k_conv(const float* __restrict__ X, const float* __restrict__ U,
                                               const float* __restrict__ M, const uint16_t* __restrict__ Wb,
@@ -48,7 +54,7 @@ __global__ void __launch_bounds__(kNB) k_conv(const float* __restrict__ X, const
   __shared__ __attribute__((aligned(16))) uint16_t wsm[KS * KS * kCOT * kCKP];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int n = blockIdx.y;
-  const int tiles_x = W >> 3;
+  const int tiles_x = (W + 7) >> 3;  // the last tile row / column may be partial
   const int ty0 = (blockIdx.x / tiles_x) * 8, tx0 = (blockIdx.x % tiles_x) * 8;
   const int co0 = blockIdx.z * kCOT;
   const int cot = min(kCOT, cout - co0);
@@ -95,15 +101,29 @@ __global__ void __launch_bounds__(kNB) k_conv(const float* __restrict__ X, const
       *reinterpret_cast<uint2*>(&wsm[(tap * kCOT + co) * kCKP + 4 * c4]) = wv;
     }
     __syncthreads();
+    if (ck == kCK) {
 #pragma unroll
-    for (int tap = 0; tap < KS * KS; tap++) {
-      const int hp = (py + tap / KS) * HW + px + tap % KS;
-      for (int sc = 0; sc < ck; sc += 16) {
-        const v4s a = *reinterpret_cast<const v4s*>(&hal[hp * kCKP + sc + 4 * kq]);
+      for (int tap = 0; tap < KS * KS; tap++) {
+        const int hp = (py + tap / KS) * HW + px + tap % KS;
+        const v8bf a = __builtin_bit_cast(v8bf, *reinterpret_cast<const v8s*>(&hal[hp * kCKP + 8 * kq]));
 #pragma unroll
         for (int cb = 0; cb < 4; cb++) {
           if (cb < ncb) {
-            const v4s b = *reinterpret_cast<const v4s*>(&wsm[(tap * kCOT + cb * 16 + li) * kCKP + sc + 4 * kq]);
+            const v8bf b = __builtin_bit_cast(
+                v8bf, *reinterpret_cast<const v8s*>(&wsm[(tap * kCOT + cb * 16 + li) * kCKP + 8 * kq]));
+            acc[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[cb], 0, 0, 0);
+          }
+        }
+      }
+    } else {
+#pragma unroll
+      for (int tap = 0; tap < KS * KS; tap++) {
+        const int hp = (py + tap / KS) * HW + px + tap % KS;
+        const v4s a = *reinterpret_cast<const v4s*>(&hal[hp * kCKP + 4 * kq]);
+#pragma unroll
+        for (int cb = 0; cb < 4; cb++) {
+          if (cb < ncb) {
+            const v4s b = *reinterpret_cast<const v4s*>(&wsm[(tap * kCOT + cb * 16 + li) * kCKP + 4 * kq]);
             acc[cb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[cb], 0, 0, 0);
           }
         }
@@ -123,7 +143,7 @@ __global__ void __launch_bounds__(kNB) k_conv(const float* __restrict__ X, const
       const int gy = ty0 + (q >> 3), gx = tx0 + (q & 7);
       float v = acc[cb][r] + bv;
       if (relu) v = v > 0.f ? v : 0.f;
-      if (co < cstore) Y[(img + (long)gy * W + gx) * cstore + co] = v;
+      if (co < cstore && gy < H && gx < W) Y[(img + (long)gy * W + gx) * cstore + co] = v;
     }
   }
 }
@@ -145,7 +165,7 @@ __global__ void __launch_bounds__(kNB) k_wgrad(const float* __restrict__ dY, con
   const int ncit = cin >> 4;
   const int co0 = (blockIdx.x / ncit) * 16, ci0 = (blockIdx.x % ncit) * 16;
   const int chunk = blockIdx.y;
-  const int tiles_x = W >> 3, tiles = (H >> 3) * tiles_x;
+  const int tiles_x = (W + 7) >> 3, tiles = ((H + 7) >> 3) * tiles_x;
   const long total = (long)tiles * batch;
   float acc[KS * KS];
 #pragma unroll
@@ -157,9 +177,13 @@ __global__ void __launch_bounds__(kNB) k_wgrad(const float* __restrict__ dY, con
     const long img = (long)n * H * W;
     for (int e = tid; e < 64 * 16; e += kNB) {
       const int p = e >> 4, c = e & 15;
-      const long pix = img + (long)(ty0 + (p >> 3)) * W + tx0 + (p & 7);
-      float v = dY[pix * cout + co0 + c];
-      if (M && !(M[pix * cout + co0 + c] > 0.f)) v = 0.f;
+      const int gy = ty0 + (p >> 3), gx = tx0 + (p & 7);
+      float v = 0.f;
+      if (gy < H && gx < W) {
+        const long pix = img + (long)gy * W + gx;
+        v = dY[pix * cout + co0 + c];
+        if (M && !(M[pix * cout + co0 + c] > 0.f)) v = 0.f;
+      }
       dsm[p][c] = bf2f(f2bf(v));
     }
     for (int e = tid; e < HS * 16; e += kNB) {
@@ -197,20 +221,44 @@ __global__ void k_chunk_sum(const float* __restrict__ part, long len, int nchunk
   }
 }
 
-// x16[p][c] = in[p][c] * scale[c] for c < 6, 0 for 6 <= c < 16
-__global__ void k_net_pack(const float* __restrict__ in, long npix, const float* __restrict__ sc,
-                           float* __restrict__ x16) {
-  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < npix * 16; i += (long)gridDim.x * blockDim.x) {
-    const long p = i >> 4;
-    const int c = (int)(i & 15);
-    x16[i] = c < WarmNet::kInCh ? in[p * WarmNet::kInCh + c] * sc[c] : 0.f;
+// Space-to-depth input: x32[n][Y][X][8 s + c] = in[n][2Y + s / 2][2X + s % 2][c] * scale[c]
+// (s = 2 dy + dx, c < 8); H, W are the image's sides.
+__global__ void k_net_pack(const float* __restrict__ in, int batch, int H, int W, const float* __restrict__ sc,
+                           float* __restrict__ x32) {
+  constexpr int C = WarmNet::kInCh, S = WarmNet::kS2D;
+  const int Hn = H / S, Wn = W / S;
+  const long total = (long)batch * Hn * Wn * S * S * C;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
+    const int c = (int)(i % C);
+    long q = i / C;
+    const int sp = (int)(q % (S * S));
+    q /= S * S;
+    const int X = (int)(q % Wn);
+    q /= Wn;
+    const int Y = (int)(q % Hn);
+    const long n = q / Hn;
+    const long src = ((n * H + (long)S * Y + sp / S) * W + (long)S * X + sp % S) * C + c;
+    x32[i] = in[src] * sc[c];
   }
 }
 
-// out[p][c] = y16[p][c] * s, c < 2
-__global__ void k_net_unpack(const float* __restrict__ y16, long npix, float s, float* __restrict__ out) {
-  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < npix * 2; i += (long)gridDim.x * blockDim.x)
-    out[i] = y16[(i >> 1) * 16 + (i & 1)] * s;
+// Depth-to-space output: out[n][2Y + s / 2][2X + s % 2][c] = y16[n][Y][X][2 s + c] * scale, c < 2
+__global__ void k_net_unpack(const float* __restrict__ y16, int batch, int H, int W, float scale,
+                             float* __restrict__ out) {
+  constexpr int S = WarmNet::kS2D;
+  const int Wn = W / S;
+  const long total = (long)batch * H * W * 2;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
+    const int c = (int)(i & 1);
+    long q = i >> 1;
+    const int x = (int)(q % W);
+    q /= W;
+    const int y = (int)(q % H);
+    const long n = q / H;
+    const int sp = (y % S) * S + x % S;
+    const long pn = (n * (H / S) + y / S) * Wn + x / S;
+    out[i] = y16[pn * 16 + 2 * sp + c] * scale;
+  }
 }
 
 // 2x2 pooling: average (sum = 0) or sum (sum = 1); H, W of the input.
@@ -243,18 +291,29 @@ __global__ void k_unpool_add(float* __restrict__ g, const float* __restrict__ gp
   }
 }
 
-// Mean squared error of channels 0,1 of y16 against target * inv: gradient
-// into g16 (zero on channels 2..15) and per-block partial sums of squares.
-__global__ void __launch_bounds__(kNB) k_loss(const float* __restrict__ y16, const float* __restrict__ t, long npix,
-                                              float inv, float* __restrict__ g16, double* __restrict__ part) {
-  const double scale = 2.0 / (double)(npix * 2);
+// Mean squared error of output channels 0..7 of y16 (2 per sub-pixel) against
+// target * inv (target [n][H][W][2], the image's sides), gradient into g16
+// (zero on channels 8..15) and per-block partial sums of squares.
+__global__ void __launch_bounds__(kNB) k_loss(const float* __restrict__ y16, const float* __restrict__ t, int batch,
+                                              int H, int W, float inv, float* __restrict__ g16,
+                                              double* __restrict__ part) {
+  constexpr int S = WarmNet::kS2D;
+  const int Hn = H / S, Wn = W / S;
+  const long npos = (long)batch * Hn * Wn;
+  const double scale = 2.0 / (double)((long)batch * H * W * 2);
   double s = 0;
-  for (long i = (long)blockIdx.x * kNB + threadIdx.x; i < npix * 16; i += (long)gridDim.x * kNB) {
+  for (long i = (long)blockIdx.x * kNB + threadIdx.x; i < npos * 16; i += (long)gridDim.x * kNB) {
     const long p = i >> 4;
-    const int c = (int)(i & 15);
+    const int j = (int)(i & 15);
     float gv = 0.f;
-    if (c < 2) {
-      const float d = y16[i] - t[p * 2 + c] * inv;
+    if (j < 2 * S * S) {
+      const int sp = j >> 1, c = j & 1;
+      const int X = (int)(p % Wn);
+      const long q = p / Wn;
+      const int Y = (int)(q % Hn);
+      const long n = q / Hn;
+      const long ti = ((n * H + (long)S * Y + sp / S) * W + (long)S * X + sp % S) * 2 + c;
+      const float d = y16[i] - t[ti] * inv;
       s += (double)d * d;
       gv = (float)(scale * d);
     }
@@ -319,7 +378,7 @@ uint64_t splitmix(uint64_t& s) {
 const std::vector<NetLayer>& WarmNet::layers() {
   static const std::vector<NetLayer> L = [] {
     // ks, cin, cout, level, relu
-    const int t[kLayers][5] = {{3, 16, 32, 0, 1},   {3, 32, 64, 1, 1},   {3, 64, 128, 2, 1}, {3, 128, 256, 3, 1},
+    const int t[kLayers][5] = {{3, 32, 32, 0, 1},   {3, 32, 64, 1, 1},   {3, 64, 128, 2, 1}, {3, 128, 256, 3, 1},
                                {1, 256, 128, 3, 0}, {3, 128, 128, 2, 1}, {1, 128, 64, 2, 0}, {3, 64, 64, 1, 1},
                                {1, 64, 32, 1, 0},   {3, 32, 32, 0, 1},   {1, 32, 16, 0, 0}};
     std::vector<NetLayer> v;
@@ -359,8 +418,8 @@ WarmNet::WarmNet(int max_batch, long max_pixels, uint64_t seed) : max_batch_(max
     err_ = "no HIP device available";
     return;
   }
-  if (max_batch < 1 || max_pixels < 4096 || max_pixels % 4096 != 0) {
-    err_ = "warm-start network: max_batch >= 1 and max_pixels a multiple of 64x64 required";
+  if (max_batch < 1 || max_pixels < (long)kTile * kTile || max_pixels % ((long)kTile * kTile) != 0) {
+    err_ = "warm-start network: max_batch >= 1 and max_pixels a multiple of 16x16 required";
     return;
   }
   const size_t np = num_params();
@@ -371,11 +430,12 @@ WarmNet::WarmNet(int max_batch, long max_pixels, uint64_t seed) : max_batch_(max
   wb_ = (uint16_t*)dalloc(np * 2);
   wt_ = (uint16_t*)dalloc(np * 2);
   // activations: channels and level of each buffer
-  const int ach[A_N][2] = {{16, 0},  {32, 0},  {32, 1},  {64, 1},  {64, 2}, {128, 2}, {128, 3}, {256, 3},
+  // (levels of the network, whose level 0 is the space-to-depth image)
+  const int ach[A_N][2] = {{32, 0},  {32, 0},  {32, 1},  {64, 1},  {64, 2}, {128, 2}, {128, 3}, {256, 3},
                            {128, 3}, {128, 2}, {64, 2}, {64, 1}, {32, 1}, {32, 0},  {16, 0}};
   const int gch[G_N][2] = {{16, 0},  {32, 0}, {32, 0}, {32, 1},  {64, 1}, {64, 1}, {64, 2},
                            {128, 2}, {128, 2}, {128, 3}, {256, 3}, {128, 3}, {64, 2}, {32, 1}};
-  const long pix = (long)max_batch * max_pixels;
+  const long pix = (long)max_batch * max_pixels / (kS2D * kS2D);
   for (int a = 0; a < A_N; a++) act_[a] = (float*)dalloc(sizeof(float) * ach[a][0] * (pix >> (2 * ach[a][1])));
   for (int a = 0; a < G_N; a++) grd_[a] = (float*)dalloc(sizeof(float) * gch[a][0] * (pix >> (2 * gch[a][1])));
   // weight-gradient partials: at most 1024 workgroups' worth per layer
@@ -391,15 +451,15 @@ WarmNet::WarmNet(int max_batch, long max_pixels, uint64_t seed) : max_batch_(max
   dscale_ = (float*)dalloc(sizeof(float) * 16);
   WN_HIP(hipHostMalloc((void**)&lpart_h_, sizeof(double) * nlpart_, hipHostMallocDefault));
   if (!ok()) return;
-  // He-normal initialisation (seeded); padded input channels of L0 and the
-  // unused output channels of L10 start (and stay) zero; the output layer
-  // starts small so the first correction is small.
+  // He-normal initialisation (seeded); the unused output channels of L10
+  // start (and stay) zero; the output layer starts small so the first
+  // correction is small.
   std::vector<float> w(np, 0.f);
   uint64_t s = seed;
   for (int li = 0; li < kLayers; li++) {
     const NetLayer& l = layers()[li];
-    const int cin_used = li == 0 ? kInCh : l.cin;
-    const int cout_used = li == kLayers - 1 ? kOutCh : l.cout;
+    const int cin_used = l.cin;
+    const int cout_used = li == kLayers - 1 ? kOutCh * kS2D * kS2D : l.cout;
     const double sd = std::sqrt(2.0 / (l.ks * l.ks * cin_used)) * (li == kLayers - 1 ? 0.1 : 1.0);
     for (int tap = 0; tap < l.ks * l.ks; tap++)
       for (int co = 0; co < cout_used; co++)
@@ -461,7 +521,7 @@ void WarmNet::conv(int li, const float* x, const float* u, const float* m, float
   const uint16_t* wb = (transpose ? wt_ : wb_) + l.woff;
   const float* bias = transpose ? nullptr : w_ + l.boff;
   const int relu = transpose ? 0 : l.relu;
-  dim3 grid((H / 8) * (W / 8), batch, (cout + kCOT - 1) / kCOT);
+  dim3 grid(((H + 7) / 8) * ((W + 7) / 8), batch, (cout + kCOT - 1) / kCOT);
   if (l.ks == 3)
     k_conv<3><<<grid, kNB, 0, s>>>(x, u, m, wb, bias, y, cin, cout, cstore, H, W, relu);
   else
@@ -472,7 +532,7 @@ void WarmNet::wgrad(int li, const float* dy, const float* m, const float* x, con
                     int W, hipStream_t s) {
   const NetLayer& l = layers()[li];
   const long pairs = (long)(l.cout / 16) * (l.cin / 16);
-  const long tiles = (long)batch * (H / 8) * (W / 8);
+  const long tiles = (long)batch * ((H + 7) / 8) * ((W + 7) / 8);
   const long nch = std::min<long>(tiles, std::max<long>(1, 1024 / pairs));
   const int per = (int)((tiles + nch - 1) / nch);
   const long wlen = (long)l.ks * l.ks * l.cout * l.cin;
@@ -486,10 +546,12 @@ void WarmNet::wgrad(int li, const float* dy, const float* m, const float* x, con
   k_chunk_sum<<<grid_for(l.cout), kNB, 0, s>>>(bpart, l.cout, (int)nch, g_ + l.boff);
 }
 
-void WarmNet::forward_all(const float* in, int batch, int H, int W, hipStream_t s) {
-  const long P = (long)H * W, np = (long)batch * P;
+// H, W: the image's sides; the network runs on the space-to-depth image.
+void WarmNet::forward_all(const float* in, int batch, int Hi, int Wi, hipStream_t s) {
   WN_HIP(hipMemcpyAsync(dscale_, scales_, sizeof(float) * kScales, hipMemcpyHostToDevice, s));
-  k_net_pack<<<grid_for(np * 16), kNB, 0, s>>>(in, np, dscale_, act_[A_X]);
+  const int H = Hi / kS2D, W = Wi / kS2D;
+  const long np = (long)batch * H * W;
+  k_net_pack<<<grid_for(np * 32), kNB, 0, s>>>(in, batch, Hi, Wi, dscale_, act_[A_X]);
   const int H1 = H / 2, W1 = W / 2, H2 = H / 4, W2 = W / 4, H3 = H / 8, W3 = W / 8;
   conv(0, act_[A_X], nullptr, nullptr, act_[A_E0], 32, false, batch, H, W, s);
   k_pool<<<grid_for(np * 32 / 4), kNB, 0, s>>>(act_[A_E0], act_[A_P0], batch, H, W, 32, 0);
@@ -509,33 +571,35 @@ void WarmNet::forward_all(const float* in, int batch, int H, int W, hipStream_t 
 }
 
 static bool shape_ok(int batch, int H, int W, int max_batch, long max_pix) {
-  return batch >= 1 && batch <= max_batch && H >= 64 && W >= 64 && H % 64 == 0 && W % 64 == 0 &&
-         (long)H * W <= max_pix;
+  constexpr int T = WarmNet::kTile;
+  return batch >= 1 && batch <= max_batch && H >= T && W >= T && H % T == 0 && W % T == 0 && (long)H * W <= max_pix;
 }
+
+static const char* kShapeMsg = "warm-start network: image shape outside the workspace (H, W multiples of 16)";
 
 void WarmNet::forward(const float* in, float* out, int batch, int H, int W, hipStream_t s) {
   if (!ok()) return;
   if (!shape_ok(batch, H, W, max_batch_, max_pix_)) {
-    err_ = "warm-start network: image shape outside the workspace (H, W multiples of 64)";
+    err_ = kShapeMsg;
     return;
   }
   forward_all(in, batch, H, W, s);
-  const long np = (long)batch * H * W;
-  k_net_unpack<<<grid_for(np * 2), kNB, 0, s>>>(act_[A_OUT], np, scales_[kInCh], out);
+  k_net_unpack<<<grid_for((long)batch * H * W * 2), kNB, 0, s>>>(act_[A_OUT], batch, H, W, scales_[kInCh], out);
   WN_HIP(hipGetLastError());
 }
 
-double WarmNet::loss_grad(const float* in, const float* target, int batch, int H, int W, hipStream_t s) {
+double WarmNet::loss_grad(const float* in, const float* target, int batch, int Hi, int Wi, hipStream_t s) {
   if (!ok()) return 0;
-  if (!shape_ok(batch, H, W, max_batch_, max_pix_)) {
-    err_ = "warm-start network: image shape outside the workspace (H, W multiples of 64)";
+  if (!shape_ok(batch, Hi, Wi, max_batch_, max_pix_)) {
+    err_ = kShapeMsg;
     return 0;
   }
-  forward_all(in, batch, H, W, s);
+  forward_all(in, batch, Hi, Wi, s);
+  const int H = Hi / kS2D, W = Wi / kS2D;
   const long np = (long)batch * H * W;
   const int H1 = H / 2, W1 = W / 2, H2 = H / 4, W2 = W / 4, H3 = H / 8, W3 = W / 8;
   const int nb = std::min(nlpart_, grid_for(np * 16));
-  k_loss<<<nb, kNB, 0, s>>>(act_[A_OUT], target, np, 1.0f / scales_[kInCh], grd_[G_OUT], lpart_);
+  k_loss<<<nb, kNB, 0, s>>>(act_[A_OUT], target, batch, Hi, Wi, 1.0f / scales_[kInCh], grd_[G_OUT], lpart_);
   WN_HIP(hipMemcpyAsync(lpart_h_, lpart_, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
   // decoder, top-down
   wgrad(10, grd_[G_OUT], nullptr, act_[A_D0], nullptr, batch, H, W, s);
@@ -570,7 +634,7 @@ double WarmNet::loss_grad(const float* in, const float* target, int batch, int H
   WN_HIP(hipStreamSynchronize(s));
   double loss = 0;
   for (int j = 0; j < nb; j++) loss += lpart_h_[j];
-  return loss / (double)(np * 2);
+  return loss / (double)((long)batch * Hi * Wi * 2);
 }
 
 void WarmNet::adam(float lr, hipStream_t s) {
@@ -582,7 +646,7 @@ void WarmNet::adam(float lr, hipStream_t s) {
   refresh_bf16(s);
 }
 
-// File: "ISGWNET1", uint32 parameter count, 7 float scales, float parameters.
+// File: "ISGWNET2", uint32 parameter count, 9 float scales, float parameters.
 bool WarmNet::save(const char* path) {
   std::vector<float> w;
   get_params(w);
@@ -593,7 +657,7 @@ bool WarmNet::save(const char* path) {
     return false;
   }
   uint32_t n = (uint32_t)w.size();
-  bool good = std::fwrite("ISGWNET1", 1, 8, f) == 8 && std::fwrite(&n, 4, 1, f) == 1 &&
+  bool good = std::fwrite("ISGWNET2", 1, 8, f) == 8 && std::fwrite(&n, 4, 1, f) == 1 &&
               std::fwrite(scales_, 4, kScales, f) == (size_t)kScales && std::fwrite(w.data(), 4, n, f) == n;
   good = (std::fclose(f) == 0) && good;
   if (!good) err_ = std::string("write failed: ") + path;
@@ -610,7 +674,7 @@ bool WarmNet::load(const char* path) {
   uint32_t n = 0;
   float sc[kScales];
   std::vector<float> w;
-  bool good = std::fread(magic, 1, 8, f) == 8 && std::memcmp(magic, "ISGWNET1", 8) == 0 &&
+  bool good = std::fread(magic, 1, 8, f) == 8 && std::memcmp(magic, "ISGWNET2", 8) == 0 &&
               std::fread(&n, 4, 1, f) == 1 && n == num_params() && std::fread(sc, 4, kScales, f) == (size_t)kScales;
   if (good) {
     w.resize(n);

@@ -1,26 +1,38 @@
 // isg_warmnet.hpp -- learned warm start for the DBIM total-field solves.
 //
 // The reference restarts every total-field solve from zero (mom.cpp:324).
-// This network maps, per transmitter, the new contrast, the incident field
-// and the previous DBIM total field (6 real channels) to a correction of the
-// previous total field (2 real channels); the solver then starts from
-// tot_prev + lambda * correction with lambda chosen per transmitter to
-// minimise the initial residual (so the start is never worse than tot_prev).
+// This network maps, per transmitter, the new contrast, the incident field,
+// the first-order start tot_prev - alpha dtot and its first-order step
+// -alpha dtot (8 real channels; dtot is the Frechet-derivative field the
+// step-length phase solved for, mom.cpp:284-290) to a correction of the
+// first-order start (2 real channels); the solver then starts from
+// first-order start + lambda * correction with lambda chosen per transmitter
+// to minimise the initial residual (so the start is never worse than the
+// first-order one).  The field channels arrive divided by the transmitter's
+// incident-field rms, the correction leaves multiplied by it.
+//
+// The image enters space-to-depth by 2 (each 2x2 pixel block becomes one
+// position with 4 x 8 = 32 channels) and leaves depth-to-space (16 channels,
+// 4 x 2 used): the network runs at half the pixel resolution, 5 positions per
+// wavelength at lambda/10 pixels, with nothing discarded.
 //
 // Network: 4-level convolutional encoder-decoder, 32 base channels.
-//   encoder  L0 conv3 16->32 (input padded 6->16), avg-pool,
+//   encoder  L0 conv3 32->32, avg-pool,
 //            L1 conv3 32->64, avg-pool, L2 conv3 64->128, avg-pool,
-//            L3 conv3 128->256 (bottleneck, 1/8 resolution)
+//            L3 conv3 128->256 (bottleneck, 1/8 of the network's resolution)
 //   decoder  L4 conv1 256->128, L5 conv3 (e2 + up(u3)) 128->128,
 //            L6 conv1 128->64,  L7 conv3 (e1 + up(u2)) 64->64,
 //            L8 conv1 64->32,   L9 conv3 (e0 + up(u1)) 32->32,
-//            L10 conv1 32->16 (channels 0,1 are the output)
+//            L10 conv1 32->16 (channels 0..7 are the output)
 // ReLU after every 3x3 convolution; up() is nearest-neighbour 2x.
 // Activations are FP32 in HBM, NHWC ([image][row][col][channel]); weights are
 // kept as FP32 masters and used as BF16; every convolution rounds its two
-// operands to BF16 and accumulates in FP32 (v_mfma_f32_16x16x16_bf16 for the
-// forward and input-gradient passes).  Images must have H and W divisible by
-// 64 (8x8-pixel tiles at the 1/8-resolution level).
+// operands to BF16 and accumulates in FP32 on gfx950's
+// v_mfma_f32_16x16x32_bf16 (32 input channels per instruction;
+// v_mfma_f32_16x16x16_bf16 for the 16-channel input of L10's backward pass).
+// Images must have H and W divisible by 16 (three 2x2 poolings after the
+// space-to-depth step); the convolutions run on 8x8-position tiles, the last
+// ones partial where a level's side is not a multiple of 8.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -42,8 +54,10 @@ struct NetLayer {
 
 class WarmNet {
  public:
-  static constexpr int kInCh = 6, kOutCh = 2, kLayers = 11, kTile = 64;
-  static constexpr int kScales = 7;  // 6 input channel scales + 1 output scale
+  static constexpr int kInCh = 8, kOutCh = 2, kLayers = 11;
+  static constexpr int kS2D = 2;        // space-to-depth factor at the input
+  static constexpr int kTile = 8 * kS2D;  // image sides must be multiples of this (3 pools after s2d)
+  static constexpr int kScales = kInCh + 1;  // input channel scales + 1 output scale
 
   // Workspace for up to max_batch images of max_pixels pixels each.
   WarmNet(int max_batch, long max_pixels, uint64_t seed);
@@ -65,7 +79,7 @@ class WarmNet {
   void set_scales(const float* s) { for (int i = 0; i < kScales; i++) scales_[i] = s[i]; }
   const float* scales() const { return scales_; }
 
-  // in: [batch][H][W][6] FP32, out: [batch][H][W][2] FP32 (device pointers).
+  // in: [batch][H][W][8] FP32, out: [batch][H][W][2] FP32 (device pointers).
   void forward(const float* in, float* out, int batch, int H, int W, hipStream_t s);
   // Forward + backward of the mean squared error against target
   // [batch][H][W][2]; gradients go to the internal gradient array.  Returns
@@ -89,7 +103,7 @@ class WarmNet {
   std::string err_;
   int max_batch_ = 0;
   long max_pix_ = 0;
-  float scales_[kScales] = {1, 1, 1, 1, 1, 1, 1};
+  float scales_[kScales] = {1, 1, 1, 1, 1, 1, 1, 1, 1};
   float* w_ = nullptr;       // FP32 masters
   float* g_ = nullptr;       // gradients
   float* m1_ = nullptr;      // Adam moments

@@ -374,7 +374,7 @@ class Solver:
         return rc, [{"status": i.status, "iters": i.iters, "relres": i.relres} for i in infos]
 
 
-_DBIM_FIELDS = {"o": 0, "od": 1, "ot": 2, "mes": 3, "mesd": 4, "inc": 5, "tot": 6}
+_DBIM_FIELDS = {"o": 0, "od": 1, "ot": 2, "mes": 3, "mesd": 4, "inc": 5, "tot": 6, "dtot": 8}
 
 
 def _stats_dict(st):
@@ -413,7 +413,8 @@ class Dbim:
 
     def _size(self, name):
         T, R, N = self.s.cfg.numtx, self.s.cfg.numrx, self.s.N
-        return {"o": N, "od": N, "ot": N, "mes": T * R, "mesd": T * R, "inc": T * N, "tot": T * N}[name]
+        return {"o": N, "od": N, "ot": N, "mes": T * R, "mesd": T * R, "inc": T * N, "tot": T * N,
+                "dtot": T * N}[name]
 
     def get(self, name):
         out = np.empty(self._size(name), np.complex128)
@@ -445,11 +446,14 @@ def _fp(a):
 
 class WarmNet:
     """Warm-start network on the GPU (csrc/isg_warmnet.hip): a 4-level conv
-    encoder-decoder from (Re/Im contrast, Re/Im incident, Re/Im previous total
-    field) to a 2-channel correction of the previous total field.  Images are
-    raster NHWC FP32 with H, W multiples of 64."""
+    encoder-decoder from 8 channels (Re/Im contrast, Re/Im incident field,
+    Re/Im first-order start, Re/Im its first-order step; fields divided by the
+    transmitter's incident-field rms) to a 2-channel correction of the
+    first-order start.  Images are raster NHWC FP32 with H, W multiples of 16
+    (the network runs on their space-to-depth-by-2 form)."""
 
-    NUM_SCALES = 7
+    NUM_SCALES = 9
+    IN_CH = 8
 
     def __init__(self, max_batch=4, max_pixels=512 * 512, seed=1, path=None):
         self.lib = load()
@@ -512,7 +516,7 @@ class WarmNet:
     def scales(self, s):
         s = np.ascontiguousarray(s, dtype=np.float32)
         if s.size != self.NUM_SCALES:
-            raise ValueError("expected 7 scales")
+            raise ValueError("expected %d scales" % self.NUM_SCALES)
         self._err(self.lib.isg_warmnet_set_scales(self.h, _fp(s)), "isg_warmnet_set_scales")
 
     def forward_dev(self, in_ptr, out_ptr, batch, H, W):
@@ -536,7 +540,7 @@ class WarmNet:
 
     # host arrays (staged through device buffers the library keeps)
     def forward(self, x):
-        """x: (batch, H, W, 6) float32 host array -> (batch, H, W, 2)."""
+        """x: (batch, H, W, 8) float32 host array -> (batch, H, W, 2)."""
         x = np.ascontiguousarray(x, dtype=np.float32)
         out = np.empty(x.shape[:3] + (2,), np.float32)
         self._err(self.lib.isg_warmnet_forward(self.h, _fp(x), _fp(out), *x.shape[:3]), "isg_warmnet_forward")

@@ -8,12 +8,14 @@ The network has no counterpart in the reference (whose total-field solves
 restart from zero, mom.cpp:324); this is a plain numpy restatement of the
 same 4-level encoder-decoder, written independently of the kernels:
 
-  L0 conv3 16->32, avg-pool, L1 conv3 32->64, avg-pool, L2 conv3 64->128,
+  input (n, H, W, 8) times per-channel scales, space-to-depth by 2:
+  position (Y, X) holds channel 8 s + c = input (2Y + s // 2, 2X + s % 2, c);
+  L0 conv3 32->32, avg-pool, L1 conv3 32->64, avg-pool, L2 conv3 64->128,
   avg-pool, L3 conv3 128->256, L4 conv1 256->128, L5 conv3(e2 + up(u3)),
   L6 conv1 128->64, L7 conv3(e1 + up(u2)), L8 conv1 64->32,
-  L9 conv3(e0 + up(u1)), L10 conv1 32->16 (channels 0,1 = output);
-  ReLU after every 3x3 convolution; up = nearest 2x; input channels 6 padded
-  to 16 and multiplied by per-channel scales, output times an output scale.
+  L9 conv3(e0 + up(u1)), L10 conv1 32->16; output channels 2 s + c (s < 4,
+  c < 2) go back depth-to-space to (n, H, W, 2), times an output scale;
+  ReLU after every 3x3 convolution; up = nearest 2x.
 
 Parameters: per layer W[ks*ks][cout][cin] then bias[cout], flat float32.
 `rnd=True` reproduces the GPU's numerics: both operands of every
@@ -28,11 +30,11 @@ and a 1x1-identity network that reproduces its input.
 import numpy as np
 
 LAYERS = [  # ks, cin, cout, level, relu
-    (3, 16, 32, 0, 1), (3, 32, 64, 1, 1), (3, 64, 128, 2, 1), (3, 128, 256, 3, 1),
+    (3, 32, 32, 0, 1), (3, 32, 64, 1, 1), (3, 64, 128, 2, 1), (3, 128, 256, 3, 1),
     (1, 256, 128, 3, 0), (3, 128, 128, 2, 1), (1, 128, 64, 2, 0), (3, 64, 64, 1, 1),
     (1, 64, 32, 1, 0), (3, 32, 32, 0, 1), (1, 32, 16, 0, 0),
 ]
-IN_CH, OUT_CH = 6, 2
+IN_CH, OUT_CH, S2D = 8, 2, 2
 
 
 def offsets():
@@ -64,6 +66,18 @@ def bf16(x):
     u = a.view(np.uint32).astype(np.uint64)
     u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
     return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def s2d(x):
+    """(n, H, W, C) -> (n, H/2, W/2, 4 C), channel C s + c with s = 2 dy + dx."""
+    n, H, W, C = x.shape
+    return x.reshape(n, H // 2, 2, W // 2, 2, C).transpose(0, 1, 3, 2, 4, 5).reshape(n, H // 2, W // 2, 4 * C)
+
+
+def d2s(y, C):
+    """Inverse of s2d for the first 4 C channels: (n, h, w, >= 4 C) -> (n, 2h, 2w, C)."""
+    n, h, w, _ = y.shape
+    return y[..., :4 * C].reshape(n, h, w, 2, 2, C).transpose(0, 1, 3, 2, 4, 5).reshape(n, 2 * h, 2 * w, C)
 
 
 def up2(u):
@@ -114,13 +128,11 @@ def wgrad(dpre, inp, ks, rnd):
     return gw, a.reshape(-1, cout).sum(axis=0)
 
 
-def forward(params, x6, scales, rnd=True):
-    """x6: (n, H, W, 6).  Returns the activations dict; 'y' = (n, H, W, 2)
+def forward(params, x8, scales, rnd=True):
+    """x8: (n, H, W, 8).  Returns the activations dict; 'y' = (n, H, W, 2)
     output times the output scale."""
     P = unpack(np.asarray(params, dtype=np.float64))
-    n, H, W, _ = x6.shape
-    x = np.zeros((n, H, W, 16))
-    x[..., :IN_CH] = x6 * np.asarray(scales[:IN_CH])[None, None, None, :]
+    x = s2d(np.asarray(x8, dtype=np.float64) * np.asarray(scales[:IN_CH], dtype=np.float64)[None, None, None, :])
     A = {"x": x}
 
     def layer(i, inp):
@@ -143,19 +155,20 @@ def forward(params, x6, scales, rnd=True):
     A["u1"] = layer(8, A["d1"])
     A["d0"] = layer(9, conv_in(A["e0"], A["u1"]))
     A["out"] = layer(10, A["d0"])
-    A["y"] = A["out"][..., :OUT_CH] * scales[IN_CH]
+    A["y"] = d2s(A["out"], OUT_CH) * scales[IN_CH]
     return A
 
 
-def loss_grad(params, x6, target, scales, rnd=True):
-    """Mean squared error of output channels 0,1 against target / out_scale
-    and its gradient with respect to the flat parameters."""
+def loss_grad(params, x8, target, scales, rnd=True):
+    """Mean squared error of the (depth-to-space) output against
+    target / out_scale and its gradient with respect to the flat parameters."""
     P = unpack(np.asarray(params, dtype=np.float64))
-    A = forward(params, x6, scales, rnd)
-    d = A["out"][..., :OUT_CH] - np.asarray(target) / scales[IN_CH]
+    A = forward(params, x8, scales, rnd)
+    k = S2D * S2D * OUT_CH
+    d = A["out"][..., :k] - s2d(np.asarray(target, dtype=np.float64) / scales[IN_CH])
     loss = float(np.mean(d ** 2))
     g_out = np.zeros_like(A["out"])
-    g_out[..., :OUT_CH] = 2.0 * d / d.size
+    g_out[..., :k] = 2.0 * d / d.size
     offs, npar = offsets()
     G = np.zeros(npar)
 

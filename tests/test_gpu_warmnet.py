@@ -1,6 +1,7 @@
 """Warm-start network kernels on the GPU against the CPU oracle.
 
-Workload: 2 images of 64x64 (and one 128x64) pixels, random inputs, seeded
+Workload: 2 images of 64x64 (and one 128x64, and one 48x80 whose levels end
+in partial 8x8 tiles) pixels with 8 input channels, random inputs, seeded
 He-normal weights.  The oracle reproduces the GPU's rounding points (both
 operands of every convolution rounded to BF16); the remaining differences are
 FP32 versus float64 accumulation, and the BF16 roundings they occasionally
@@ -14,7 +15,7 @@ from conftest import relerr
 
 pytestmark = pytest.mark.gpu
 
-SCALES = np.array([3.0, 3.0, 1.0, 1.0, 1.0, 1.0, 0.05], np.float32)
+SCALES = np.array([3.0, 3.0, 1.0, 1.0, 1.0, 1.0, 2.0, 2.0, 0.05], np.float32)
 
 
 @pytest.fixture(scope="module")
@@ -27,12 +28,12 @@ def net():
 
 def data(shape, seed):
     rng = np.random.default_rng(seed)
-    x = rng.standard_normal(shape + (6,)).astype(np.float32)
+    x = rng.standard_normal(shape + (8,)).astype(np.float32)
     t = (0.05 * rng.standard_normal(shape + (2,))).astype(np.float32)
     return x, t
 
 
-@pytest.mark.parametrize("shape", [(2, 64, 64), (1, 128, 64)])
+@pytest.mark.parametrize("shape", [(2, 64, 64), (1, 128, 64), (1, 48, 80)])
 def test_forward_matches_oracle(net, shape):
     from oracle import warmnet_oracle as wo
     print("warm-start network forward", shape, flush=True)

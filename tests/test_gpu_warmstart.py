@@ -76,16 +76,16 @@ def test_first_order_start_matches_oracle_and_saves_iterations(case, precision):
 
 @pytest.mark.parametrize("precision", [0, 1])
 def test_learned_start_falls_back_with_untrained_network(case, precision):
-    """An untrained network gives a useless correction; the residual-minimising
-    blend must keep the solve at (about) the previous-iterate start's cost and
-    the inversion on the oracle's trajectory."""
+    """An untrained network gives a useless correction of the first-order
+    start; the residual-minimising blend must keep the solve at (about) the
+    first-order start's cost and the inversion on the oracle's trajectory."""
     isg, otrue, o_ref = case
     print("learned start, untrained network, precision", precision, flush=True)
     net = isg.WarmNet(max_batch=8, max_pixels=64 * 64, seed=5)
     o_l, it_l = run(isg, otrue, 3, precision, net=net, key="field_iters")
-    _, it_p = run(isg, otrue, 1, precision, key="field_iters")
+    _, it_f = run(isg, otrue, 2, precision, key="field_iters")
     assert relerr(o_l, o_ref) < 1e-4
-    assert it_l <= it_p + 3 * NTX, (it_l, it_p)
+    assert it_l <= it_f + 3 * NTX, (it_l, it_f)
 
 
 def test_learned_start_needs_a_network(case):
@@ -96,38 +96,47 @@ def test_learned_start_needs_a_network(case):
         d.iterate()
 
 
-WEIGHTS = os.path.join(os.path.dirname(__file__), "..", "inverse-solver-gpu_amd", "weights", "warmnet_512.bin")
+WEIGHTS = os.path.join(os.path.dirname(__file__), "..", "inverse-solver-gpu_amd", "weights", "warmnet.bin")
 
 
-def test_trained_network_gain():
-    """The committed network (trained on 512x512 ellipse phantoms) on a
-    held-out 256x256 ellipse phantom, 16 Tx, iterations 2-3: the learned start
-    must beat the zero start of the reference and stay within 10% of the
-    previous-iterate start (it does not beat it; see the 512x512 evaluation),
-    all starts ending on the same contrast."""
+@pytest.mark.timeout(300)
+@pytest.mark.parametrize("radius", [30.0, 0.0])
+def test_trained_network_gain(radius):
+    """The committed network on held-out 512x512 ellipse phantoms (seed
+    900000; 64 transmitters and 256 receivers, line sources on a 30-wavelength
+    circle or plane waves), DBIM iterations 2-4: the learned start needs fewer
+    total-field BiCGSTAB iterations than the first-order start it corrects
+    (and than the previous iterate), the network's correction is used (mean
+    |lambda| of the blend between 0.5 and 1.5), and every start ends on the
+    contrast of the reference's zero start within 1e-4."""
     import isg
-    print("trained warm-start network gain, 256x256", flush=True)
-    level, ntx, nrx = 6, 16, 64
+    print("trained warm-start network gain, 512x512, radius", radius, flush=True)
+    level, ntx, nrx = 7, 64, 256
     side = 8 << (level - 1)
-    net = isg.WarmNet(max_batch=16, max_pixels=side * side, path=WEIGHTS)
-
-    def solver():
-        return isg.Solver(level=level, numrx=nrx, numtx=ntx, toli=TOL, iti=300, regular=REG, max_rhs=ntx,
-                          precision=1)
-
-    s = solver()
+    net = isg.WarmNet(max_batch=8, max_pixels=side * side, path=WEIGHTS)
+    s = isg.Solver(level=level, numrx=nrx, numtx=ntx, toli=TOL, iti=300, regular=REG, max_rhs=ntx, precision=1,
+                   radius=radius)
     _, r2t = s.geometry()
-    otrue = isg.raster_to_tree(isg.phantom_ellipses(side, 8, 900000), r2t)
+    otrue = isg.raster_to_tree(isg.phantom_ellipses(side, 20, 900000), r2t)
     mes = isg.Dbim(s, otrue, field_start=2).get("mes")
-    iters, objs = {}, {}
-    for kind, fs in (("zero", 0), ("previous", 1), ("learned", 3)):
+    iters, objs, lam = {}, {}, []
+    for kind, fs in (("zero", 0), ("previous", 1), ("first-order", 2), ("learned", 3)):
         d = isg.Dbim(s, otrue, field_start=fs, measurements=mes)
         if fs == 3:
             d.set_warmnet(net)
         d.iterate()
-        iters[kind] = sum(d.iterate()["field_iters"] for _ in range(2))
+        iters[kind] = 0
+        for _ in range(3):
+            st = d.iterate()
+            iters[kind] += st["field_iters"]
+            if fs == 3:
+                lam.append(st["lambda_abs"])
         objs[kind] = d.get("o")
-    print("field iterations", iters)
-    assert iters["learned"] < iters["zero"]
-    assert iters["learned"] <= 1.1 * iters["previous"]
-    assert relerr(objs["learned"], objs["zero"]) < 1e-4
+        del d
+    s.close()
+    print("field iterations", iters, "mean |lambda|", np.mean(lam), flush=True)
+    assert iters["learned"] < iters["first-order"] < iters["zero"]
+    assert iters["learned"] < iters["previous"]
+    assert 0.5 < np.mean(lam) < 1.5
+    for kind in ("previous", "first-order", "learned"):
+        assert relerr(objs[kind], objs["zero"]) < 1e-4, kind
