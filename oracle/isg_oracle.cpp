@@ -832,6 +832,13 @@ double dbim_iterate(Oracle& o, Dbim& d) {
 
 // ================================================================ C ABI ====
 // Complex arrays are interleaved (re, im) doubles, i.e. std::complex<double>.
+// copy a table into a caller's array (nothing for an empty table, whose data()
+// may be null)
+template <typename T, typename U>
+void copy_out(U* out, const std::vector<T>& v) {
+  if (!v.empty()) memcpy(out, v.data(), sizeof(T) * v.size());
+}
+
 extern "C" {
 
 void* orc_create(int level, double res, double scale, int beta, int ninter, int numrx) {
@@ -871,59 +878,58 @@ void orc_centres(void* h, int i, double* out) {
 }
 void orc_near_list(void* h, int i, int* out) {
   Oracle* o = (Oracle*)h;
-  memcpy(out, o->cnear[i].data(), sizeof(int) * o->cnear[i].size());
+  copy_out(out, o->cnear[i]);
 }
 void orc_far_list(void* h, int i, int* out) {
   Oracle* o = (Oracle*)h;
-  memcpy(out, o->cfar[i].data(), sizeof(int) * o->cfar[i].size());
+  copy_out(out, o->cfar[i]);
 }
 void orc_traid(void* h, int i, int* out) {
   Oracle* o = (Oracle*)h;
-  memcpy(out, o->traid[i].data(), sizeof(int) * o->traid[i].size());
+  copy_out(out, o->traid[i]);
 }
 void orc_near_blocks(void* h, double* out) {
   Oracle* o = (Oracle*)h;
-  memcpy(out, o->nearb.data(), sizeof(cd) * o->nearb.size());
+  copy_out(out, o->nearb);
 }
 void orc_coeff(void* h, double* out) {
   Oracle* o = (Oracle*)h;
-  memcpy(out, o->coeff.data(), sizeof(cd) * o->coeff.size());
+  copy_out(out, o->coeff);
 }
 void orc_basis(void* h, double* out) {
   Oracle* o = (Oracle*)h;
-  memcpy(out, o->basis.data(), sizeof(cd) * o->basis.size());
+  copy_out(out, o->basis);
 }
 void orc_trans(void* h, int i, double* out) {
   Oracle* o = (Oracle*)h;
-  memcpy(out, o->trans[i].data(), sizeof(cd) * o->trans[i].size());
+  copy_out(out, o->trans[i]);
 }
 void orc_shift(void* h, int i, double* mul, double* loc) {
   Oracle* o = (Oracle*)h;
-  memcpy(mul, o->shm[i].data(), sizeof(cd) * o->shm[i].size());
-  memcpy(loc, o->shl[i].data(), sizeof(cd) * o->shl[i].size());
+  copy_out(mul, o->shm[i]);
+  copy_out(loc, o->shl[i]);
 }
 void orc_interp(void* h, int i, double* w, int* idx) {
   Oracle* o = (Oracle*)h;
-  memcpy(w, o->interp[i].data(), sizeof(double) * o->interp[i].size());
-  memcpy(idx, o->intind[i].data(), sizeof(int) * o->intind[i].size());
+  copy_out(w, o->interp[i]);
+  copy_out(idx, o->intind[i]);
 }
 void orc_anterp(void* h, int i, double* w, int* idx) {
   Oracle* o = (Oracle*)h;
-  memcpy(w, o->anterp[i].data(), sizeof(double) * o->anterp[i].size());
-  memcpy(idx, o->antind[i].data(), sizeof(int) * o->antind[i].size());
+  copy_out(w, o->anterp[i]);
+  copy_out(idx, o->antind[i]);
 }
 void orc_rx_interp(void* h, double* w, int* idx, double* aw, int* aidx) {
   Oracle* o = (Oracle*)h;
-  memcpy(w, o->interph.data(), sizeof(double) * o->interph.size());
-  memcpy(idx, o->intindh.data(), sizeof(int) * o->intindh.size());
-  memcpy(aw, o->anterph.data(), sizeof(double) * o->anterph.size());
-  memcpy(aidx, o->antindh.data(), sizeof(int) * o->antindh.size());
+  copy_out(w, o->interph);
+  copy_out(idx, o->intindh);
+  copy_out(aw, o->anterph);
+  copy_out(aidx, o->antindh);
 }
 // multipoles (which=0) or locals (which=1) of level i after the last operator call
 void orc_stage(void* h, int which, int i, double* out) {
   Oracle* o = (Oracle*)h;
-  const auto& v = which == 0 ? o->agm[i] : o->agl[i];
-  memcpy(out, v.data(), sizeof(cd) * v.size());
+  copy_out(out, which == 0 ? o->agm[i] : o->agl[i]);
 }
 
 void orc_green(void* h, double tr, double ti, double br, double bi, double* out) {
@@ -996,7 +1002,7 @@ void orc_dbim_get(void* h, void* dd, int which, double* out) {
     default: return;
   }
   (void)o;
-  memcpy(out, v->data(), sizeof(cd) * v->size());
+  copy_out(out, *v);
 }
 void orc_dbim_counts(void* dd, long* out) {
   Dbim* d = (Dbim*)dd;
