@@ -14,6 +14,7 @@
 #include <rocblas/rocblas.h>
 
 #include "isg_kernels.hpp"
+#include "isg_setup_kernels.hpp"
 #include "isg_solver_kernels.hpp"
 
 namespace isg {
@@ -168,20 +169,45 @@ void Engine::upload(DevOp<T>& d) {
     return p;
   };
   int L = P_.L;
-  d.near = up_c(P_.near);
-  d.coeffT = up_c(P_.coeffT);
-  d.basisT = up_c(P_.basisT);
   d.interpT.assign(L, nullptr);
   d.anterpT.assign(L, nullptr);
   d.shiftM.assign(L, nullptr);
   d.shiftL.assign(L, nullptr);
   d.trans.assign(L, nullptr);
+  if (cfg_.plan.host_tables) {
+    d.near = up_c(P_.near);
+    d.coeffT = up_c(P_.coeffT);
+    d.basisT = up_c(P_.basisT);
+    for (int i = 0; i < L; i++) {
+      if (!P_.shiftM[i].empty()) d.shiftM[i] = up_c(P_.shiftM[i]);
+      if (!P_.shiftL[i].empty()) d.shiftL[i] = up_c(P_.shiftL[i]);
+      if (!P_.trans[i].empty()) d.trans[i] = up_c(P_.trans[i]);
+    }
+  } else if constexpr (sizeof(T) == 8) {
+    device_tables(d);
+  } else {
+    // the FP32 operator's tables: the FP64 device tables rounded
+    auto cvt = [&](const zd* src, size_t n) -> cplx<T>* {
+      cplx<T>* p = (cplx<T>*)dalloc(n * sizeof(cplx<T>));
+      if (p && src)
+        setup::k_table_f32<<<(unsigned)std::min<size_t>(4096, (n + 255) / 256), 256, 0, stream_>>>(src, p, (long)n);
+      return p;
+    };
+    const int Sl = P_.nsamp[L - 1];
+    d.near = cvt(op64_.near, (size_t)9 * 64 * 64);
+    d.coeffT = cvt(op64_.coeffT, (size_t)64 * Sl);
+    d.basisT = cvt(op64_.basisT, (size_t)Sl * 64);
+    for (int i = 0; i < L; i++) {
+      if (op64_.shiftM[i]) d.shiftM[i] = cvt(op64_.shiftM[i], (size_t)4 * P_.nsamp[i]);
+      if (op64_.shiftL[i]) d.shiftL[i] = cvt(op64_.shiftL[i], (size_t)4 * P_.nsamp[i]);
+      if (op64_.trans[i]) d.trans[i] = cvt(op64_.trans[i], (size_t)49 * P_.nsamp[i]);
+    }
+    ISG_HIP(hipGetLastError());
+    ISG_HIP(hipStreamSynchronize(stream_));
+  }
   for (int i = 0; i < L; i++) {
     if (!P_.interpT[i].empty()) d.interpT[i] = up_r(P_.interpT[i]);
     if (!P_.anterpT[i].empty()) d.anterpT[i] = up_r(P_.anterpT[i]);
-    if (!P_.shiftM[i].empty()) d.shiftM[i] = up_c(P_.shiftM[i]);
-    if (!P_.shiftL[i].empty()) d.shiftL[i] = up_c(P_.shiftL[i]);
-    if (!P_.trans[i].empty()) d.trans[i] = up_c(P_.trans[i]);
   }
   d.rxT = up_r(P_.rxT);
   d.rxaT = up_r(P_.rxaT);
@@ -279,6 +305,48 @@ Engine::Engine(const EngineConfig& cfg) : cfg_(cfg) {
     ISG_HIP(hipMemset(active_, 0, (size_t)R * sizeof(int)));
   }
   if (ok() && cfg_.plan.radius > 0) setup_receivers();
+}
+
+// Operator set-up on the device (setup_mlfma.cpp:453-631): near blocks,
+// P2M/L2P patterns, shifts and translation operators computed by kernels into
+// the FP64 tables; the Hankel functions of each level's 40 translation
+// offsets come from one thread per offset (isg_setup_kernels.hpp).
+void Engine::device_tables(DevOp<double>& d) {
+  const int L = P_.L, Sl = P_.nsamp[L - 1];
+  const double k0 = P_.k0, res = cfg_.plan.res;
+  d.near = (zd*)dalloc((size_t)9 * 64 * 64 * sizeof(zd));
+  d.coeffT = (zd*)dalloc((size_t)64 * Sl * sizeof(zd));
+  d.basisT = (zd*)dalloc((size_t)Sl * 64 * sizeof(zd));
+  for (int i = 0; i < L - 1; i++) {
+    d.shiftM[i] = (zd*)dalloc((size_t)4 * P_.nsamp[i] * sizeof(zd));
+    d.shiftL[i] = (zd*)dalloc((size_t)4 * P_.nsamp[i] * sizeof(zd));
+  }
+  int Pmax = 0, Mmax = 0;
+  std::vector<int> Mlev(L, 0);
+  for (int i = 2; i < L; i++) {
+    d.trans[i] = (zd*)dalloc((size_t)49 * P_.nsamp[i] * sizeof(zd));
+    // backward-recurrence start: past the largest order and argument of the level
+    const double xmax = k0 * P_.csize[i] * std::sqrt(18.0);
+    const int top = std::max(P_.nterm[i], (int)std::ceil(xmax));
+    Mlev[i] = top + 32 + (int)std::sqrt(40.0 * top);
+    Pmax = std::max(Pmax, P_.nterm[i]);
+    Mmax = std::max(Mmax, Mlev[i]);
+  }
+  zd* h = (zd*)dalloc((size_t)49 * (Pmax + 1) * sizeof(zd));
+  double* jw = (double*)dalloc((size_t)49 * (Mmax + 2) * sizeof(double));
+  if (!ok()) return;
+  setup::k_near_table<<<(9 * 64 * 64 + 255) / 256, 256, 0, stream_>>>(k0, res, d.near);
+  setup::k_p2m_table<<<(64 * Sl + 255) / 256, 256, 0, stream_>>>(k0, res, P_.csize[L - 1], Sl, d.coeffT, d.basisT);
+  for (int i = 0; i < L - 1; i++)
+    setup::k_shift_table<<<(4 * P_.nsamp[i] + 255) / 256, 256, 0, stream_>>>(k0, P_.csize[i] / 4, P_.nsamp[i],
+                                                                            d.shiftM[i], d.shiftL[i]);
+  for (int i = 2; i < L; i++) {
+    const int S = P_.nsamp[i], Pn = P_.nterm[i];
+    setup::k_hankel<<<1, 64, 0, stream_>>>(k0, P_.csize[i], Pn, Mlev[i], jw, h);
+    setup::k_trans_table<<<dim3((S + 255) / 256, 49), 256, 0, stream_>>>(S, Pn, P_.csize[i], h, d.trans[i]);
+  }
+  ISG_HIP(hipGetLastError());
+  ISG_HIP(hipStreamSynchronize(stream_));
 }
 
 // Finite-radius receivers: the numrx x N operator on the device (FP64) and a

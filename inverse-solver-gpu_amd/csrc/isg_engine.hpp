@@ -221,6 +221,7 @@ class Engine {
   std::vector<InterpShape>& m2m_shapes(bool fp32) { return fp32 ? m2m32_ : m2m64_; }
 
   void apply_mlfma(const zd* y, zd* r, int nrhs, const int* act, int prec);
+  void device_tables(DevOp<double>& d);
   void setup_receivers();
   void rx_apply(const zd* x, zd* r, int nrhs);
   void rx_apply_h(const zd* r, zd* x, int nrhs);

@@ -125,8 +125,9 @@ bool build_plan(const PlanConfig& cfg, Plan& P) {
 
   // near-field blocks: slot s = 3 (dY + 1) + (dX + 1) for the source box at
   // (X + dX, Y + dY); entry [basis n][test m] (setup_mlfma.cpp:459-505)
-  P.near.assign((size_t)9 * 64 * 64, cd(0, 0));
-  for (int s = 0; s < 9; s++) {
+  const bool ht = cfg.host_tables;
+  if (ht) P.near.assign((size_t)9 * 64 * 64, cd(0, 0));
+  for (int s = 0; s < 9 && ht; s++) {
     int dY = s / 3 - 1, dX = s % 3 - 1;
     for (int n = 0; n < 64; n++)
       for (int m = 0; m < 64; m++) {
@@ -148,9 +149,11 @@ bool build_plan(const PlanConfig& cfg, Plan& P) {
 
   // P2M radiation and L2P receiving patterns about the box centre
   // (integrate.cpp:43-69, setup_mlfma.cpp:546-553)
-  P.coeffT.assign((size_t)64 * Sl, cd(0, 0));
-  P.basisT.assign((size_t)Sl * 64, cd(0, 0));
-  for (int n = 0; n < 64; n++) {
+  if (ht) {
+    P.coeffT.assign((size_t)64 * Sl, cd(0, 0));
+    P.basisT.assign((size_t)Sl * 64, cd(0, 0));
+  }
+  for (int n = 0; n < 64 && ht; n++) {
     cd d((n % 8 + 0.5) * res - a_leaf / 2, a_leaf / 2 - (n / 8 + 0.5) * res);
     for (int k = 0; k < Sl; k++) {
       double ang = 2 * M_PI * k / Sl;
@@ -165,7 +168,7 @@ bool build_plan(const PlanConfig& cfg, Plan& P) {
   // shifts between a parent centre and its children (setup_mlfma.cpp:556-570)
   P.shiftM.assign(L, {});
   P.shiftL.assign(L, {});
-  for (int i = 0; i < L - 1; i++) {
+  for (int i = 0; i < L - 1 && ht; i++) {
     int S = P.nsamp[i];
     double q = P.csize[i] / 4;
     P.shiftM[i].resize(4 * S);
@@ -238,9 +241,9 @@ bool build_plan(const PlanConfig& cfg, Plan& P) {
   for (int i = 2; i < L; i++) {
     int S = P.nsamp[i], Pn = P.nterm[i];
     double a = P.csize[i];
-    P.trans[i].assign((size_t)49 * S, cd(0, 0));
+    if (ht) P.trans[i].assign((size_t)49 * S, cd(0, 0));
     std::vector<cd> h(Pn + 1);
-    for (int dY = -3; dY <= 3; dY++)
+    for (int dY = -3; dY <= 3 && ht; dY++)
       for (int dX = -3; dX <= 3; dX++) {
         if (std::abs(dX) <= 1 && std::abs(dY) <= 1) continue;
         cd d(-dX * a, dY * a);  // target centre minus source centre

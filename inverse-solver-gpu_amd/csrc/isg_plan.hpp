@@ -37,6 +37,9 @@ struct PlanConfig {
   // > 0: line-source transmitters and point receivers on a circle of this
   // radius (wavelengths), equispaced from angle 0
   double radius = 0.0;
+  // false: the near blocks, P2M/L2P patterns, shifts and translation
+  // operators are left to the device set-up (their host vectors stay empty)
+  bool host_tables = true;
 };
 
 struct Plan {
