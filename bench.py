@@ -203,7 +203,7 @@ def run_paths(isg, args, wl, common, otrue, mes, paths, warmup, steps, timing):
             try:
                 d = isg.Dbim(s, otrue, field_start=START_KINDS[start], measurements=mes)
                 if start == "learned":
-                    d.set_warmnet(isg.WarmNet(max_batch=8, max_pixels=side * side, path=WARMNET_FILE))
+                    d.set_warmnet(isg.WarmNet(max_batch=16, max_pixels=side * side, path=WARMNET_FILE))
             except (isg.IsgError, OSError) as e:
                 errors[p] = str(e)
                 log("[%s] skipped: %s" % (p, e))
@@ -319,6 +319,11 @@ def main():
                     help="also time the learned and plain paths in the reference geometry (plane waves, "
                          "far-field receivers), one 5-iteration cycle each")
     ap.add_argument("--cpu-baseline", default="on", choices=["on", "off"])
+    ap.add_argument("--save-measurements", default=None,
+                    help="write the synthesized measurements (numtx x numrx complex, .npy) to this file")
+    ap.add_argument("--measurements", default=None,
+                    help="read the measurements from a file written by --save-measurements instead of "
+                         "synthesizing them (profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: there is no multi-GPU path (sharding the transmitters "
@@ -346,7 +351,16 @@ def main():
     _, r2t = probe.geometry()
     probe.close()
     otrue = isg.raster_to_tree(isg.phantom_ellipses(side, wl["nell"], wl["seed"]), r2t)
-    mes, synth, numsamp = synthesize(isg, common, batch, otrue, args.synth_iti)
+    if args.measurements:
+        mes = np.load(args.measurements)
+        synth = {"from_file": args.measurements}
+        probe = isg.Solver(max_rhs=1, **common)
+        numsamp = list(probe.numsamp)
+        probe.close()
+    else:
+        mes, synth, numsamp = synthesize(isg, common, batch, otrue, args.synth_iti)
+    if args.save_measurements:
+        np.save(args.save_measurements, mes)
     S_leaf = numsamp[-1]
     flops_mv, pairs = near_flops_per_rhs(level, S_leaf)
 

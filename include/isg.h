@@ -91,6 +91,10 @@ typedef struct isg_config {
                       the same transmitter scaled by the residual-minimising lambda (one operator
                       application; the start is never worse than zero); the stores are dropped by
                       isg_dbim_setup* and isg_dbim_set */
+  int setup;       /* operator set-up (setup_mlfma.cpp:453-631): 1 (default) the near blocks, P2M/L2P
+                      patterns, shifts and translation operators are computed by device kernels in FP64;
+                      0 on the host (FP64) and uploaded.  Interpolators, anterpolators and receiver
+                      weights are built on the host either way. */
 } isg_config;
 
 /* Per-solve result. */
@@ -188,6 +192,8 @@ typedef struct isg_dbim_stats {
   double field_seconds;  /* wall time of that phase (network and blend included) */
   double lambda_abs;     /* learned start: mean |lambda| of the blend */
   double net_seconds;    /* learned start: network inference time */
+  double start_ratio;    /* learned start: mean over transmitters of the blend's initial-residual
+                            ratio |r(first-order start + lambda d)| / |r(first-order start)| */
 } isg_dbim_stats;
 
 /* Incident plane waves for numtx transmitters (setup_born.cpp:62-65), synthetic

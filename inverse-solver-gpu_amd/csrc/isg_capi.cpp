@@ -91,6 +91,7 @@ void isg_config_default(isg_config* c) {
   c->radius = 0.0;
   c->products = 0;
   c->solve_start = 0;
+  c->setup = 1;
 }
 
 int isg_create(const isg_config* cfg, isg_context** out) {
@@ -102,6 +103,7 @@ int isg_create(const isg_config* cfg, isg_context** out) {
   if (!(cfg->radius >= 0)) return ISG_E_ARG;
   if (cfg->products != 0 && !(cfg->products == 1 && cfg->precision == 1)) return ISG_E_ARG;
   if (cfg->solve_start != 0 && cfg->solve_start != 1) return ISG_E_ARG;
+  if (cfg->setup != 0 && cfg->setup != 1) return ISG_E_ARG;
   isg_context* c = new isg_context();
   c->cfg = *cfg;
   isg::EngineConfig ec;
@@ -113,6 +115,7 @@ int isg_create(const isg_config* cfg, isg_context** out) {
   ec.plan.ninter = cfg->ninter;
   ec.plan.numrx = cfg->numrx;
   ec.plan.radius = cfg->radius;
+  ec.plan.host_tables = cfg->setup == 0;
   ec.max_rhs = cfg->max_rhs;
   ec.precision = cfg->precision;
   ec.mixed_margin = cfg->mixed_margin;
@@ -378,6 +381,7 @@ static void fill_stats(const isg::DbimStats& s, isg_dbim_stats* o) {
   o->field_iters = s.field_iters;
   o->field_seconds = s.field_seconds;
   o->lambda_abs = s.lambda_abs;
+  o->start_ratio = s.start_ratio;
   o->net_seconds = s.net_seconds;
 }
 

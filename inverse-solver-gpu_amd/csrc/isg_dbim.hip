@@ -138,7 +138,7 @@ __global__ void __launch_bounds__(kBlock) k_net_in(const zd* o, const zd* inc, c
   }
 }
 
-// d[t][r2t[q]] = out[t][q] * rms[t] (complex from 2 floats)
+// d[t][r2t[q]] = out[t][q] * scale[t] (complex from 2 floats)
 __global__ void __launch_bounds__(kBlock) k_net_dir(const float* out, const double* rms, const int* r2t, zd* d,
                                                     long N) {
   int t = blockIdx.y;
@@ -146,6 +146,25 @@ __global__ void __launch_bounds__(kBlock) k_net_dir(const float* out, const doub
   DB_LOOP(N) {
     const float* p = out + (t * N + i) * 2;
     d[t * N + r2t[i]] = mkc<double>(p[0] * s, p[1] * s);
+  }
+}
+
+// part[t][block] = partial sum of |a[t][n]|^2 over the block's share of row t
+__global__ void __launch_bounds__(kBlock) k_row_sq(const zd* a, long N, double* part) {
+  const int t = blockIdx.y;
+  double v = 0;
+  DB_LOOP(N) {
+    const zd x = a[t * N + i];
+    v += x.x * x.x + x.y * x.y;
+  }
+  __shared__ double sh[kBlock / 64];
+  const double w = wave_sum(v);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = w;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0;
+    for (int j = 0; j < kBlock / 64; j++) s += sh[j];
+    part[(long)t * gridDim.x + blockIdx.x] = s;
   }
 }
 
@@ -256,6 +275,8 @@ Dbim::~Dbim() {
   (void)hipStreamSynchronize(eng_.stream());
   for (void* p : allocs_) (void)hipFree(p);
   if (part_h_) (void)hipHostFree(part_h_);
+  if (rowpart_h_) (void)hipHostFree(rowpart_h_);
+  if (step_rms_h_) (void)hipHostFree(step_rms_h_);
 }
 
 void Dbim::solve(zd* x, const zd* b, bool herm, int nb, bool x_zero, DbimStats& st) {
@@ -536,6 +557,10 @@ void Dbim::set_warmnet(WarmNet* net) {
     r2t_ = (int*)dalloc(sizeof(int) * N_);
     wdir_ = (zd*)dalloc(sizeof(zd) * (size_t)N_ * B_);
     inc_rms_ = (double*)dalloc(sizeof(double) * 2 * T_);
+    rowpart_ = (double*)dalloc(sizeof(double) * nblk_ * B_);
+    step_rms_ = (double*)dalloc(sizeof(double) * B_);
+    DB_HIP(hipHostMalloc((void**)&rowpart_h_, sizeof(double) * nblk_ * B_, hipHostMallocDefault));
+    DB_HIP(hipHostMalloc((void**)&step_rms_h_, sizeof(double) * B_, hipHostMallocDefault));
     if (!ok()) return;
     DB_HIP(hipMemcpy(r2t_, P.raster2tree.data(), sizeof(int) * N_, hipMemcpyHostToDevice));
   }
@@ -583,6 +608,17 @@ void Dbim::learned_solve(zd* tot, int t0, int nb, double alpha, DbimStats& st) {
   const zd* inc = inc_ + (size_t)t0 * N;
   const zd* dtot = dtot_ + (size_t)t0 * N;
   auto tn = std::chrono::steady_clock::now();
+  // output scale of each transmitter: the rms of its first-order step alpha dtot_t
+  // (the network predicts the correction in units of the step it refines)
+  k_row_sq<<<dim3(nblk_, nb), dim3(kBlock), 0, s>>>(dtot, N, rowpart_);
+  DB_HIP(hipMemcpyAsync(rowpart_h_, rowpart_, sizeof(double) * nblk_ * nb, hipMemcpyDeviceToHost, s));
+  DB_HIP(hipStreamSynchronize(s));
+  for (int t = 0; t < nb; t++) {
+    double e = 0;
+    for (int j = 0; j < nblk_; j++) e += rowpart_h_[(size_t)t * nblk_ + j];
+    step_rms_h_[t] = std::fabs(alpha) * std::sqrt(e / N);
+  }
+  DB_HIP(hipMemcpyAsync(step_rms_, step_rms_h_, sizeof(double) * nb, hipMemcpyHostToDevice, s));
   for (int c0 = 0; c0 < nb; c0 += net_chunk_) {
     int nc = std::min(net_chunk_, nb - c0);
     k_net_in<<<dim3(nblk_, nc), dim3(kBlock), 0, s>>>(o_, inc + (size_t)c0 * N, tot + (size_t)c0 * N,
@@ -593,17 +629,20 @@ void Dbim::learned_solve(zd* tot, int t0, int nb, double alpha, DbimStats& st) {
       if (err_.empty()) err_ = net_->error();
       return;
     }
-    k_net_dir<<<dim3(nblk_, nc), dim3(kBlock), 0, s>>>(net_out_, inc_rms_ + t0 + c0, r2t_, wdir_ + (size_t)c0 * N,
-                                                        N);
+    k_net_dir<<<dim3(nblk_, nc), dim3(kBlock), 0, s>>>(net_out_, step_rms_ + c0, r2t_, wdir_ + (size_t)c0 * N, N);
   }
   DB_HIP(hipStreamSynchronize(s));
   st.net_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - tn).count();
   std::vector<SolveStat> ss;
   std::vector<zd> lam;
-  eng_.bicgs_blend(tot, wdir_, g_, 0, inc, false, nb, opt_.tol, opt_.maxit, ss, opt_.precision, &lam);
-  double la = 0;
+  std::vector<double> ratio;
+  eng_.bicgs_blend(tot, wdir_, g_, 0, inc, false, nb, opt_.tol, opt_.maxit, ss, opt_.precision, &lam, false,
+                   &ratio);
+  double la = 0, ra = 0;
   for (auto& l : lam) la += std::sqrt(l.x * l.x + l.y * l.y);
+  for (double r : ratio) ra += r;
   st.lambda_abs += lam.empty() ? 0 : la / T_;
+  st.start_ratio += ratio.empty() ? 0 : ra / T_;
   for (auto& q : ss) {
     st.solves++;
     st.solve_iters += q.iters;

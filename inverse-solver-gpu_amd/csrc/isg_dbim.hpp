@@ -45,6 +45,8 @@ struct DbimStats {
   // start, the mean |lambda| and the network time
   long long field_iters = 0;
   double field_seconds = 0, lambda_abs = 0, net_seconds = 0;
+  // learned start: mean over transmitters of |r(first-order + lambda d)| / |r(first-order)|
+  double start_ratio = 0;
 };
 
 class Dbim {
@@ -110,6 +112,8 @@ class Dbim {
   zd* wdir_ = nullptr;                             // [max_rhs][N] learned corrections
   int* r2t_ = nullptr;                             // raster -> tree index
   double* inc_rms_ = nullptr;                      // [2][numtx]: rms of inc_t, then 1 / rms
+  double *rowpart_ = nullptr, *rowpart_h_ = nullptr;   // per-row partial sums [max_rhs][nblk]
+  double *step_rms_ = nullptr, *step_rms_h_ = nullptr; // [max_rhs]: rms of alpha dtot_t (output scale)
   bool rms_ok_ = false;
   double* part_ = nullptr;
   double* part_h_ = nullptr;

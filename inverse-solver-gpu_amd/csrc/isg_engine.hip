@@ -294,6 +294,7 @@ Engine::Engine(const EngineConfig& cfg) : cfg_(cfg) {
   active_ = (int*)dalloc((size_t)R * sizeof(int));
   tolv_ = (double*)dalloc((size_t)R * sizeof(double));
   lam_ = (zd*)dalloc((size_t)R * sizeof(zd));
+  bratio_ = (double*)dalloc((size_t)R * sizeof(double));
   ISG_HIP(hipHostMalloc((void**)&active_h_, (size_t)R * sizeof(int), hipHostMallocDefault));
   ISG_HIP(hipHostMalloc((void**)&state_h_, (size_t)R * sizeof(BiState), hipHostMallocDefault));
   ISG_HIP(hipHostMalloc((void**)&tol_h_, (size_t)R * sizeof(double), hipHostMallocDefault));
@@ -1140,7 +1141,8 @@ void Engine::bicgs_static(zd* x, const zd* g, long gs, const zd* b, bool herm, i
 // r_x = b - A x and A d with the solver's operator, lambda = <Ad, r_x> / <Ad, Ad>,
 // x += lambda d, r0 = r_x - lambda Ad; BiCGSTAB then starts from (x, r0).
 void Engine::bicgs_blend(zd* x, const zd* d, const zd* g, long gs, const zd* b, bool herm, int nrhs, double tol,
-                         int maxit, std::vector<SolveStat>& out, int prec, std::vector<zd>* lam, bool x_zero) {
+                         int maxit, std::vector<SolveStat>& out, int prec, std::vector<zd>* lam, bool x_zero,
+                         std::vector<double>* ratio) {
   out.assign(nrhs, SolveStat{-1, 0, 0.0});
   if (!ok()) return;
   if (nrhs > cfg_.max_rhs) {
@@ -1158,6 +1160,10 @@ void Engine::bicgs_blend(zd* x, const zd* d, const zd* g, long gs, const zd* b, 
   if (lam) {
     lam->resize(nrhs);
     ISG_HIP(hipMemcpyAsync(lam->data(), lam_, nrhs * sizeof(zd), hipMemcpyDeviceToHost, stream_));
+    if (ratio) {
+      ratio->resize(nrhs);
+      ISG_HIP(hipMemcpyAsync(ratio->data(), bratio_, nrhs * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    }
     ISG_HIP(hipStreamSynchronize(stream_));
   }
   solve(x, g, gs, b, herm, nrhs, false, tol, maxit, out, prec, true);
@@ -1179,7 +1185,7 @@ void Engine::blend(zd* x, const zd* d, const zd* g, long gs, const zd* b, bool h
   k_prep<V><<<grid, blk, 0, stream_>>>(d, g, gs, y, N, h);
   apply_vec<V>(y, op, nrhs, nullptr, prec);
   k_bl_dots<V><<<grid, blk, 0, stream_>>>(d, op, g, gs, res, ad, N, h, part_);
-  k_bl_fin<<<dim3(nrhs), blk, 0, stream_>>>(part_, nblk_, lam_);
+  k_bl_fin<<<dim3(nrhs), blk, 0, stream_>>>(part_, nblk_, lam_, bratio_);
   k_bl_apply<V><<<grid, blk, 0, stream_>>>(x, d, res, ad, lam_, N);
 }
 

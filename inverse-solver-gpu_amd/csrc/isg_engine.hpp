@@ -106,9 +106,11 @@ class Engine {
   // The same solve started from x + lambda d, lambda chosen per system to
   // minimise |b - A (x + lambda d)| (so the start is never worse than x); two
   // operator applications replace the usual one (one when x_zero: x holds
-  // zeros and r_x = b).  lam (host, may be null) receives lambda.
+  // zeros and r_x = b).  lam (host, may be null) receives lambda, ratio (may
+  // be null) |r_x - lambda A d| / |r_x|.
   void bicgs_blend(zd* x, const zd* d, const zd* g, long g_stride, const zd* b, bool herm, int nrhs, double tol,
-                   int maxit, std::vector<SolveStat>& out, int prec, std::vector<zd>* lam, bool x_zero = false);
+                   int maxit, std::vector<SolveStat>& out, int prec, std::vector<zd>* lam, bool x_zero = false,
+                   std::vector<double>* ratio = nullptr);
 
   // bicgs() without device allocations: the solver's partial sums, scalars,
   // flags, tolerances and blend coefficients live in this GPU's static scratch
@@ -282,6 +284,7 @@ class Engine {
   int* active_ = nullptr;         // [max_rhs]
   double* tolv_ = nullptr;        // [max_rhs] per-system tolerance
   zd* lam_ = nullptr;             // [max_rhs] blend coefficients
+  double* bratio_ = nullptr;      // [max_rhs] blend residual ratios
   int* active_h_ = nullptr;       // pinned host mirrors
   BiState* state_h_ = nullptr;
   double* tol_h_ = nullptr;

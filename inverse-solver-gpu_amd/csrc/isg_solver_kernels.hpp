@@ -205,7 +205,7 @@ __global__ void __launch_bounds__(kBlock) k_bi_fin(int phase, BiState* __restric
 }
 
 // Blended start: ad = A d (post of the operator product), partials
-// <ad, r> (= sum conj(ad) r) and <ad, ad>.
+// <ad, r> (= sum conj(ad) r), <ad, ad> and <r, r>.
 template <typename V>
 __global__ void __launch_bounds__(kBlock) k_bl_dots(const zd* __restrict__ d, const cplx<V>* __restrict__ gy,
                                                     const zd* __restrict__ g, long gs, const cplx<V>* __restrict__ r,
@@ -220,32 +220,47 @@ __global__ void __launch_bounds__(kBlock) k_bl_dots(const zd* __restrict__ d, co
     acc[0].x += a.x * rv.x + a.y * rv.y;
     acc[0].y += a.x * rv.y - a.y * rv.x;
     acc[1].x += a.x * a.x + a.y * a.y;
+    acc[1].y += rv.x * rv.x + rv.y * rv.y;
   }
   block_store<2>(acc, part + (long)rhs * kSlots * gridDim.x, gridDim.x, blockIdx.x);
 }
 
-// lambda = <ad, r> / <ad, ad> per right-hand side (0 when ad = 0), fixed-order sums
-__global__ void __launch_bounds__(kBlock) k_bl_fin(const double2* __restrict__ part, int nblk, zd* __restrict__ lam) {
+// lambda = <ad, r> / <ad, ad> per right-hand side, fixed-order sums; 0 when
+// ad = 0 or the sums are not finite (a correction d holding NaN or Inf is
+// then ignored: the solve starts from x alone).  ratio (may be null) gets
+// |r - lambda ad| / |r|, the blend's reduction of the initial residual.
+__global__ void __launch_bounds__(kBlock) k_bl_fin(const double2* __restrict__ part, int nblk, zd* __restrict__ lam,
+                                                   double* __restrict__ ratio) {
   int rhs = blockIdx.x;
   const double2* pr = part + (long)rhs * kSlots * nblk;
-  __shared__ double sh[3][kBlock / 64];
-  double v[3] = {0, 0, 0};
+  __shared__ double sh[4][kBlock / 64];
+  double v[4] = {0, 0, 0, 0};
   for (int j = threadIdx.x; j < nblk; j += kBlock) {
     v[0] += pr[j].x;
     v[1] += pr[j].y;
     v[2] += pr[nblk + j].x;
+    v[3] += pr[nblk + j].y;
   }
   int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  for (int s = 0; s < 3; s++) {
+  for (int s = 0; s < 4; s++) {
     double a = wave_sum(v[s]);
     if (lane == 0) sh[s][w] = a;
   }
   __syncthreads();
   if (threadIdx.x != 0) return;
-  double t[3] = {0, 0, 0};
-  for (int s = 0; s < 3; s++)
+  double t[4] = {0, 0, 0, 0};
+  for (int s = 0; s < 4; s++)
     for (int i = 0; i < kBlock / 64; i++) t[s] += sh[s][i];
-  lam[rhs] = t[2] > 0 ? mkc<double>(t[0] / t[2], t[1] / t[2]) : mkc<double>(0, 0);
+  zd l = mkc<double>(0, 0);
+  if (t[2] > 0 && isfinite(t[2])) l = mkc<double>(t[0] / t[2], t[1] / t[2]);
+  if (!(isfinite(l.x) && isfinite(l.y))) l = mkc<double>(0, 0);
+  lam[rhs] = l;
+  if (ratio) {
+    // |r - lambda ad|^2 = |r|^2 - |<ad, r>|^2 / <ad, ad> at the minimising lambda
+    double q = 1.0;
+    if ((l.x != 0 || l.y != 0) && t[3] > 0) q = sqrt(fmax(0.0, 1.0 - (t[0] * t[0] + t[1] * t[1]) / (t[2] * t[3])));
+    ratio[rhs] = q;
+  }
 }
 
 // x += lambda d, r -= lambda ad
@@ -256,6 +271,7 @@ __global__ void __launch_bounds__(kBlock) k_bl_apply(zd* __restrict__ x, const z
   int rhs = blockIdx.y;
   long o = rhs * N;
   zd l = lam[rhs], ml = mkc<double>(-l.x, -l.y);
+  if (l.x == 0 && l.y == 0) return;  // nothing to add (and d may hold non-finite values)
   ISG_VEC_LOOP(N) {
     x[o + i] = cfma(l, d[o + i], x[o + i]);
     stv(r, o + i, cfma(ml, ldv(ad, o + i), ldv(r, o + i)));

@@ -36,38 +36,42 @@ __host__ __device__ __forceinline__ float bf2f(uint16_t b) {
 // Convolution (KS = 1 or 3, stride 1, zero padding KS/2) on the matrix cores.
 //   Y[n][p][co] = act(bias[co] + sum_{tap,ci} bf16(In[n][p+tap][ci]) bf16(Wb[tap][co][ci]))
 //   In = X (+ up2(U)) (* [M > 0])
-// One workgroup: an 8x8 pixel tile of one image and up to 64 output channels;
-// wave w owns tile pixels 16w..16w+15 (the MFMA's 16 rows), every 16-channel
-// output block is one accumulator.  Input halo and weights are staged through
+// One workgroup: a 16x16-position tile of one image and up to 64 output
+// channels (NCB blocks of 16).  Waves split the output blocks and the tile's
+// 16 rows: wave w owns block w / (4 / NCB) and rows w % (4 / NCB) + (4 / NCB) j,
+// one MFMA row block (16 positions of a tile row) each, so a wave keeps its
+// block's weights of one input chunk (KS * KS fragments) in registers and
+// reuses them over 16 * NCB / 4 row blocks.  The input halo is staged through
 // LDS in chunks of 32 (or 16) input channels as BF16; a chunk of 32 is one
-// v_mfma_f32_16x16x32_bf16 per tap and output block (lane l supplies row or
-// column l % 16 and the 8 channels 8 (l / 16) .. +7 of both operands; the
-// same channel order on both sides, so the sum over the chunk is exact in
-// any lane-to-channel assignment), a chunk of 16 one v_mfma_f32_16x16x16_bf16.
-template <int KS>
+// v_mfma_f32_16x16x32_bf16 per tap and row block (lane l supplies position
+// or output channel l % 16 and the 8 input channels 8 (l / 16) .. +7 of both
+// operands; the same channel order on both sides, so the sum over the chunk
+// is exact in any lane-to-channel assignment), a chunk of 16 one
+// v_mfma_f32_16x16x16_bf16.  Positions outside the image are zero on input
+// and not stored (partial tiles).
+template <int KS, int NCB>
 __global__ void __launch_bounds__(kNB) k_conv(const float* __restrict__ X, const float* __restrict__ U,
                                               const float* __restrict__ M, const uint16_t* __restrict__ Wb,
                                               const float* __restrict__ bias, float* __restrict__ Y, int cin,
                                               int cout, int cstore, int H, int W, int relu) {
-  constexpr int R = KS / 2, HW = 8 + KS - 1, HS = HW * HW;
+  constexpr int TS = 16, R = KS / 2, HW = TS + KS - 1, HS = HW * HW;
+  constexpr int WPC = 4 / NCB;     // waves per output block
+  constexpr int NRB = TS / WPC;    // row blocks per wave
   __shared__ __attribute__((aligned(16))) uint16_t hal[HS * kCKP];
-  __shared__ __attribute__((aligned(16))) uint16_t wsm[KS * KS * kCOT * kCKP];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int n = blockIdx.y;
-  const int tiles_x = (W + 7) >> 3;  // the last tile row / column may be partial
-  const int ty0 = (blockIdx.x / tiles_x) * 8, tx0 = (blockIdx.x % tiles_x) * 8;
-  const int co0 = blockIdx.z * kCOT;
-  const int cot = min(kCOT, cout - co0);
-  const int ncb = cot >> 4;
+  const int tiles_x = (W + TS - 1) / TS;  // the last tile row / column may be partial
+  const int ty0 = (blockIdx.x / tiles_x) * TS, tx0 = (blockIdx.x % tiles_x) * TS;
+  const int cb = w / WPC, part = w % WPC;
+  const int co = blockIdx.z * kCOT + cb * 16 + (lane & 15);  // this lane's B column / D channel
   const int ck = (cin % kCK == 0) ? kCK : 16;
   const int ck4 = ck >> 2;
   const int li = lane & 15, kq = lane >> 4;
-  const int tp = 16 * w + li;
-  const int py = tp >> 3, px = tp & 7;
-  v4f acc[4];
+  v4f acc[NRB];
 #pragma unroll
-  for (int c = 0; c < 4; c++) acc[c] = v4f{0, 0, 0, 0};
+  for (int j = 0; j < NRB; j++) acc[j] = v4f{0, 0, 0, 0};
   const long img = (long)n * H * W;
+  const uint16_t* wrow = Wb + (long)co * cin;  // + tap * cout * cin + ci
   for (int ci0 = 0; ci0 < cin; ci0 += ck) {
     for (int e = tid; e < HS * ck4; e += kNB) {
       const int hp = e / ck4, c4 = e - hp * ck4;
@@ -94,54 +98,53 @@ __global__ void __launch_bounds__(kNB) k_conv(const float* __restrict__ X, const
       pk.y = (uint32_t)f2bf(v.z) | ((uint32_t)f2bf(v.w) << 16);
       *reinterpret_cast<uint2*>(&hal[hp * kCKP + 4 * c4]) = pk;
     }
-    for (int e = tid; e < KS * KS * cot * ck4; e += kNB) {
-      const int c4 = e % ck4, r = e / ck4;
-      const int co = r % cot, tap = r / cot;
-      uint2 wv = *reinterpret_cast<const uint2*>(Wb + ((long)tap * cout + co0 + co) * cin + ci0 + 4 * c4);
-      *reinterpret_cast<uint2*>(&wsm[(tap * kCOT + co) * kCKP + 4 * c4]) = wv;
-    }
-    __syncthreads();
     if (ck == kCK) {
+      // this lane's weight fragments of the chunk: 8 input channels per tap
+      v8bf b[KS * KS];
+#pragma unroll
+      for (int tap = 0; tap < KS * KS; tap++)
+        b[tap] = __builtin_bit_cast(
+            v8bf, *reinterpret_cast<const v8s*>(wrow + (long)tap * cout * cin + ci0 + 8 * kq));
+      __syncthreads();
 #pragma unroll
       for (int tap = 0; tap < KS * KS; tap++) {
-        const int hp = (py + tap / KS) * HW + px + tap % KS;
-        const v8bf a = __builtin_bit_cast(v8bf, *reinterpret_cast<const v8s*>(&hal[hp * kCKP + 8 * kq]));
+        const int dy = tap / KS, dx = tap % KS;
 #pragma unroll
-        for (int cb = 0; cb < 4; cb++) {
-          if (cb < ncb) {
-            const v8bf b = __builtin_bit_cast(
-                v8bf, *reinterpret_cast<const v8s*>(&wsm[(tap * kCOT + cb * 16 + li) * kCKP + 8 * kq]));
-            acc[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[cb], 0, 0, 0);
-          }
+        for (int j = 0; j < NRB; j++) {
+          const int rb = part + WPC * j;
+          const v8bf a = __builtin_bit_cast(
+              v8bf, *reinterpret_cast<const v8s*>(&hal[((rb + dy) * HW + li + dx) * kCKP + 8 * kq]));
+          acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[tap], acc[j], 0, 0, 0);
         }
       }
     } else {
+      v4s b[KS * KS];
+#pragma unroll
+      for (int tap = 0; tap < KS * KS; tap++)
+        b[tap] = *reinterpret_cast<const v4s*>(wrow + (long)tap * cout * cin + ci0 + 4 * kq);
+      __syncthreads();
 #pragma unroll
       for (int tap = 0; tap < KS * KS; tap++) {
-        const int hp = (py + tap / KS) * HW + px + tap % KS;
-        const v4s a = *reinterpret_cast<const v4s*>(&hal[hp * kCKP + 4 * kq]);
+        const int dy = tap / KS, dx = tap % KS;
 #pragma unroll
-        for (int cb = 0; cb < 4; cb++) {
-          if (cb < ncb) {
-            const v4s b = *reinterpret_cast<const v4s*>(&wsm[(tap * kCOT + cb * 16 + li) * kCKP + 4 * kq]);
-            acc[cb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[cb], 0, 0, 0);
-          }
+        for (int j = 0; j < NRB; j++) {
+          const int rb = part + WPC * j;
+          const v4s a = *reinterpret_cast<const v4s*>(&hal[((rb + dy) * HW + li + dx) * kCKP + 4 * kq]);
+          acc[j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b[tap], acc[j], 0, 0, 0);
         }
       }
     }
     __syncthreads();
   }
-  // D[i][j]: lane j + 16 (i / 4), register i % 4; i = tile pixel (within the wave), j = channel
+  // D[i][j]: lane j + 16 (i / 4), register i % 4; i = position within the row block, j = channel
+  const float bv = bias ? bias[co] : 0.f;
 #pragma unroll
-  for (int cb = 0; cb < 4; cb++) {
-    if (cb >= ncb) continue;
-    const int co = co0 + cb * 16 + li;
-    const float bv = bias ? bias[co] : 0.f;
+  for (int j = 0; j < NRB; j++) {
+    const int gy = ty0 + part + WPC * j;
 #pragma unroll
     for (int r = 0; r < 4; r++) {
-      const int q = 16 * w + 4 * kq + r;
-      const int gy = ty0 + (q >> 3), gx = tx0 + (q & 7);
-      float v = acc[cb][r] + bv;
+      const int gx = tx0 + 4 * kq + r;
+      float v = acc[j][r] + bv;
       if (relu) v = v > 0.f ? v : 0.f;
       if (co < cstore && gy < H && gx < W) Y[(img + (long)gy * W + gx) * cstore + co] = v;
     }
@@ -521,11 +524,16 @@ void WarmNet::conv(int li, const float* x, const float* u, const float* m, float
   const uint16_t* wb = (transpose ? wt_ : wb_) + l.woff;
   const float* bias = transpose ? nullptr : w_ + l.boff;
   const int relu = transpose ? 0 : l.relu;
-  dim3 grid(((H + 7) / 8) * ((W + 7) / 8), batch, (cout + kCOT - 1) / kCOT);
-  if (l.ks == 3)
-    k_conv<3><<<grid, kNB, 0, s>>>(x, u, m, wb, bias, y, cin, cout, cstore, H, W, relu);
-  else
-    k_conv<1><<<grid, kNB, 0, s>>>(x, u, m, wb, bias, y, cin, cout, cstore, H, W, relu);
+  dim3 grid(((H + 15) / 16) * ((W + 15) / 16), batch, (cout + kCOT - 1) / kCOT);
+  // output blocks of 16 channels per workgroup: 1, 2 or 4 (every layer's widths are 16, 32 or multiples of 64)
+  const int ncb = std::min(4, cout / 16);
+#define WN_CONV(KS, NCB) k_conv<KS, NCB><<<grid, kNB, 0, s>>>(x, u, m, wb, bias, y, cin, cout, cstore, H, W, relu)
+  if (l.ks == 3) {
+    if (ncb == 4) WN_CONV(3, 4); else if (ncb == 2) WN_CONV(3, 2); else WN_CONV(3, 1);
+  } else {
+    if (ncb == 4) WN_CONV(1, 4); else if (ncb == 2) WN_CONV(1, 2); else WN_CONV(1, 1);
+  }
+#undef WN_CONV
 }
 
 void WarmNet::wgrad(int li, const float* dy, const float* m, const float* x, const float* u, int batch, int H,

@@ -9,7 +9,9 @@
 // first-order start + lambda * correction with lambda chosen per transmitter
 // to minimise the initial residual (so the start is never worse than the
 // first-order one).  The field channels arrive divided by the transmitter's
-// incident-field rms, the correction leaves multiplied by it.
+// incident-field rms; the correction comes out in units of the rms of the
+// transmitter's first-order step alpha dtot (so that small late steps and the
+// large first one weigh alike in training).
 //
 // The image enters space-to-depth by 2 (each 2x2 pixel block becomes one
 // position with 4 x 8 = 32 channels) and leaves depth-to-space (16 channels,

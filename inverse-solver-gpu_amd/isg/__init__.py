@@ -44,6 +44,7 @@ class Config(ctypes.Structure):
         ("radius", ctypes.c_double),
         ("products", ctypes.c_int),
         ("solve_start", ctypes.c_int),
+        ("setup", ctypes.c_int),
     ]
 
 
@@ -71,6 +72,7 @@ class DbimStats(ctypes.Structure):
         ("field_seconds", ctypes.c_double),
         ("lambda_abs", ctypes.c_double),
         ("net_seconds", ctypes.c_double),
+        ("start_ratio", ctypes.c_double),
     ]
 
 

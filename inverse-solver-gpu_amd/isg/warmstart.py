@@ -13,7 +13,8 @@ start; for each iteration and a random subset of transmitters one sample is
 input  (Re/Im new contrast, Re/Im incident field, Re/Im first-order start
         tot_prev - alpha dtot, Re/Im its step -alpha dtot), the field channels
         divided by the transmitter's incident-field rms,
-target (new total field - first-order start) / that rms,
+target (new total field - first-order start) / the rms of the first-order
+        step (the correction in units of the step it refines),
 cropped to `crop` x `crop` pixels.  The network is fully convolutional, so
 crops train it and whole images run through it.  Training augments with the 8
 symmetries of the square (a rotation or reflection of the whole scene maps
@@ -94,7 +95,10 @@ def gen_samples(level, ntx, nrx, seeds, radii, iters, crop, tx_per, crops_per, r
                 step = -alpha * dtot[t]
                 fo = prev[t] + step
                 xin = sample_inputs(o, inc[t], fo, step, rms[t])
-                dv = (new[t] - fo) / rms[t]
+                srms = float(np.sqrt(np.mean(np.abs(step) ** 2)))
+                if not srms > 0:
+                    continue
+                dv = (new[t] - fo) / srms
                 for _ in range(crops_per):
                     y0, x0 = rng.integers(0, side - crop + 1, 2)
                     sl = (slice(y0, y0 + crop), slice(x0, x0 + crop))
@@ -210,9 +214,10 @@ def train(args):
 def evaluate(args):
     """Zero / previous / first-order / learned total-field starts on held-out phantoms."""
     side = 8 << (args.level - 1)
-    net = isg.WarmNet(max_batch=8, max_pixels=side * side, path=args.weights) if args.weights else None
+    net = isg.WarmNet(max_batch=16, max_pixels=side * side, path=args.weights) if args.weights else None
     out = {k: {"field_iters": 0, "field_seconds": 0.0, "iter_seconds": 0.0, "net_seconds": 0.0,
-               "solve_iters": 0, "lambda_abs": []} for k in STARTS if k != "learned" or net is not None}
+               "solve_iters": 0, "lambda_abs": [], "start_ratio": []}
+           for k in STARTS if k != "learned" or net is not None}
     misfit = {k: [] for k in out}
     for i in range(args.phantoms):
         seed = EVAL_SEED0 + i
@@ -236,6 +241,7 @@ def evaluate(args):
                 r["solve_iters"] += st["solve_iters"]
                 if kind == "learned":
                     r["lambda_abs"].append(st["lambda_abs"])
+                    r["start_ratio"].append(st["start_ratio"])
             misfit[kind].append(st["misfit"])
             del d
         s.close()
@@ -258,6 +264,8 @@ def evaluate(args):
         }
         if r["lambda_abs"]:
             res["per_dbim_iteration"][k]["mean_abs_lambda"] = round(float(np.mean(r["lambda_abs"])), 3)
+            # |initial residual of first-order start + lambda * network| / |of the first-order start|
+            res["per_dbim_iteration"][k]["start_residual_ratio"] = round(float(np.mean(r["start_ratio"])), 3)
     print(json.dumps(res), flush=True)
 
 

@@ -5,8 +5,11 @@ in partial 8x8 tiles) pixels with 8 input channels, random inputs, seeded
 He-normal weights.  The oracle reproduces the GPU's rounding points (both
 operands of every convolution rounded to BF16); the remaining differences are
 FP32 versus float64 accumulation, and the BF16 roundings they occasionally
-flip downstream.  Bars: output 5e-3 relative (norm), loss 1e-4, per-layer
-weight gradient 1e-2, Adam step 1e-5 given the GPU's own gradient.
+flip downstream (one BF16 ulp is 3.9e-3 relative; a flipped rounding in an
+early layer moves the output by ~1e-3 of its norm).  Bars: output 1e-2
+relative (norm) -- a wrong tile, halo or channel mapping moves whole 8x8
+tiles and fails it by orders of magnitude --, loss 1e-4, per-layer weight
+gradient 1e-2, Adam step 1e-5 given the GPU's own gradient.
 """
 import numpy as np
 import pytest
@@ -42,7 +45,7 @@ def test_forward_matches_oracle(net, shape):
     ref = wo.forward(net.params, x, SCALES, rnd=True)["y"]
     err = relerr(y, ref)
     print("forward relative error %.3e" % err)
-    assert err < 5e-3
+    assert err < 1e-2
 
 
 def test_gradient_matches_oracle(net):
