@@ -49,8 +49,8 @@ __host__ __device__ __forceinline__ float bf2f(uint16_t b) {
 // is exact in any lane-to-channel assignment), a chunk of 16 one
 // v_mfma_f32_16x16x16_bf16.  Positions outside the image are zero on input
 // and not stored (partial tiles).
-template <int KS, int NCB>
-__global__ void __launch_bounds__(kNB) k_conv(const float* __restrict__ X, const float* __restrict__ U,
+template <int KS, int NCB, int MODE>  // MODE 0: In = X, 1: X + up2(U), 2: X * [M > 0]
+__global__ void __launch_bounds__(kNB) __attribute__((amdgpu_waves_per_eu(2))) k_conv(const float* __restrict__ X, const float* __restrict__ U,
                                               const float* __restrict__ M, const uint16_t* __restrict__ Wb,
                                               const float* __restrict__ bias, float* __restrict__ Y, int cin,
                                               int cout, int cstore, int H, int W, int relu) {
@@ -65,72 +65,89 @@ __global__ void __launch_bounds__(kNB) k_conv(const float* __restrict__ X, const
   const int cb = w / WPC, part = w % WPC;
   const int co = blockIdx.z * kCOT + cb * 16 + (lane & 15);  // this lane's B column / D channel
   const int ck = (cin % kCK == 0) ? kCK : 16;
-  const int ck4 = ck >> 2;
   const int li = lane & 15, kq = lane >> 4;
   v4f acc[NRB];
 #pragma unroll
   for (int j = 0; j < NRB; j++) acc[j] = v4f{0, 0, 0, 0};
   const long img = (long)n * H * W;
   const uint16_t* wrow = Wb + (long)co * cin;  // + tap * cout * cin + ci
+  const int sh4 = ck == kCK ? 3 : 2;  // log2 of the float4 groups per position in a chunk
   for (int ci0 = 0; ci0 < cin; ci0 += ck) {
-    for (int e = tid; e < HS * ck4; e += kNB) {
-      const int hp = e / ck4, c4 = e - hp * ck4;
+    // halo staging: every load of the thread's share is issued before the
+    // first conversion, so their latencies overlap (NE float4 groups each)
+    constexpr int NE = (HS * (kCK / 4) + kNB - 1) / kNB;
+    float4 v[NE], a2[MODE ? NE : 1];  // a2: the upsampled addend (MODE 1) or the mask (MODE 2)
+#pragma unroll
+    for (int q = 0; q < NE; q++) {
+      const int e = tid + q * kNB;
+      const int hp = e >> sh4, c4 = e & ((1 << sh4) - 1);
       const int gy = ty0 + hp / HW - R, gx = tx0 + hp % HW - R;
-      float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-      if (gy >= 0 && gy < H && gx >= 0 && gx < W) {
+      v[q] = make_float4(0.f, 0.f, 0.f, 0.f);
+      if constexpr (MODE != 0) a2[q] = v[q];
+      if (hp < HS && gy >= 0 && gy < H && gx >= 0 && gx < W) {
         const long pix = img + (long)gy * W + gx;
-        v = *reinterpret_cast<const float4*>(X + pix * cin + ci0 + 4 * c4);
-        if (U) {
+        v[q] = *reinterpret_cast<const float4*>(X + pix * cin + ci0 + 4 * c4);
+        if constexpr (MODE == 1) {
           const long up = (long)n * (H >> 1) * (W >> 1) + (long)(gy >> 1) * (W >> 1) + (gx >> 1);
-          float4 u = *reinterpret_cast<const float4*>(U + up * cin + ci0 + 4 * c4);
-          v.x += u.x; v.y += u.y; v.z += u.z; v.w += u.w;
+          a2[q] = *reinterpret_cast<const float4*>(U + up * cin + ci0 + 4 * c4);
         }
-        if (M) {
-          float4 m = *reinterpret_cast<const float4*>(M + pix * cin + ci0 + 4 * c4);
-          v.x = m.x > 0.f ? v.x : 0.f;
-          v.y = m.y > 0.f ? v.y : 0.f;
-          v.z = m.z > 0.f ? v.z : 0.f;
-          v.w = m.w > 0.f ? v.w : 0.f;
-        }
+        if constexpr (MODE == 2) a2[q] = *reinterpret_cast<const float4*>(M + pix * cin + ci0 + 4 * c4);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < NE; q++) {
+      const int e = tid + q * kNB;
+      const int hp = e >> sh4, c4 = e & ((1 << sh4) - 1);
+      if (hp >= HS) continue;
+      float4 x = v[q];
+      if constexpr (MODE == 1) {
+        x.x += a2[q].x; x.y += a2[q].y; x.z += a2[q].z; x.w += a2[q].w;
+      }
+      if constexpr (MODE == 2) {
+        x.x = a2[q].x > 0.f ? x.x : 0.f;
+        x.y = a2[q].y > 0.f ? x.y : 0.f;
+        x.z = a2[q].z > 0.f ? x.z : 0.f;
+        x.w = a2[q].w > 0.f ? x.w : 0.f;
       }
       uint2 pk;
-      pk.x = (uint32_t)f2bf(v.x) | ((uint32_t)f2bf(v.y) << 16);
-      pk.y = (uint32_t)f2bf(v.z) | ((uint32_t)f2bf(v.w) << 16);
+      pk.x = (uint32_t)f2bf(x.x) | ((uint32_t)f2bf(x.y) << 16);
+      pk.y = (uint32_t)f2bf(x.z) | ((uint32_t)f2bf(x.w) << 16);
       *reinterpret_cast<uint2*>(&hal[hp * kCKP + 4 * c4]) = pk;
     }
+    // this lane's weight fragment of one tap (8 input channels of its output
+    // channel), the next tap's loaded while the current one's MFMAs run
     if (ck == kCK) {
-      // this lane's weight fragments of the chunk: 8 input channels per tap
-      v8bf b[KS * KS];
-#pragma unroll
-      for (int tap = 0; tap < KS * KS; tap++)
-        b[tap] = __builtin_bit_cast(
-            v8bf, *reinterpret_cast<const v8s*>(wrow + (long)tap * cout * cin + ci0 + 8 * kq));
+      const uint16_t* wp = wrow + ci0 + 8 * kq;
+      v8s bn = *reinterpret_cast<const v8s*>(wp);
       __syncthreads();
-#pragma unroll
+#pragma unroll 1
       for (int tap = 0; tap < KS * KS; tap++) {
+        const v8bf b = __builtin_bit_cast(v8bf, bn);
+        if (tap + 1 < KS * KS) bn = *reinterpret_cast<const v8s*>(wp + (long)(tap + 1) * cout * cin);
         const int dy = tap / KS, dx = tap % KS;
+        const uint16_t* hb = &hal[(dy * HW + li + dx) * kCKP + 8 * kq];
 #pragma unroll
         for (int j = 0; j < NRB; j++) {
           const int rb = part + WPC * j;
-          const v8bf a = __builtin_bit_cast(
-              v8bf, *reinterpret_cast<const v8s*>(&hal[((rb + dy) * HW + li + dx) * kCKP + 8 * kq]));
-          acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[tap], acc[j], 0, 0, 0);
+          const v8bf a = __builtin_bit_cast(v8bf, *reinterpret_cast<const v8s*>(hb + rb * HW * kCKP));
+          acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[j], 0, 0, 0);
         }
       }
     } else {
-      v4s b[KS * KS];
-#pragma unroll
-      for (int tap = 0; tap < KS * KS; tap++)
-        b[tap] = *reinterpret_cast<const v4s*>(wrow + (long)tap * cout * cin + ci0 + 4 * kq);
+      const uint16_t* wp = wrow + ci0 + 4 * kq;
+      v4s bn = *reinterpret_cast<const v4s*>(wp);
       __syncthreads();
-#pragma unroll
+#pragma unroll 1
       for (int tap = 0; tap < KS * KS; tap++) {
+        const v4s b = bn;
+        if (tap + 1 < KS * KS) bn = *reinterpret_cast<const v4s*>(wp + (long)(tap + 1) * cout * cin);
         const int dy = tap / KS, dx = tap % KS;
+        const uint16_t* hb = &hal[(dy * HW + li + dx) * kCKP + 4 * kq];
 #pragma unroll
         for (int j = 0; j < NRB; j++) {
           const int rb = part + WPC * j;
-          const v4s a = *reinterpret_cast<const v4s*>(&hal[((rb + dy) * HW + li + dx) * kCKP + 4 * kq]);
-          acc[j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b[tap], acc[j], 0, 0, 0);
+          const v4s a = *reinterpret_cast<const v4s*>(hb + rb * HW * kCKP);
+          acc[j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[j], 0, 0, 0);
         }
       }
     }
@@ -527,12 +544,24 @@ void WarmNet::conv(int li, const float* x, const float* u, const float* m, float
   dim3 grid(((H + 15) / 16) * ((W + 15) / 16), batch, (cout + kCOT - 1) / kCOT);
   // output blocks of 16 channels per workgroup: 1, 2 or 4 (every layer's widths are 16, 32 or multiples of 64)
   const int ncb = std::min(4, cout / 16);
-#define WN_CONV(KS, NCB) k_conv<KS, NCB><<<grid, kNB, 0, s>>>(x, u, m, wb, bias, y, cin, cout, cstore, H, W, relu)
-  if (l.ks == 3) {
-    if (ncb == 4) WN_CONV(3, 4); else if (ncb == 2) WN_CONV(3, 2); else WN_CONV(3, 1);
-  } else {
-    if (ncb == 4) WN_CONV(1, 4); else if (ncb == 2) WN_CONV(1, 2); else WN_CONV(1, 1);
+  // input mode: plain, plus the upsampled decoder input (skip connection), or masked by a ReLU output
+  if (u && m) {
+    if (err_.empty()) err_ = "warm-start network: a convolution input cannot be both skip-added and masked";
+    return;
   }
+  const int mode = u ? 1 : (m ? 2 : 0);
+#define WN_CONV(KS, NCB, MODE) \
+  k_conv<KS, NCB, MODE><<<grid, kNB, 0, s>>>(x, u, m, wb, bias, y, cin, cout, cstore, H, W, relu)
+#define WN_CONV_NCB(KS, MODE) \
+  do { if (ncb == 4) WN_CONV(KS, 4, MODE); else if (ncb == 2) WN_CONV(KS, 2, MODE); else WN_CONV(KS, 1, MODE); } while (0)
+#define WN_CONV_MODE(KS) \
+  do { if (mode == 1) WN_CONV_NCB(KS, 1); else if (mode == 2) WN_CONV_NCB(KS, 2); else WN_CONV_NCB(KS, 0); } while (0)
+  if (l.ks == 3)
+    WN_CONV_MODE(3);
+  else
+    WN_CONV_MODE(1);
+#undef WN_CONV_MODE
+#undef WN_CONV_NCB
 #undef WN_CONV
 }
 
