@@ -114,39 +114,61 @@ __global__ void __launch_bounds__(kBlock) k_axpy_rows(zd* a, const zd* b, zd s, 
   DB_LOOP(N) a[t * N + i] = cadd(a[t * N + i], cmul(s, b[t * N + i]));
 }
 
-// Network input of transmitters t0..t0+nc-1 in raster order, the field
-// channels divided by the transmitter's incident-field rms (inv[t]):
-// in[t][q] = (o, inc_t, tot_t, -alpha dtot_t)[r2t[q]] as 8 floats, tot_t
-// being the first-order start tot_prev - alpha dtot.
+// Network input of transmitters t0..t0+nc-1, written straight into the
+// network's space-to-depth layout: position (Y, X) of an image holds channel
+// 8 s + c = raster pixel (2Y + s / 2, 2X + s % 2), channel c of
+// (o, inc_t, tot_t, -alpha dtot_t) -- the field channels divided by the
+// transmitter's incident-field rms (inv[t]) -- times the network's input
+// scale sc[c]; tot_t is the first-order start tot_prev - alpha dtot.  One
+// thread per (position, sub-pixel).
+struct NetScales {
+  float in[WarmNet::kInCh];
+};
 __global__ void __launch_bounds__(kBlock) k_net_in(const zd* o, const zd* inc, const zd* tot, const zd* dtot,
-                                                   double alpha, const double* inv, const int* r2t, float* in,
-                                                   long N) {
-  int t = blockIdx.y;
+                                                   double alpha, const double* inv, const int* r2t, int side,
+                                                   NetScales sc, float* x32) {
+  const int t = blockIdx.y;
+  const long N = (long)side * side;
   const double s = inv[t];
+  const int hs = side / 2;
   DB_LOOP(N) {
-    int n = r2t[i];
-    zd a = o[n], b = inc[t * N + n], c = tot[t * N + n], d = dtot[t * N + n];
-    float* p = in + (t * N + i) * 8;
-    p[0] = (float)a.x;
-    p[1] = (float)a.y;
-    p[2] = (float)(b.x * s);
-    p[3] = (float)(b.y * s);
-    p[4] = (float)(c.x * s);
-    p[5] = (float)(c.y * s);
-    p[6] = (float)(-alpha * d.x * s);
-    p[7] = (float)(-alpha * d.y * s);
+    const int sp = (int)(i & 3);
+    const long pos = i >> 2;
+    const int X = (int)(pos % hs), Y = (int)(pos / hs);
+    const long q = (long)(2 * Y + (sp >> 1)) * side + 2 * X + (sp & 1);
+    const int n = r2t[q];
+    const zd a = o[n], b = inc[t * N + n], c = tot[t * N + n], d = dtot[t * N + n];
+    float* p = x32 + (t * (N >> 2) + pos) * 32 + 8 * sp;
+    p[0] = (float)a.x * sc.in[0];
+    p[1] = (float)a.y * sc.in[1];
+    p[2] = (float)(b.x * s) * sc.in[2];
+    p[3] = (float)(b.y * s) * sc.in[3];
+    p[4] = (float)(c.x * s) * sc.in[4];
+    p[5] = (float)(c.y * s) * sc.in[5];
+    p[6] = (float)(-alpha * d.x * s) * sc.in[6];
+    p[7] = (float)(-alpha * d.y * s) * sc.in[7];
   }
 }
 
-// d[t][r2t[q]] = out[t][q] * scale[t] (complex from 2 floats)
-__global__ void __launch_bounds__(kBlock) k_net_dir(const float* out, const double* rms, const int* r2t, zd* d,
-                                                    long N) {
-  int t = blockIdx.y;
-  const double s = rms[t];
+// d[t][r2t[q]] = the network's output for raster pixel q (channels 2 s, 2 s + 1
+// of its position in the packed output, s its sub-pixel) * scale[t]
+__global__ void __launch_bounds__(kBlock) k_net_dir(const float* y16, const double* scale, const int* r2t, int side,
+                                                    zd* d) {
+  const int t = blockIdx.y;
+  const long N = (long)side * side;
+  const double s = scale[t];
+  const int hs = side / 2;
   DB_LOOP(N) {
-    const float* p = out + (t * N + i) * 2;
+    const int y = (int)(i / side), x = (int)(i % side);
+    const int sp = 2 * (y & 1) + (x & 1);
+    const float* p = y16 + (t * (N >> 2) + (long)(y >> 1) * hs + (x >> 1)) * 16 + 2 * sp;
     d[t * N + r2t[i]] = mkc<double>(p[0] * s, p[1] * s);
   }
+}
+
+// out[t] = a[t] * f, t < n
+__global__ void k_scale_rows(const double* a, double f, int n, double* out) {
+  for (int t = threadIdx.x; t < n; t += blockDim.x) out[t] = a[t] * f;
 }
 
 // part[t][block] = partial sum of |a[t][n]|^2 over the block's share of row t
@@ -559,16 +581,13 @@ void Dbim::set_warmnet(WarmNet* net) {
     inc_rms_ = (double*)dalloc(sizeof(double) * 2 * T_);
     rowpart_ = (double*)dalloc(sizeof(double) * nblk_ * B_);
     step_rms_ = (double*)dalloc(sizeof(double) * B_);
+    out_scale_ = (double*)dalloc(sizeof(double) * B_);
     DB_HIP(hipHostMalloc((void**)&rowpart_h_, sizeof(double) * nblk_ * B_, hipHostMallocDefault));
     DB_HIP(hipHostMalloc((void**)&step_rms_h_, sizeof(double) * B_, hipHostMallocDefault));
     if (!ok()) return;
     DB_HIP(hipMemcpy(r2t_, P.raster2tree.data(), sizeof(int) * N_, hipMemcpyHostToDevice));
   }
-  if (net->max_batch() > net_chunk_) {
-    net_chunk_ = std::min(net->max_batch(), B_);
-    net_in_ = (float*)dalloc(sizeof(float) * WarmNet::kInCh * (size_t)N_ * net_chunk_);
-    net_out_ = (float*)dalloc(sizeof(float) * 2 * (size_t)N_ * net_chunk_);
-  }
+  net_chunk_ = std::min(net->max_batch(), B_);
   rms_ok_ = false;
 }
 
@@ -597,7 +616,7 @@ void Dbim::incident_rms() {
 // -alpha dtot_t) into raster images, run the network, scatter its correction
 // back to tree order, then one blended solve for the batch.
 void Dbim::learned_solve(zd* tot, int t0, int nb, double alpha, DbimStats& st) {
-  if (!net_ || !net_in_) {
+  if (!net_ || net_chunk_ < 1) {
     if (err_.empty()) err_ = "field_start 3 needs a warm-start network (isg_dbim_set_warmnet)";
     return;
   }
@@ -619,17 +638,28 @@ void Dbim::learned_solve(zd* tot, int t0, int nb, double alpha, DbimStats& st) {
     step_rms_h_[t] = std::fabs(alpha) * std::sqrt(e / N);
   }
   DB_HIP(hipMemcpyAsync(step_rms_, step_rms_h_, sizeof(double) * nb, hipMemcpyHostToDevice, s));
+  NetScales sc;
+  for (int c = 0; c < WarmNet::kInCh; c++) sc.in[c] = net_->scales()[c];
   for (int c0 = 0; c0 < nb; c0 += net_chunk_) {
     int nc = std::min(net_chunk_, nb - c0);
+    float* x32 = net_->packed_input(nc, side, side);
+    if (!x32) {
+      if (err_.empty()) err_ = net_->error();
+      return;
+    }
     k_net_in<<<dim3(nblk_, nc), dim3(kBlock), 0, s>>>(o_, inc + (size_t)c0 * N, tot + (size_t)c0 * N,
                                                        dtot + (size_t)c0 * N, alpha, inc_rms_ + T_ + t0 + c0, r2t_,
-                                                       net_in_, N);
-    net_->forward(net_in_, net_out_, nc, side, side, s);
+                                                       side, sc, x32);
+    net_->forward_packed(nc, side, side, s);
     if (!net_->ok()) {
       if (err_.empty()) err_ = net_->error();
       return;
     }
-    k_net_dir<<<dim3(nblk_, nc), dim3(kBlock), 0, s>>>(net_out_, step_rms_ + c0, r2t_, wdir_ + (size_t)c0 * N, N);
+    // correction = packed output * output scale * rms of the transmitter's first-order step
+    k_scale_rows<<<dim3(1), dim3(kBlock), 0, s>>>(step_rms_ + c0, (double)net_->scales()[WarmNet::kInCh], nc,
+                                                   out_scale_);
+    k_net_dir<<<dim3(nblk_, nc), dim3(kBlock), 0, s>>>(net_->packed_output(), out_scale_, r2t_, side,
+                                                        wdir_ + (size_t)c0 * N);
   }
   DB_HIP(hipStreamSynchronize(s));
   st.net_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - tn).count();

@@ -108,12 +108,13 @@ class Dbim {
   bool dtot_ok_ = false, adj_ok_ = false;  // the stores hold a previous iteration's solutions
   WarmNet* net_ = nullptr;
   int net_chunk_ = 0;
-  float *net_in_ = nullptr, *net_out_ = nullptr;  // [chunk][side][side][6], [..][2]
+
   zd* wdir_ = nullptr;                             // [max_rhs][N] learned corrections
   int* r2t_ = nullptr;                             // raster -> tree index
   double* inc_rms_ = nullptr;                      // [2][numtx]: rms of inc_t, then 1 / rms
   double *rowpart_ = nullptr, *rowpart_h_ = nullptr;   // per-row partial sums [max_rhs][nblk]
   double *step_rms_ = nullptr, *step_rms_h_ = nullptr; // [max_rhs]: rms of alpha dtot_t (output scale)
+  double* out_scale_ = nullptr;                        // [max_rhs]: step rms * the network's output scale
   bool rms_ok_ = false;
   double* part_ = nullptr;
   double* part_h_ = nullptr;

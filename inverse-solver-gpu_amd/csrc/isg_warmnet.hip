@@ -583,12 +583,44 @@ void WarmNet::wgrad(int li, const float* dy, const float* m, const float* x, con
   k_chunk_sum<<<grid_for(l.cout), kNB, 0, s>>>(bpart, l.cout, (int)nch, g_ + l.boff);
 }
 
+static bool shape_ok(int batch, int H, int W, int max_batch, long max_pix) {
+  constexpr int T = WarmNet::kTile;
+  return batch >= 1 && batch <= max_batch && H >= T && W >= T && H % T == 0 && W % T == 0 && (long)H * W <= max_pix;
+}
+
+static const char* kShapeMsg = "warm-start network: image shape outside the workspace (H, W multiples of 16)";
+
 // H, W: the image's sides; the network runs on the space-to-depth image.
 void WarmNet::forward_all(const float* in, int batch, int Hi, int Wi, hipStream_t s) {
   WN_HIP(hipMemcpyAsync(dscale_, scales_, sizeof(float) * kScales, hipMemcpyHostToDevice, s));
   const int H = Hi / kS2D, W = Wi / kS2D;
   const long np = (long)batch * H * W;
   k_net_pack<<<grid_for(np * 32), kNB, 0, s>>>(in, batch, Hi, Wi, dscale_, act_[A_X]);
+  forward_layers(batch, H, W, s);
+}
+
+float* WarmNet::packed_input(int batch, int Hi, int Wi) {
+  if (!ok()) return nullptr;
+  if (!shape_ok(batch, Hi, Wi, max_batch_, max_pix_)) {
+    err_ = kShapeMsg;
+    return nullptr;
+  }
+  return act_[A_X];
+}
+
+void WarmNet::forward_packed(int batch, int Hi, int Wi, hipStream_t s) {
+  if (!ok()) return;
+  if (!shape_ok(batch, Hi, Wi, max_batch_, max_pix_)) {
+    err_ = kShapeMsg;
+    return;
+  }
+  forward_layers(batch, Hi / kS2D, Wi / kS2D, s);
+  WN_HIP(hipGetLastError());
+}
+
+// every layer from the packed input act_[A_X] (H, W: the network's level-0 sides)
+void WarmNet::forward_layers(int batch, int H, int W, hipStream_t s) {
+  const long np = (long)batch * H * W;
   const int H1 = H / 2, W1 = W / 2, H2 = H / 4, W2 = W / 4, H3 = H / 8, W3 = W / 8;
   conv(0, act_[A_X], nullptr, nullptr, act_[A_E0], 32, false, batch, H, W, s);
   k_pool<<<grid_for(np * 32 / 4), kNB, 0, s>>>(act_[A_E0], act_[A_P0], batch, H, W, 32, 0);
@@ -607,12 +639,6 @@ void WarmNet::forward_all(const float* in, int batch, int Hi, int Wi, hipStream_
   WN_HIP(hipGetLastError());
 }
 
-static bool shape_ok(int batch, int H, int W, int max_batch, long max_pix) {
-  constexpr int T = WarmNet::kTile;
-  return batch >= 1 && batch <= max_batch && H >= T && W >= T && H % T == 0 && W % T == 0 && (long)H * W <= max_pix;
-}
-
-static const char* kShapeMsg = "warm-start network: image shape outside the workspace (H, W multiples of 16)";
 
 void WarmNet::forward(const float* in, float* out, int batch, int H, int W, hipStream_t s) {
   if (!ok()) return;

@@ -83,6 +83,14 @@ class WarmNet {
 
   // in: [batch][H][W][8] FP32, out: [batch][H][W][2] FP32 (device pointers).
   void forward(const float* in, float* out, int batch, int H, int W, hipStream_t s);
+  // The same for a caller that writes the network's input itself: packed_input
+  // is the space-to-depth input [batch][H/2][W/2][32] (channel 8 (2 dy + dx) + c,
+  // already multiplied by the input scales), forward_packed runs the layers and
+  // packed_output holds [batch][H/2][W/2][16] (channel 2 (2 dy + dx) + c, c < 2,
+  // before the output scale).  packed_input returns null on a bad shape.
+  float* packed_input(int batch, int H, int W);
+  void forward_packed(int batch, int H, int W, hipStream_t s);
+  const float* packed_output() const { return act_[A_OUT]; }
   // Forward + backward of the mean squared error against target
   // [batch][H][W][2]; gradients go to the internal gradient array.  Returns
   // the loss (synchronises s).
@@ -95,6 +103,7 @@ class WarmNet {
 
  private:
   void forward_all(const float* in, int batch, int H, int W, hipStream_t s);
+  void forward_layers(int batch, int H, int W, hipStream_t s);
   void conv(int li, const float* x, const float* u, const float* m, float* y, int cstore, bool transpose,
             int batch, int H, int W, hipStream_t s);
   void wgrad(int li, const float* dy, const float* m, const float* x, const float* u, int batch, int H, int W,
