@@ -457,6 +457,11 @@ def main():
         "misfit_by_iteration": misfit_by_iteration(stats),
         "synthesis": synth,
     }
+    if timing:
+        # GPU time of the live-timed kernel groups per step (HIP events on the solver's stream;
+        # near/m2l are the FP64 operator, *_fp32 the FP32 one, p2m/interp either precision,
+        # receivers the receiver operator calls)
+        out["kernel_timers_ms_per_step"] = {k: round(v[0] / args.steps, 3) for k, v in prof.items() if v[1] > 0}
     if main_path == "learned":
         out["network_s_per_iter"] = round(sum(s["net_seconds"] for s in stats) / len(stats), 4)
         out["mean_abs_lambda"] = round(sum(s["lambda_abs"] for s in stats) / len(stats), 4)

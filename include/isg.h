@@ -278,6 +278,10 @@ int isg_tuning_load(isg_context* ctx, const char* path);
  * kernel 1: M2L over all levels of one matvec
  * kernel 2: BiCGSTAB x/r update with its two dot products
  * kernel 3, 4: as 0, 1 for the FP32 operator of the mixed path
+ * kernel 5: P2M of one matvec (either precision)
+ * kernel 6: M2M and L2L interpolation of one matvec (either precision)
+ * kernel 7: receiver operator (isg_aggregate / isg_aggregateh and the DBIM
+ *           loop's receiver products), one launch set per call
  * isg_profile(ctx, 1) clears the totals and starts recording. */
 int isg_profile(isg_context* ctx, int enable);
 int isg_profile_read(isg_context* ctx, int kernel, double* total_ms, long long* launches);

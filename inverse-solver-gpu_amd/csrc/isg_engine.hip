@@ -771,8 +771,12 @@ template <typename T, typename TI>
 void Engine::apply(const cplx<TI>* y, cplx<TI>* r, int nrhs, const int* act, const NearPost<TI>* post) {
   const int L = P_.L, Lf = L - 1;
   // post mode 3 (residual check): y is the iterate, weighted while P2M and the near field load it
+  prof_begin(KT_P2M);
   launch_p2m<T, TI>(y, nrhs, act, -1, (post && post->mode == 3) ? post : nullptr);
+  prof_end(KT_P2M);
+  prof_begin(KT_INTERP);
   for (int i = L - 2; i >= 2; i--) launch_m2m<T>(i, nrhs, act);
+  prof_end(KT_INTERP);
   const int kt = sizeof(T) == 4 ? KT_M2L32 : KT_M2L;
   // with the fused leaf kernel the last L2L (level Lf-1 -> Lf) runs inside the
   // leaf M2L, after the L2L passes above it
@@ -781,7 +785,9 @@ void Engine::apply(const cplx<TI>* y, cplx<TI>* r, int nrhs, const int* act, con
   prof_begin(kt);
   for (int i = 2; i <= mtop; i++) launch_m2l<T>(i, nrhs, act);
   prof_end(kt);
+  prof_begin(KT_INTERP, false);  // timed with the M2M launch set of this product
   for (int i = 2; i <= ltop; i++) launch_l2l<T>(i, true, nrhs, act);
+  prof_end(KT_INTERP);
   if (fuse) {
     prof_begin(kt, false);  // timed with the M2L launch set of this product
     launch_m2l_leaf<T>(nrhs, act);
@@ -835,35 +841,40 @@ void Engine::mlfma(const zd* x, zd* r, int nrhs, const int* act, int prec) {
 // on the device as well (FP64).
 void Engine::aggregate(const zd* x, zd* r, int nrhs) {
   if (!ok()) return;
+  prof_begin(KT_RX);
   if (cfg_.plan.radius > 0) {
     rx_apply(x, r, nrhs);
-    return;
+  } else {
+    const int L = P_.L;
+    launch_p2m<double, double>(x, nrhs, nullptr);
+    for (int i = L - 2; i >= 0; i--) launch_m2m<double>(i, nrhs, nullptr);
+    int R = cfg_.plan.numrx;
+    k_rx_interp<double><<<dim3(nblocks(R), nrhs), dim3(kBlock), 0, stream_>>>(
+        op64_.agm + lvl_off_[0], (long)lvl_size_[0], r, R, op64_.rxT, rx0_, P_.nsamp[0], R, cfg_.plan.ninter);
+    ISG_HIP(hipGetLastError());
   }
-  const int L = P_.L;
-  launch_p2m<double, double>(x, nrhs, nullptr);
-  for (int i = L - 2; i >= 0; i--) launch_m2m<double>(i, nrhs, nullptr);
-  int R = cfg_.plan.numrx;
-  k_rx_interp<double><<<dim3(nblocks(R), nrhs), dim3(kBlock), 0, stream_>>>(
-      op64_.agm + lvl_off_[0], (long)lvl_size_[0], r, R, op64_.rxT, rx0_, P_.nsamp[0], R, cfg_.plan.ninter);
-  ISG_HIP(hipGetLastError());
+  prof_end(KT_RX);
 }
 
 // matvec_gpu.cu:369-470 (argument order out x, in r as in the reference),
 // every level on the device (FP64).
 void Engine::aggregateh(const zd* r, zd* x, int nrhs) {
   if (!ok()) return;
+  prof_begin(KT_RX);
   if (cfg_.plan.radius > 0) {
     rx_apply_h(r, x, nrhs);
-    return;
+  } else {
+    const int L = P_.L;
+    int R = cfg_.plan.numrx;
+    double sc = (double)R * cfg_.plan.res * cfg_.plan.res;
+    k_rx_anterp<double><<<dim3(nblocks(P_.nsamp[0]), nrhs), dim3(kBlock), 0, stream_>>>(
+        r, R, op64_.agl + lvl_off_[0], (long)lvl_size_[0], op64_.rxaT, rxaI_, P_.nsamp[0], 2 * cfg_.plan.ninter,
+        sc);
+    for (int i = 0; i <= L - 2; i++) launch_l2l<double>(i, false, nrhs, nullptr);
+    launch_near<double, double>(nullptr, x, nrhs, nullptr, false);
+    ISG_HIP(hipGetLastError());
   }
-  const int L = P_.L;
-  int R = cfg_.plan.numrx;
-  double sc = (double)R * cfg_.plan.res * cfg_.plan.res;
-  k_rx_anterp<double><<<dim3(nblocks(P_.nsamp[0]), nrhs), dim3(kBlock), 0, stream_>>>(
-      r, R, op64_.agl + lvl_off_[0], (long)lvl_size_[0], op64_.rxaT, rxaI_, P_.nsamp[0], 2 * cfg_.plan.ninter, sc);
-  for (int i = 0; i <= L - 2; i++) launch_l2l<double>(i, false, nrhs, nullptr);
-  launch_near<double, double>(nullptr, x, nrhs, nullptr, false);
-  ISG_HIP(hipGetLastError());
+  prof_end(KT_RX);
 }
 
 void Engine::launch_prep(const zd* x, const zd* g, long gs, zd* y, bool herm, int nrhs) {

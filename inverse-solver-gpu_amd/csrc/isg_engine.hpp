@@ -139,8 +139,10 @@ class Engine {
   void launch_prep(const zd* x, const zd* g, long g_stride, zd* y, bool herm, int nrhs);
 
   // Live per-kernel timing with HIP events on the engine stream.
-  // KT_NEAR/KT_M2L: FP64 operator launches; KT_NEAR32/KT_M2L32: FP32 ones.
-  enum { KT_NEAR = 0, KT_M2L = 1, KT_VEC = 2, KT_NEAR32 = 3, KT_M2L32 = 4, KT_COUNT = 5 };
+  // KT_NEAR/KT_M2L: FP64 operator launches; KT_NEAR32/KT_M2L32: FP32 ones;
+  // KT_P2M, KT_INTERP (M2M + L2L), KT_RX (receiver operator): either precision.
+  enum { KT_NEAR = 0, KT_M2L = 1, KT_VEC = 2, KT_NEAR32 = 3, KT_M2L32 = 4, KT_P2M = 5, KT_INTERP = 6, KT_RX = 7,
+         KT_COUNT = 8 };
   void profile(bool on);
   void profile_read(int id, double& ms, long long& launches);
 

@@ -300,7 +300,8 @@ class Solver:
         self._err(self.lib.isg_stage(self.h, which, level, rhs, _dp(out)), "isg_stage")
         return out.reshape(4 ** level, self.numsamp[level])
 
-    KERNELS = {"near": 0, "m2l": 1, "bicgs_update": 2, "near_fp32": 3, "m2l_fp32": 4}
+    KERNELS = {"near": 0, "m2l": 1, "bicgs_update": 2, "near_fp32": 3, "m2l_fp32": 4, "p2m": 5, "interp": 6,
+               "receivers": 7}
 
     def profile(self, enable=True):
         """Start (clearing totals) or stop live HIP-event timing of the hot kernels."""
