@@ -163,6 +163,8 @@ def synthesize(isg, common, batch, otrue, iti):
     kw = dict(common)
     kw["iti"] = iti
     gen = isg.Solver(max_rhs=batch, precision=1, mixed_margin=0.5, **kw)
+    gen.set_progress(lambda done, total: log("measurements: %d of %d transmitters solved (%.1fs)"
+                                             % (done, total, time.perf_counter() - t0)))
     d = isg.Dbim(gen, otrue, field_start=1)
     mes = d.get("mes")
     s = d.setup_stats

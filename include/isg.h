@@ -213,6 +213,12 @@ int isg_dbim_setup(isg_context* ctx, const double* o_true, int field_start, isg_
  * NULL (object_error then reports sum|o|^2). */
 int isg_dbim_setup_measured(isg_context* ctx, const double* mes, const double* o_true, int field_start,
                             isg_dbim_stats* st);
+/* Progress of isg_dbim_setup's measurement synthesis (no counterpart in the
+ * reference): fn(user, done, total) after each batch of transmitters has been
+ * solved and its measurements formed, done of total transmitters; fn NULL
+ * turns it off.  Kept by the context for later isg_dbim_setup calls. */
+typedef void (*isg_progress_fn)(void* user, int done, int total);
+int isg_set_progress(isg_context* ctx, isg_progress_fn fn, void* user);
 /* One DBIM iteration (mom.cpp:277-400). */
 int isg_dbim_iterate(isg_context* ctx, isg_dbim_stats* st);
 /* Back to the state isg_dbim_setup* left (zero object, total fields = incident

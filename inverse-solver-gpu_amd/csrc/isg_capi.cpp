@@ -36,6 +36,9 @@ struct isg_context {
   isg::zd* drx = nullptr;
   // per-system results of isg_bicgs_dev_static (capacity max_rhs, reserved at creation)
   std::vector<isg::SolveStat> sst;
+  // isg_set_progress
+  isg_progress_fn progress = nullptr;
+  void* progress_user = nullptr;
 };
 
 namespace {
@@ -401,6 +404,11 @@ int isg_dbim_setup(isg_context* c, const double* o_true, int field_start, isg_db
   c->dbim.reset(new isg::Dbim(*c->eng, opt));
   if (!c->dbim->ok()) return fail(c, ISG_E_DEVICE, c->dbim->error());
   isg::DbimStats s;
+  if (c->progress) {
+    isg_progress_fn fn = c->progress;
+    void* user = c->progress_user;
+    c->dbim->set_progress([fn, user](int done, int total) { fn(user, done, total); });
+  }
   c->dbim->setup((const isg::zd*)o_true, s);
   if (!c->dbim->ok()) return fail(c, ISG_E_DEVICE, c->dbim->error());
   fill_stats(s, st);
@@ -482,6 +490,13 @@ int isg_dbim_iterate(isg_context* c, isg_dbim_stats* st) {
   c->dbim->iterate(s);
   if (!c->dbim->ok()) return fail(c, ISG_E_DEVICE, c->dbim->error());
   fill_stats(s, st);
+  return ISG_OK;
+}
+
+int isg_set_progress(isg_context* c, isg_progress_fn fn, void* user) {
+  if (!check(c)) return ISG_E_STATE;
+  c->progress = fn;
+  c->progress_user = user;
   return ISG_OK;
 }
 

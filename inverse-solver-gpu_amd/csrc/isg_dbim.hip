@@ -403,6 +403,11 @@ void Dbim::setup(const zd* otrue, DbimStats& st) {
     zd* wy = rxw_ ? rxw_ + (size_t)t0 * N : wy_;
     k_rowmul<<<dim3(nblk_, nb), blk, 0, s>>>(wx_, g_, wy, N);
     if (!rxw_) eng_.aggregate(wy, mes_ + (size_t)t0 * R_, nb);
+    if (progress_) {
+      DB_HIP(hipStreamSynchronize(s));
+      if (!ok()) return;
+      progress_(t0 + nb, T_);
+    }
   }
   if (rxw_) eng_.aggregate(rxw_, mes_, T_);
   std::vector<zd> mes((size_t)T_ * R_);

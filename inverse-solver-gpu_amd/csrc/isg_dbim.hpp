@@ -7,6 +7,7 @@
 // and a batch of every transmitter is one multi-RHS solve per phase.
 #pragma once
 
+#include <functional>
 #include <vector>
 
 #include "isg_engine.hpp"
@@ -74,6 +75,9 @@ class Dbim {
   int iteration() const { return iter_; }
   // Network used by field_start 3 (not owned; must outlive the iterations).
   void set_warmnet(WarmNet* net);
+  // Called by setup() after each batch of synthesized measurements with
+  // (transmitters done, numtx); the stream has been synchronized.
+  void set_progress(std::function<void(int, int)> fn) { progress_ = std::move(fn); }
 
  private:
   void start(const zd* mes_host, const zd* otrue_host, DbimStats& st);
@@ -107,6 +111,7 @@ class Dbim {
   zd* adj_ = nullptr;   // [numtx][N] adjoint solutions (solve_start 1)
   bool dtot_ok_ = false, adj_ok_ = false;  // the stores hold a previous iteration's solutions
   WarmNet* net_ = nullptr;
+  std::function<void(int, int)> progress_;
   int net_chunk_ = 0;
 
   zd* wdir_ = nullptr;                             // [max_rhs][N] learned corrections

@@ -48,9 +48,13 @@ __global__ void __launch_bounds__(kBlock) k_clear_words(unsigned* __restrict__ p
 
 // Finite-radius receivers (line-source geometry): K[m][n] = (i/4) H0^(1)(k0
 // |rho_m - r_n|) res^2 with rho_m = radius (cos, sin)(2 pi m / R), the
-// midpoint-rule Green's function weight of pixel n seen from receiver m.
-__global__ void __launch_bounds__(kBlock) k_rx_matrix(const zd* __restrict__ pos, long N, int R, double radius,
-                                                      double k0, double res2, zd* __restrict__ K) {
+// midpoint-rule Green's function weight of pixel n seen from receiver m,
+// stored in chunks of kc unknowns: Kb[b][m][j] = K[m][b kc + j].  (rocBLAS's
+// GEMM kernels slow down threefold once one operand spans 4 GiB -- the
+// unchunked 2048^2 x 512 operator is 34 GB -- so every operand is kept
+// chunk-compact.)
+__global__ void __launch_bounds__(kBlock) k_rx_matrix(const zd* __restrict__ pos, long N, int R, long kc, double radius,
+                                                      double k0, double res2, zd* __restrict__ Kb) {
   const int m = blockIdx.y;
   double sa, ca;
   sincos(2.0 * M_PI * m / R, &sa, &ca);
@@ -58,8 +62,19 @@ __global__ void __launch_bounds__(kBlock) k_rx_matrix(const zd* __restrict__ pos
   for (long n = (long)blockIdx.x * kBlock + threadIdx.x; n < N; n += (long)gridDim.x * kBlock) {
     const double dx = rx - pos[n].x, dy = ry - pos[n].y;
     const double kd = k0 * sqrt(dx * dx + dy * dy);
+    const long b = n / kc, j = n - b * kc;
     // (i/4)(J0 + i Y0) res^2
-    K[(long)m * N + n] = mkc<double>(-0.25 * y0(kd) * res2, 0.25 * j0(kd) * res2);
+    Kb[(b * R + m) * kc + j] = mkc<double>(-0.25 * y0(kd) * res2, 0.25 * j0(kd) * res2);
+  }
+}
+
+// xb[b][t][j] = x[t][b kc + j], t < T (the receiver product's input in the operator's chunking)
+__global__ void __launch_bounds__(kBlock) k_rx_chunk(const zd* __restrict__ x, long N, int T, long kc,
+                                                     zd* __restrict__ xb) {
+  const long tot = N * T;
+  for (long i = (long)blockIdx.x * kBlock + threadIdx.x; i < tot; i += (long)gridDim.x * kBlock) {
+    const long t = i / N, n = i - t * N, b = n / kc, j = n - b * kc;
+    xb[(b * T + t) * kc + j] = x[i];
   }
 }
 
@@ -357,12 +372,13 @@ void Engine::setup_receivers() {
   const long N = P_.N;
   rxm_ = (zd*)dalloc((size_t)Rx * N * sizeof(zd));
   zd* pos = (zd*)dalloc((size_t)N * sizeof(zd));
-  rx_kc_ = (int)std::min<long>(N, 4096);
+  rx_kc_ = (int)std::min<long>(N, 16384);  // N is 64 * 4^k: a multiple of the chunk
   rxpart_ = (zd*)dalloc((size_t)(N / rx_kc_) * Rx * R * sizeof(zd));
   rxc_ = (zd*)dalloc((size_t)Rx * R * sizeof(zd));
+  rxin_ = (zd*)dalloc((size_t)N * R * sizeof(zd));
   if (!ok()) return;
   ISG_HIP(hipMemcpy(pos, P_.pos.data(), (size_t)N * sizeof(zd), hipMemcpyHostToDevice));
-  k_rx_matrix<<<dim3(nblocks(N), Rx), dim3(kBlock), 0, stream_>>>(pos, N, Rx, cfg_.plan.radius, P_.k0,
+  k_rx_matrix<<<dim3(nblocks(N), Rx), dim3(kBlock), 0, stream_>>>(pos, N, Rx, rx_kc_, cfg_.plan.radius, P_.k0,
                                                                    cfg_.plan.res * cfg_.plan.res, rxm_);
   ISG_HIP(hipGetLastError());
   rocblas_handle h = nullptr;
@@ -374,17 +390,20 @@ void Engine::setup_receivers() {
   ISG_HIP(hipStreamSynchronize(stream_));
 }
 
-// r[t][m] = sum_n K[m][n] x[t][n]: split-K strided-batched ZGEMM (one batch
-// per chunk of rx_kc_ unknowns) and a fixed-order sum of the chunks.
+// r[t][m] = sum_n K[m][n] x[t][n]: x copied into the operator's chunking, one
+// strided-batched ZGEMM (batch b: the Rx x kc block of chunk b times the
+// kc x nrhs block of x) and a fixed-order sum of the chunks.
 void Engine::rx_apply(const zd* x, zd* r, int nrhs) {
   const int Rx = cfg_.plan.numrx;
-  const long N = P_.N;
-  const int nb = (int)(N / rx_kc_);
+  const long N = P_.N, kc = rx_kc_;
+  const int nb = (int)(N / kc);
   const rocblas_double_complex one = {1.0, 0.0}, zero = {0.0, 0.0};
+  k_rx_chunk<<<nblocks(N * nrhs), kBlock, 0, stream_>>>(x, N, nrhs, kc, rxin_);
   rocblas_status st = rocblas_zgemm_strided_batched(
-      (rocblas_handle)blas_, rocblas_operation_transpose, rocblas_operation_none, Rx, nrhs, rx_kc_, &one,
-      (const rocblas_double_complex*)rxm_, (rocblas_int)N, rx_kc_, (const rocblas_double_complex*)x, (rocblas_int)N,
-      rx_kc_, &zero, (rocblas_double_complex*)rxpart_, Rx, (rocblas_stride)Rx * nrhs, nb);
+      (rocblas_handle)blas_, rocblas_operation_transpose, rocblas_operation_none, Rx, nrhs, (rocblas_int)kc, &one,
+      (const rocblas_double_complex*)rxm_, (rocblas_int)kc, (rocblas_stride)Rx * kc,
+      (const rocblas_double_complex*)rxin_, (rocblas_int)kc, (rocblas_stride)nrhs * kc, &zero,
+      (rocblas_double_complex*)rxpart_, Rx, (rocblas_stride)Rx * nrhs, nb);
   if (st != rocblas_status_success) {
     set_error("rocblas_zgemm_strided_batched failed");
     return;
@@ -394,19 +413,23 @@ void Engine::rx_apply(const zd* x, zd* r, int nrhs) {
   ISG_HIP(hipGetLastError());
 }
 
-// x[t][n] = sum_m conj(K[m][n]) r[t][m] = conj(sum_m K[m][n] conj(r[t][m])): one ZGEMM
-// with K read as the N x numrx column-major matrix, conjugations before and after.
+// x[t][n] = sum_m conj(K[m][n]) r[t][m] = conj(sum_m K[m][n] conj(r[t][m])): one
+// strided-batched ZGEMM over the chunks (batch b: chunk b read as the kc x numrx
+// column-major block, times the conjugated receiver values, into x[.][b kc ..]),
+// conjugations before and after.
 void Engine::rx_apply_h(const zd* r, zd* x, int nrhs) {
   const int Rx = cfg_.plan.numrx;
-  const long N = P_.N;
+  const long N = P_.N, kc = rx_kc_;
+  const int nb = (int)(N / kc);
   const rocblas_double_complex one = {1.0, 0.0}, zero = {0.0, 0.0};
   k_conj_copy<<<nblocks((long)Rx * nrhs), kBlock, 0, stream_>>>(r, rxc_, (long)Rx * nrhs);
-  rocblas_status st = rocblas_zgemm((rocblas_handle)blas_, rocblas_operation_none, rocblas_operation_none,
-                                    (rocblas_int)N, nrhs, Rx, &one, (const rocblas_double_complex*)rxm_,
-                                    (rocblas_int)N, (const rocblas_double_complex*)rxc_, Rx, &zero,
-                                    (rocblas_double_complex*)x, (rocblas_int)N);
+  rocblas_status st = rocblas_zgemm_strided_batched(
+      (rocblas_handle)blas_, rocblas_operation_none, rocblas_operation_none, (rocblas_int)kc, nrhs, Rx, &one,
+      (const rocblas_double_complex*)rxm_, (rocblas_int)kc, (rocblas_stride)Rx * kc,
+      (const rocblas_double_complex*)rxc_, Rx, (rocblas_stride)0, &zero, (rocblas_double_complex*)x,
+      (rocblas_int)N, (rocblas_stride)kc, nb);
   if (st != rocblas_status_success) {
-    set_error("rocblas_zgemm failed");
+    set_error("rocblas_zgemm_strided_batched failed (adjoint receiver product)");
     return;
   }
   k_conj_copy<<<nblocks(N * nrhs), kBlock, 0, stream_>>>(x, x, N * nrhs);

@@ -273,10 +273,13 @@ class Engine {
   std::vector<zd*> vec_;          // solver vectors, each [max_rhs][N]
   std::vector<cplx<float>*> vec32_;  // FP32 copies of the work vectors (mixed solver)
   // finite-radius receivers: dense operator K[m][n] = (i/4) H0(k0 |rho_m - r_n|) res^2,
-  // applied with rocBLAS (split-K partial products rxpart_, conjugated inputs rxc_)
+  // stored in chunks of rx_kc_ unknowns, rxm_[b][m][j] = K[m][b rx_kc_ + j], and applied
+  // with rocBLAS: split-K partial products rxpart_, inputs copied into the same chunking
+  // (rxin_[b][t][j]), conjugated receiver values rxc_
   zd* rxm_ = nullptr;
   zd* rxpart_ = nullptr;
   zd* rxc_ = nullptr;
+  zd* rxin_ = nullptr;
   int rx_kc_ = 0;
   void* blas_ = nullptr;  // rocblas_handle
   double2* part_ = nullptr;       // reduction partials [max_rhs][3][part_blocks_]

@@ -115,6 +115,7 @@ _SIGS = {
     "isg_profile_read": (ctypes.c_int, [_V, ctypes.c_int, _D, _P(ctypes.c_longlong)]),
     "isg_dbim_iterate": (ctypes.c_int, [_V, _P(DbimStats)]),
     "isg_dbim_restart": (ctypes.c_int, [_V]),
+    "isg_set_progress": (ctypes.c_int, [_V, _V, _V]),
     "isg_dbim_get": (ctypes.c_int, [_V, ctypes.c_int, _D]),
     "isg_dbim_set": (ctypes.c_int, [_V, ctypes.c_int, _D]),
     "isg_warmnet_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_long, ctypes.c_ulonglong, _P(_V)]),
@@ -337,6 +338,15 @@ class Solver:
     def tuning_load(self, path=None):
         """Load a tuning cache (default: the committed MI355X cache)."""
         self._err(self.lib.isg_tuning_load(self.h, str(path or self.TUNING_FILE).encode()), "isg_tuning_load")
+
+    _PROGRESS = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_int, ctypes.c_int)
+
+    def set_progress(self, fn=None):
+        """fn(done, total) after each batch of the measurement synthesis of a
+        later Dbim(...) without measurements; None turns it off."""
+        self._progress_cb = self._PROGRESS(lambda user, done, total: fn(done, total)) if fn else None
+        ptr = ctypes.cast(self._progress_cb, ctypes.c_void_p) if fn else None
+        self._err(self.lib.isg_set_progress(self.h, ptr, None), "isg_set_progress")
 
     def counters(self):
         out = (ctypes.c_longlong * 5)()

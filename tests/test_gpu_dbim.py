@@ -139,3 +139,18 @@ def test_cg_files(case, tmp_path):
     e.read_cg(tmp_path / "cg_1.bin")
     np.testing.assert_array_equal(e.get("o"), d.get("o"))
     assert len((tmp_path / "hist.txt").read_text().splitlines()) == 1
+
+
+def test_synthesis_progress_reports_each_batch(case):
+    """isg_set_progress: the measurement synthesis calls back once per batch of
+    transmitters (16 transmitters in batches of 6: 6, 12, 16 of 16), and the
+    measurements equal those of a run without the callback bitwise."""
+    isg, otrue, _, _ = case
+    kw = dict(level=LEVEL, numrx=NRX, numtx=NTX, toli=TOL, iti=300, regular=REG, max_rhs=6)
+    s = isg.Solver(**kw)
+    calls = []
+    s.set_progress(lambda done, total: calls.append((done, total)))
+    d = isg.Dbim(s, otrue)
+    assert calls == [(6, NTX), (12, NTX), (16, NTX)]
+    plain = isg.Dbim(isg.Solver(**kw), otrue)
+    np.testing.assert_array_equal(d.get("mes"), plain.get("mes"))
