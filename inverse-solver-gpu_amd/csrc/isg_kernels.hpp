@@ -574,30 +574,26 @@ __global__ void __launch_bounds__(kBlock) k_interp_band(const cplx<T>* __restric
   const int li = lane & 15, lk = lane >> 4;
   const int row = K + 1;
   const long p0 = (long)xcd_block() * (NC / 4);
-  // Element e = col * K + j of the NC x K block goes to tile[col * (K + 1) + j]
-  // = tile[e + col].  Each thread starts at e = tid and advances by kBlock;
-  // (col, j) are stepped rather than divided out of e, and the loads of U
-  // strides are issued before the first store.
-  {
+  // M2M: element e = col * K + j of the NC x K block of children goes to
+  // tile[col * (K + 1) + j] = tile[e + col].  L2L: element e = p * K + j of the
+  // NC/4 parents is loaded once and gives the 4 children's columns
+  // tile[(4 p + c) * (K + 1) + j] = shiftL[c][j] parent[p][j].  Each thread
+  // starts at e = tid and advances by kBlock; (col, j) are stepped rather than
+  // divided out of e, and the loads of U strides are issued before the first
+  // store.
+  if constexpr (MODE == 0) {
     constexpr int U = 8;
     const int n = NC * K, dcol = kBlock / K, dj = kBlock - dcol * K;
     int col = tid / K, j = tid - col * K;
-    const cplx<T>* s = src + rhs * ss + (MODE == 0 ? 4 * p0 : p0) * K;
+    const cplx<T>* s = src + rhs * ss + 4 * p0 * K;
     for (int e0 = tid; e0 < n; e0 += U * kBlock) {
-      cplx<T> a[U], b[U];
+      cplx<T> a[U];
       int cu[U];
 #pragma unroll
       for (int u = 0; u < U; u++) {
         const int e = e0 + u * kBlock;
         cu[u] = col;
-        if (e < n) {
-          if (MODE == 0) {
-            a[u] = s[e];  // NC consecutive children of length K
-          } else {
-            a[u] = shift[(col & 3) * K + j];  // NC/4 parents of length K
-            b[u] = s[(long)(col >> 2) * K + j];
-          }
-        }
+        if (e < n) a[u] = s[e];  // NC consecutive children of length K
         j += dj;
         col += dcol;
         if (j >= K) {
@@ -608,7 +604,38 @@ __global__ void __launch_bounds__(kBlock) k_interp_band(const cplx<T>* __restric
 #pragma unroll
       for (int u = 0; u < U; u++) {
         const int e = e0 + u * kBlock;
-        if (e < n) tile[e + cu[u]] = (MODE == 0) ? a[u] : cmul(a[u], b[u]);
+        if (e < n) tile[e + cu[u]] = a[u];
+      }
+    }
+  } else {
+    constexpr int U = 4;
+    const int n = (NC / 4) * K, dp = kBlock / K, dj = kBlock - dp * K;
+    int pp = tid / K, j = tid - pp * K;
+    const cplx<T>* s = src + rhs * ss + p0 * K;
+    for (int e0 = tid; e0 < n; e0 += U * kBlock) {
+      cplx<T> a[U];
+      int pu[U], ju[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int e = e0 + u * kBlock;
+        pu[u] = pp;
+        ju[u] = j;
+        if (e < n) a[u] = s[e];  // NC/4 consecutive parents of length K
+        j += dj;
+        pp += dp;
+        if (j >= K) {
+          j -= K;
+          pp++;
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int e = e0 + u * kBlock;
+        if (e < n) {
+          cplx<T>* t = tile + 4 * pu[u] * row + ju[u];
+#pragma unroll
+          for (int c = 0; c < 4; c++) t[c * row] = cmul(shift[c * K + ju[u]], a[u]);
+        }
       }
     }
   }

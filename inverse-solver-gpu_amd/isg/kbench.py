@@ -5,7 +5,8 @@
 Runs `reps` batched matvecs (nrhs right-hand sides; FP64 operator, FP32
 operator on FP64 vectors, or with --vec32 the FP32 operator on FP32 vectors
 as the mixed-precision solver runs it) with live kernel timing and prints
-one JSON line with the near-field and M2L launch times.  --copy-mib adds a
+one JSON line with the near-field, M2L, P2M and interpolation (M2M + L2L)
+launch times.  --copy-mib adds a
 plain device-to-device copy of that many MiB, whose memory counters
 calibrate FETCH_SIZE / WRITE_SIZE readings of the same profiler run.
 Device buffers come from torch, imported before the library starts HIP.
@@ -62,9 +63,13 @@ def main():
     m2l = "m2l_fp32" if a.precision == 1 else "m2l"
     nms, nn = s.profile_read(near)
     mms, mn = s.profile_read(m2l)
+    pms, pn = s.profile_read("p2m")
+    ims, im = s.profile_read("interp")
     print(json.dumps({"level": a.level, "nrhs": a.nrhs, "precision": a.precision, "vec32": a.vec32, "reps": a.reps,
                       "near_avg_ms": nms / max(nn, 1), "near_launches": nn,
-                      "m2l_avg_ms": mms / max(mn, 1), "m2l_launchsets": mn}), flush=True)
+                      "m2l_avg_ms": mms / max(mn, 1), "m2l_launchsets": mn,
+                      "p2m_avg_ms": pms / max(pn, 1), "interp_avg_ms": ims / max(im, 1),
+                      "interp_launchsets": im}), flush=True)
 
 
 if __name__ == "__main__":
