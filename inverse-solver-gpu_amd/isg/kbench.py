@@ -29,6 +29,8 @@ def main():
     ap.add_argument("--vec32", action="store_true", help="FP32 vectors (needs --precision 1)")
     ap.add_argument("--reps", type=int, default=3)
     ap.add_argument("--copy-mib", type=int, default=0)
+    ap.add_argument("--near", type=int, default=None,
+                    help="force the near-field variant (16, 32 or 64 boxes per tile)")
     a = ap.parse_args()
     import torch
     import isg
@@ -39,6 +41,8 @@ def main():
         torch.cuda.synchronize()
         print("copy %d MiB done" % a.copy_mib, file=sys.stderr, flush=True)
     s = isg.Solver(level=a.level, numrx=16, max_rhs=a.nrhs, precision=a.precision)
+    if a.near is not None:
+        s.tuning_set(a.precision, 1 if a.nrhs >= 16 else 0, "near", 0, a.near)
     rng = np.random.default_rng(0)
     X = (rng.standard_normal((a.nrhs, s.N)) + 1j * rng.standard_normal((a.nrhs, s.N)))
     if a.vec32:
