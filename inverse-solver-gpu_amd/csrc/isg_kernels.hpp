@@ -1058,6 +1058,12 @@ __global__ void __launch_bounds__(kBlock) k_m2l_leaf(const cplx<T>* __restrict__
 // every wave gets 5 units (whole row blocks per wave would leave three waves
 // idle half the time).  PREP (FP64 only): y holds x and the contrast product
 // prep(x) (g x forward, conj(x) adjoint) is formed while loading it.
+// With S <= kP2mRows (at most 5 row blocks) the units' results stay in
+// registers until every wave is done, then go through LDS so that the tile's
+// contiguous output range [b0 S, (b0 + NB) S) is written with whole,
+// coalesced lines (stored from the MFMA layout, a box's row blocks came from
+// different waves and partial lines reached HBM separately).
+constexpr int kP2mRows = 80;
 template <typename T, typename TI, int NB, bool PREP = false>
 __global__ void __launch_bounds__(kBlock) k_p2m_mfma(const cplx<TI>* __restrict__ y, long ys,
                                                      cplx<T>* __restrict__ agm, long as,
@@ -1066,7 +1072,8 @@ __global__ void __launch_bounds__(kBlock) k_p2m_mfma(const cplx<TI>* __restrict_
                                                      long gs = 0, int herm = 0) {
   typedef typename Mfma<T>::v4 v4;
   constexpr int NN = NB / 16;
-  __shared__ cplx<T> tile[NB * 65];
+  constexpr int MAXU = (5 * NN + 3) / 4;  // units per wave with 5 row blocks
+  __shared__ cplx<T> tile[NB * kP2mRows];  // input rows of 65, later the output rows of S
   const int rhs = blockIdx.y;
   if (rhs_off(act, rhs)) return;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -1084,8 +1091,11 @@ __global__ void __launch_bounds__(kBlock) k_p2m_mfma(const cplx<TI>* __restrict_
       [&](int idx, cplx<TI> v) { tile[(idx >> 6) * 65 + (idx & 63)] = ccast<T>(v); });
   __syncthreads();
   const int nrb = (S + 15) / 16, nunit = nrb * NN;
+  const bool staged = S <= kP2mRows;
   cplx<T>* out = agm + rhs * as;
-  for (int u = w; u < nunit; u += 4) {
+  cplx<T> res[MAXU][4];
+  // one unit: rows 16 rb .. +15 of the 16 boxes 16 j .. +15
+  auto unit = [&](int u, cplx<T>(&o)[4]) {
     const int rb = u / NN, j = u - rb * NN;
     v4 p1 = v4{0, 0, 0, 0}, p2 = v4{0, 0, 0, 0}, p3 = v4{0, 0, 0, 0};
     const int krow = 16 * rb + li;
@@ -1103,11 +1113,36 @@ __global__ void __launch_bounds__(kBlock) k_p2m_mfma(const cplx<TI>* __restrict_
         Mfma<T>::cmac3(trow[4 * (k0 + v) + lk], a[v], asum, p1, p2, p3);
       }
     }
-    if (krow < S) {
 #pragma unroll
-      for (int q = 0; q < 4; q++) {
-        const long box = b0 + 16 * j + Mfma<T>::row(lk, q);
-        out[box * S + krow] = mkc<T>(p1[q] - p2[q], p3[q] - p1[q] - p2[q]);
+    for (int q = 0; q < 4; q++) o[q] = mkc<T>(p1[q] - p2[q], p3[q] - p1[q] - p2[q]);
+  };
+  if (staged) {
+#pragma unroll
+    for (int c = 0; c < MAXU; c++) {
+      const int u = w + 4 * c;
+      if (u < nunit) unit(u, res[c]);
+    }
+    __syncthreads();  // the input tile is no longer read: it takes the output
+#pragma unroll
+    for (int c = 0; c < MAXU; c++) {
+      const int u = w + 4 * c;
+      const int rb = u / NN, j = u - rb * NN, krow = 16 * rb + li;
+      if (u < nunit && krow < S) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) tile[(16 * j + Mfma<T>::row(lk, q)) * S + krow] = res[c][q];
+      }
+    }
+    __syncthreads();
+    cplx<T>* ob = out + b0 * S;
+    for (int e = tid; e < NB * S; e += kBlock) ob[e] = tile[e];
+  } else {
+    for (int u = w; u < nunit; u += 4) {
+      cplx<T> o[4];
+      unit(u, o);
+      const int rb = u / NN, j = u - rb * NN, krow = 16 * rb + li;
+      if (krow < S) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) out[(b0 + 16 * j + Mfma<T>::row(lk, q)) * S + krow] = o[q];
       }
     }
   }
