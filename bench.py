@@ -55,7 +55,7 @@ import numpy as np  # noqa: E402
 METRIC = "s per DBIM iteration, 1024² px ×64 Tx, rel-res 1e-6: plain vs batched+learned"
 PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3}
 PEAK_HBM_GBS = 8000.0
-CYCLE = 5  # DBIM iterations of one inversion (the workload's 5)
+CYCLE = 5  # DBIM iterations of one inversion (the workload's 5; --cycle sets it)
 
 # total-field start (DbimOptions::field_start)
 START_KINDS = {"zero": 0, "previous": 1, "first-order": 2, "learned": 3}
@@ -116,12 +116,13 @@ def run_path(solver, dbim, warmup, steps, label, timing=True):
     the inversion, the state going back to the start before every
     CYCLE-th step (inside the timed region when it falls there)."""
     pos = [0]
+    cyc = CYCLE
 
     def step():
-        if pos[0] > 0 and pos[0] % CYCLE == 0:
+        if pos[0] > 0 and pos[0] % cyc == 0:
             dbim.restart()
         st = dbim.iterate()
-        st["iteration"] = pos[0] % CYCLE + 1
+        st["iteration"] = pos[0] % cyc + 1
         pos[0] += 1
         return st
 
@@ -291,6 +292,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--cycle", type=int, default=5,
+                    help="DBIM iterations of one inversion: step p is iteration p mod cycle + 1, the state going "
+                         "back to the start before every cycle-th step (10 with --steps 10 --warmup 0: one "
+                         "10-iteration inversion per path, the full-scale configuration's)")
     ap.add_argument("--workload", default="dbim-1024", choices=sorted(WORKLOADS))
     ap.add_argument("--paths", default="learned,fast,plain",
                     help="learned: the headline path; fast: the same with the first-order total-field start "
@@ -331,6 +336,11 @@ def main():
         print("bench.py measures one GPU: there is no multi-GPU path (sharding the transmitters "
               "over RCCL is designed but not built); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.cycle < 1:
+        print("--cycle: at least 1", file=sys.stderr)
+        sys.exit(2)
+    global CYCLE
+    CYCLE = args.cycle
     paths = [p for p in args.paths.split(",") if p]
     if not paths or not set(paths) <= {"fast", "plain", "learned"}:
         print("--paths: a comma list of learned, fast, plain", file=sys.stderr)
