@@ -69,7 +69,9 @@ typedef struct isg_config {
   int precision;   /* 0: FP64 operator (plain).  1: mixed -- FP32 operator tables, multipoles and
                       matrix-core arithmetic inside BiCGSTAB; FP64 vectors, dot products and
                       scalars; FP64 true-residual check with defect correction until the FP64
-                      relative residual meets toli.  isg_mlfma/aggregate(h) stay FP64. */
+                      relative residual meets toli.  isg_mlfma/aggregate(h) stay FP64.
+                      Refused (isg_create fails) where the translation series amplify FP32
+                      rounding beyond 1e-5: beta >= 7 with the default box and resolution. */
   double mixed_margin; /* mixed precision only: the FP32 solve runs to mixed_margin * toli on its
                           recursive residual before the FP64 check, so that the check rarely
                           needs a defect-correction round; 0 < mixed_margin <= 1 (default 0.5) */

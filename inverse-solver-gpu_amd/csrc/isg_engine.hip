@@ -270,8 +270,22 @@ Engine::Engine(const EngineConfig& cfg) : cfg_(cfg) {
     lvl_off_[i] = tot;
     tot += lvl_size_[i] * R;
   }
-  upload(op64_);
   has32_ = cfg_.precision == PREC_MIXED;
+  if (has32_) {
+    // The nearest far-field translation of level i (boxes two apart, x = 2 k0 a_i)
+    // runs to order P_i; once P_i exceeds x its terms grow like |Y_P(x)|, the
+    // locals carry components that cancel in L2P, and FP32 rounding (6e-8) is
+    // amplified by that factor (43 at the default beta 5, 2.1e4 at beta 9).
+    // The FP32 operator is refused where that exceeds its 1e-5 accuracy.
+    double amp = 0;
+    for (int i = 2; i < L; i++) amp = std::max(amp, std::fabs(yn(P_.nterm[i], 2 * P_.k0 * P_.csize[i])));
+    if (!(6e-8 * amp <= 1e-5)) {
+      set_error("precision 1: the FP32 operator would lose accuracy at this beta (translation terms up to " +
+                std::to_string(amp) + " amplify FP32 rounding beyond 1e-5); use precision 0 or beta <= 6");
+      return;
+    }
+  }
+  upload(op64_);
   if (has32_) upload(op32_);
   auto up_i = [&](const std::vector<int>& src) -> int* {
     int* p = (int*)dalloc(src.size() * sizeof(int));

@@ -47,6 +47,42 @@ def test_mlfma_level7_matches_oracle(isg):
         assert relerr(s.stage(1, i), o.stage(1, i)) < TOL, ("locals", i)
 
 
+@pytest.mark.parametrize("beta", [3, 9])
+def test_mlfma_beta_matches_oracle(isg, beta):
+    """The excess-bandwidth parameter changes every sample count: beta 3 gives
+    56 leaf samples, beta 9 gives 88 (more than the 80 rows whose P2M output
+    is staged through LDS, so the direct-store P2M runs; wider interpolation
+    windows).  4 right-hand sides; FP32 products within 1e-5 of the oracle
+    built with the same beta where the library admits the FP32 operator (beta
+    3); at beta 9 it is refused (its rounding would be amplified to ~1e-3 by
+    the terms below, measured 7e-4).  FP64 bar: the translation series of the nearest
+    far-field boxes (distance two leaf boxes, x = 2 k0 a) runs to order
+    P = S_leaf / 4, and once P exceeds x its terms grow like |Y_P(x)| (43 at
+    beta 5, 2.1e4 at beta 9); the locals then carry large components that
+    cancel in L2P, so two FP64 evaluations in different summation orders
+    agree only to ~eps |Y_P(x)|.  Bar max(1e-12, 2e-15 |Y_P(x)|) -- every
+    stage checkpoint still agrees to 1e-14."""
+    from scipy.special import yv
+    level = 5
+    print("mlfma parity, beta", beta, flush=True)
+    o = oracle_for(level, beta=beta)
+    X = rand_c(np.random.default_rng(40 + beta), (4, o.N))
+    ref = [o.mlfma(X[k]) for k in range(4)]
+    for prec in (0, 1):
+        if prec == 1 and beta == 9:
+            with pytest.raises(isg.IsgError, match="FP32 operator would lose accuracy"):
+                isg.Solver(level=level, numrx=16, max_rhs=4, precision=1, beta=beta)
+            continue
+        s = isg.Solver(level=level, numrx=16, max_rhs=4, precision=prec, beta=beta)
+        P = s.numsamp[level - 1] // 4
+        tol = max(TOL, 2e-15 * abs(yv(P, 2 * 2 * np.pi * 0.8))) if prec == 0 else 1e-5
+        R = s.mlfma_many(X)
+        errs = [relerr(R[k], ref[k]) for k in range(4)]
+        print("precision %d: max relative difference %.2e (bar %.1e)" % (prec, max(errs), tol), flush=True)
+        assert max(errs) < tol
+        s.close()
+
+
 @pytest.mark.parametrize("level,nrhs", [(5, 4), (6, 16)])
 def test_mlfma_batched_matches_oracle(isg, level, nrhs):
     """Batched passes (every launch shape that depends on the RHS count)."""
