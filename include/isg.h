@@ -58,7 +58,7 @@ typedef struct isg_config {
   int box;         /* pixels per leaf-box side; must be 8 */
   double res;      /* pixel size in wavelengths */
   double scale;    /* frequency scale: k0 = 2 pi scale */
-  int beta;        /* excess-bandwidth parameter of the sample-count rule */
+  int beta;        /* excess-bandwidth parameter of the sample-count rule, in [1, 32] */
   int ninter;      /* Lagrange interpolation taps */
   int numrx;       /* far-field receivers, equispaced on 360 degrees */
   int numtx;       /* plane-wave transmitters, equispaced on 360 degrees */

@@ -83,6 +83,7 @@ bool build_plan(const PlanConfig& cfg, Plan& P) {
   if (cfg.ninter < 2 || cfg.ninter % 2 || cfg.ninter > 64) { P.error = "ninter must be even in [2, 64]"; return false; }
   if (cfg.numrx < 1) { P.error = "numrx must be >= 1"; return false; }
   if (!(cfg.res > 0) || !(cfg.scale > 0)) { P.error = "res and scale must be positive"; return false; }
+  if (cfg.beta < 1 || cfg.beta > 32) { P.error = "beta must be in [1, 32]"; return false; }
 
   const int L = cfg.level, ni = cfg.ninter;
   P.L = L;

@@ -57,6 +57,9 @@ def test_bad_config_rejected():
         isg.Solver(level=4, precision=1, products=-1)
     with pytest.raises(isg.IsgError):
         isg.Solver(level=4, solve_start=2)
+    for beta in (0, -3, 33):  # a non-positive beta would make the sample-count rule NaN
+        with pytest.raises(isg.IsgError, match="beta must be"):
+            isg.Solver(level=4, beta=beta)
 
 
 class SplitMix64:
