@@ -83,6 +83,25 @@ def test_mlfma_beta_matches_oracle(isg, beta):
         s.close()
 
 
+@pytest.mark.parametrize("scale,res", [(1.5, 0.1), (1.0, 0.05)])
+def test_mlfma_frequency_and_pixel_size(isg, scale, res):
+    """Other frequency scales and pixel sizes (the wavenumber and the box size
+    in wavelengths enter every table and sample count): FP64 products within
+    1e-12 of the oracle built alike, FP32 within 1e-5, 2 right-hand sides."""
+    level = 5
+    print("mlfma parity, scale", scale, "res", res, flush=True)
+    o = oracle_for(level, scale=scale, res=res)
+    X = rand_c(np.random.default_rng(60), (2, o.N))
+    ref = [o.mlfma(X[k]) for k in range(2)]
+    for prec, tol in ((0, TOL), (1, 1e-5)):
+        s = isg.Solver(level=level, numrx=16, max_rhs=2, precision=prec, scale=scale, res=res)
+        R = s.mlfma_many(X)
+        errs = [relerr(R[k], ref[k]) for k in range(2)]
+        print("precision %d: max relative difference %.2e" % (prec, max(errs)), flush=True)
+        assert max(errs) < tol
+        s.close()
+
+
 @pytest.mark.parametrize("level,nrhs", [(5, 4), (6, 16)])
 def test_mlfma_batched_matches_oracle(isg, level, nrhs):
     """Batched passes (every launch shape that depends on the RHS count)."""
