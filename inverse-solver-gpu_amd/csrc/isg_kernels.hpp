@@ -338,7 +338,7 @@ __global__ void __launch_bounds__(kBlock) k_near_mfma(const cplx<TI>* __restrict
       for (int u = 0; u < U; u++) {
         const int e = e0 + u * kBlock;
         ju[u] = jb;
-        if (e < n) v[u] = lr[e];
+        v[u] = lr[e < n ? e : n - 1];  // unconditional (clamped) load: the array stays in registers
         kq += dk;
         jb += dj;
         if (kq >= S) {
@@ -582,7 +582,7 @@ __global__ void __launch_bounds__(kBlock) k_interp_band(const cplx<T>* __restric
   // divided out of e, and the loads of U strides are issued before the first
   // store.
   if constexpr (MODE == 0) {
-    constexpr int U = 8;
+    constexpr int U = sizeof(T) == 4 ? 8 : 4;
     const int n = NC * K, dcol = kBlock / K, dj = kBlock - dcol * K;
     int col = tid / K, j = tid - col * K;
     const cplx<T>* s = src + rhs * ss + 4 * p0 * K;
@@ -593,7 +593,7 @@ __global__ void __launch_bounds__(kBlock) k_interp_band(const cplx<T>* __restric
       for (int u = 0; u < U; u++) {
         const int e = e0 + u * kBlock;
         cu[u] = col;
-        if (e < n) a[u] = s[e];  // NC consecutive children of length K
+        a[u] = s[e < n ? e : n - 1];  // NC consecutive children of length K (clamped load)
         j += dj;
         col += dcol;
         if (j >= K) {
@@ -620,7 +620,7 @@ __global__ void __launch_bounds__(kBlock) k_interp_band(const cplx<T>* __restric
         const int e = e0 + u * kBlock;
         pu[u] = pp;
         ju[u] = j;
-        if (e < n) a[u] = s[e];  // NC/4 consecutive parents of length K
+        a[u] = s[e < n ? e : n - 1];  // NC/4 consecutive parents of length K (clamped load)
         j += dj;
         pp += dp;
         if (j >= K) {
@@ -988,10 +988,8 @@ __global__ void __launch_bounds__(kBlock) k_m2l_leaf(const cplx<T>* __restrict__
         const int e = e0 + u * kBlock;
         int j = s0 + jj;
         if (j >= Sp) j -= Sp;
-        if (e < 16 * KB)
-          v[u] = pr[(long)p * Sp + j];
-        else if (e < n)
-          v[u] = shift[(p - 16) * Sp + j];
+        // unconditional loads (past n: a clamped, unused shift element), so v stays in registers
+        v[u] = e < 16 * KB ? pr[(long)p * Sp + j] : shift[min(p - 16, 3) * Sp + j];
         jj += dq;
         p += dp;
         if (jj >= KB) {
