@@ -171,25 +171,31 @@ __global__ void __launch_bounds__(kNB) __attribute__((amdgpu_waves_per_eu(2))) k
 // Weight-gradient partial sums of one chunk of pixel tiles:
 //   part[chunk][tap][co][ci] = sum_p bf16(dY[p][co] [M>0]) bf16(In[p+tap][ci])
 //   bpart[chunk][co]         = sum_p bf16(dY[p][co] [M>0])       (ci tile 0 only)
-// with In = X (+ up2(U)).  Threads: 16 output x 16 input channels.
+// with In = X (+ up2(U)).  One workgroup: 16 output x 16 input channels; per
+// 8x8 pixel tile the products are v_mfma_f32_16x16x32_bf16 over the tile's
+// positions (two k-steps of 32 = four tile rows each): A = dY^T (rows co,
+// lane l supplies channel l % 16 and the 8 positions of tile row
+// 4 c + l / 16), B = the input shifted by the tap (columns ci, the same
+// positions); wave w owns taps w, w + 4, w + 8.  The tile's dY and input halo
+// are staged in LDS as BF16, channel-major.  Fixed order over the chunk's
+// tiles: deterministic.
 template <int KS>
 __global__ void __launch_bounds__(kNB) k_wgrad(const float* __restrict__ dY, const float* __restrict__ M,
                                                const float* __restrict__ X, const float* __restrict__ U, int cin,
                                                int cout, int H, int W, int batch, int per_chunk,
                                                float* __restrict__ part, float* __restrict__ bpart) {
-  constexpr int R = KS / 2, HW = 8 + KS - 1, HS = HW * HW;
-  __shared__ float dsm[64][17];
-  __shared__ float xsm[HS][17];
-  const int tid = threadIdx.x;
-  const int tco = tid >> 4, tci = tid & 15;
+  constexpr int R = KS / 2, HW = 8 + KS - 1, HS = HW * HW, NT = KS * KS, TPW = (NT + 3) / 4;
+  __shared__ __attribute__((aligned(16))) uint16_t dsm[16][64];  // dY[co][p], BF16
+  __shared__ uint16_t xsm[16][HS];                              // In[ci][halo position], BF16
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, li = lane & 15, lk = lane >> 4;
   const int ncit = cin >> 4;
   const int co0 = (blockIdx.x / ncit) * 16, ci0 = (blockIdx.x % ncit) * 16;
   const int chunk = blockIdx.y;
   const int tiles_x = (W + 7) >> 3, tiles = ((H + 7) >> 3) * tiles_x;
   const long total = (long)tiles * batch;
-  float acc[KS * KS];
+  v4f acc[TPW];
 #pragma unroll
-  for (int t = 0; t < KS * KS; t++) acc[t] = 0.f;
+  for (int q = 0; q < TPW; q++) acc[q] = v4f{0, 0, 0, 0};
   float bacc = 0.f;
   for (long tt = (long)chunk * per_chunk; tt < total && tt < (long)(chunk + 1) * per_chunk; tt++) {
     const int n = (int)(tt / tiles), tl = (int)(tt % tiles);
@@ -204,7 +210,7 @@ __global__ void __launch_bounds__(kNB) k_wgrad(const float* __restrict__ dY, con
         v = dY[pix * cout + co0 + c];
         if (M && !(M[pix * cout + co0 + c] > 0.f)) v = 0.f;
       }
-      dsm[p][c] = bf2f(f2bf(v));
+      dsm[c][p] = f2bf(v);
     }
     for (int e = tid; e < HS * 16; e += kNB) {
       const int hp = e >> 4, c = e & 15;
@@ -214,22 +220,41 @@ __global__ void __launch_bounds__(kNB) k_wgrad(const float* __restrict__ dY, con
         v = X[(img + (long)gy * W + gx) * cin + ci0 + c];
         if (U) v += U[((long)n * (H >> 1) * (W >> 1) + (long)(gy >> 1) * (W >> 1) + (gx >> 1)) * cin + ci0 + c];
       }
-      xsm[hp][c] = bf2f(f2bf(v));
+      xsm[c][hp] = f2bf(v);
     }
     __syncthreads();
-    for (int p = 0; p < 64; p++) {
-      const float d = dsm[p][tco];
-      const int py = p >> 3, px = p & 7;
-      bacc += d;
+    if (tid < 16)
+      for (int p = 0; p < 64; p++) bacc += bf2f(dsm[tid][p]);
 #pragma unroll
-      for (int t = 0; t < KS * KS; t++) acc[t] += d * xsm[(py + t / KS) * HW + px + t % KS][tci];
+    for (int q = 0; q < TPW; q++) {
+      const int t = w + 4 * q;
+      if (t < NT) {
+        const int dy = t / KS, dx = t % KS;
+#pragma unroll
+        for (int c = 0; c < 2; c++) {
+          const int row = 4 * c + lk;  // tile row of this lane's 8 positions
+          const v8bf a = __builtin_bit_cast(v8bf, *reinterpret_cast<const v8s*>(&dsm[li][8 * row]));
+          const uint16_t* xr = &xsm[li][(row + dy) * HW + dx];
+          v8s bs;
+#pragma unroll
+          for (int j = 0; j < 8; j++) bs[j] = (short)xr[j];
+          acc[q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, __builtin_bit_cast(v8bf, bs), acc[q], 0, 0, 0);
+        }
+      }
     }
     __syncthreads();
   }
-  const long base = (long)chunk * KS * KS * cout * cin;
+  const long base = (long)chunk * NT * cout * cin;
+  // D[co][ci]: lane l holds rows co0 + 4 (l / 16) + i, column ci0 + l % 16
 #pragma unroll
-  for (int t = 0; t < KS * KS; t++) part[base + ((long)t * cout + co0 + tco) * cin + ci0 + tci] = acc[t];
-  if (ci0 == 0 && tci == 0) bpart[(long)chunk * cout + co0 + tco] = bacc;
+  for (int q = 0; q < TPW; q++) {
+    const int t = w + 4 * q;
+    if (t < NT) {
+#pragma unroll
+      for (int i = 0; i < 4; i++) part[base + ((long)t * cout + co0 + 4 * lk + i) * cin + ci0 + li] = acc[q][i];
+    }
+  }
+  if (ci0 == 0 && tid < 16) bpart[(long)chunk * cout + co0 + tid] = bacc;
 }
 
 // out[i] = sum_c part[c][i], fixed order (deterministic)
