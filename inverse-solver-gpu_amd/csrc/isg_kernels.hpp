@@ -923,6 +923,30 @@ __device__ __forceinline__ void par_translate(const cplx<T> (*tr)[kParKC], int k
     }
 }
 
+// The same sums source by source: each of the 36 sources is read once and
+// added into every child it is a far-field source of, with that offset's
+// operator (read from LDS per use): registers for 4 accumulators instead of
+// the 36-source neighbourhood.
+template <typename T>
+__device__ __forceinline__ void par_translate_stream(const cplx<T> (*src)[kParKC], const cplx<T> (*tr)[kParKC],
+                                                     int lx0, int ly0, int kk, cplx<T> (&acc)[4]) {
+#pragma unroll
+  for (int c = 0; c < 4; c++) acc[c] = mkc<T>(0, 0);
+#pragma unroll
+  for (int sy = -2; sy <= 3; sy++)
+#pragma unroll
+    for (int sx = -2; sx <= 3; sx++) {
+      const cplx<T> v = src[(ly0 + sy + 2) * kParSB + lx0 + sx + 2][kk];
+#pragma unroll
+      for (int c = 0; c < 4; c++) {
+        const int ex = sx - (c & 1), ey = sy - (c >> 1);  // offset of the source from child c
+        if (ex < -3 || ex > 3 || ey < -3 || ey > 3) continue;
+        if (ex >= -1 && ex <= 1 && ey >= -1 && ey <= 1) continue;  // near offsets
+        acc[c] = cfma(tr[7 * (ey + 3) + ex + 3][kk], v, acc[c]);
+      }
+    }
+}
+
 // (at least 2 waves per SIMD: the FP64 instantiation otherwise takes 256+ registers
 // and runs one wave per SIMD; measured 2.00 -> 1.52 ms per FP64 launch set)
 template <typename T>
@@ -938,9 +962,14 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))
   __syncthreads();
   const int kk = threadIdx.x % kParKC, pp = threadIdx.x / kParKC;
   const int lx0 = 2 * (pp & 3), ly0 = 2 * (pp >> 2);  // first child of the parent, tile coordinates
-  cplx<T> sv[6][6], acc[4];
-  par_load<T>(src, lx0, ly0, kk, sv);
-  par_translate<T>(tr, kk, sv, acc);
+  cplx<T> acc[4];
+  if constexpr (sizeof(T) == 8) {
+    par_translate_stream<T>(src, tr, lx0, ly0, kk, acc);
+  } else {
+    cplx<T> sv[6][6];
+    par_load<T>(src, lx0, ly0, kk, sv);
+    par_translate<T>(tr, kk, sv, acc);
+  }
   if (B.k0 + kk < S) {
     cplx<T>* out = agl + rhs * st + (long)(B.m0 + 4 * morton2((unsigned)(pp & 3), (unsigned)(pp >> 2))) * S;
 #pragma unroll
