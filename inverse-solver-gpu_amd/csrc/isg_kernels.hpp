@@ -993,6 +993,9 @@ __global__ void __launch_bounds__(kBlock) k_m2l_leaf(const cplx<T>* __restrict__
                                                      const cplx<T>* __restrict__ par, long ps, int Sp,
                                                      const T* __restrict__ AB, const int* __restrict__ s0v, int KB,
                                                      const cplx<T>* __restrict__ shift, const int* __restrict__ act) {
+  // FP64: the translation streams its sources (par_translate_stream); the
+  // neighbourhood in registers would take 256 registers, one wave per SIMD
+  constexpr bool STREAM_LEAF = sizeof(T) == 8;
   typedef typename Mfma<T>::v4 v4;
   __shared__ cplx<T> src[kParSB * kParSB][kParKC];
   __shared__ cplx<T> tr[49][kParKC];
@@ -1068,9 +1071,14 @@ __global__ void __launch_bounds__(kBlock) k_m2l_leaf(const cplx<T>* __restrict__
 #pragma unroll
   for (int q = 0; q < 4; q++) ll[(16 * w + Mfma<T>::row(lk, q)) * kParKC + li] = mkc<T>(cr[q], ci[q]);
   __syncthreads();
-  cplx<T> sv[6][6], acc[4];
-  par_load<T>(src, 2 * (pp & 3), 2 * (pp >> 2), kk, sv);
-  par_translate<T>(tr, kk, sv, acc);
+  cplx<T> acc[4];
+  if constexpr (STREAM_LEAF) {
+    par_translate_stream<T>(src, tr, 2 * (pp & 3), 2 * (pp >> 2), kk, acc);
+  } else {
+    cplx<T> sv[6][6];
+    par_load<T>(src, 2 * (pp & 3), 2 * (pp >> 2), kk, sv);
+    par_translate<T>(tr, kk, sv, acc);
+  }
   const unsigned pm = morton2((unsigned)(pp & 3), (unsigned)(pp >> 2));
   if (B.k0 + kk < S) {
     cplx<T>* out = agl + rhs * st + (long)(B.m0 + 4 * pm) * S;
