@@ -1005,6 +1005,17 @@ __global__ void __launch_bounds__(kBlock) k_m2l_leaf(const cplx<T>* __restrict__
   const int rhs = blockIdx.y;
   if (rhs_off(act, rhs)) return;
   const ParBlock B(S);
+  // the band rows of the window (the interpolation's A operand; no LDS
+  // needed), issued before the staging so their latency overlaps it
+  const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
+  constexpr int KQ = 24;  // windows of up to 96 samples (the L2L band limit)
+  const int nkq = KB >> 2;
+  T a[KQ];
+  {
+    const T* ab = AB + (long)B.chunk * KB * 16;
+#pragma unroll
+    for (int q = 0; q < KQ; q++) a[q] = (q < nkq) ? ab[(4 * q + lk) * 16 + li] : (T)0;
+  }
   m2l_stage<T, kParTS, kParKC>(src, tr, agm + rhs * st, trans, B.X0, B.Y0, B.k0, 1 << lev, S);
   const int s0 = s0v[B.chunk];
   const cplx<T>* pr = par + rhs * ps + (long)(B.m0 >> 2) * Sp;  // the tile's 16 parents, Morton order
@@ -1046,16 +1057,9 @@ __global__ void __launch_bounds__(kBlock) k_m2l_leaf(const cplx<T>* __restrict__
   // wave w owns columns 16w .. 16w+15 (children 16w + i, i.e.
   // parent 4w + i/4, child position i%4); lane (li, lk) feeds column li at
   // window sample 4q + lk and receives rows Mfma::row(lk, q) at sample li
-  const int lane = tid & 63, w = tid >> 6, li = lane & 15, lk = lane >> 4;
-  const T* ab = AB + (long)B.chunk * KB * 16;
+  const int w = tid >> 6;
   const cplx<T>* pcol = pw + (4 * w + (li >> 2)) * KB;
   const cplx<T>* scol = sw + (li & 3) * KB;
-  // the band rows of the window, loaded together ahead of the MFMA chain
-  constexpr int KQ = 24;  // windows of up to 96 samples (the L2L band limit)
-  const int nkq = KB >> 2;
-  T a[KQ];
-#pragma unroll
-  for (int q = 0; q < KQ; q++) a[q] = (q < nkq) ? ab[(4 * q + lk) * 16 + li] : (T)0;
   v4 cr = v4{0, 0, 0, 0}, ci = v4{0, 0, 0, 0};
 #pragma unroll
   for (int q = 0; q < KQ; q++) {
