@@ -1156,10 +1156,40 @@ __global__ void __launch_bounds__(kBlock) k_p2m_mfma(const cplx<TI>* __restrict_
     for (int q = 0; q < 4; q++) o[q] = mkc<T>(p1[q] - p2[q], p3[q] - p1[q] - p2[q]);
   };
   if (staged) {
+    // the wave's units as one software pipeline over (unit, k-batch) steps:
+    // the coefficients of the next step are loaded while the current step's
+    // MFMAs run (coeffT comes from L2; one step waiting for its own loads per
+    // batch was the kernel's critical path)
+    constexpr int PRE = sizeof(T) == 4 ? 8 : 4;
+    constexpr int NBAT = 16 / PRE;  // k-batches per unit
+    cplx<T> ab[2][PRE];
+    auto load_step = [&](int c, int b, cplx<T>(&a)[PRE]) {
+      const int u = w + 4 * c, krow = 16 * (u / NN) + li;
+#pragma unroll
+      for (int v = 0; v < PRE; v++)
+        a[v] = (u < nunit && krow < S) ? coeffT[(4 * (b * PRE + v) + lk) * S + krow] : mkc<T>(0, 0);
+    };
+    load_step(0, 0, ab[0]);
 #pragma unroll
     for (int c = 0; c < MAXU; c++) {
       const int u = w + 4 * c;
-      if (u < nunit) unit(u, res[c]);
+      const int j = u - (u / NN) * NN;
+      const cplx<T>* trow = tile + (16 * j + li) * 65;
+      v4 p1 = v4{0, 0, 0, 0}, p2 = v4{0, 0, 0, 0}, p3 = v4{0, 0, 0, 0};
+#pragma unroll
+      for (int b = 0; b < NBAT; b++) {
+        const int i = c * NBAT + b;
+        if (i + 1 < MAXU * NBAT) load_step((i + 1) / NBAT, (i + 1) % NBAT, ab[(i + 1) & 1]);
+        if (u < nunit) {
+#pragma unroll
+          for (int v = 0; v < PRE; v++) {
+            const cplx<T> a = ab[i & 1][v];
+            Mfma<T>::cmac3(trow[4 * (b * PRE + v) + lk], a, a.x + a.y, p1, p2, p3);
+          }
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < 4; q++) res[c][q] = mkc<T>(p1[q] - p2[q], p3[q] - p1[q] - p2[q]);
     }
     __syncthreads();  // the input tile is no longer read: it takes the output
 #pragma unroll
