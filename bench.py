@@ -206,7 +206,8 @@ def run_paths(isg, args, wl, common, otrue, mes, paths, warmup, steps, timing):
             try:
                 d = isg.Dbim(s, otrue, field_start=START_KINDS[start], measurements=mes)
                 if start == "learned":
-                    d.set_warmnet(isg.WarmNet(max_batch=16, max_pixels=side * side, path=WARMNET_FILE))
+                    d.set_warmnet(isg.WarmNet(max_batch=args.net_batch, max_pixels=side * side,
+                                              path=WARMNET_FILE))
             except (isg.IsgError, OSError) as e:
                 errors[p] = str(e)
                 log("[%s] skipped: %s" % (p, e))
@@ -292,6 +293,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--net-batch", type=int, default=16,
+                    help="transmitters per warm-start network launch (its workspace grows with it)")
     ap.add_argument("--cycle", type=int, default=5,
                     help="DBIM iterations of one inversion: step p is iteration p mod cycle + 1, the state going "
                          "back to the start before every cycle-th step (10 with --steps 10 --warmup 0: one "
