@@ -1,5 +1,6 @@
 """bench.py's argument handling (CPU): a request for more than one GPU and an
-unknown path are refused with exit status 2 before any device work."""
+unknown path or a non-positive --cycle are refused with exit status 2
+before any device work."""
 import os
 import subprocess
 import sys
@@ -24,3 +25,10 @@ def test_unknown_path_refused():
         p = run("--paths", paths)
         assert p.returncode == 2, paths
         assert "--paths" in p.stderr
+
+
+def test_bad_cycle_refused():
+    for cycle in ("0", "-2"):
+        p = run("--cycle", cycle)
+        assert p.returncode == 2, cycle
+        assert "--cycle" in p.stderr
