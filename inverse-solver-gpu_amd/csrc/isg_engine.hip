@@ -14,6 +14,7 @@
 #include <rocblas/rocblas.h>
 
 #include "isg_kernels.hpp"
+#include "isg_near_h16.hpp"
 #include "isg_setup_kernels.hpp"
 #include "isg_solver_kernels.hpp"
 
@@ -286,7 +287,10 @@ Engine::Engine(const EngineConfig& cfg) : cfg_(cfg) {
     }
   }
   upload(op64_);
-  if (has32_) upload(op32_);
+  if (has32_) {
+    upload(op32_);
+    build_split16();
+  }
   auto up_i = [&](const std::vector<int>& src) -> int* {
     int* p = (int*)dalloc(src.size() * sizeof(int));
     if (p) ISG_HIP(hipMemcpy(p, src.data(), src.size() * sizeof(int), hipMemcpyHostToDevice));
@@ -463,6 +467,32 @@ Engine::~Engine() {
   if (tol_h_) (void)hipHostFree(tol_h_);
   if (part_h_) (void)hipHostFree(part_h_);
   if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+// Split-FP16 tables of k_near_h16 from the FP32 operator's near-field and
+// L2P tables, each with the power-of-two scale that puts its largest
+// component just below 2^14.
+void Engine::build_split16() {
+  if (!ok()) return;
+  const int S = P_.nsamp[P_.L - 1], nlc = (S + 31) / 32;
+  std::vector<zf> nf((size_t)9 * 4096), bf((size_t)S * 64);
+  ISG_HIP(hipMemcpy(nf.data(), op32_.near, nf.size() * sizeof(zf), hipMemcpyDeviceToHost));
+  ISG_HIP(hipMemcpy(bf.data(), op32_.basisT, bf.size() * sizeof(zf), hipMemcpyDeviceToHost));
+  if (!ok()) return;
+  auto absmax = [](const std::vector<zf>& v) {
+    float m = 0;
+    for (const zf& c : v) m = std::max(m, std::max(std::fabs(c.x), std::fabs(c.y)));
+    return m;
+  };
+  near16_et_ = h16_scale_exp(absmax(nf));
+  bas16_eb_ = h16_scale_exp(absmax(bf));
+  near16_ = dalloc((size_t)18 * 1024 * 16);
+  bas16_ = dalloc((size_t)nlc * 1024 * 16);
+  if (!near16_ || !bas16_) return;
+  k_split16<<<(18 * 256 + 255) / 256, 256, 0, stream_>>>(op32_.near, 2, 4096, 64, near16_et_, 18, (v8h*)near16_);
+  k_split16<<<(nlc * 256 + 255) / 256, 256, 0, stream_>>>(op32_.basisT, nlc, 0, S, bas16_eb_, nlc, (v8h*)bas16_);
+  ISG_HIP(hipGetLastError());
+  ISG_HIP(hipStreamSynchronize(stream_));
 }
 
 // ------------------------------------------------------- interpolation plan
@@ -742,14 +772,38 @@ int Engine::launch_near(const cplx<TI>* y, cplx<TI>* r, int nrhs, const int* act
   const long N = P_.N, ls = (long)lvl_size_[Lf];
   const int tuned = tune(sizeof(T) == 4 ? PREC_MIXED : PREC_FP64, nrhs).near_nb;
   int nb = force > 0 ? force : (tuned > 0 ? tuned : (B >= 32 ? 32 : 16));
-  if (nb == 64 && sizeof(T) != 4) nb = 32;  // 64-box tiles: FP32 only (LDS)
-  if (B % nb != 0) nb = 16;
   DevOp<T>& d = ops<T>();
   const cplx<T>* loc = d.agl + lvl_off_[Lf];
+  const NearPost<TI> pp = post ? *post : NearPost<TI>();
+  const int kt = sizeof(T) == 4 ? KT_NEAR32 : KT_NEAR;
+  if (nb == kNearSplit) {
+    // split-FP16 kernel: FP32 operator on FP32 vectors, near field included,
+    // 64-box tiles, BiCGSTAB post steps (not the FP64 residual check)
+    if constexpr (sizeof(T) == 4 && sizeof(TI) == 4) {
+      if (with_near && B % 64 == 0 && near16_ && bas16_ && pp.mode != 3) {
+        dim3 grid(B / 64, (nrhs + 1) / 2), blk(kBlock);
+        auto go = [&](auto modec) {
+          k_near_h16<decltype(modec)::value><<<grid, blk, kH16Lds, stream_>>>(
+              y, N, loc, ls, r, N, (const v8h*)near16_, near16_et_, (const v8h*)bas16_, bas16_eb_, S, 1 << Lf, nrhs,
+              act, pp);
+        };
+        prof_begin(kt);
+        if (pp.mode == 1)
+          go(std::integral_constant<int, 1>());
+        else if (pp.mode == 2)
+          go(std::integral_constant<int, 2>());
+        else
+          go(std::integral_constant<int, 0>());
+        prof_end(kt);
+        return B / 64;
+      }
+    }
+    nb = B >= 32 ? 32 : 16;
+  }
+  if (nb == 64 && sizeof(T) != 4) nb = 32;  // 64-box tiles: FP32 only (LDS)
+  if (B % nb != 0) nb = 16;
   size_t lds = (size_t)near_lds_elems(nb, S) * sizeof(cplx<T>);
   dim3 grid(B / nb, nrhs), blk(kBlock);
-  const int kt = sizeof(T) == 4 ? KT_NEAR32 : KT_NEAR;
-  const NearPost<TI> pp = post ? *post : NearPost<TI>();
   if (pp.mode == 3 && !(sizeof(T) == 8 && sizeof(TI) == 8 && nb <= 32)) {
     set_error("near field: the residual-check epilogue is FP64 only");
     return B / nb;
@@ -1342,9 +1396,16 @@ void Engine::autotune_prec(int nrhs, int reps) {
     }
     return best;
   };
-  const int widths[3] = {16, 32, 64};
-  auto width_ok = [&](int c) { return P_.B % widths[c] == 0 && (widths[c] < 64 || sizeof(T) == 4); };
-  int c = pick(3, width_ok, [&](int c) { launch_near<T, double>(y, op, nrhs, nullptr, true, widths[c]); });
+  // near field: tile widths, and for the FP32 operator the split-FP16 kernel,
+  // timed on vectors of the operator's own type (those of the solver's products)
+  const int widths[4] = {16, 32, 64, kNearSplit};
+  auto width_ok = [&](int c) {
+    if (c == 3) return sizeof(T) == 4 && P_.B % 64 == 0 && near16_ != nullptr;
+    return P_.B % widths[c] == 0 && (widths[c] < 64 || sizeof(T) == 4);
+  };
+  int c = pick(4, width_ok, [&](int c) {
+    launch_near<T, T>((const cplx<T>*)y, (cplx<T>*)op, nrhs, nullptr, true, widths[c]);
+  });
   tu.near_nb = c < 0 ? -1 : widths[c];
   c = pick(3, width_ok, [&](int c) { launch_p2m<T, double>(y, nrhs, nullptr, widths[c]); });
   tu.p2m_nb = c < 0 ? -1 : widths[c];
@@ -1395,7 +1456,8 @@ bool Engine::tuning_set(int prec, int cls, int stage, int level, int value) {
   if ((prec != PREC_FP64 && prec != PREC_MIXED) || cls < 0 || cls > 1 || stage < 0 || stage >= ST_COUNT) return false;
   Tuning& t = tune_[prec == PREC_MIXED ? 1 : 0][cls];
   if (stage == ST_NEAR || stage == ST_P2M) {
-    if (value != -1 && value != 16 && value != 32 && value != 64) return false;
+    const bool split = stage == ST_NEAR && prec == PREC_MIXED && value == kNearSplit;
+    if (value != -1 && value != 16 && value != 32 && value != 64 && !split) return false;
     (stage == ST_NEAR ? t.near_nb : t.p2m_nb) = value;
     return true;
   }

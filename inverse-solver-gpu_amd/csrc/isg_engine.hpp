@@ -194,6 +194,8 @@ class Engine {
   void upload(DevOp<T>& d);
   template <typename T>
   void plan_interp(DevOp<T>& d, std::vector<InterpShape>& m2m, std::vector<InterpShape>& l2l);
+  // split-FP16 near-field and L2P tables of the FP32 operator (k_near_h16)
+  void build_split16();
   // operator with tables / multipoles of type T on vectors of type TI
   template <typename T, typename TI>
   void apply(const cplx<TI>* y, cplx<TI>* r, int nrhs, const int* act, const NearPost<TI>* post = nullptr);
@@ -264,6 +266,11 @@ class Engine {
   bool checked_h_ = false;
   DevOp<double> op64_;
   DevOp<float> op32_;
+  // k_near_h16 tables ([9 x 2][4][4][64] and [ceil(S / 32)][4][4][64] groups
+  // of 8 FP16) and their power-of-two scale exponents
+  void* near16_ = nullptr;
+  void* bas16_ = nullptr;
+  int near16_et_ = 0, bas16_eb_ = 0;
   std::vector<InterpShape> m2m64_, l2l64_, m2m32_, l2l32_;  // per parent level
   std::vector<int*> interp0_, anterp0_;
   int* rx0_ = nullptr;

@@ -1,0 +1,462 @@
+// isg_near_h16.hpp -- near field + L2P of the FP32 operator on gfx950's FP16
+// matrix cores with split operands (v_mfma_f32_16x16x32_f16).
+//
+// Every FP32 operand x is carried as two FP16 numbers, hi = rne16(2^e x) and
+// lo = rne16(2^e x - hi), with a power-of-two scale 2^e chosen so that
+// |2^e x| < 2^14 -- per table, and per workgroup and right-hand side for the
+// data.  hi + lo holds x to 2^-22 relative (elements below 2^-18 of the
+// largest one to 2^-40 of the largest, absolutely).  A product x a is formed
+// as hi.hi + hi.lo + lo.hi in FP32 accumulators (the dropped lo.lo term is
+// below 2^-22 relative): three FP16 MFMAs, each 16 times the FP32 rate, for
+// one FP32 MFMA's worth of products.  Complex products take the four-product
+// form (cr += dr ar - di ai, ci += dr ai + di ar): 12 MFMAs per 16 x 16 x 32
+// block.  The scales are exact powers of two and are divided out of the FP32
+// accumulators at the end.
+//
+// Workgroup: 8 x 8 Morton-consecutive leaf boxes (64) of two right-hand sides
+// = 128 MFMA rows, so each table fragment read from L2 feeds 8 row blocks
+// (the table, 12 B per complex entry and part pair, is what limits a split
+// kernel; the FP32 kernel reads 8 B per entry for 16 or 32 rows).  Wave w
+// owns right-hand side w / 2 of the pair (its 4 row blocks of 16 boxes) and
+// output pixels 32 (w % 2) .. +31 (2 column blocks): every A fragment read
+// from LDS feeds 2 column blocks, every B fragment 4 row blocks.
+//
+// Reference counterpart: the near-field and L2P stages of matvec_gpu.cu
+// (kernels.cu:32-81, :292-319); the result equals the FP32 kernel's to the
+// split's 2^-22.
+#pragma once
+
+#include "isg_kernels.hpp"
+
+namespace isg {
+
+typedef _Float16 v8h __attribute__((ext_vector_type(8)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef unsigned v4u __attribute__((ext_vector_type(4)));
+
+// Tuning value of the near stage that selects this kernel (FP32 operator).
+constexpr int kNearSplit = 128;
+// Halo of the 8 x 8 box tile: 10 x 10 boxes at a row pitch of 12 positions,
+// so the 16 boxes of a row block (a 4 x 4 block) read by 16 lanes fall in 16
+// different 16-byte bank groups.
+constexpr int kH16Pitch = 12;
+constexpr int kH16Pos = kH16Pitch * 10;
+// LDS (bytes): half of the halo (32 of the 64 pixels of each box) as
+// [rhs 2][part 4][k group 4][position 120] x 8 FP16; reused for a 32-local
+// chunk of the L2P ([rhs][part][k group][box 64] x 8 FP16) and for one
+// right-hand side's 64 x 64 complex output tile.
+constexpr size_t kH16Lds = (size_t)2 * 4 * 4 * kH16Pos * 16;
+static_assert(kH16Lds >= (size_t)64 * 64 * 8 && kH16Lds >= (size_t)2 * 4 * 4 * 64 * 16, "LDS reuse");
+
+// Power-of-two exponent e with |2^e m| < 2^14 for the largest magnitude m
+// (0 for m = 0 or m not finite: Inf / NaN then propagate as in FP32).
+__host__ __device__ inline int h16_scale_exp(float m) {
+  if (!(m > 0.f) || !(m <= 3.0e38f)) return 0;
+  int e;
+  (void)frexpf(m, &e);  // m = f 2^e, f in [0.5, 1)
+  e = 14 - e;
+  return e > 126 ? 126 : (e < -126 ? -126 : e);
+}
+
+// element e of hi and lo: the split of v
+__device__ __forceinline__ void h16_split(float v, v8h& hi, v8h& lo, int e) {
+  const _Float16 h = (_Float16)v;
+  hi[e] = h;
+  lo[e] = (_Float16)(v - (float)h);  // exact difference: at most 13 significant bits
+}
+
+// Four complex values (two float4) scaled by s and split into the four parts
+// (re hi, re lo, im hi, im lo), elements e0 .. e0 + 3 of each part.
+__device__ __forceinline__ void h16_split4(float4 a, float4 b, float s, v8h (&ph)[4], int e0) {
+  h16_split(a.x * s, ph[0], ph[1], e0);
+  h16_split(a.y * s, ph[2], ph[3], e0);
+  h16_split(a.z * s, ph[0], ph[1], e0 + 1);
+  h16_split(a.w * s, ph[2], ph[3], e0 + 1);
+  h16_split(b.x * s, ph[0], ph[1], e0 + 2);
+  h16_split(b.y * s, ph[2], ph[3], e0 + 2);
+  h16_split(b.z * s, ph[0], ph[1], e0 + 3);
+  h16_split(b.w * s, ph[2], ph[3], e0 + 3);
+}
+
+__device__ __forceinline__ float h16_absmax(float4 v) {
+  return fmaxf(fmaxf(fabsf(v.x), fabsf(v.y)), fmaxf(fabsf(v.z), fabsf(v.w)));
+}
+
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+// Workgroup maxima of two values; every thread gets both.
+__device__ __forceinline__ void block_max2(float (&m)[2], float (*red)[kBlock / 64]) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  m[0] = wave_max(m[0]);
+  m[1] = wave_max(m[1]);
+  __syncthreads();  // red may still be read by an earlier call
+  if (lane == 0) {
+    red[0][w] = m[0];
+    red[1][w] = m[1];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < 2; q++) {
+    float v = red[q][0];
+#pragma unroll
+    for (int i = 1; i < kBlock / 64; i++) v = fmaxf(v, red[q][i]);
+    m[q] = v;
+  }
+}
+
+// a[q] for a runtime q without indexing a private array (which would live in scratch)
+template <typename T>
+__device__ __forceinline__ T sel2(const T (&a)[2], int q) {
+  return q ? a[1] : a[0];
+}
+
+__device__ __forceinline__ v8h h16_neg(v8h v) {
+  return __builtin_bit_cast(v8h, __builtin_bit_cast(v4u, v) ^ 0x80008000u);
+}
+
+// cr + i ci += (a0 + a1 + i (a2 + a3)) (b0 + b1 + i (b2 + b3)) without the
+// lo.lo terms; a = data parts (re hi, re lo, im hi, im lo), b = table parts,
+// nb = -b2, -b3.  The two accumulators alternate so that consecutive MFMAs
+// are independent.
+__device__ __forceinline__ void h16_cmac(const v8h (&a)[4], const v8h (&b)[4], const v8h (&nb)[2], v4f& cr,
+                                         v4f& ci) {
+  cr = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[0], b[0], cr, 0, 0, 0);
+  ci = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[0], b[2], ci, 0, 0, 0);
+  cr = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[0], b[1], cr, 0, 0, 0);
+  ci = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[0], b[3], ci, 0, 0, 0);
+  cr = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[1], b[0], cr, 0, 0, 0);
+  ci = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[1], b[2], ci, 0, 0, 0);
+  cr = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[2], nb[0], cr, 0, 0, 0);
+  ci = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[2], b[0], ci, 0, 0, 0);
+  cr = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[2], nb[1], cr, 0, 0, 0);
+  ci = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[2], b[1], ci, 0, 0, 0);
+  cr = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[3], nb[0], cr, 0, 0, 0);
+  ci = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[3], b[0], ci, 0, 0, 0);
+}
+
+// Split-FP16 table: out[b][part][g][m] = 8 FP16 parts of 2^ex src[(b / per)
+// bstride + k 64 + m], k = 32 (b % per) + 8 g + e, e = 0..7 (zero for
+// k >= kmax); one thread per (b, g, m).
+static __global__ void __launch_bounds__(256) k_split16(const zf* __restrict__ src, int per, long bstride, int kmax,
+                                                        int ex, int nblk, v8h* __restrict__ out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nblk * 256) return;
+  const int b = t >> 8, g = (t >> 6) & 3, m = t & 63;
+  const float s = ldexpf(1.f, ex);
+  v8h ph[4];
+#pragma unroll
+  for (int e = 0; e < 8; e++) {
+    const int k = 32 * (b % per) + 8 * g + e;
+    const zf v = k < kmax ? src[(long)(b / per) * bstride + (long)k * 64 + m] : mkc<float>(0.f, 0.f);
+    h16_split(v.x * s, ph[0], ph[1], e);
+    h16_split(v.y * s, ph[2], ph[3], e);
+  }
+#pragma unroll
+  for (int P = 0; P < 4; P++) out[((long)(b * 4 + P) * 4 + g) * 64 + m] = ph[P];
+}
+
+// Split tables, built once per operator (k_split16):
+//   tab[s][kc][part][g][m] (8 FP16 each): 2^et near[s][32 kc + 8 g + e][m], e = 0..7
+//   bas[kc][part][g][m]    (8 FP16 each): 2^eb basisT[32 kc + 8 g + e][m] (0 for k >= S)
+// Grid (B / 64, ceil(nrhs / 2)), kBlock threads, kH16Lds bytes of dynamic LDS.
+template <int POST>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2)))
+k_near_h16(const zf* __restrict__ y, long ys, const zf* __restrict__ loc, long ls, zf* __restrict__ r, long rs,
+           const v8h* __restrict__ tab, int et, const v8h* __restrict__ bas, int eb, int S, int nbs, int nrhs,
+           const int* __restrict__ act, NearPost<float> post) {
+  extern __shared__ __attribute__((aligned(16))) char smem_raw[];
+  v8h* hal = reinterpret_cast<v8h*>(smem_raw);
+  __shared__ int hoff[100];
+  __shared__ float red[2][kBlock / 64];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  const int rw = w >> 1, mb0 = 2 * (w & 1);
+  const int rhs[2] = {2 * (int)blockIdx.y, 2 * (int)blockIdx.y + 1};
+  bool on[2];
+#pragma unroll
+  for (int q = 0; q < 2; q++) on[q] = rhs[q] < nrhs && !rhs_off(act, rhs[q]);
+  if (!on[0] && !on[1]) return;
+  const bool mine = sel2(on, rw);  // wave-uniform
+  // a right-hand side that loads may use when its own is off (never read then)
+  const int rv[2] = {on[0] ? rhs[0] : rhs[1], on[1] ? rhs[1] : rhs[0]};
+  const long b0 = (long)xcd_block() * 64;
+  const int X0 = (int)compact2((unsigned)b0), Y0 = (int)compact2((unsigned)b0 >> 1);
+  for (int h = tid; h < 100; h += kBlock) {
+    const int sx = X0 - 1 + h % 10, sy = Y0 - 1 + h / 10;
+    hoff[h] = (sx >= 0 && sy >= 0 && sx < nbs && sy < nbs) ? (int)morton2(sx, sy) * 64 : -1;
+  }
+  __syncthreads();
+
+  // data scale of each right-hand side: largest |re|, |im| over the halo
+  int es[2];
+  {
+    float m[2] = {0.f, 0.f};
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+      const float4* yq = reinterpret_cast<const float4*>(y + (long)sel2(rv, q) * ys);
+      for (int e0 = tid; e0 < 3200; e0 += 4 * kBlock) {  // 100 boxes x 32 float4
+        float4 v[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          const int e = e0 + u * kBlock, ec = e < 3200 ? e : 3199;
+          const int off = hoff[ec >> 5];
+          v[u] = yq[((off < 0 ? 0 : off) >> 1) + (ec & 31)];
+          if (!(sel2(on, q) && e < 3200 && off >= 0)) v[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) m[q] = fmaxf(m[q], h16_absmax(v[u]));
+      }
+    }
+    block_max2(m, red);
+    es[0] = h16_scale_exp(m[0]);
+    es[1] = h16_scale_exp(m[1]);
+  }
+
+  v4f cr[4][2], ci[4][2];
+#pragma unroll
+  for (int j = 0; j < 4; j++)
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+      cr[j][c] = v4f{0, 0, 0, 0};
+      ci[j][c] = v4f{0, 0, 0, 0};
+    }
+  // halo position of box 16 j + li (this lane's A row) for the centre slot
+  int hc[4];
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    const unsigned c = (unsigned)(16 * j + li);
+    hc[j] = ((int)compact2(c >> 1) + 1) * kH16Pitch + (int)compact2(c) + 1;
+  }
+
+  // ---- near field: two halves of the source pixels (k = 32 kc .. +31)
+  // B fragments of slot s (this lane's 8 k of column li, for both column
+  // blocks); the next slot's are loaded while the current slot's MFMAs run
+  const v8h* tl = tab + lk * 64 + 16 * mb0 + li;  // + (s * 2 + kc) * 1024 + P * 256 + 16 c
+  auto load_b = [&](v8h (&b)[2][4], const v8h* t) {
+#pragma unroll
+    for (int c = 0; c < 2; c++)
+#pragma unroll
+      for (int P = 0; P < 4; P++) b[c][P] = t[P * 256 + 16 * c];
+  };
+  v8h bn[2][4];
+#pragma unroll 1
+  for (int kc = 0; kc < 2; kc++) {
+    __syncthreads();  // the previous half is no longer read
+    if (mine) load_b(bn, tl + (long)kc * 1024);  // in flight during the staging
+    {
+      // units (q, h, g) of 8 source pixels 32 kc + 8 g .. +7 of halo box h
+      float4 v[4][4];
+#pragma unroll
+      for (int t = 0; t < 4; t++) {
+        const int u = tid + t * kBlock, uc = u < 800 ? u : 799;
+        const int q = uc / 400, h = (uc >> 2) % 100, g = uc & 3;
+        const int off = hoff[h];
+        const float4* src = reinterpret_cast<const float4*>(y + (long)sel2(rv, q) * ys + (off < 0 ? 0 : off) + 32 * kc + 8 * g);
+        const bool ok = u < 800 && sel2(on, q) && off >= 0;
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+          v[t][c] = src[c];
+          if (!ok) v[t][c] = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+      }
+#pragma unroll
+      for (int t = 0; t < 4; t++) {
+        const int u = tid + t * kBlock;
+        if (u < 800) {
+          const int q = u / 400, h = (u >> 2) % 100, g = u & 3;
+          const float s = ldexpf(1.f, sel2(es, q));
+          v8h ph[4];
+          h16_split4(v[t][0], v[t][1], s, ph, 0);
+          h16_split4(v[t][2], v[t][3], s, ph, 4);
+          const int p = (h / 10) * kH16Pitch + h % 10;
+#pragma unroll
+          for (int P = 0; P < 4; P++) hal[((q * 4 + P) * 4 + g) * kH16Pos + p] = ph[P];
+        }
+      }
+    }
+    __syncthreads();
+    if (mine) {
+      const v8h* hb = hal + (rw * 16 + lk) * kH16Pos;  // + P * 4 * kH16Pos + position
+#pragma unroll 1
+      for (int s = 0; s < 9; s++) {
+        const int off = (s / 3 - 1) * kH16Pitch + (s % 3 - 1);
+        v8h b[2][4], nb[2][2];
+#pragma unroll
+        for (int c = 0; c < 2; c++)
+#pragma unroll
+          for (int P = 0; P < 4; P++) b[c][P] = bn[c][P];
+        if (s + 1 < 9) load_b(bn, tl + (long)((s + 1) * 2 + kc) * 1024);
+#pragma unroll
+        for (int c = 0; c < 2; c++) {
+          nb[c][0] = h16_neg(b[c][2]);
+          nb[c][1] = h16_neg(b[c][3]);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          const int p = hc[j] + off;
+          v8h a[4];
+#pragma unroll
+          for (int P = 0; P < 4; P++) a[P] = hb[P * 4 * kH16Pos + p];
+#pragma unroll
+          for (int c = 0; c < 2; c++) h16_cmac(a, b[c], nb[c], cr[j][c], ci[j][c]);
+        }
+      }
+    }
+  }
+
+  // ---- L2P: locals of the tile's boxes, in chunks of 32
+  int el[2];
+  {
+    float m[2] = {0.f, 0.f};
+    const int n4 = 32 * S;  // 64 boxes x S locals = 32 S float4 (S even)
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+      const float4* lq = reinterpret_cast<const float4*>(loc + (long)sel2(rv, q) * ls + b0 * S);
+      for (int e0 = tid; e0 < n4; e0 += 4 * kBlock) {
+        float4 v[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          const int e = e0 + u * kBlock;
+          v[u] = lq[e < n4 ? e : n4 - 1];
+          if (!(sel2(on, q) && e < n4)) v[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) m[q] = fmaxf(m[q], h16_absmax(v[u]));
+      }
+    }
+    block_max2(m, red);
+    el[0] = h16_scale_exp(m[0]);
+    el[1] = h16_scale_exp(m[1]);
+  }
+  if (mine) {
+    // near accumulators carry 2^(es + et); bring them to the L2P's 2^(el + eb)
+    const int n = (sel2(el, rw) + eb) - (sel2(es, rw) + et);
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+#pragma unroll
+      for (int c = 0; c < 2; c++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          cr[j][c][i] = ldexpf(cr[j][c][i], n);
+          ci[j][c][i] = ldexpf(ci[j][c][i], n);
+        }
+  }
+  const int nlc = (S + 31) / 32;
+#pragma unroll 1
+  for (int kc = 0; kc < nlc; kc++) {
+    __syncthreads();
+    if (mine) load_b(bn, bas + (long)kc * 1024 + lk * 64 + 16 * mb0 + li);  // in flight during the staging
+    {
+      // units (q, box, g): locals 32 kc + 8 g .. +7 of box b0 + box
+      float4 v[2][4];
+#pragma unroll
+      for (int t = 0; t < 2; t++) {
+        const int u = tid + t * kBlock;
+        const int q = u >> 8, bx = (u >> 2) & 63, g = u & 3;
+        const zf* lb = loc + (long)sel2(rv, q) * ls + (b0 + bx) * S;
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+          const int k = 32 * kc + 8 * g + 2 * c;  // pairs (k, k + 1) lie within S or beyond it together
+          v[t][c] = *reinterpret_cast<const float4*>(lb + (k < S ? k : S - 2));
+          if (!(sel2(on, q) && k < S)) v[t][c] = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+      }
+#pragma unroll
+      for (int t = 0; t < 2; t++) {
+        const int u = tid + t * kBlock;
+        const int q = u >> 8, bx = (u >> 2) & 63, g = u & 3;
+        const float s = ldexpf(1.f, sel2(el, q));
+        v8h ph[4];
+        h16_split4(v[t][0], v[t][1], s, ph, 0);
+        h16_split4(v[t][2], v[t][3], s, ph, 4);
+#pragma unroll
+        for (int P = 0; P < 4; P++) hal[((q * 4 + P) * 4 + g) * 64 + bx] = ph[P];
+      }
+    }
+    __syncthreads();
+    if (mine) {
+      const v8h* hb = hal + (rw * 16 + lk) * 64 + li;  // + P * 256 + 16 j
+      v8h nb[2][2];
+#pragma unroll
+      for (int c = 0; c < 2; c++) {
+        nb[c][0] = h16_neg(bn[c][2]);
+        nb[c][1] = h16_neg(bn[c][3]);
+      }
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        v8h a[4];
+#pragma unroll
+        for (int P = 0; P < 4; P++) a[P] = hb[P * 256 + 16 * j];
+#pragma unroll
+        for (int c = 0; c < 2; c++) h16_cmac(a, bn[c], nb[c], cr[j][c], ci[j][c]);
+      }
+    }
+  }
+
+  // ---- output, one right-hand side at a time through LDS (coalesced stores
+  // and, for a post step, coalesced loads); D[box][m]: box 16 j + 4 lk + i
+  // (register i), pixel m = 16 (mb0 + c) + li
+  zf* tile = reinterpret_cast<zf*>(smem_raw);
+#pragma unroll 1
+  for (int q = 0; q < 2; q++) {
+    if (!sel2(on, q)) continue;  // uniform
+    __syncthreads();
+    if (rw == q) {
+      const int n = -(sel2(el, q) + eb);
+#pragma unroll
+      for (int j = 0; j < 4; j++)
+#pragma unroll
+        for (int c = 0; c < 2; c++)
+#pragma unroll
+          for (int i = 0; i < 4; i++)
+            tile[(16 * j + 4 * lk + i) * 64 + 16 * (mb0 + c) + li] =
+                mkc<float>(ldexpf(cr[j][c][i], n), ldexpf(ci[j][c][i], n));
+    }
+    __syncthreads();
+    const long o = (long)sel2(rhs, q) * rs + b0 * 64;
+    if constexpr (POST == 0) {
+#pragma unroll 4
+      for (int e = tid; e < 64 * 64; e += kBlock) r[o + e] = tile[e];
+    } else {
+      const zd* gr = post.g + (long)sel2(rhs, q) * post.gs + b0 * 64;
+      zd acc[2] = {mkc<double>(0, 0), mkc<double>(0, 0)};
+      struct In {
+        zf x, aux;  // aux: r~ (mode 1)
+        zd g;
+      };
+      const bool herm = post.herm != 0;
+      stage<2, In>(
+          64 * 64,
+          [&](int e) {
+            In v;
+            v.x = post.x[o + e];
+            if (POST == 1) v.aux = post.rt[o + e];
+            if (herm) v.g = gr[e];  // the forward post step does not use g
+            return v;
+          },
+          [&](int e, In v) {
+            const zf gt = tile[e];
+            const zd gy = mkc<double>((double)gt.x, (double)gt.y);
+            const zd xv = ccast<double>(v.x);
+            const zd px = herm ? csub(xv, cconj(cmul(v.g, gy))) : csub(xv, gy);
+            const zd res = stv(r, o + e, px);
+            if (POST == 1) {
+              const zd a = ccast<double>(v.aux);
+              acc[0].x += a.x * res.x + a.y * res.y;
+              acc[0].y += a.x * res.y - a.y * res.x;
+            } else {
+              acc[0].x += res.x * xv.x + res.y * xv.y;
+              acc[0].y += res.x * xv.y - res.y * xv.x;
+              acc[1].x += res.x * res.x + res.y * res.y;
+            }
+          });
+      block_store<2>(acc, post.part + (long)sel2(rhs, q) * kSlots * gridDim.x, gridDim.x, blockIdx.x);
+    }
+  }
+}
+
+}  // namespace isg
