@@ -53,7 +53,7 @@ sys.path.insert(0, ROOT)
 import numpy as np  # noqa: E402
 
 METRIC = "s per DBIM iteration, 1024² px ×64 Tx, rel-res 1e-6: plain vs batched+learned"
-PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3}
+PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3, "f16": 2500.0}
 PEAK_HBM_GBS = 8000.0
 CYCLE = 5  # DBIM iterations of one inversion (the workload's 5; --cycle sets it)
 
@@ -86,16 +86,17 @@ def near_flops_per_rhs(level, S_leaf):
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_fp32_operator_1024x64rhs.txt")
 
 
-def measured_traffic():
+def measured_traffic(split):
     """HBM bytes (FETCH_SIZE + WRITE_SIZE, copy-calibrated) of one 64-RHS
-    launch of the FP32 near-field kernel at level 8, averaged over the
-    launches of a timed fast-path iteration (most carry the fused BiCGSTAB
-    post step), from the committed rocprofv3 counter summary; None when that
-    summary is absent."""
+    launch of the FP32 operator's near-field kernel at level 8 (split: the
+    split-FP16 kernel), averaged over the launches of a timed fast-path
+    iteration (most carry the fused BiCGSTAB post step), from the committed
+    rocprofv3 counter summary; None when that summary is absent."""
+    key = "traffic_bytes_per_launch_64rhs%s=" % ("_split" if split else "")
     try:
         with open(TRAFFIC_FILE) as f:
             for line in f:
-                if line.startswith("traffic_bytes_per_launch_64rhs="):
+                if line.startswith(key):
                     return int(line.split("=")[1])
     except OSError:
         pass
@@ -183,14 +184,15 @@ def synthesize(isg, common, batch, otrue, iti):
 
 def run_paths(isg, args, wl, common, otrue, mes, paths, warmup, steps, timing):
     """Time the requested paths from the same measurements; returns per-path
-    (s/step, stats, kernel timers), the contrasts and the leaf-fusion flag."""
+    (s/step, stats, kernel timers), the contrasts, the leaf-fusion flag and
+    whether the FP32 near field ran on the split-FP16 kernel."""
     batch = wl.get("batch", wl["numtx"])
     fast_prec = 1 if args.fast_precision == "mixed" else 0
     fast_kw = dict(max_rhs=batch, precision=fast_prec, mixed_margin=args.mixed_margin,
                    products=1 if args.fast_products == "fp32" and fast_prec == 1 else 0,
                    solve_start=1 if args.fast_solve_start == "previous" else 0)
     res, contrast, errors = {}, {}, {}
-    fused = False
+    fused = split = False
     side = 8 << (wl["level"] - 1)
     for p in paths:
         if p == "plain":
@@ -215,11 +217,14 @@ def run_paths(isg, args, wl, common, otrue, mes, paths, warmup, steps, timing):
                 continue
             ptime = timing
             fused = fused or s.tuning_get(fast_prec, 1, "leaf") == 1
+            # k_near_h16 serves launches of >= 4 right-hand sides on a grid of >= 8 x 8 leaf boxes
+            split = split or (fast_prec == 1 and batch >= 4 and wl["level"] >= 4
+                              and s.tuning_get(1, 1 if batch >= 16 else 0, "near") == 128)
         res[p] = run_path(s, d, warmup, steps, p, ptime)
         contrast[p] = d.get("o")
         del d
         s.close()
-    return res, contrast, fused, errors
+    return res, contrast, fused, split, errors
 
 
 def cpu_baselines(isg, level, radius, o_rec, inc0, plain_iter_solves):
@@ -379,7 +384,7 @@ def main():
     S_leaf = numsamp[-1]
     flops_mv, pairs = near_flops_per_rhs(level, S_leaf)
 
-    res, contrast, fast_leaf_fused, errors = run_paths(isg, args, wl, common, otrue, mes, paths,
+    res, contrast, fast_leaf_fused, fast_split, errors = run_paths(isg, args, wl, common, otrue, mes, paths,
                                                        args.warmup, args.steps, timing)
     if not res:
         print(json.dumps({"metric": METRIC, "value": None, "error": errors}), flush=True)
@@ -404,7 +409,9 @@ def main():
     m2l_gbs = (n_near * m2l_bytes_per_rhs(level, numsamp, cb, fused) / (m2l_ms * 1e-3) / 1e9
                if m2l_ms > 0 else 0.0)
     iters = sum(s["solve_iters"] for s in stats)
-    traffic = measured_traffic() if (mixed and level == 8 and ntx == 64) else None
+    split = mixed and fast_split
+    traffic = measured_traffic(split) if (mixed and level == 8 and ntx == 64) else None
+    peak = PEAK_TFLOPS["f16" if split else prec]
     geom = (("on a circle of radius %g lambda (line sources)" % args.radius) if args.radius > 0
             else "(plane waves, far-field receivers)")
 
@@ -440,19 +447,25 @@ def main():
             "kernel_tuning": args.tuning,
         },
         "roofline": {
-            "kernel": "k_near_mfma (P2P + L2P on v_mfma_%s_16x16x4)" % prec,
+            "kernel": ("k_near_h16 (P2P + L2P on v_mfma_f32_16x16x32_f16, FP32 operands split into FP16 hi + lo)"
+                       if split else "k_near_mfma (P2P + L2P on v_mfma_%s_16x16x4)" % prec),
             "bound": "mfma",
             "achieved": round(achieved, 2),
-            "peak": PEAK_TFLOPS[prec],
+            "peak": peak,
             "unit": "TFLOP/s",
-            "frac": round(achieved / PEAK_TFLOPS[prec], 4),
+            "frac": round(achieved / peak, 4),
             "traffic": traffic,
             "traffic_launch": "64 RHS, level 8, mean over the near-field launches of a fast-path DBIM iteration "
                               "(rocprofv3 FETCH_SIZE x2 + WRITE_SIZE x1, copy-calibrated)"
                               if traffic else None,
             "flops_per_rhs_launch": flops_mv,
-            "note": "complex products in the three-product form: the matrix cores execute 6 of the 8 "
-                    "algorithmic flops per complex multiply-add, so the executed MFMA rate is 0.75 x achieved",
+            "note": ("achieved counts 8 flops per complex multiply-add; the split kernel executes each real product "
+                     "as three FP16 products (hi.hi + hi.lo + lo.hi) in the four-product complex form, 24 FP16 "
+                     "flops per complex multiply-add, so its executed MFMA rate is 3 x achieved and its ceiling "
+                     "for this arithmetic is peak / 3" if split else
+                     "complex products in the three-product form: the matrix cores execute 6 of the 8 "
+                     "algorithmic flops per complex multiply-add, so the executed MFMA rate is 0.75 x achieved"),
+            "frac_of_split_ceiling": round(3 * achieved / peak, 4) if split else None,
             "avg_launch_ms": round(near_ms / max(near_n, 1), 4),
             "launches": near_n,
         },

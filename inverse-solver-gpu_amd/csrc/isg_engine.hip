@@ -780,7 +780,8 @@ int Engine::launch_near(const cplx<TI>* y, cplx<TI>* r, int nrhs, const int* act
     // split-FP16 kernel: FP32 operator on FP32 vectors, near field included,
     // 64-box tiles, BiCGSTAB post steps (not the FP64 residual check)
     if constexpr (sizeof(T) == 4 && sizeof(TI) == 4) {
-      if (with_near && B % 64 == 0 && near16_ && bas16_ && pp.mode != 3) {
+      // (pairs of right-hand sides: below 4 the workgroups are few and half idle)
+      if (with_near && B % 64 == 0 && near16_ && bas16_ && pp.mode != 3 && (nrhs >= 4 || force == kNearSplit)) {
         dim3 grid(B / 64, (nrhs + 1) / 2), blk(kBlock);
         auto go = [&](auto modec) {
           k_near_h16<decltype(modec)::value><<<grid, blk, kH16Lds, stream_>>>(

@@ -38,7 +38,10 @@ typedef unsigned v4u __attribute__((ext_vector_type(4)));
 constexpr int kNearSplit = 128;
 // Halo of the 8 x 8 box tile: 10 x 10 boxes at a row pitch of 12 positions,
 // so the 16 boxes of a row block (a 4 x 4 block) read by 16 lanes fall in 16
-// different 16-byte bank groups.
+// different 16-byte bank groups.  Positions (and the boxes of the L2P chunk)
+// are stored XOR 2 g for k group g, so that the four groups of one box, which
+// consecutive lanes write, also fall in different bank groups (a bijection on
+// the low bits: readers of one group keep distinct banks; 119 ^ 6 < 120).
 constexpr int kH16Pitch = 12;
 constexpr int kH16Pos = kH16Pitch * 10;
 // LDS (bytes): half of the halo (32 of the 64 pixels of each box) as
@@ -194,23 +197,26 @@ k_near_h16(const zf* __restrict__ y, long ys, const zf* __restrict__ loc, long l
   // data scale of each right-hand side: largest |re|, |im| over the halo
   int es[2];
   {
+    // 100 boxes x 32 float4 per right-hand side, every load of both issued
+    // before the first comparison (the accumulators are not live yet)
+    constexpr int NU = (3200 + kBlock - 1) / kBlock;
     float m[2] = {0.f, 0.f};
+    float4 v[2][NU];
 #pragma unroll
     for (int q = 0; q < 2; q++) {
       const float4* yq = reinterpret_cast<const float4*>(y + (long)sel2(rv, q) * ys);
-      for (int e0 = tid; e0 < 3200; e0 += 4 * kBlock) {  // 100 boxes x 32 float4
-        float4 v[4];
 #pragma unroll
-        for (int u = 0; u < 4; u++) {
-          const int e = e0 + u * kBlock, ec = e < 3200 ? e : 3199;
-          const int off = hoff[ec >> 5];
-          v[u] = yq[((off < 0 ? 0 : off) >> 1) + (ec & 31)];
-          if (!(sel2(on, q) && e < 3200 && off >= 0)) v[u] = make_float4(0.f, 0.f, 0.f, 0.f);
-        }
-#pragma unroll
-        for (int u = 0; u < 4; u++) m[q] = fmaxf(m[q], h16_absmax(v[u]));
+      for (int u = 0; u < NU; u++) {
+        const int e = tid + u * kBlock, ec = e < 3200 ? e : 3199;
+        const int off = hoff[ec >> 5];
+        v[q][u] = yq[((off < 0 ? 0 : off) >> 1) + (ec & 31)];
+        if (!(sel2(on, q) && e < 3200 && off >= 0)) v[q][u] = make_float4(0.f, 0.f, 0.f, 0.f);
       }
     }
+#pragma unroll
+    for (int q = 0; q < 2; q++)
+#pragma unroll
+      for (int u = 0; u < NU; u++) m[q] = fmaxf(m[q], h16_absmax(v[q][u]));
     block_max2(m, red);
     es[0] = h16_scale_exp(m[0]);
     es[1] = h16_scale_exp(m[1]);
@@ -274,7 +280,7 @@ k_near_h16(const zf* __restrict__ y, long ys, const zf* __restrict__ loc, long l
           h16_split4(v[t][2], v[t][3], s, ph, 4);
           const int p = (h / 10) * kH16Pitch + h % 10;
 #pragma unroll
-          for (int P = 0; P < 4; P++) hal[((q * 4 + P) * 4 + g) * kH16Pos + p] = ph[P];
+          for (int P = 0; P < 4; P++) hal[((q * 4 + P) * 4 + g) * kH16Pos + (p ^ (2 * g))] = ph[P];
         }
       }
     }
@@ -297,7 +303,7 @@ k_near_h16(const zf* __restrict__ y, long ys, const zf* __restrict__ loc, long l
         }
 #pragma unroll
         for (int j = 0; j < 4; j++) {
-          const int p = hc[j] + off;
+          const int p = (hc[j] + off) ^ (2 * lk);
           v8h a[4];
 #pragma unroll
           for (int P = 0; P < 4; P++) a[P] = hb[P * 4 * kH16Pos + p];
@@ -312,20 +318,25 @@ k_near_h16(const zf* __restrict__ y, long ys, const zf* __restrict__ loc, long l
   int el[2];
   {
     float m[2] = {0.f, 0.f};
-    const int n4 = 32 * S;  // 64 boxes x S locals = 32 S float4 (S even)
+    const int n4 = 32 * S;  // 64 boxes x S locals = 32 S float4 (S even), per right-hand side
+    constexpr int U = 9;
+    for (int e0 = tid; e0 < 2 * n4; e0 += U * kBlock) {
+      float4 v[U];
 #pragma unroll
-    for (int q = 0; q < 2; q++) {
-      const float4* lq = reinterpret_cast<const float4*>(loc + (long)sel2(rv, q) * ls + b0 * S);
-      for (int e0 = tid; e0 < n4; e0 += 4 * kBlock) {
-        float4 v[4];
+      for (int u = 0; u < U; u++) {
+        const int e = e0 + u * kBlock, q = e >= n4 ? 1 : 0, ei = e - q * n4;
+        const float4* lq = reinterpret_cast<const float4*>(loc + (long)sel2(rv, q) * ls + b0 * S);
+        v[u] = lq[ei < n4 ? ei : n4 - 1];
+        if (!(sel2(on, q) && e < 2 * n4)) v[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+      }
 #pragma unroll
-        for (int u = 0; u < 4; u++) {
-          const int e = e0 + u * kBlock;
-          v[u] = lq[e < n4 ? e : n4 - 1];
-          if (!(sel2(on, q) && e < n4)) v[u] = make_float4(0.f, 0.f, 0.f, 0.f);
-        }
-#pragma unroll
-        for (int u = 0; u < 4; u++) m[q] = fmaxf(m[q], h16_absmax(v[u]));
+      for (int u = 0; u < U; u++) {
+        const int e = e0 + u * kBlock;
+        const float a = h16_absmax(v[u]);
+        if (e >= n4)
+          m[1] = fmaxf(m[1], a);
+        else
+          m[0] = fmaxf(m[0], a);
       }
     }
     block_max2(m, red);
@@ -374,12 +385,12 @@ k_near_h16(const zf* __restrict__ y, long ys, const zf* __restrict__ loc, long l
         h16_split4(v[t][0], v[t][1], s, ph, 0);
         h16_split4(v[t][2], v[t][3], s, ph, 4);
 #pragma unroll
-        for (int P = 0; P < 4; P++) hal[((q * 4 + P) * 4 + g) * 64 + bx] = ph[P];
+        for (int P = 0; P < 4; P++) hal[((q * 4 + P) * 4 + g) * 64 + (bx ^ (2 * g))] = ph[P];
       }
     }
     __syncthreads();
     if (mine) {
-      const v8h* hb = hal + (rw * 16 + lk) * 64 + li;  // + P * 256 + 16 j
+      const v8h* hb = hal + (rw * 16 + lk) * 64;  // + P * 256 + box
       v8h nb[2][2];
 #pragma unroll
       for (int c = 0; c < 2; c++) {
@@ -390,7 +401,7 @@ k_near_h16(const zf* __restrict__ y, long ys, const zf* __restrict__ loc, long l
       for (int j = 0; j < 4; j++) {
         v8h a[4];
 #pragma unroll
-        for (int P = 0; P < 4; P++) a[P] = hb[P * 256 + 16 * j];
+        for (int P = 0; P < 4; P++) a[P] = hb[P * 256 + ((16 * j + li) ^ (2 * lk))];
 #pragma unroll
         for (int c = 0; c < 2; c++) h16_cmac(a, bn[c], nb[c], cr[j][c], ci[j][c]);
       }

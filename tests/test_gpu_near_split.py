@@ -65,22 +65,34 @@ def test_split_near_matches_fp32_kernel_and_oracle(isg, level):
 
 def test_split_near_mixed_bicgs_reaches_fp64_residual(isg):
     """The solver's post-step epilogues (modes 1 and 2) in the split kernel:
-    a mixed solve with it meets the FP64 true residual (recomputed with the
-    oracle), forward and adjoint."""
-    level, tol = 5, 1e-6
+    a batched mixed solve of 5 systems (plane waves from 5 directions, so
+    they converge at different iterations and deflation masks workgroup
+    pairs half off) meets the FP64 true residual (recomputed with the
+    oracle) in every system, forward and adjoint."""
+    import torch
+    level, tol, nrhs = 5, 1e-6, 5
     o = oracle_for(level)
-    s = isg.Solver(level=level, numrx=16, toli=tol, iti=300, precision=1)
+    s = isg.Solver(level=level, numrx=16, toli=tol, iti=300, precision=1, max_rhs=nrhs)
     s.tuning_set(1, 0, "near", 0, 128)
     pos = o.pos()
     g = 0.4 * np.exp(-np.abs(pos) ** 2 / (2 * 2.5 ** 2)) * (1 + 0.05j) * o.k0 ** 2
-    b = np.exp(1j * o.k0 * (0.6 * pos.real + 0.8 * pos.imag))
+    ang = np.linspace(0, 2 * np.pi, nrhs, endpoint=False) + 0.3
+    B = np.stack([np.exp(1j * o.k0 * (np.cos(a) * pos.real + np.sin(a) * pos.imag)) for a in ang])
+    g_d = torch.from_numpy(g).cuda()
+    B_d = torch.from_numpy(B).cuda()
     for herm in (False, True):
-        x, info = s.bicgs(g, b, herm=herm)
-        assert info["status"] == 0, info
-        ax = x - np.conj(g * o.mlfma(np.conj(x))) if herm else x - o.mlfma(g * x)
-        res = np.linalg.norm(b - ax) / np.linalg.norm(b)
-        print("herm %d: %d iterations, FP64 residual %.2e" % (herm, info["iters"], res))
-        assert res < tol * 1.05, res
+        X_d = torch.zeros_like(B_d)
+        torch.cuda.synchronize()
+        _, infos = s.bicgs_dev(X_d.data_ptr(), g_d.data_ptr(), B_d.data_ptr(), nrhs, herm, 0, True)
+        s.synchronize()
+        X = X_d.cpu().numpy()
+        for k in range(nrhs):
+            assert infos[k]["status"] == 0, infos[k]
+            x = X[k]
+            ax = x - np.conj(g * o.mlfma(np.conj(x))) if herm else x - o.mlfma(g * x)
+            res = np.linalg.norm(B[k] - ax) / np.linalg.norm(B[k])
+            print("herm %d rhs %d: %d iterations, FP64 residual %.2e" % (herm, k, infos[k]["iters"], res))
+            assert res < tol * 1.05, (k, res)
 
 
 def test_split_near_batched_dbim_matches_oracle(isg):
