@@ -91,19 +91,18 @@ __device__ __forceinline__ float wave_max(float v) {
   return v;
 }
 
-// Workgroup maxima of two values; every thread gets both.
-__device__ __forceinline__ void block_max2(float (&m)[2], float (*red)[kBlock / 64]) {
+// Workgroup maxima of N values; every thread gets all of them.
+template <int N>
+__device__ __forceinline__ void block_max(float (&m)[N], float (*red)[kBlock / 64]) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  m[0] = wave_max(m[0]);
-  m[1] = wave_max(m[1]);
-  __syncthreads();  // red may still be read by an earlier call
-  if (lane == 0) {
-    red[0][w] = m[0];
-    red[1][w] = m[1];
-  }
+#pragma unroll
+  for (int q = 0; q < N; q++) m[q] = wave_max(m[q]);
+  if (lane == 0)
+#pragma unroll
+    for (int q = 0; q < N; q++) red[q][w] = m[q];
   __syncthreads();
 #pragma unroll
-  for (int q = 0; q < 2; q++) {
+  for (int q = 0; q < N; q++) {
     float v = red[q][0];
 #pragma unroll
     for (int i = 1; i < kBlock / 64; i++) v = fmaxf(v, red[q][i]);
@@ -174,7 +173,7 @@ k_near_h16(const zf* __restrict__ y, long ys, const zf* __restrict__ loc, long l
   extern __shared__ __attribute__((aligned(16))) char smem_raw[];
   v8h* hal = reinterpret_cast<v8h*>(smem_raw);
   __shared__ int hoff[100];
-  __shared__ float red[2][kBlock / 64];
+  __shared__ float red[4][kBlock / 64];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int li = lane & 15, lk = lane >> 4;
   const int rw = w >> 1, mb0 = 2 * (w & 1);
@@ -194,13 +193,13 @@ k_near_h16(const zf* __restrict__ y, long ys, const zf* __restrict__ loc, long l
   }
   __syncthreads();
 
-  // data scale of each right-hand side: largest |re|, |im| over the halo
-  int es[2];
+  // scales of each right-hand side: largest |re|, |im| over the halo (data,
+  // es) and over the tile's leaf locals (el), from one pass whose loads are
+  // all issued before the first comparison (the accumulators are not live yet)
+  int es[2], el[2];
   {
-    // 100 boxes x 32 float4 per right-hand side, every load of both issued
-    // before the first comparison (the accumulators are not live yet)
-    constexpr int NU = (3200 + kBlock - 1) / kBlock;
-    float m[2] = {0.f, 0.f};
+    constexpr int NU = (3200 + kBlock - 1) / kBlock;  // 100 boxes x 32 float4 per right-hand side
+    float m[4] = {0.f, 0.f, 0.f, 0.f};
     float4 v[2][NU];
 #pragma unroll
     for (int q = 0; q < 2; q++) {
@@ -213,13 +212,36 @@ k_near_h16(const zf* __restrict__ y, long ys, const zf* __restrict__ loc, long l
         if (!(sel2(on, q) && e < 3200 && off >= 0)) v[q][u] = make_float4(0.f, 0.f, 0.f, 0.f);
       }
     }
+    const int n4 = 32 * S;  // 64 boxes x S locals = 32 S float4 (S even), per right-hand side
+    constexpr int U = 9;
+    for (int e0 = tid; e0 < 2 * n4; e0 += U * kBlock) {
+      float4 t[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int e = e0 + u * kBlock, q = e >= n4 ? 1 : 0, ei = e - q * n4;
+        const float4* lq = reinterpret_cast<const float4*>(loc + (long)sel2(rv, q) * ls + b0 * S);
+        t[u] = lq[ei < n4 ? ei : n4 - 1];
+        if (!(sel2(on, q) && e < 2 * n4)) t[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const float a = h16_absmax(t[u]);
+        if (e0 + u * kBlock >= n4)
+          m[3] = fmaxf(m[3], a);
+        else
+          m[2] = fmaxf(m[2], a);
+      }
+    }
 #pragma unroll
     for (int q = 0; q < 2; q++)
 #pragma unroll
       for (int u = 0; u < NU; u++) m[q] = fmaxf(m[q], h16_absmax(v[q][u]));
-    block_max2(m, red);
-    es[0] = h16_scale_exp(m[0]);
-    es[1] = h16_scale_exp(m[1]);
+    block_max<4>(m, red);
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+      es[q] = h16_scale_exp(m[q]);
+      el[q] = h16_scale_exp(m[2 + q]);
+    }
   }
 
   v4f cr[4][2], ci[4][2];
@@ -262,12 +284,8 @@ k_near_h16(const zf* __restrict__ y, long ys, const zf* __restrict__ loc, long l
         const int q = uc / 400, h = (uc >> 2) % 100, g = uc & 3;
         const int off = hoff[h];
         const float4* src = reinterpret_cast<const float4*>(y + (long)sel2(rv, q) * ys + (off < 0 ? 0 : off) + 32 * kc + 8 * g);
-        const bool ok = u < 800 && sel2(on, q) && off >= 0;
 #pragma unroll
-        for (int c = 0; c < 4; c++) {
-          v[t][c] = src[c];
-          if (!ok) v[t][c] = make_float4(0.f, 0.f, 0.f, 0.f);
-        }
+        for (int c = 0; c < 4; c++) v[t][c] = src[c];
       }
 #pragma unroll
       for (int t = 0; t < 4; t++) {
@@ -275,9 +293,12 @@ k_near_h16(const zf* __restrict__ y, long ys, const zf* __restrict__ loc, long l
         if (u < 800) {
           const int q = u / 400, h = (u >> 2) % 100, g = u & 3;
           const float s = ldexpf(1.f, sel2(es, q));
+          // zero outside the grid and for a right-hand side that is off (its loads used stand-in addresses)
+          const bool ok = sel2(on, q) && hoff[h] >= 0;
+          const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
           v8h ph[4];
-          h16_split4(v[t][0], v[t][1], s, ph, 0);
-          h16_split4(v[t][2], v[t][3], s, ph, 4);
+          h16_split4(ok ? v[t][0] : z, ok ? v[t][1] : z, s, ph, 0);
+          h16_split4(ok ? v[t][2] : z, ok ? v[t][3] : z, s, ph, 4);
           const int p = (h / 10) * kH16Pitch + h % 10;
 #pragma unroll
           for (int P = 0; P < 4; P++) hal[((q * 4 + P) * 4 + g) * kH16Pos + (p ^ (2 * g))] = ph[P];
@@ -314,35 +335,23 @@ k_near_h16(const zf* __restrict__ y, long ys, const zf* __restrict__ loc, long l
     }
   }
 
-  // ---- L2P: locals of the tile's boxes, in chunks of 32
-  int el[2];
-  {
-    float m[2] = {0.f, 0.f};
-    const int n4 = 32 * S;  // 64 boxes x S locals = 32 S float4 (S even), per right-hand side
-    constexpr int U = 9;
-    for (int e0 = tid; e0 < 2 * n4; e0 += U * kBlock) {
-      float4 v[U];
+  // ---- L2P: locals of the tile's boxes, in chunks of 32; units (q, box, g):
+  // locals 32 kc + 8 g .. +7 of box b0 + box, loaded a chunk ahead
+  float4 lv[2][4];
+  auto load_loc = [&](int kc) {
 #pragma unroll
-      for (int u = 0; u < U; u++) {
-        const int e = e0 + u * kBlock, q = e >= n4 ? 1 : 0, ei = e - q * n4;
-        const float4* lq = reinterpret_cast<const float4*>(loc + (long)sel2(rv, q) * ls + b0 * S);
-        v[u] = lq[ei < n4 ? ei : n4 - 1];
-        if (!(sel2(on, q) && e < 2 * n4)) v[u] = make_float4(0.f, 0.f, 0.f, 0.f);
-      }
+    for (int t = 0; t < 2; t++) {
+      const int u = tid + t * kBlock;
+      const int q = u >> 8, bx = (u >> 2) & 63, g = u & 3;
+      const zf* lb = loc + (long)sel2(rv, q) * ls + (b0 + bx) * S;
 #pragma unroll
-      for (int u = 0; u < U; u++) {
-        const int e = e0 + u * kBlock;
-        const float a = h16_absmax(v[u]);
-        if (e >= n4)
-          m[1] = fmaxf(m[1], a);
-        else
-          m[0] = fmaxf(m[0], a);
+      for (int c = 0; c < 4; c++) {
+        const int k = 32 * kc + 8 * g + 2 * c;
+        lv[t][c] = *reinterpret_cast<const float4*>(lb + (k < S ? k : S - 2));
       }
     }
-    block_max2(m, red);
-    el[0] = h16_scale_exp(m[0]);
-    el[1] = h16_scale_exp(m[1]);
-  }
+  };
+  load_loc(0);
   if (mine) {
     // near accumulators carry 2^(es + et); bring them to the L2P's 2^(el + eb)
     const int n = (sel2(el, rw) + eb) - (sel2(es, rw) + et);
@@ -362,33 +371,27 @@ k_near_h16(const zf* __restrict__ y, long ys, const zf* __restrict__ loc, long l
     __syncthreads();
     if (mine) load_b(bn, bas + (long)kc * 1024 + lk * 64 + 16 * mb0 + li);  // in flight during the staging
     {
-      // units (q, box, g): locals 32 kc + 8 g .. +7 of box b0 + box
-      float4 v[2][4];
-#pragma unroll
-      for (int t = 0; t < 2; t++) {
-        const int u = tid + t * kBlock;
-        const int q = u >> 8, bx = (u >> 2) & 63, g = u & 3;
-        const zf* lb = loc + (long)sel2(rv, q) * ls + (b0 + bx) * S;
-#pragma unroll
-        for (int c = 0; c < 4; c++) {
-          const int k = 32 * kc + 8 * g + 2 * c;  // pairs (k, k + 1) lie within S or beyond it together
-          v[t][c] = *reinterpret_cast<const float4*>(lb + (k < S ? k : S - 2));
-          if (!(sel2(on, q) && k < S)) v[t][c] = make_float4(0.f, 0.f, 0.f, 0.f);
-        }
-      }
+      // converts the chunk's locals, loaded during the previous chunk's MFMAs
 #pragma unroll
       for (int t = 0; t < 2; t++) {
         const int u = tid + t * kBlock;
         const int q = u >> 8, bx = (u >> 2) & 63, g = u & 3;
         const float s = ldexpf(1.f, sel2(el, q));
+        float4 v[4];
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+          const int k = 32 * kc + 8 * g + 2 * c;  // pairs (k, k + 1) lie within S or beyond it together
+          v[c] = (sel2(on, q) && k < S) ? lv[t][c] : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
         v8h ph[4];
-        h16_split4(v[t][0], v[t][1], s, ph, 0);
-        h16_split4(v[t][2], v[t][3], s, ph, 4);
+        h16_split4(v[0], v[1], s, ph, 0);
+        h16_split4(v[2], v[3], s, ph, 4);
 #pragma unroll
         for (int P = 0; P < 4; P++) hal[((q * 4 + P) * 4 + g) * 64 + (bx ^ (2 * g))] = ph[P];
       }
     }
     __syncthreads();
+    if (kc + 1 < nlc) load_loc(kc + 1);
     if (mine) {
       const v8h* hb = hal + (rw * 16 + lk) * 64;  // + P * 256 + box
       v8h nb[2][2];
