@@ -183,9 +183,10 @@ def synthesize(isg, common, batch, otrue, iti):
 
 
 def run_paths(isg, args, wl, common, otrue, mes, paths, warmup, steps, timing):
-    """Time the requested paths from the same measurements; returns per-path
-    (s/step, stats, kernel timers), the contrasts, the leaf-fusion flag and
-    whether the FP32 near field ran on the split-FP16 kernel."""
+    """Time the requested paths from the same measurements; returns a dict:
+    res (per path: s/step, stats, kernel timers), contrast (per path), fused
+    (leaf M2L + L2L fused), split (the FP32 near field ran on the split-FP16
+    kernel) and errors (paths skipped)."""
     batch = wl.get("batch", wl["numtx"])
     fast_prec = 1 if args.fast_precision == "mixed" else 0
     fast_kw = dict(max_rhs=batch, precision=fast_prec, mixed_margin=args.mixed_margin,
@@ -224,7 +225,7 @@ def run_paths(isg, args, wl, common, otrue, mes, paths, warmup, steps, timing):
         contrast[p] = d.get("o")
         del d
         s.close()
-    return res, contrast, fused, split, errors
+    return {"res": res, "contrast": contrast, "fused": fused, "split": split, "errors": errors}
 
 
 def cpu_baselines(isg, level, radius, o_rec, inc0, plain_iter_solves):
@@ -384,8 +385,9 @@ def main():
     S_leaf = numsamp[-1]
     flops_mv, pairs = near_flops_per_rhs(level, S_leaf)
 
-    res, contrast, fast_leaf_fused, fast_split, errors = run_paths(isg, args, wl, common, otrue, mes, paths,
-                                                       args.warmup, args.steps, timing)
+    runs = run_paths(isg, args, wl, common, otrue, mes, paths, args.warmup, args.steps, timing)
+    res, contrast, errors = runs["res"], runs["contrast"], runs["errors"]
+    fast_leaf_fused, fast_split = runs["fused"], runs["split"]
     if not res:
         print(json.dumps({"metric": METRIC, "value": None, "error": errors}), flush=True)
         sys.exit(1)
@@ -516,7 +518,8 @@ def main():
         rg_common["radius"] = 0.0
         rg_mes, rg_synth, _ = synthesize(isg, rg_common, batch, otrue, args.synth_iti)
         rg_paths = [p for p in ("learned", "plain") if p in paths]
-        rres, rcon, _, rerr = run_paths(isg, args, wl, rg_common, otrue, rg_mes, rg_paths, 1, CYCLE, False)
+        rg_runs = run_paths(isg, args, wl, rg_common, otrue, rg_mes, rg_paths, 1, CYCLE, False)
+        rres, rcon, rerr = rg_runs["res"], rg_runs["contrast"], rg_runs["errors"]
         rg = {"note": "plane waves and far-field receivers (the reference's geometry), one warm-up step and "
                       "one 5-iteration cycle per path, no live kernel timers", "synthesis": rg_synth}
         for p in rres:
