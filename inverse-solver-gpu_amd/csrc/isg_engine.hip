@@ -469,15 +469,16 @@ Engine::~Engine() {
   if (stream_) (void)hipStreamDestroy(stream_);
 }
 
-// Split-FP16 tables of k_near_h16 from the FP32 operator's near-field and
-// L2P tables, each with the power-of-two scale that puts its largest
-// component just below 2^14.
+// Split-FP16 tables of k_near_h16 and k_p2m_h16 from the FP32 operator's
+// near-field, L2P and P2M tables, each with the power-of-two scale that puts
+// its largest component just below 2^14.
 void Engine::build_split16() {
   if (!ok()) return;
-  const int S = P_.nsamp[P_.L - 1], nlc = (S + 31) / 32;
-  std::vector<zf> nf((size_t)9 * 4096), bf((size_t)S * 64);
+  const int S = P_.nsamp[P_.L - 1], nlc = (S + 31) / 32, ncolp = 16 * ((S + 15) / 16);
+  std::vector<zf> nf((size_t)9 * 4096), bf((size_t)S * 64), cf((size_t)64 * S);
   ISG_HIP(hipMemcpy(nf.data(), op32_.near, nf.size() * sizeof(zf), hipMemcpyDeviceToHost));
   ISG_HIP(hipMemcpy(bf.data(), op32_.basisT, bf.size() * sizeof(zf), hipMemcpyDeviceToHost));
+  ISG_HIP(hipMemcpy(cf.data(), op32_.coeffT, cf.size() * sizeof(zf), hipMemcpyDeviceToHost));
   if (!ok()) return;
   auto absmax = [](const std::vector<zf>& v) {
     float m = 0;
@@ -486,11 +487,19 @@ void Engine::build_split16() {
   };
   near16_et_ = h16_scale_exp(absmax(nf));
   bas16_eb_ = h16_scale_exp(absmax(bf));
+  p2m16_ec_ = h16_scale_exp(absmax(cf));
   near16_ = dalloc((size_t)18 * 1024 * 16);
   bas16_ = dalloc((size_t)nlc * 1024 * 16);
+  p2m16_ = S <= 16 * kP2mH16MaxCB ? dalloc((size_t)2 * 16 * ncolp * 16) : nullptr;
   if (!near16_ || !bas16_) return;
-  k_split16<<<(18 * 256 + 255) / 256, 256, 0, stream_>>>(op32_.near, 2, 4096, 64, near16_et_, 18, (v8h*)near16_);
-  k_split16<<<(nlc * 256 + 255) / 256, 256, 0, stream_>>>(op32_.basisT, nlc, 0, S, bas16_eb_, nlc, (v8h*)bas16_);
+  // near[s][k][m] (k = source pixel), basisT[k][m] (k = local sample), coeffT[n][s] (n = pixel)
+  k_split16<<<(18 * 256 + 255) / 256, 256, 0, stream_>>>(op32_.near, 2, 4096, 64, 64, 64, 64, near16_et_, 18,
+                                                          (v8h*)near16_);
+  k_split16<<<(nlc * 256 + 255) / 256, 256, 0, stream_>>>(op32_.basisT, nlc, 0, S, 64, 64, 64, bas16_eb_, nlc,
+                                                          (v8h*)bas16_);
+  if (p2m16_)
+    k_split16<<<(2 * 4 * ncolp + 255) / 256, 256, 0, stream_>>>(op32_.coeffT, 2, 0, 64, S, S, ncolp, p2m16_ec_, 2,
+                                                                (v8h*)p2m16_);
   ISG_HIP(hipGetLastError());
   ISG_HIP(hipStreamSynchronize(stream_));
 }
@@ -577,12 +586,24 @@ void Engine::launch_p2m(const cplx<TI>* y, int nrhs, const int* act, int force, 
   const int Lf = P_.L - 1, S = P_.nsamp[Lf], B = P_.B;
   const int tuned = tune(sizeof(T) == 4 ? PREC_MIXED : PREC_FP64, nrhs).p2m_nb;
   int nb = force > 0 ? force : (tuned > 0 ? tuned : (B >= 32 ? 32 : 16));
-  if (nb == 64 && sizeof(T) != 4) nb = 32;  // 64-box tiles: FP32 only (LDS)
-  if (B % nb != 0) nb = 16;
-  dim3 grid(B / nb, nrhs), blk(kBlock);
   DevOp<T>& d = ops<T>();
   cplx<T>* out = d.agm + lvl_off_[Lf];
   long as = (long)lvl_size_[Lf];
+  if (nb == kNearSplit) {
+    // split-FP16 P2M: FP32 operator on FP32 vectors, 32-box tiles of two
+    // right-hand sides (below 4 the workgroups are few and half idle)
+    if constexpr (sizeof(T) == 4 && sizeof(TI) == 4) {
+      if (!prep && B % 32 == 0 && p2m16_ && (nrhs >= 4 || force == kNearSplit)) {
+        dim3 grid(B / 32, (nrhs + 1) / 2), blk(kBlock);
+        k_p2m_h16<<<grid, blk, 0, stream_>>>(y, P_.N, out, as, (const v8h*)p2m16_, p2m16_ec_, S, nrhs, act);
+        return;
+      }
+    }
+    nb = B >= 32 ? 32 : 16;
+  }
+  if (nb == 64 && sizeof(T) != 4) nb = 32;  // 64-box tiles: FP32 only (LDS)
+  if (B % nb != 0) nb = 16;
+  dim3 grid(B / nb, nrhs), blk(kBlock);
   if (prep) {  // residual check: y is the iterate, contrast-weighted while loading (FP64 only)
     if constexpr (sizeof(T) == 8 && sizeof(TI) == 8) {
       if (nb == 32)
@@ -1408,7 +1429,11 @@ void Engine::autotune_prec(int nrhs, int reps) {
     launch_near<T, T>((const cplx<T>*)y, (cplx<T>*)op, nrhs, nullptr, true, widths[c]);
   });
   tu.near_nb = c < 0 ? -1 : widths[c];
-  c = pick(3, width_ok, [&](int c) { launch_p2m<T, double>(y, nrhs, nullptr, widths[c]); });
+  auto p2m_ok = [&](int c) {
+    if (c == 3) return sizeof(T) == 4 && P_.B % 32 == 0 && p2m16_ != nullptr;
+    return P_.B % widths[c] == 0 && (widths[c] < 64 || sizeof(T) == 4);
+  };
+  c = pick(4, p2m_ok, [&](int c) { launch_p2m<T, T>((const cplx<T>*)y, nrhs, nullptr, widths[c]); });
   tu.p2m_nb = c < 0 ? -1 : widths[c];
   for (int i = 2; i <= Lf; i++)
     tu.m2l[i] = pick(
@@ -1457,7 +1482,7 @@ bool Engine::tuning_set(int prec, int cls, int stage, int level, int value) {
   if ((prec != PREC_FP64 && prec != PREC_MIXED) || cls < 0 || cls > 1 || stage < 0 || stage >= ST_COUNT) return false;
   Tuning& t = tune_[prec == PREC_MIXED ? 1 : 0][cls];
   if (stage == ST_NEAR || stage == ST_P2M) {
-    const bool split = stage == ST_NEAR && prec == PREC_MIXED && value == kNearSplit;
+    const bool split = prec == PREC_MIXED && value == kNearSplit;
     if (value != -1 && value != 16 && value != 32 && value != 64 && !split) return false;
     (stage == ST_NEAR ? t.near_nb : t.p2m_nb) = value;
     return true;

@@ -271,6 +271,9 @@ class Engine {
   void* near16_ = nullptr;
   void* bas16_ = nullptr;
   int near16_et_ = 0, bas16_eb_ = 0;
+  // k_p2m_h16 table ([2][4][4][16 ceil(S / 16)] groups of 8 FP16; null for S > 96) and its exponent
+  void* p2m16_ = nullptr;
+  int p2m16_ec_ = 0;
   std::vector<InterpShape> m2m64_, l2l64_, m2m32_, l2l32_;  // per parent level
   std::vector<int*> interp0_, anterp0_;
   int* rx0_ = nullptr;

@@ -140,25 +140,26 @@ __device__ __forceinline__ void h16_cmac(const v8h (&a)[4], const v8h (&b)[4], c
   ci = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[3], b[0], ci, 0, 0, 0);
 }
 
-// Split-FP16 table: out[b][part][g][m] = 8 FP16 parts of 2^ex src[(b / per)
-// bstride + k 64 + m], k = 32 (b % per) + 8 g + e, e = 0..7 (zero for
-// k >= kmax); one thread per (b, g, m).
+// Split-FP16 table: out[b][part][g][m] (m < ncolp) = 8 FP16 parts of
+// 2^ex src[(b / per) bstride + k srow + m], k = 32 (b % per) + 8 g + e,
+// e = 0..7 (zero for k >= kmax or m >= ncol); one thread per (b, g, m).
 static __global__ void __launch_bounds__(256) k_split16(const zf* __restrict__ src, int per, long bstride, int kmax,
-                                                        int ex, int nblk, v8h* __restrict__ out) {
+                                                        int srow, int ncol, int ncolp, int ex, int nblk,
+                                                        v8h* __restrict__ out) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nblk * 256) return;
-  const int b = t >> 8, g = (t >> 6) & 3, m = t & 63;
+  if (t >= nblk * 4 * ncolp) return;
+  const int m = t % ncolp, g = (t / ncolp) & 3, b = t / (4 * ncolp);
   const float s = ldexpf(1.f, ex);
   v8h ph[4];
 #pragma unroll
   for (int e = 0; e < 8; e++) {
     const int k = 32 * (b % per) + 8 * g + e;
-    const zf v = k < kmax ? src[(long)(b / per) * bstride + (long)k * 64 + m] : mkc<float>(0.f, 0.f);
+    const zf v = (k < kmax && m < ncol) ? src[(long)(b / per) * bstride + (long)k * srow + m] : mkc<float>(0.f, 0.f);
     h16_split(v.x * s, ph[0], ph[1], e);
     h16_split(v.y * s, ph[2], ph[3], e);
   }
 #pragma unroll
-  for (int P = 0; P < 4; P++) out[((long)(b * 4 + P) * 4 + g) * 64 + m] = ph[P];
+  for (int P = 0; P < 4; P++) out[((long)(b * 4 + P) * 4 + g) * ncolp + m] = ph[P];
 }
 
 // Split tables, built once per operator (k_split16):
@@ -469,6 +470,112 @@ k_near_h16(const zf* __restrict__ y, long ys, const zf* __restrict__ loc, long l
             }
           });
       block_store<2>(acc, post.part + (long)sel2(rhs, q) * kSlots * gridDim.x, gridDim.x, blockIdx.x);
+    }
+  }
+}
+
+// P2M of the FP32 operator on split FP16: agm[box][s] = sum_n coeffT[n][s]
+// y[box 64 + n] for 32 Morton-consecutive boxes of two right-hand sides.
+// The workgroup's input (2 x 32 boxes x 64 pixels) is loaded once into
+// registers (units of 8 pixels: 2 per thread), its largest component per
+// right-hand side sets the data scale, and it is staged split in LDS as
+// [rhs][part][pixel group 8][box 32] x 8 FP16 (boxes XOR 2 (g mod 4), as in
+// k_near_h16).  Wave w owns right-hand side w / 2, row block w % 2 (16
+// boxes) and all ceil(S / 16) column blocks (samples); K = 64 pixels in two
+// MFMA chunks of 32.  The coefficient table coef[kc][part][g][s] (s padded to
+// 16 ceil(S / 16), zero beyond S) carries its own scale 2^ec.  Results go
+// straight from the accumulators to HBM: lane li stores sample 16 cb + li of
+// box 16 j + 4 (lane / 16) + i, 16 consecutive samples per row.
+// Grid (B / 32, ceil(nrhs / 2)), kBlock threads.
+constexpr int kP2mH16MaxCB = 6;  // column blocks held per wave (S <= 96)
+// (at least 4 waves per SIMD: unbounded, the compiler hoists every
+// coefficient fragment of both K chunks and takes 256 registers)
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) k_p2m_h16(const zf* __restrict__ y, long ys, zf* __restrict__ agm, long as,
+                                                    const v8h* __restrict__ coef, int ec, int S, int nrhs,
+                                                    const int* __restrict__ act) {
+  __shared__ v8h hal[2 * 4 * 8 * 32];
+  __shared__ float red[4][kBlock / 64];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  const int rw = w >> 1, j = w & 1;
+  const int rhs[2] = {2 * (int)blockIdx.y, 2 * (int)blockIdx.y + 1};
+  bool on[2];
+#pragma unroll
+  for (int q = 0; q < 2; q++) on[q] = rhs[q] < nrhs && !rhs_off(act, rhs[q]);
+  if (!on[0] && !on[1]) return;
+  const bool mine = sel2(on, rw);
+  const int rv[2] = {on[0] ? rhs[0] : rhs[1], on[1] ? rhs[1] : rhs[0]};
+  const long b0 = (long)xcd_block() * 32;
+  const int ncb = (S + 15) / 16, ncolp = 16 * ncb;
+  // units (q, box, g8): pixels 8 g8 .. +7 of box b0 + box of right-hand side q; unit t of this thread
+  // is tid + 256 t, so q = t and g8 = tid % 8 for both
+  const int bx = (tid >> 3) & 31, g8 = tid & 7;
+  float4 v[2][4];
+#pragma unroll
+  for (int q = 0; q < 2; q++) {
+    const float4* src = reinterpret_cast<const float4*>(y + (long)sel2(rv, q) * ys + (b0 + bx) * 64 + 8 * g8);
+#pragma unroll
+    for (int c = 0; c < 4; c++) v[q][c] = src[c];
+  }
+  int es[2];
+  {
+    float m[2];
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+      m[q] = 0.f;
+      if (sel2(on, q))
+#pragma unroll
+        for (int c = 0; c < 4; c++) m[q] = fmaxf(m[q], h16_absmax(v[q][c]));
+    }
+    block_max<2>(m, red);
+    es[0] = h16_scale_exp(m[0]);
+    es[1] = h16_scale_exp(m[1]);
+  }
+#pragma unroll
+  for (int q = 0; q < 2; q++) {
+    const float s = ldexpf(1.f, sel2(es, q));
+    const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+    const bool ok = sel2(on, q);
+    v8h ph[4];
+    h16_split4(ok ? v[q][0] : z, ok ? v[q][1] : z, s, ph, 0);
+    h16_split4(ok ? v[q][2] : z, ok ? v[q][3] : z, s, ph, 4);
+#pragma unroll
+    for (int P = 0; P < 4; P++) hal[((q * 4 + P) * 8 + g8) * 32 + (bx ^ (2 * (g8 & 3)))] = ph[P];
+  }
+  __syncthreads();
+  if (!mine) return;  // no barrier follows
+  v4f cr[kP2mH16MaxCB], ci[kP2mH16MaxCB];
+#pragma unroll
+  for (int c = 0; c < kP2mH16MaxCB; c++) {
+    cr[c] = v4f{0, 0, 0, 0};
+    ci[c] = v4f{0, 0, 0, 0};
+  }
+  const v8h* tl = coef + lk * ncolp + li;  // + (kc * 4 + P) * 4 * ncolp + 16 cb
+#pragma unroll
+  for (int kc = 0; kc < 2; kc++) {
+    v8h a[4];
+#pragma unroll
+    for (int P = 0; P < 4; P++) a[P] = hal[((rw * 4 + P) * 8 + 4 * kc + lk) * 32 + ((16 * j + li) ^ (2 * lk))];
+#pragma unroll
+    for (int cb = 0; cb < kP2mH16MaxCB; cb++) {
+      if (cb < ncb) {
+        v8h b[4], nb[2];
+#pragma unroll
+        for (int P = 0; P < 4; P++) b[P] = tl[(long)(kc * 4 + P) * 4 * ncolp + 16 * cb];
+        nb[0] = h16_neg(b[2]);
+        nb[1] = h16_neg(b[3]);
+        h16_cmac(a, b, nb, cr[cb], ci[cb]);
+      }
+    }
+  }
+  const int n = -(sel2(es, rw) + ec);
+  zf* out = agm + (long)sel2(rhs, rw) * as + (b0 + 16 * j + 4 * lk) * S;
+#pragma unroll
+  for (int cb = 0; cb < kP2mH16MaxCB; cb++) {
+    const int sm = 16 * cb + li;
+    if (cb < ncb && sm < S) {
+#pragma unroll
+      for (int i = 0; i < 4; i++) out[(long)i * S + sm] = mkc<float>(ldexpf(cr[cb][i], n), ldexpf(ci[cb][i], n));
     }
   }
 }

@@ -31,6 +31,9 @@ def main():
     ap.add_argument("--copy-mib", type=int, default=0)
     ap.add_argument("--near", type=int, default=None,
                     help="force the near-field variant (16, 32 or 64 boxes per tile; 128: the split-FP16 kernel of the FP32 operator)")
+    ap.add_argument("--p2m", type=int, default=None,
+                    help="force the P2M variant (16, 32 or 64 boxes per tile; 128: the split-FP16 kernel of the FP32 "
+                         "operator)")
     a = ap.parse_args()
     import torch
     import isg
@@ -43,6 +46,8 @@ def main():
     s = isg.Solver(level=a.level, numrx=16, max_rhs=a.nrhs, precision=a.precision)
     if a.near is not None:
         s.tuning_set(a.precision, 1 if a.nrhs >= 16 else 0, "near", 0, a.near)
+    if a.p2m is not None:
+        s.tuning_set(a.precision, 1 if a.nrhs >= 16 else 0, "p2m", 0, a.p2m)
     rng = np.random.default_rng(0)
     X = (rng.standard_normal((a.nrhs, s.N)) + 1j * rng.standard_normal((a.nrhs, s.N)))
     if a.vec32:
