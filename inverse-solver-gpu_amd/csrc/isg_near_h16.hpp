@@ -194,13 +194,13 @@ k_near_h16(const zf* __restrict__ y, long ys, const zf* __restrict__ loc, long l
   }
   __syncthreads();
 
-  // scales of each right-hand side: largest |re|, |im| over the halo (data,
-  // es) and over the tile's leaf locals (el), from one pass whose loads are
-  // all issued before the first comparison (the accumulators are not live yet)
+  // data scale of each right-hand side: largest |re|, |im| over the halo,
+  // from one pass whose loads are all issued before the first comparison
+  // (the accumulators are not live yet)
   int es[2], el[2];
   {
     constexpr int NU = (3200 + kBlock - 1) / kBlock;  // 100 boxes x 32 float4 per right-hand side
-    float m[4] = {0.f, 0.f, 0.f, 0.f};
+    float m[2] = {0.f, 0.f};
     float4 v[2][NU];
 #pragma unroll
     for (int q = 0; q < 2; q++) {
@@ -213,36 +213,13 @@ k_near_h16(const zf* __restrict__ y, long ys, const zf* __restrict__ loc, long l
         if (!(sel2(on, q) && e < 3200 && off >= 0)) v[q][u] = make_float4(0.f, 0.f, 0.f, 0.f);
       }
     }
-    const int n4 = 32 * S;  // 64 boxes x S locals = 32 S float4 (S even), per right-hand side
-    constexpr int U = 9;
-    for (int e0 = tid; e0 < 2 * n4; e0 += U * kBlock) {
-      float4 t[U];
-#pragma unroll
-      for (int u = 0; u < U; u++) {
-        const int e = e0 + u * kBlock, q = e >= n4 ? 1 : 0, ei = e - q * n4;
-        const float4* lq = reinterpret_cast<const float4*>(loc + (long)sel2(rv, q) * ls + b0 * S);
-        t[u] = lq[ei < n4 ? ei : n4 - 1];
-        if (!(sel2(on, q) && e < 2 * n4)) t[u] = make_float4(0.f, 0.f, 0.f, 0.f);
-      }
-#pragma unroll
-      for (int u = 0; u < U; u++) {
-        const float a = h16_absmax(t[u]);
-        if (e0 + u * kBlock >= n4)
-          m[3] = fmaxf(m[3], a);
-        else
-          m[2] = fmaxf(m[2], a);
-      }
-    }
 #pragma unroll
     for (int q = 0; q < 2; q++)
 #pragma unroll
       for (int u = 0; u < NU; u++) m[q] = fmaxf(m[q], h16_absmax(v[q][u]));
-    block_max<4>(m, red);
+    block_max<2>(m, red);
 #pragma unroll
-    for (int q = 0; q < 2; q++) {
-      es[q] = h16_scale_exp(m[q]);
-      el[q] = h16_scale_exp(m[2 + q]);
-    }
+    for (int q = 0; q < 2; q++) es[q] = h16_scale_exp(m[q]);
   }
 
   v4f cr[4][2], ci[4][2];
@@ -353,6 +330,35 @@ k_near_h16(const zf* __restrict__ y, long ys, const zf* __restrict__ loc, long l
     }
   };
   load_loc(0);
+  // locals scale of each right-hand side, read after the near field so that the
+  // staging below finds them in L2
+  {
+    float m[2] = {0.f, 0.f};
+    const int n4 = 32 * S;  // 64 boxes x S locals = 32 S float4 (S even), per right-hand side
+    constexpr int U = 9;
+    for (int e0 = tid; e0 < 2 * n4; e0 += U * kBlock) {
+      float4 t[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int e = e0 + u * kBlock, q = e >= n4 ? 1 : 0, ei = e - q * n4;
+        const float4* lq = reinterpret_cast<const float4*>(loc + (long)sel2(rv, q) * ls + b0 * S);
+        t[u] = lq[ei < n4 ? ei : n4 - 1];
+        if (!(sel2(on, q) && e < 2 * n4)) t[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const float a = h16_absmax(t[u]);
+        if (e0 + u * kBlock >= n4)
+          m[1] = fmaxf(m[1], a);
+        else
+          m[0] = fmaxf(m[0], a);
+      }
+    }
+    __syncthreads();  // red is read by the data scale above
+    block_max<2>(m, red);
+#pragma unroll
+    for (int q = 0; q < 2; q++) el[q] = h16_scale_exp(m[q]);
+  }
   if (mine) {
     // near accumulators carry 2^(es + et); bring them to the L2P's 2^(el + eb)
     const int n = (sel2(el, rw) + eb) - (sel2(es, rw) + et);
