@@ -3,8 +3,9 @@ order as the eager launch, so every result must be bit-identical to graph = 0.
 
 Workload: 64x64 (level 4) single solves, forward and adjoint, FP64; a
 16-transmitter batched mixed-precision DBIM (3 seeded blobs) for 2 iterations
-with the first-order start, which replays each captured iteration graph
-several times per solve, with the driver's own eager work between replays.
+with the first-order start (FP32-MFMA kernels, and the split-FP16 near field
+and P2M), which replays each captured iteration graph several times per
+solve, with the driver's own eager work between replays.
 """
 import numpy as np
 import pytest
@@ -35,14 +36,21 @@ def test_graph_solve_bit_identical(isg, herm):
     np.testing.assert_array_equal(out[0][0], out[1][0])
 
 
-def test_graph_dbim_bit_identical(isg):
-    print("graph vs eager batched DBIM", flush=True)
+@pytest.mark.parametrize("split", [False, True])
+def test_graph_dbim_bit_identical(isg, split):
+    """split: the FP32 operator's near field, L2P and P2M on the split-FP16
+    kernels (tuning value 128), captured and replayed like the others."""
+    print("graph vs eager batched DBIM, split kernels", split, flush=True)
     o = oracle_for(4, numrx=16)
     otrue = isg.raster_to_tree(isg.phantom_blobs(o.side, 3, 1), o.unkmap())
     res = []
     for graph in (0, 1):
         s = isg.Solver(level=4, numrx=16, numtx=16, toli=1e-6, iti=300, regular=float(np.sqrt(1e-3)), max_rhs=16,
                        precision=1, graph=graph)
+        if split:
+            for cls in (0, 1):
+                s.tuning_set(1, cls, "near", 0, 128)
+                s.tuning_set(1, cls, "p2m", 0, 128)
         d = isg.Dbim(s, otrue, field_start=2)
         its = [d.iterate()["solve_iters"] for _ in range(2)]
         res.append((d.get("o"), its))
