@@ -1,5 +1,6 @@
-// isg_near_h16.hpp -- near field + L2P of the FP32 operator on gfx950's FP16
-// matrix cores with split operands (v_mfma_f32_16x16x32_f16).
+// isg_near_h16.hpp -- near field + L2P (k_near_h16) and P2M (k_p2m_h16) of the
+// FP32 operator on gfx950's FP16 matrix cores with split operands
+// (v_mfma_f32_16x16x32_f16).
 //
 // Every FP32 operand x is carried as two FP16 numbers, hi = rne16(2^e x) and
 // lo = rne16(2^e x - hi), with a power-of-two scale 2^e chosen so that
@@ -15,8 +16,9 @@
 //
 // Workgroup: 8 x 8 Morton-consecutive leaf boxes (64) of two right-hand sides
 // = 128 MFMA rows, so each table fragment read from L2 feeds 8 row blocks
-// (the table, 12 B per complex entry and part pair, is what limits a split
-// kernel; the FP32 kernel reads 8 B per entry for 16 or 32 rows).  Wave w
+// (the table, 8 B per complex entry in its four FP16 parts, is what limits a
+// split kernel: the matrix cores consume it ~5 times faster than the FP32
+// kernel does, which reads the same 8 B per entry for 16 or 32 rows).  Wave w
 // owns right-hand side w / 2 of the pair (its 4 row blocks of 16 boxes) and
 // output pixels 32 (w % 2) .. +31 (2 column blocks): every A fragment read
 // from LDS feeds 2 column blocks, every B fragment 4 row blocks.
