@@ -119,6 +119,10 @@ def test_bench_json_line_with_stub_library(monkeypatch):
     assert abs(r["achieved"] - 2 * 250 * flops / 10e-3 / 1e12) < 0.01
     assert d["speedup_vs_plain"] > 0 and "fast" in d and "plain" in d
     assert "reference_geometry" in d and "learned_s_per_iter" in d["reference_geometry"]
+    # warm-up 1 + 2 timed steps of the 5-iteration cycle end on iteration 3; the reference
+    # geometry's timed cycle ends on the cycle's last iteration
+    assert d["contrast_after_iteration"] == 3
+    assert d["reference_geometry"]["contrast_after_iteration"] == 5
     assert "cpu_baseline" not in d
 
 
