@@ -89,10 +89,10 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_fp32_operator_1024x64
 def measured_traffic(split):
     """HBM bytes (FETCH_SIZE + WRITE_SIZE, copy-calibrated) of one 64-RHS
     launch of the FP32 operator's near-field kernel at level 8 (split: the
-    split-FP16 kernel), averaged over the launches of a timed fast-path
+    split-FP16 kernel's tuning value, 128 or 256), averaged over the launches of a timed fast-path
     iteration (most carry the fused BiCGSTAB post step), from the committed
     rocprofv3 counter summary; None when that summary is absent."""
-    key = "traffic_bytes_per_launch_64rhs%s=" % ("_split" if split else "")
+    key = "traffic_bytes_per_launch_64rhs%s=" % {128: "_split", 256: "_splitw"}.get(split, "")
     try:
         with open(TRAFFIC_FILE) as f:
             for line in f:
@@ -218,9 +218,11 @@ def run_paths(isg, args, wl, common, otrue, mes, paths, warmup, steps, timing):
                 continue
             ptime = timing
             fused = fused or s.tuning_get(fast_prec, 1, "leaf") == 1
-            # k_near_h16 serves launches of >= 4 right-hand sides on a grid of >= 8 x 8 leaf boxes
-            split = split or (fast_prec == 1 and batch >= 4 and wl["level"] >= 4
-                              and s.tuning_get(1, 1 if batch >= 16 else 0, "near") == 128)
+            # the split-FP16 near field: k_near_h16 (tuning value 128) serves launches of >= 4
+            # right-hand sides on a grid of >= 8 x 8 leaf boxes, k_near_h16w (256) grids of >= 16 x 8
+            nv = s.tuning_get(1, 1 if batch >= 16 else 0, "near")
+            if fast_prec == 1 and ((nv == 128 and batch >= 4 and wl["level"] >= 4) or (nv == 256 and wl["level"] >= 5)):
+                split = split or nv
         res[p] = run_path(s, d, warmup, steps, p, ptime)
         contrast[p] = d.get("o")
         del d
@@ -411,7 +413,7 @@ def main():
     m2l_gbs = (n_near * m2l_bytes_per_rhs(level, numsamp, cb, fused) / (m2l_ms * 1e-3) / 1e9
                if m2l_ms > 0 else 0.0)
     iters = sum(s["solve_iters"] for s in stats)
-    split = mixed and fast_split
+    split = fast_split if mixed else False
     traffic = measured_traffic(split) if (mixed and level == 8 and ntx == 64) else None
     peak = PEAK_TFLOPS["f16" if split else prec]
     geom = (("on a circle of radius %g lambda (line sources)" % args.radius) if args.radius > 0
@@ -449,7 +451,8 @@ def main():
             "kernel_tuning": args.tuning,
         },
         "roofline": {
-            "kernel": ("k_near_h16 (P2P + L2P on v_mfma_f32_16x16x32_f16, FP32 operands split into FP16 hi + lo)"
+            "kernel": ("%s (P2P + L2P on v_mfma_f32_16x16x32_f16, FP32 operands split into FP16 hi + lo)"
+                       % ("k_near_h16w" if split == 256 else "k_near_h16")
                        if split else "k_near_mfma (P2P + L2P on v_mfma_%s_16x16x4)" % prec),
             "bound": "mfma",
             "achieved": round(achieved, 2),

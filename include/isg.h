@@ -261,7 +261,8 @@ int isg_raster_bin_read(const char* path, int numunk, double* raster);
  * stage 0 near-field tile width (16, 32 or 64 boxes; 64 for the FP32
  * operator only; 128, FP32 operator only: the split-FP16 kernel, 64-box tiles
  * of two right-hand sides on the FP16 matrix cores with FP32 operands carried
- * as hi + lo, used for launches of 4 or more right-hand sides), 1 P2M tile
+ * as hi + lo, used for launches of 4 or more right-hand sides; 256, likewise:
+ * 128-box tiles of one right-hand side, grids of 16 x 8 leaf boxes or more), 1 P2M tile
  * width (16, 32 or 64; 128, FP32 operator only: the split-FP16 P2M, 32-box
  * tiles of two right-hand sides, 4 or more right-hand sides), 2 M2L tile
  * variant of `level` (0: 4x4 boxes x 8 samples, 1: 8x8 x 8, 2: 8x8 x 16,

@@ -797,6 +797,29 @@ int Engine::launch_near(const cplx<TI>* y, cplx<TI>* r, int nrhs, const int* act
   const cplx<T>* loc = d.agl + lvl_off_[Lf];
   const NearPost<TI> pp = post ? *post : NearPost<TI>();
   const int kt = sizeof(T) == 4 ? KT_NEAR32 : KT_NEAR;
+  if (nb == kNearSplitWide) {
+    // split-FP16 kernel with 128-box tiles of one right-hand side
+    if constexpr (sizeof(T) == 4 && sizeof(TI) == 4) {
+      if (with_near && B % 128 == 0 && near16_ && bas16_ && pp.mode != 3) {
+        dim3 grid(B / 128, nrhs), blk(kBlock);
+        auto go = [&](auto modec) {
+          k_near_h16w<decltype(modec)::value><<<grid, blk, kH16WLds, stream_>>>(
+              y, N, loc, ls, r, N, (const v8h*)near16_, near16_et_, (const v8h*)bas16_, bas16_eb_, S, 1 << Lf, act,
+              pp);
+        };
+        prof_begin(kt);
+        if (pp.mode == 1)
+          go(std::integral_constant<int, 1>());
+        else if (pp.mode == 2)
+          go(std::integral_constant<int, 2>());
+        else
+          go(std::integral_constant<int, 0>());
+        prof_end(kt);
+        return B / 128;
+      }
+    }
+    nb = kNearSplit;
+  }
   if (nb == kNearSplit) {
     // split-FP16 kernel: FP32 operator on FP32 vectors, near field included,
     // 64-box tiles, BiCGSTAB post steps (not the FP64 residual check)
@@ -1420,12 +1443,13 @@ void Engine::autotune_prec(int nrhs, int reps) {
   };
   // near field: tile widths, and for the FP32 operator the split-FP16 kernel,
   // timed on vectors of the operator's own type (those of the solver's products)
-  const int widths[4] = {16, 32, 64, kNearSplit};
+  const int widths[5] = {16, 32, 64, kNearSplit, kNearSplitWide};
   auto width_ok = [&](int c) {
     if (c == 3) return sizeof(T) == 4 && P_.B % 64 == 0 && near16_ != nullptr;
+    if (c == 4) return sizeof(T) == 4 && P_.B % 128 == 0 && near16_ != nullptr;
     return P_.B % widths[c] == 0 && (widths[c] < 64 || sizeof(T) == 4);
   };
-  int c = pick(4, width_ok, [&](int c) {
+  int c = pick(5, width_ok, [&](int c) {
     launch_near<T, T>((const cplx<T>*)y, (cplx<T>*)op, nrhs, nullptr, true, widths[c]);
   });
   tu.near_nb = c < 0 ? -1 : widths[c];
@@ -1482,7 +1506,7 @@ bool Engine::tuning_set(int prec, int cls, int stage, int level, int value) {
   if ((prec != PREC_FP64 && prec != PREC_MIXED) || cls < 0 || cls > 1 || stage < 0 || stage >= ST_COUNT) return false;
   Tuning& t = tune_[prec == PREC_MIXED ? 1 : 0][cls];
   if (stage == ST_NEAR || stage == ST_P2M) {
-    const bool split = prec == PREC_MIXED && value == kNearSplit;
+    const bool split = prec == PREC_MIXED && (value == kNearSplit || (stage == ST_NEAR && value == kNearSplitWide));
     if (value != -1 && value != 16 && value != 32 && value != 64 && !split) return false;
     (stage == ST_NEAR ? t.near_nb : t.p2m_nb) = value;
     return true;

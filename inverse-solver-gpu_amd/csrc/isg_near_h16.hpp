@@ -164,6 +164,25 @@ static __global__ void __launch_bounds__(256) k_split16(const zf* __restrict__ s
   for (int P = 0; P < 4; P++) out[((long)(b * 4 + P) * 4 + g) * ncolp + m] = ph[P];
 }
 
+// As h16_cmac with the negation on the data side: na = -a2, -a3 (the table
+// fragments are then used as loaded; fewer live registers when one table
+// fragment serves several data fragments one after another).
+__device__ __forceinline__ void h16_cmac_na(const v8h (&a)[4], const v8h (&na)[2], const v8h (&b)[4], v4f& cr,
+                                            v4f& ci) {
+  cr = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[0], b[0], cr, 0, 0, 0);
+  ci = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[0], b[2], ci, 0, 0, 0);
+  cr = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[0], b[1], cr, 0, 0, 0);
+  ci = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[0], b[3], ci, 0, 0, 0);
+  cr = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[1], b[0], cr, 0, 0, 0);
+  ci = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[1], b[2], ci, 0, 0, 0);
+  cr = __builtin_amdgcn_mfma_f32_16x16x32_f16(na[0], b[2], cr, 0, 0, 0);
+  ci = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[2], b[0], ci, 0, 0, 0);
+  cr = __builtin_amdgcn_mfma_f32_16x16x32_f16(na[0], b[3], cr, 0, 0, 0);
+  ci = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[2], b[1], ci, 0, 0, 0);
+  cr = __builtin_amdgcn_mfma_f32_16x16x32_f16(na[1], b[2], cr, 0, 0, 0);
+  ci = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[3], b[0], ci, 0, 0, 0);
+}
+
 // Split tables, built once per operator (k_split16):
 //   tab[s][kc][part][g][m] (8 FP16 each): 2^et near[s][32 kc + 8 g + e][m], e = 0..7
 //   bas[kc][part][g][m]    (8 FP16 each): 2^eb basisT[32 kc + 8 g + e][m] (0 for k >= S)
@@ -480,6 +499,301 @@ k_near_h16(const zf* __restrict__ y, long ys, const zf* __restrict__ loc, long l
       block_store<2>(acc, post.part + (long)sel2(rhs, q) * kSlots * gridDim.x, gridDim.x, blockIdx.x);
     }
   }
+}
+
+// ---------------------------------------------------------------------------
+// Split-FP16 near field + L2P with 128-box tiles of ONE right-hand side
+// (tuning value kNearSplitWide): the same arithmetic and table as k_near_h16,
+// the same 128 MFMA rows per table fragment, but a 16 x 8 box tile (halo
+// 18 x 10 boxes: 1.41 x the tile's pixels staged instead of 1.56 x), no
+// half-idle pairs when systems of a batch have converged, and 51 KB of LDS
+// and at most 168 registers, so three workgroups share a CU (k_near_h16: two).
+// Wave w owns row blocks 4 (w / 2) .. +3 (16 boxes each) and column blocks
+// 2 (w % 2), +1.  Halo positions at a row pitch of 20 (= 4 mod 8, as 12 is),
+// stored XOR 2 g as in k_near_h16.  Grid (B / 128, nrhs).
+constexpr int kNearSplitWide = 256;
+constexpr int kH16WPitch = 20;
+constexpr int kH16WPos = kH16WPitch * 10;
+constexpr size_t kH16WLds = (size_t)4 * 4 * kH16WPos * 16;
+static_assert(kH16WLds >= (size_t)64 * 64 * 8 && kH16WLds >= (size_t)4 * 4 * 128 * 16, "LDS reuse");
+
+template <int POST>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2)))
+k_near_h16w(const zf* __restrict__ y, long ys, const zf* __restrict__ loc, long ls, zf* __restrict__ r, long rs,
+            const v8h* __restrict__ tab, int et, const v8h* __restrict__ bas, int eb, int S, int nbs,
+            const int* __restrict__ act, NearPost<float> post) {
+  extern __shared__ __attribute__((aligned(16))) char smem_raw[];
+  v8h* hal = reinterpret_cast<v8h*>(smem_raw);
+  __shared__ int hoff[180];
+  __shared__ float red[4][kBlock / 64];
+  const int rhs = blockIdx.y;
+  if (rhs_off(act, rhs)) return;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  const int rbh = w >> 1, mb0 = 2 * (w & 1);
+  const long b0 = (long)xcd_block() * 128;
+  const int X0 = (int)compact2((unsigned)b0), Y0 = (int)compact2((unsigned)b0 >> 1);
+  for (int h = tid; h < 180; h += kBlock) {
+    const int sx = X0 - 1 + h % 18, sy = Y0 - 1 + h / 18;
+    hoff[h] = (sx >= 0 && sy >= 0 && sx < nbs && sy < nbs) ? (int)morton2(sx, sy) * 64 : -1;
+  }
+  __syncthreads();
+  const zf* yr = y + (long)rhs * ys;
+
+  // data scale: largest |re|, |im| over the halo, all loads issued first
+  int es;
+  {
+    constexpr int NU = 12;  // 180 boxes x 32 float4 = 5760 in two batches of 12 per thread
+    float m[1] = {0.f};
+    const float4* yq = reinterpret_cast<const float4*>(yr);
+#pragma unroll 1
+    for (int e0 = tid; e0 < 5760; e0 += NU * kBlock) {
+      float4 v[NU];
+#pragma unroll
+      for (int u = 0; u < NU; u++) {
+        const int e = e0 + u * kBlock, ec = e < 5760 ? e : 5759;
+        const int off = hoff[ec >> 5];
+        v[u] = yq[((off < 0 ? 0 : off) >> 1) + (ec & 31)];
+        if (!(e < 5760 && off >= 0)) v[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+      }
+#pragma unroll
+      for (int u = 0; u < NU; u++) m[0] = fmaxf(m[0], h16_absmax(v[u]));
+    }
+    block_max<1>(m, red);
+    es = h16_scale_exp(m[0]);
+  }
+
+  v4f cr[4][2], ci[4][2];
+#pragma unroll
+  for (int j = 0; j < 4; j++)
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+      cr[j][c] = v4f{0, 0, 0, 0};
+      ci[j][c] = v4f{0, 0, 0, 0};
+    }
+  // halo position of box 16 (4 rbh + j) + li for the centre slot
+  int hc[4];
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    const unsigned c = (unsigned)(16 * (4 * rbh + j) + li);
+    hc[j] = ((int)compact2(c >> 1) + 1) * kH16WPitch + (int)compact2(c) + 1;
+  }
+
+  const v8h* tl = tab + lk * 64 + 16 * mb0 + li;  // + (s * 2 + kc) * 1024 + P * 256 + 16 c
+  auto load_b = [&](v8h (&b)[2][4], const v8h* t) {
+#pragma unroll
+    for (int c = 0; c < 2; c++)
+#pragma unroll
+      for (int P = 0; P < 4; P++) b[c][P] = t[P * 256 + 16 * c];
+  };
+  const float sd = ldexpf(1.f, es);
+  v8h bn[2][4];
+#pragma unroll 1
+  for (int kc = 0; kc < 2; kc++) {
+    __syncthreads();  // the previous half is no longer read
+    load_b(bn, tl + (long)kc * 1024);  // in flight during the staging
+    {
+      // units (h, g) of 8 source pixels 32 kc + 8 g .. +7 of halo box h
+      float4 v[3][4];
+#pragma unroll
+      for (int t = 0; t < 3; t++) {
+        const int u = tid + t * kBlock, uc = u < 720 ? u : 719;
+        const int h = uc >> 2, g = uc & 3;
+        const int off = hoff[h];
+        const float4* src = reinterpret_cast<const float4*>(yr + (off < 0 ? 0 : off) + 32 * kc + 8 * g);
+#pragma unroll
+        for (int c = 0; c < 4; c++) v[t][c] = src[c];
+      }
+#pragma unroll
+      for (int t = 0; t < 3; t++) {
+        const int u = tid + t * kBlock;
+        if (u < 720) {
+          const int h = u >> 2, g = u & 3;
+          const bool ok = hoff[h] >= 0;  // zero outside the grid
+          const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+          v8h ph[4];
+          h16_split4(ok ? v[t][0] : z, ok ? v[t][1] : z, sd, ph, 0);
+          h16_split4(ok ? v[t][2] : z, ok ? v[t][3] : z, sd, ph, 4);
+          const int p = (h / 18) * kH16WPitch + h % 18;
+#pragma unroll
+          for (int P = 0; P < 4; P++) hal[(P * 4 + g) * kH16WPos + (p ^ (2 * g))] = ph[P];
+        }
+      }
+    }
+    __syncthreads();
+    const v8h* hb = hal + lk * kH16WPos;  // + P * 4 * kH16WPos + position
+#pragma unroll 1
+    for (int s = 0; s < 9; s++) {
+      const int off = (s / 3 - 1) * kH16WPitch + (s % 3 - 1);
+      v8h b[2][4];
+#pragma unroll
+      for (int c = 0; c < 2; c++)
+#pragma unroll
+        for (int P = 0; P < 4; P++) b[c][P] = bn[c][P];
+      if (s + 1 < 9) load_b(bn, tl + (long)((s + 1) * 2 + kc) * 1024);  // the next slot's, during these MFMAs
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const int p = (hc[j] + off) ^ (2 * lk);
+        v8h a[4], na[2];
+#pragma unroll
+        for (int P = 0; P < 4; P++) a[P] = hb[P * 4 * kH16WPos + p];
+        na[0] = h16_neg(a[2]);
+        na[1] = h16_neg(a[3]);
+#pragma unroll
+        for (int c = 0; c < 2; c++) h16_cmac_na(a, na, b[c], cr[j][c], ci[j][c]);
+      }
+    }
+  }
+
+  // ---- L2P: locals of the tile's 128 boxes in chunks of 32; units (box, g):
+  // locals 32 kc + 8 g .. +7 of box b0 + box, loaded a chunk ahead
+  const zf* lr = loc + (long)rhs * ls + b0 * S;
+  float4 lv[2][4];
+  auto load_loc = [&](int kc) {
+#pragma unroll
+    for (int t = 0; t < 2; t++) {
+      const int u = tid + t * kBlock;
+      const int bx = u >> 2, g = u & 3;
+      const zf* lb = lr + (long)bx * S;
+#pragma unroll
+      for (int c = 0; c < 4; c++) {
+        const int k = 32 * kc + 8 * g + 2 * c;
+        lv[t][c] = *reinterpret_cast<const float4*>(lb + (k < S ? k : S - 2));
+      }
+    }
+  };
+  load_loc(0);
+  int el;
+  {
+    float m[1] = {0.f};
+    const int n4 = 64 * S;  // 128 boxes x S locals = 64 S float4 (S even)
+    constexpr int U = 9;
+    const float4* lq = reinterpret_cast<const float4*>(lr);
+    for (int e0 = tid; e0 < n4; e0 += U * kBlock) {
+      float4 t[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int e = e0 + u * kBlock;
+        t[u] = lq[e < n4 ? e : n4 - 1];
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++) m[0] = fmaxf(m[0], h16_absmax(t[u]));  // clamped repeats change no maximum
+    }
+    __syncthreads();  // red is read by the data scale above
+    block_max<1>(m, red);
+    el = h16_scale_exp(m[0]);
+  }
+  {
+    // near accumulators carry 2^(es + et); bring them to the L2P's 2^(el + eb)
+    const int n = (el + eb) - (es + et);
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+#pragma unroll
+      for (int c = 0; c < 2; c++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          cr[j][c][i] = ldexpf(cr[j][c][i], n);
+          ci[j][c][i] = ldexpf(ci[j][c][i], n);
+        }
+  }
+  const float sl = ldexpf(1.f, el);
+  const int nlc = (S + 31) / 32;
+#pragma unroll 1
+  for (int kc = 0; kc < nlc; kc++) {
+    __syncthreads();
+    v8h b[2][4];
+    load_b(b, bas + (long)kc * 1024 + lk * 64 + 16 * mb0 + li);  // in flight during the staging
+    {
+#pragma unroll
+      for (int t = 0; t < 2; t++) {
+        const int u = tid + t * kBlock;
+        const int bx = u >> 2, g = u & 3;
+        float4 v[4];
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+          const int k = 32 * kc + 8 * g + 2 * c;  // pairs (k, k + 1) lie within S or beyond it together
+          v[c] = k < S ? lv[t][c] : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+        v8h ph[4];
+        h16_split4(v[0], v[1], sl, ph, 0);
+        h16_split4(v[2], v[3], sl, ph, 4);
+#pragma unroll
+        for (int P = 0; P < 4; P++) hal[(P * 4 + g) * 128 + (bx ^ (2 * g))] = ph[P];
+      }
+    }
+    __syncthreads();
+    if (kc + 1 < nlc) load_loc(kc + 1);
+    const v8h* hb = hal + lk * 128;  // + P * 512 + box
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      v8h a[4], na[2];
+#pragma unroll
+      for (int P = 0; P < 4; P++) a[P] = hb[P * 512 + ((16 * (4 * rbh + j) + li) ^ (2 * lk))];
+      na[0] = h16_neg(a[2]);
+      na[1] = h16_neg(a[3]);
+#pragma unroll
+      for (int c = 0; c < 2; c++) h16_cmac_na(a, na, b[c], cr[j][c], ci[j][c]);
+    }
+  }
+
+  // ---- output in two halves of 64 boxes through LDS; D[box][m]: box
+  // 16 (4 rbh + j) + 4 lk + i (register i), pixel m = 16 (mb0 + c) + li
+  zf* tile = reinterpret_cast<zf*>(smem_raw);
+  const int n = -(el + eb);
+  zd acc[2] = {mkc<double>(0, 0), mkc<double>(0, 0)};
+  const bool herm = post.herm != 0;
+#pragma unroll 1
+  for (int hh = 0; hh < 2; hh++) {
+    __syncthreads();
+    if (rbh == hh) {
+#pragma unroll
+      for (int j = 0; j < 4; j++)
+#pragma unroll
+        for (int c = 0; c < 2; c++)
+#pragma unroll
+          for (int i = 0; i < 4; i++)
+            tile[(16 * j + 4 * lk + i) * 64 + 16 * (mb0 + c) + li] =
+                mkc<float>(ldexpf(cr[j][c][i], n), ldexpf(ci[j][c][i], n));
+    }
+    __syncthreads();
+    const long o = (long)rhs * rs + (b0 + 64 * hh) * 64;
+    if constexpr (POST == 0) {
+#pragma unroll 4
+      for (int e = tid; e < 64 * 64; e += kBlock) r[o + e] = tile[e];
+    } else {
+      const zd* gr = post.g + (long)rhs * post.gs + (b0 + 64 * hh) * 64;
+      struct In {
+        zf x, aux;  // aux: r~ (mode 1)
+        zd g;
+      };
+      stage<2, In>(
+          64 * 64,
+          [&](int e) {
+            In v;
+            v.x = post.x[o + e];
+            if (POST == 1) v.aux = post.rt[o + e];
+            if (herm) v.g = gr[e];  // the forward post step does not use g
+            return v;
+          },
+          [&](int e, In v) {
+            const zf gt = tile[e];
+            const zd gy = mkc<double>((double)gt.x, (double)gt.y);
+            const zd xv = ccast<double>(v.x);
+            const zd px = herm ? csub(xv, cconj(cmul(v.g, gy))) : csub(xv, gy);
+            const zd res = stv(r, o + e, px);
+            if (POST == 1) {
+              const zd a = ccast<double>(v.aux);
+              acc[0].x += a.x * res.x + a.y * res.y;
+              acc[0].y += a.x * res.y - a.y * res.x;
+            } else {
+              acc[0].x += res.x * xv.x + res.y * xv.y;
+              acc[0].y += res.x * xv.y - res.y * xv.x;
+              acc[1].x += res.x * res.x + res.y * res.y;
+            }
+          });
+    }
+  }
+  if constexpr (POST != 0) block_store<2>(acc, post.part + (long)rhs * kSlots * gridDim.x, gridDim.x, blockIdx.x);
 }
 
 // P2M of the FP32 operator on split FP16: agm[box][s] = sum_n coeffT[n][s]

@@ -16,7 +16,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def stub_library(split=True):
+def stub_library(split=128):
     m = types.ModuleType("isg")
 
     class IsgError(RuntimeError):
@@ -54,7 +54,7 @@ def stub_library(split=True):
 
         def tuning_get(self, precision, batch_class, stage, level=0):
             if stage == "near":
-                return 128 if split else 32
+                return split or 32
             return 1 if stage == "leaf" else -1
 
     class Dbim:
@@ -87,7 +87,7 @@ def stub_library(split=True):
     return m
 
 
-def run_bench(monkeypatch, *args, split=True):
+def run_bench(monkeypatch, *args, split=128):
     monkeypatch.setitem(sys.modules, "isg", stub_library(split))
     monkeypatch.syspath_prepend(ROOT)
     import bench
@@ -128,8 +128,16 @@ def test_bench_json_line_with_stub_library(monkeypatch):
 
 def test_bench_roofline_fp32_kernel_when_not_split(monkeypatch):
     d = run_bench(monkeypatch, "--workload", "dbim-256", "--steps", "1", "--warmup", "0", "--cpu-baseline", "off",
-                  "--ref-geometry", "off", split=False)
+                  "--ref-geometry", "off", split=0)
     r = d["roofline"]
     assert r["kernel"].startswith("k_near_mfma") and r["peak"] == 157.3
     assert r["frac_of_split_ceiling"] is None
     assert "reference_geometry" not in d
+
+
+def test_bench_roofline_wide_split_kernel(monkeypatch):
+    d = run_bench(monkeypatch, "--workload", "dbim-256", "--steps", "1", "--warmup", "0", "--cpu-baseline", "off",
+                  "--ref-geometry", "off", split=256)
+    r = d["roofline"]
+    assert r["kernel"].startswith("k_near_h16w") and r["peak"] == 2500.0
+    assert abs(r["frac_of_split_ceiling"] - 3 * r["frac"]) < 1e-3

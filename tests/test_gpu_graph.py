@@ -36,20 +36,22 @@ def test_graph_solve_bit_identical(isg, herm):
     np.testing.assert_array_equal(out[0][0], out[1][0])
 
 
-@pytest.mark.parametrize("split", [False, True])
-def test_graph_dbim_bit_identical(isg, split):
+@pytest.mark.parametrize("level,split", [(4, 0), (4, 128), (5, 256)])
+def test_graph_dbim_bit_identical(isg, level, split):
     """split: the FP32 operator's near field, L2P and P2M on the split-FP16
-    kernels (tuning value 128), captured and replayed like the others."""
-    print("graph vs eager batched DBIM, split kernels", split, flush=True)
-    o = oracle_for(4, numrx=16)
+    kernels (near tuning value 128: pairs of right-hand sides on 64-box
+    tiles; 256: one right-hand side on 128-box tiles, which needs level 5),
+    captured and replayed like the others."""
+    print("graph vs eager batched DBIM, level", level, "split kernels", split, flush=True)
+    o = oracle_for(level, numrx=16)
     otrue = isg.raster_to_tree(isg.phantom_blobs(o.side, 3, 1), o.unkmap())
     res = []
     for graph in (0, 1):
-        s = isg.Solver(level=4, numrx=16, numtx=16, toli=1e-6, iti=300, regular=float(np.sqrt(1e-3)), max_rhs=16,
-                       precision=1, graph=graph)
+        s = isg.Solver(level=level, numrx=16, numtx=16, toli=1e-6, iti=300, regular=float(np.sqrt(1e-3)),
+                       max_rhs=16, precision=1, graph=graph)
         if split:
             for cls in (0, 1):
-                s.tuning_set(1, cls, "near", 0, 128)
+                s.tuning_set(1, cls, "near", 0, split)
                 s.tuning_set(1, cls, "p2m", 0, 128)
         d = isg.Dbim(s, otrue, field_start=2)
         its = [d.iterate()["solve_iters"] for _ in range(2)]

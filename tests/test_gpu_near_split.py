@@ -1,6 +1,6 @@
-"""Split-FP16 kernels of the FP32 operator (tuning value 128: k_near_h16 for
-the near field + L2P, k_p2m_h16 for P2M) against the FP32 kernels and the
-FP64 oracle.
+"""Split-FP16 kernels of the FP32 operator (near tuning value 128: k_near_h16,
+256: k_near_h16w for the near field + L2P; P2M value 128: k_p2m_h16) against
+the FP32 kernels and the FP64 oracle.
 
 The kernels carry every FP32 operand as two FP16 numbers (hi + lo, 22
 significant bits) under power-of-two scales and multiply in the
@@ -63,7 +63,7 @@ def test_split_near_matches_fp32_kernel_and_oracle(isg, level):
 
     R32 = products(32, 32)
     refs = [o.mlfma(X[k]) for k in range(nrhs)]
-    for near, p2m in ((128, 32), (32, 128), (128, 128)):
+    for near, p2m in ((128, 32), (32, 128), (128, 128), (256, 128)):
         R16 = products(near, p2m)
         for k in range(nrhs):
             e16, e32, d = relerr(R16[k], refs[k]), relerr(R32[k], refs[k]), relerr(R16[k], R32[k])
@@ -73,8 +73,8 @@ def test_split_near_matches_fp32_kernel_and_oracle(isg, level):
 
 
 def test_split_near_mixed_bicgs_reaches_fp64_residual(isg):
-    """The solver's post-step epilogues (modes 1 and 2) in the split kernel
-    (and the split P2M):
+    """The solver's post-step epilogues (modes 1 and 2) in the split kernels
+    (pairs on 64-box tiles and single systems on 128-box tiles; split P2M):
     a batched mixed solve of 5 systems (plane waves from 5 directions, so
     they converge at different iterations and deflation masks workgroup
     pairs half off) meets the FP64 true residual (recomputed with the
@@ -82,16 +82,16 @@ def test_split_near_mixed_bicgs_reaches_fp64_residual(isg):
     import torch
     level, tol, nrhs = 5, 1e-6, 5
     o = oracle_for(level)
-    s = isg.Solver(level=level, numrx=16, toli=tol, iti=300, precision=1, max_rhs=nrhs)
-    s.tuning_set(1, 0, "near", 0, 128)
-    s.tuning_set(1, 0, "p2m", 0, 128)
     pos = o.pos()
     g = 0.4 * np.exp(-np.abs(pos) ** 2 / (2 * 2.5 ** 2)) * (1 + 0.05j) * o.k0 ** 2
     ang = np.linspace(0, 2 * np.pi, nrhs, endpoint=False) + 0.3
     B = np.stack([np.exp(1j * o.k0 * (np.cos(a) * pos.real + np.sin(a) * pos.imag)) for a in ang])
     g_d = torch.from_numpy(g).cuda()
     B_d = torch.from_numpy(B).cuda()
-    for herm in (False, True):
+    for near, herm in ((128, False), (128, True), (256, False), (256, True)):
+        s = isg.Solver(level=level, numrx=16, toli=tol, iti=300, precision=1, max_rhs=nrhs)
+        s.tuning_set(1, 0, "near", 0, near)
+        s.tuning_set(1, 0, "p2m", 0, 128)
         X_d = torch.zeros_like(B_d)
         torch.cuda.synchronize()
         _, infos = s.bicgs_dev(X_d.data_ptr(), g_d.data_ptr(), B_d.data_ptr(), nrhs, herm, 0, True)
@@ -102,7 +102,7 @@ def test_split_near_mixed_bicgs_reaches_fp64_residual(isg):
             x = X[k]
             ax = x - np.conj(g * o.mlfma(np.conj(x))) if herm else x - o.mlfma(g * x)
             res = np.linalg.norm(B[k] - ax) / np.linalg.norm(B[k])
-            print("herm %d rhs %d: %d iterations, FP64 residual %.2e" % (herm, k, infos[k]["iters"], res))
+            print("near %d herm %d rhs %d: %d iterations, FP64 residual %.2e" % (near, herm, k, infos[k]["iters"], res))
             assert res < tol * 1.05, (k, res)
 
 
