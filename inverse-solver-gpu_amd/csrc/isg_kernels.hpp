@@ -642,15 +642,19 @@ __global__ void __launch_bounds__(kBlock) k_interp_band(const cplx<T>* __restric
   __syncthreads();
   const int nrb = (Mrows + 15) / 16;
   const int nkq = KB >> 2;
-  // (row block, column block) items spread evenly over the 4 waves
-  for (int item = w; item < nrb * NCB; item += 4) {
-    const int rb = item / NCB, cb = item - rb * NCB;
+  // (row block, column block) items spread evenly over the 4 waves; the M2M
+  // with two column blocks takes both of a row block in one item, so that its
+  // band rows and sibling shifts (which do not depend on the column) are
+  // loaded once for both
+  constexpr bool RB_ITEMS = MODE == 0 && NCB == 2;
+  constexpr int CPI = RB_ITEMS ? NCB : 1;  // column blocks per item
+  for (int item = w; item < nrb * NCB / CPI; item += 4) {
+    const int rb = RB_ITEMS ? item : item / NCB, cb0 = RB_ITEMS ? 0 : item - rb * NCB;
     const T* ab = AB + (long)rb * KB * 16;
     T a[KQ];
 #pragma unroll
     for (int q = 0; q < KQ; q++) a[q] = (q < nkq) ? ab[(4 * q + lk) * 16 + li] : (T)0;
     const int s0 = s0v[rb];
-    const cplx<T>* tcol = tile + (16 * cb + li) * row;
     // the epilogue's operands (M2M: sibling shifts; L2L: the child locals it
     // adds to) are loaded here, so their latency overlaps the MFMA chain
     const int k = 16 * rb + li;
@@ -663,56 +667,61 @@ __global__ void __launch_bounds__(kBlock) k_interp_band(const cplx<T>* __restric
           if (!Mfma<T>::kRowsAcrossLanes || q == 0)
             pre[q] = shift[(Mfma<T>::kRowsAcrossLanes ? lk : q) * Mrows + k];
         } else if (accumulate) {
-          pre[q] = dst[rhs * ds + (4 * p0 + 16 * cb + Mfma<T>::row(lk, q)) * (long)Mrows + k];
+          pre[q] = dst[rhs * ds + (4 * p0 + 16 * cb0 + Mfma<T>::row(lk, q)) * (long)Mrows + k];
         }
       }
     }
-    v4 cr0 = v4{0, 0, 0, 0}, ci0 = v4{0, 0, 0, 0};
 #pragma unroll
-    for (int q = 0; q < KQ; q++) {
-      if (q < nkq) {
-        int j = s0 + 4 * q + lk;
-        if (j >= K) j -= K;
-        const cplx<T> bv = tcol[j];
-        cr0 = Mfma<T>::mac(bv.x, a[q], cr0);
-        ci0 = Mfma<T>::mac(bv.y, a[q], ci0);
+    for (int cc = 0; cc < CPI; cc++) {
+      const int cb = cb0 + cc;
+      const cplx<T>* tcol = tile + (16 * cb + li) * row;
+      v4 cr0 = v4{0, 0, 0, 0}, ci0 = v4{0, 0, 0, 0};
+#pragma unroll
+      for (int q = 0; q < KQ; q++) {
+        if (q < nkq) {
+          int j = s0 + 4 * q + lk;
+          if (j >= K) j -= K;
+          const cplx<T> bv = tcol[j];
+          cr0 = Mfma<T>::mac(bv.x, a[q], cr0);
+          ci0 = Mfma<T>::mac(bv.y, a[q], ci0);
+        }
       }
-    }
-    // D[col][k]: lane li holds target sample k = 16 rb + li; col = 16 cb + row(lk, q)
-    if (MODE == 0) {
-      // parent = sum over the 4 sibling columns of shiftM[c][k] * D[col][k]
-      if (Mfma<T>::kRowsAcrossLanes) {
-        // col = 16cb + lk + 4q: sibling c = lk, parent 4cb + q
-        const cplx<T> s1 = pre[0];
+      // D[col][k]: lane li holds target sample k = 16 rb + li; col = 16 cb + row(lk, q)
+      if (MODE == 0) {
+        // parent = sum over the 4 sibling columns of shiftM[c][k] * D[col][k]
+        if (Mfma<T>::kRowsAcrossLanes) {
+          // col = 16cb + lk + 4q: sibling c = lk, parent 4cb + q
+          const cplx<T> s1 = pre[0];
+#pragma unroll
+          for (int q = 0; q < 4; q++) {
+            T vr = cr0[q], vi = ci0[q];
+            T xr = vr * s1.x - vi * s1.y, xi = vr * s1.y + vi * s1.x;
+            xr += __shfl_xor(xr, 16, 64);
+            xi += __shfl_xor(xi, 16, 64);
+            xr += __shfl_xor(xr, 32, 64);
+            xi += __shfl_xor(xi, 32, 64);
+            if (lk == 0 && k < Mrows) dst[rhs * ds + (p0 + 4 * cb + q) * Mrows + k] = mkc<T>(xr, xi);
+          }
+        } else {
+          // col = 16cb + 4lk + q: sibling c = q, parent 4cb + lk
+          T xr = 0, xi = 0;
+#pragma unroll
+          for (int q = 0; q < 4; q++) {
+            const cplx<T> sq = pre[q];
+            T vr = cr0[q], vi = ci0[q];
+            xr += vr * sq.x - vi * sq.y;
+            xi += vr * sq.y + vi * sq.x;
+          }
+          if (k < Mrows) dst[rhs * ds + (p0 + 4 * cb + lk) * Mrows + k] = mkc<T>(xr, xi);
+        }
+      } else if (k < Mrows) {
 #pragma unroll
         for (int q = 0; q < 4; q++) {
-          T vr = cr0[q], vi = ci0[q];
-          T xr = vr * s1.x - vi * s1.y, xi = vr * s1.y + vi * s1.x;
-          xr += __shfl_xor(xr, 16, 64);
-          xi += __shfl_xor(xi, 16, 64);
-          xr += __shfl_xor(xr, 32, 64);
-          xi += __shfl_xor(xi, 32, 64);
-          if (lk == 0 && k < Mrows) dst[rhs * ds + (p0 + 4 * cb + q) * Mrows + k] = mkc<T>(xr, xi);
+          const int col = 16 * cb + Mfma<T>::row(lk, q);
+          cplx<T>* o = dst + rhs * ds + (4 * p0 + col) * (long)Mrows + k;
+          cplx<T> v = mkc<T>(cr0[q], ci0[q]);
+          *o = accumulate ? cadd(pre[q], v) : v;
         }
-      } else {
-        // col = 16cb + 4lk + q: sibling c = q, parent 4cb + lk
-        T xr = 0, xi = 0;
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-          const cplx<T> sq = pre[q];
-          T vr = cr0[q], vi = ci0[q];
-          xr += vr * sq.x - vi * sq.y;
-          xi += vr * sq.y + vi * sq.x;
-        }
-        if (k < Mrows) dst[rhs * ds + (p0 + 4 * cb + lk) * Mrows + k] = mkc<T>(xr, xi);
-      }
-    } else if (k < Mrows) {
-#pragma unroll
-      for (int q = 0; q < 4; q++) {
-        const int col = 16 * cb + Mfma<T>::row(lk, q);
-        cplx<T>* o = dst + rhs * ds + (4 * p0 + col) * (long)Mrows + k;
-        cplx<T> v = mkc<T>(cr0[q], ci0[q]);
-        *o = accumulate ? cadd(pre[q], v) : v;
       }
     }
   }
