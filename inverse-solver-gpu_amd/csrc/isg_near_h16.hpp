@@ -5,8 +5,8 @@
 // Every FP32 operand x is carried as two FP16 numbers, hi = rne16(2^e x) and
 // lo = rne16(2^e x - hi), with a power-of-two scale 2^e chosen so that
 // |2^e x| < 2^14 -- per table, and per workgroup and right-hand side for the
-// data.  hi + lo holds x to 2^-22 relative (elements below 2^-18 of the
-// largest one to 2^-40 of the largest, absolutely).  A product x a is formed
+// data.  hi + lo holds x to 2^-22 relative (elements below 2^-16 of the
+// largest one, whose lo is subnormal, to 2^-38 of the largest, absolutely).  A product x a is formed
 // as hi.hi + hi.lo + lo.hi in FP32 accumulators (the dropped lo.lo term is
 // below 2^-22 relative): three FP16 MFMAs, each 16 times the FP32 rate, for
 // one FP32 MFMA's worth of products.  Complex products take the four-product
