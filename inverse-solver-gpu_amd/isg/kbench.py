@@ -30,7 +30,8 @@ def main():
     ap.add_argument("--reps", type=int, default=3)
     ap.add_argument("--copy-mib", type=int, default=0)
     ap.add_argument("--near", type=int, default=None,
-                    help="force the near-field variant (16, 32 or 64 boxes per tile; 128: the split-FP16 kernel of the FP32 operator)")
+                    help="force the near-field variant (16, 32 or 64 boxes per tile; 128: the split-FP16 kernel of the FP32 "
+                         "operator on pairs of right-hand sides, 256: the same on 128-box tiles of one)")
     ap.add_argument("--p2m", type=int, default=None,
                     help="force the P2M variant (16, 32 or 64 boxes per tile; 128: the split-FP16 kernel of the FP32 "
                          "operator)")
